@@ -1,0 +1,162 @@
+/*
+ * qann_hip.h -- the C ABI between the host index objects and the MI355X
+ * (gfx950) search kernels of qann_amd.
+ *
+ * The reference (maxtmiller/qann) has no device boundary: its search runs
+ * in-process behind the C++ virtual hooks DenseIndex::query_batch_impl /
+ * query_impl (include/vecengine/index.hpp:277-297). This header is the seam a
+ * host implementation of those hooks calls instead of its CPU loops; every
+ * entry point below names the reference code whose work it takes over.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only. Pointers are host memory unless a
+ *     parameter says "device".
+ *   - Every int-returning function returns a status: QANN_OK, or an error
+ *     whose message qann_hip_last_error() returns on the calling thread.
+ *     QANN_EINVAL maps to the reference's std::invalid_argument (Python
+ *     ValueError), QANN_ESTATE / QANN_EDEVICE to std::runtime_error /
+ *     std::logic_error (RuntimeError), QANN_ENOMEM to MemoryError.
+ *   - Queries arrive prepared as DenseIndex::query_prepared prepares them
+ *     (src/index.cpp:170-184): unit length for Metric::Cosine. Rotation of a
+ *     rotated IVF index (src/ivf_index.cpp:200-204) is applied on the device.
+ *   - Distances come back in the reference's internal form: squared L2 between
+ *     prepared vectors, or -(q . x) for inner product; the caller halves them
+ *     for Metric::Cosine (l2_to_cosine, src/index.cpp:23-25).
+ *   - Result rows are padded with id -1 and distance +inf, as results_padded
+ *     does (bindings/python_bindings.cpp:582-594).
+ *   - Host input buffers are borrowed for the duration of the call; the index
+ *     keeps its own device copies.
+ *   - qann_hip_search / qann_hip_search_async may run concurrently on one
+ *     handle (the reference's shared-lock queries); the create / set calls need
+ *     exclusive access, which the host holds as write_locked does.
+ */
+#ifndef QANN_HIP_H
+#define QANN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    QANN_OK = 0,
+    QANN_EINVAL = 1,
+    QANN_ESTATE = 2,
+    QANN_ENOMEM = 3,
+    QANN_EDEVICE = 4,
+};
+
+/* Metric values are the reference's file-format values (index.hpp:57-62). */
+enum { QANN_METRIC_L2 = 0, QANN_METRIC_COSINE = 1, QANN_METRIC_IP = 2 };
+
+typedef struct qann_hip_index qann_hip_index;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qann_hip_last_error(void);
+
+/* Number of visible GPUs (0 without a usable device). */
+int qann_hip_device_count(int* count);
+
+/*
+ * An IVF index with 4-bit PQ fast-scan codes: what IVFIndex::load_body builds
+ * (src/ivf_index.cpp:976-1122) after enable_pq(m, 16, fast_scan=true), moved
+ * into HBM. The device derives what IVFIndex derives on load: centroid and
+ * residual-centre squared norms (update_centroid_sqnorms, :297-303), the PQ
+ * transpose (PQCodebook::build_tables, src/pq.cpp:266-275) and, when
+ * use_precomputed is set, the per-list term B (build_precomputed, :66-81 with
+ * PQCodebook::compute_list_term, src/pq.cpp:206-223).
+ */
+typedef struct {
+    uint32_t dim;
+    uint32_t nlist;           /* 100 .. 65535 */
+    uint32_t metric;          /* QANN_METRIC_* */
+    uint32_t m;               /* PQ subspaces: even, <= 1024, dim % m == 0 */
+    uint32_t cosine_ip;       /* Metric::Cosine scored by inner product (ivf_index.hpp:116) */
+    uint32_t use_precomputed; /* build B (the caller applies the reference's rule and 256 MiB cap) */
+    const float* centroids;        /* nlist * dim, coarse centroids */
+    const float* residual_centers; /* nlist * dim, or NULL (format 13, Cosine PQ) */
+    const float* pq_centroids;     /* m * 16 * (dim / m), the saved [s][c][sub] layout */
+    const float* rotation;         /* dim * dim row-major R (x * R), or NULL */
+} qann_hip_ivf_desc;
+
+/* Creates the device index on the current GPU (replaces IVFIndex::load_body's
+ * in-RAM structures for the query path). */
+int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out);
+
+/*
+ * Uploads every inverted list at once (InvertedList::ids / codes,
+ * ivf_index.hpp:32-50): sizes[l] entries for list l; slots holds the lists'
+ * slot ids back to back (sum(sizes) of them); codes holds each list's
+ * ceil(sizes[l] / 32) * m * 16 fast-scan bytes back to back, in the
+ * reference's block layout (src/fast_scan.hpp:19-34).
+ */
+int qann_hip_ivf_set_lists(qann_hip_index* index, const uint32_t* sizes, const uint32_t* slots, const uint8_t* codes);
+
+/* A flat index of raw rows (FlatIndex, src/flat_index.cpp): slots rows of dim floats. */
+int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint64_t slots, qann_hip_index** out);
+
+/*
+ * A RefineIndex over an IVF fast-scan device index (src/refine_index.cpp):
+ * holds the raw rows it re-ranks with (VectorStore, one row per slot). base
+ * must outlive the refine handle.
+ */
+int qann_hip_refine_create(qann_hip_index* base, const float* rows, uint64_t slots, qann_hip_index** out);
+
+/*
+ * The slot -> id map and deleted flags (detail::IdMap, include/vecengine/
+ * id_map.hpp:81-89): labels is NULL when ids equal slots; deleted holds one
+ * byte (0/1) per slot, or NULL for none deleted.
+ */
+int qann_hip_set_ids(qann_hip_index* index, uint64_t slots, const int64_t* labels, const uint8_t* deleted);
+
+typedef struct {
+    uint32_t k;                    /* results per query */
+    uint32_t nprobe;               /* IVF: lists probed (IVFIndex::nprobe) */
+    uint32_t k_factor;             /* Refine: candidates per result (RefineIndex::k_factor) */
+    uint32_t filter_probe_scaling; /* IVFIndex::filter_probe_scaling */
+    /* Filter (detail::SlotFilter, id_map.hpp:20-35): NULL for none, else
+     * ceil(slots / 64) words of allowed live slots (host memory for
+     * qann_hip_search, device memory for qann_hip_search_async) and their count. */
+    const uint64_t* allowed_words;
+    uint64_t allowed_count;
+} qann_hip_search_params;
+
+/*
+ * Top-k search of nq prepared queries (nq * dim floats): the work of
+ * DenseIndex::query_batch -> {IVFIndex, RefineIndex, FlatIndex}::
+ * query_batch_impl (src/ivf_index.cpp:258-295, src/refine_index.cpp:92-110,
+ * src/index.cpp:186-191). Writes nq * k ids and distances. Synchronous.
+ */
+int qann_hip_search(qann_hip_index* index, const float* queries, uint64_t nq, const qann_hip_search_params* params,
+                    int64_t* out_ids, float* out_dists);
+
+/*
+ * qann_hip_search with queries, results and allowed words in device memory,
+ * enqueued on stream (a hipStream_t; NULL = the default stream) and returning
+ * before the work finishes. Calls on one stream run in order; calls on
+ * different streams may overlap on the device.
+ */
+int qann_hip_search_async(qann_hip_index* index, const float* d_queries, uint64_t nq,
+                          const qann_hip_search_params* params, int64_t* d_out_ids, float* d_out_dists, void* stream);
+
+/*
+ * Nearest centroid of each of n rows (detail::assign_nearest without BLAS,
+ * src/kmeans.cpp:34-100 / nearest_centroid): the smallest exact squared L2
+ * distance, the lowest centroid index on ties. Used when vectors are added
+ * to an IVF index (IVFIndex::add_batch_impl, src/ivf_index.cpp:124-125) and by
+ * training. Host buffers; synchronous.
+ */
+int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const float* centroids, uint32_t k,
+                            uint32_t* out);
+
+/* Device bytes the index holds. */
+int qann_hip_index_bytes(qann_hip_index* index, uint64_t* bytes);
+
+void qann_hip_index_destroy(qann_hip_index* index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QANN_HIP_H */

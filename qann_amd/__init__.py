@@ -1,0 +1,42 @@
+"""qann_amd -- qann's IVF 4-bit PQ fast-scan search with exact re-ranking on
+AMD MI355X (gfx950).
+
+The API mirrors the reference's Python module ``qann`` for the dense indexes
+on this path: ``FlatIndex``, ``IVFIndex`` (fast-scan PQ), ``RefineIndex``,
+``Metric`` and ``load``. Files saved by ``qann`` load here unchanged and
+files saved here load in ``qann``. Every query runs on the GPU through the
+HIP library ``libqann_hip.so``; there is no CPU search path, so importing
+this package fails loudly when the native libraries are not built.
+"""
+from __future__ import annotations
+
+import pathlib
+
+_here = pathlib.Path(__file__).resolve().parent
+try:
+    from . import _qann  # noqa: F401  (loads libqann_hip.so through its rpath)
+except ImportError as e:  # pragma: no cover - exercised only on a broken build
+    raise ImportError(
+        f"qann_amd's native extension is not built or cannot load ({e}); run `python qann_amd/build.py`"
+    ) from e
+
+from ._qann import (  # noqa: E402
+    DenseIndex,
+    FlatIndex,
+    IVFIndex,
+    Metric,
+    RefineIndex,
+    get_num_threads,
+    load,
+    set_num_threads,
+)
+
+Index = DenseIndex
+__version__ = "0.1.0"
+__all__ = ["DenseIndex", "Index", "FlatIndex", "IVFIndex", "RefineIndex", "Metric", "load", "set_num_threads",
+           "get_num_threads", "native_library"]
+
+
+def native_library() -> pathlib.Path:
+    """Path of the HIP library every search runs through."""
+    return _here / "libqann_hip.so"

@@ -1,0 +1,692 @@
+// The C ABI of include/qann_hip.h: device indexes in HBM and the search
+// pipelines that run on them.
+#include "../../../include/qann_hip.h"
+#include "device_common.hpp"
+#include "ivf_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    const int code = e == hipErrorOutOfMemory ? QANN_ENOMEM : QANN_EDEVICE;
+    throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void check_launch(const char* what) { check(hipGetLastError(), what); }
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return QANN_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of host memory";
+        return QANN_ENOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return QANN_ESTATE;
+    }
+}
+
+void invalid(const std::string& m) { throw Error(QANN_EINVAL, m); }
+
+// A bump allocator over one device buffer, with the stream its work runs on.
+struct Scratch {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    char* io = nullptr; // staging for the synchronous entry point's inputs and outputs
+    size_t io_cap = 0;
+
+    char* reserve_io(size_t bytes) {
+        if (bytes > io_cap) {
+            if (io) check(hipFree(io), "hipFree");
+            io = nullptr;
+            io_cap = 0;
+            check(hipMalloc(&io, bytes), "hipMalloc staging");
+            io_cap = bytes;
+        }
+        return io;
+    }
+
+    void reserve(size_t bytes) {
+        if (bytes <= cap) return;
+        if (base) {
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            check(hipFree(base), "hipFree");
+            base = nullptr;
+            cap = 0;
+        }
+        const size_t want = std::max(bytes, cap * 3 / 2);
+        check(hipMalloc(&base, want), "hipMalloc scratch");
+        cap = want;
+    }
+    template <class T>
+    T* take(size_t n) {
+        const size_t bytes = (n * sizeof(T) + 255) & ~size_t{255};
+        if (used + bytes > cap) throw Error(QANN_ESTATE, "scratch plan too small");
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        return p;
+    }
+};
+
+size_t pad256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+} // namespace
+
+struct qann_hip_index {
+    int kind = 0; // 1 flat, 2 ivf, 3 refine
+    int device = 0;
+    uint32_t dim = 0, metric = 0;
+    // IVF
+    qd::DevIvf ivf{};
+    bool use_pre = false;
+    std::vector<uint32_t> host_list_sizes;
+    // ids
+    uint64_t slots = 0, live = 0;
+    int64_t* labels = nullptr;
+    uint64_t* deleted = nullptr;
+    // raw rows (flat, refine)
+    float* rows = nullptr;
+    qann_hip_index* base = nullptr;
+    std::vector<void*> allocs;
+    uint64_t bytes = 0;
+
+    std::mutex mu;
+    std::vector<Scratch*> free_scratch;         // for synchronous calls
+    std::map<hipStream_t, Scratch*> stream_scratch; // for async calls, per caller stream
+
+    template <class T>
+    T* upload(const T* host, size_t n) {
+        if (n == 0) return nullptr;
+        T* d = alloc<T>(n);
+        check(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+        return d;
+    }
+    template <class T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        check(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 1)), "hipMalloc");
+        allocs.push_back(p);
+        bytes += n * sizeof(T);
+        return static_cast<T*>(p);
+    }
+    void free_ptr(void* p) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) {
+            (void)hipFree(p);
+            allocs.erase(it);
+        }
+    }
+    ~qann_hip_index() {
+        for (Scratch* s : free_scratch) {
+            if (s->base) (void)hipFree(s->base);
+            if (s->io) (void)hipFree(s->io);
+            if (s->own_stream) (void)hipStreamDestroy(s->stream);
+            delete s;
+        }
+        for (auto& kv : stream_scratch) {
+            (void)hipStreamSynchronize(kv.first);
+            if (kv.second->base) (void)hipFree(kv.second->base);
+            delete kv.second;
+        }
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+// Deleted flags (one byte per slot) as a bitmap.
+std::vector<uint64_t> to_bitmap(const uint8_t* flags, uint64_t slots, uint64_t& set) {
+    std::vector<uint64_t> w((slots + 63) / 64, 0);
+    set = 0;
+    if (!flags) return w;
+    for (uint64_t s = 0; s < slots; ++s)
+        if (flags[s]) {
+            w[s >> 6] |= uint64_t{1} << (s & 63);
+            ++set;
+        }
+    return w;
+}
+
+// The set slots of a bitmap, in any order (the selections that read them key
+// every candidate by its slot). Writes at most cap.
+__global__ void k_bitmap_slots(const uint64_t* words, uint64_t nwords, uint32_t* out, uint32_t* count, uint32_t cap) {
+    const uint64_t w = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (w >= nwords) return;
+    uint64_t bits = words[w];
+    if (!bits) return;
+    uint32_t pos = atomicAdd(count, static_cast<uint32_t>(__popcll(bits)));
+    while (bits && pos < cap) {
+        const int b = __builtin_ctzll(bits);
+        bits &= bits - 1;
+        out[pos++] = static_cast<uint32_t>(w * 64 + b);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Compacts the filter bitmap into slot_list (cap entries).
+void bitmap_slots(const uint64_t* d_words, uint64_t nwords, uint32_t* slot_list, uint32_t* cnt, uint32_t cap,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, cnt, 1u);
+    hipLaunchKernelGGL(k_bitmap_slots, dim3(static_cast<unsigned>((nwords + 255) / 256)), dim3(256), 0, st, d_words,
+                       nwords, slot_list, cnt, cap);
+    check_launch("k_bitmap_slots");
+}
+
+__global__ void k_fill_results(int64_t* ids, float* dists, size_t n) {
+    const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    ids[i] = -1;
+    dists[i] = INFINITY;
+}
+
+__global__ void k_copy_results(const int64_t* src_ids, const float* src_d, int kp, int k, int nq, int64_t* ids,
+                               float* dists) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * k) return;
+    const int i = t / k, j = t % k;
+    ids[t] = j < kp ? src_ids[static_cast<size_t>(i) * kp + j] : -1;
+    dists[t] = j < kp ? src_d[static_cast<size_t>(i) * kp + j] : INFINITY;
+}
+
+void fill_empty(int64_t* d_ids, float* d_dists, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_results, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, d_ids, d_dists, n);
+    check_launch("k_fill_results");
+}
+
+// The device search of an index for nq prepared queries already in HBM.
+void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip_search_params& p,
+                const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc) {
+    hipStream_t st = sc.stream;
+    const uint32_t k = p.k;
+    if (nq == 0 || k == 0) return;
+    const bool filtered = p.allowed_words != nullptr;
+    // query_prepared: an empty filter returns empty rows (src/index.cpp:172)
+    if (filtered && p.allowed_count == 0) {
+        fill_empty(d_ids, d_dists, nq * k, st);
+        return;
+    }
+    const int ip = h->metric == QANN_METRIC_IP;
+    const uint64_t nwords = (h->slots + 63) / 64;
+
+    if (h->kind == 1) { // FlatIndex::query_impl
+        sc.used = 0;
+        const bool f = filtered;
+        sc.reserve(f ? pad256(h->slots * 4) + 256 : 256);
+        uint32_t* slot_list = nullptr;
+        uint32_t n_slots = static_cast<uint32_t>(h->slots);
+        if (f) {
+            slot_list = sc.take<uint32_t>(h->slots);
+            uint32_t* cnt = sc.take<uint32_t>(1);
+            n_slots = static_cast<uint32_t>(std::min<uint64_t>(p.allowed_count, h->slots));
+            bitmap_slots(d_filter, nwords, slot_list, cnt, n_slots, st);
+        }
+        qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, h->rows, slot_list, n_slots,
+                                f ? nullptr : h->deleted, 0, 0, static_cast<int>(k), h->labels, d_ids, d_dists, st);
+        check_launch("k_block_select");
+        return;
+    }
+
+    qann_hip_index* ivfh = h->kind == 3 ? h->base : h;
+    const qd::DevIvf& ix = ivfh->ivf;
+    const bool refine = h->kind == 3;
+    uint32_t kp = k;
+    if (refine) {
+        if (h->live == 0) {
+            fill_empty(d_ids, d_dists, nq * k, st);
+            return;
+        }
+        const uint64_t cap = filtered ? p.allowed_count : h->live;
+        kp = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(k) * p.k_factor, cap));
+        // RefineIndex::use_direct (src/refine_index.cpp:59-69): score the allowed rows directly
+        if (filtered && p.allowed_count <= 6ull * ix.nlist) {
+            sc.used = 0;
+            sc.reserve(pad256(p.allowed_count * 4) + 256);
+            uint32_t* slot_list = sc.take<uint32_t>(p.allowed_count);
+            uint32_t* cnt = sc.take<uint32_t>(1);
+            bitmap_slots(d_filter, nwords, slot_list, cnt, static_cast<uint32_t>(p.allowed_count), st);
+            qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, h->rows, slot_list,
+                                    static_cast<uint32_t>(p.allowed_count), nullptr, 1, 1, static_cast<int>(k),
+                                    h->labels, d_ids, d_dists, st);
+            check_launch("k_block_select");
+            return;
+        }
+    }
+    if (kp == 0) {
+        fill_empty(d_ids, d_dists, nq * k, st);
+        return;
+    }
+    const uint32_t nlist = static_cast<uint32_t>(ix.nlist);
+    const uint32_t nprobe = p.nprobe;
+    if (nprobe == 0 || nprobe > nlist) invalid("nprobe must be in [1, nlist]");
+    const uint32_t M = filtered ? std::min<uint32_t>(nlist, qd::kSelectMax) : nprobe;
+    if (M > static_cast<uint32_t>(qd::kSelectMax)) invalid("nprobe above 4096 is not supported");
+    const bool cos_pq = ix.metric == QANN_METRIC_COSINE && ix.cosine_ip;
+    const bool ip_pq = ix.metric == QANN_METRIC_IP || cos_pq;
+    const bool use_pre = !ip_pq && ix.B != nullptr;
+    const uint32_t m = static_cast<uint32_t>(ix.m);
+    const size_t lds = qd::scan_lds_bytes(static_cast<int>(kp), ix.m, ix.dim);
+    if (lds > 64 * 1024) invalid("k * k_factor too large for the on-chip top-k buffer");
+
+    // IVFIndex::scan's filter parameters (src/ivf_index.cpp:462, 502, 510-514)
+    int min_probes = static_cast<int>(nprobe), wanted = 0, block_filter = 0;
+    if (filtered) {
+        wanted = static_cast<int>(std::min<uint64_t>(kp, p.allowed_count));
+        block_filter = p.allowed_count * 16 < ivfh->slots;
+        if (p.filter_probe_scaling && p.allowed_count > 0) {
+            const double want = std::ceil(0.75 * double(nprobe) * std::cbrt(double(ivfh->live) / double(p.allowed_count)));
+            min_probes = static_cast<int>(std::clamp<size_t>(static_cast<size_t>(std::min(want, double(nlist))), nprobe, nlist));
+        }
+    }
+
+    // scratch plan
+    const size_t d = ix.dim;
+    size_t need = 0;
+    need += ix.rotation ? pad256(nq * d * 4) : 0;
+    need += pad256(nq * nlist * 4);       // coarse distances
+    need += pad256(nq * M * 2) + 3 * pad256(nq * M * 4);
+    need += use_pre ? pad256(nq * m * 16 * 4) : 0;
+    need += ip_pq ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
+    need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + 2 * pad256(nq * 4);
+    sc.used = 0;
+    sc.reserve(need);
+    const float* qr = d_q;
+    if (ix.rotation) {
+        float* r = sc.take<float>(nq * d);
+        qd::launch_rotate(d_q, ix.rotation, static_cast<int>(nq), ix.dim, r, st);
+        check_launch("k_rotate");
+        qr = r;
+    }
+    float* D = sc.take<float>(nq * nlist);
+    qd::launch_coarse_dist(qr, ix.centroids, static_cast<int>(nq), ix.nlist, ix.dim, ip, D, st);
+    check_launch("k_coarse_dist");
+    uint16_t* pidx = sc.take<uint16_t>(nq * M);
+    float* pval = sc.take<float>(nq * M);
+    qd::launch_select_sorted(D, static_cast<int>(nq), ix.nlist, static_cast<int>(M), pidx, pval, st);
+    check_launch("k_select_sorted");
+    float* pA = sc.take<float>(nq * M);
+    float* pbase = sc.take<float>(nq * M);
+    qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
+    check_launch("k_probe_base");
+    float* C = nullptr;
+    uint8_t* lut = nullptr;
+    float *lscale = nullptr, *loff = nullptr;
+    if (use_pre) {
+        C = sc.take<float>(nq * m * 16);
+        qd::launch_query_tables(qr, ix, static_cast<int>(nq), 0, C, nullptr, nullptr, nullptr, st);
+        check_launch("k_query_tables");
+    } else if (ip_pq) {
+        lut = sc.take<uint8_t>(nq * m * 16);
+        lscale = sc.take<float>(nq);
+        loff = sc.take<float>(nq);
+        qd::launch_query_tables(qr, ix, static_cast<int>(nq), 1, nullptr, lut, lscale, loff, st);
+        check_launch("k_query_tables");
+    }
+    qd::ScanArgs a{};
+    a.ix = ix;
+    if (!use_pre) a.ix.B = nullptr;
+    a.kp = static_cast<int>(kp);
+    a.nprobe = static_cast<int>(nprobe);
+    a.M = static_cast<int>(M);
+    a.min_probes = min_probes;
+    a.wanted = wanted;
+    a.block_filter = block_filter;
+    a.q = qr;
+    a.probe_idx = pidx;
+    a.probe_A = pA;
+    a.probe_base = pbase;
+    a.C = C;
+    a.ip_lut = lut;
+    a.ip_scale = lscale;
+    a.ip_offset = loff;
+    a.filter = filtered ? d_filter : nullptr;
+    float* cd = sc.take<float>(nq * kp);
+    int64_t* cl = sc.take<int64_t>(nq * kp);
+    int* ccount = sc.take<int>(nq);
+    int* status = sc.take<int>(nq);
+    a.out_dist = cd;
+    a.out_label = cl;
+    a.out_count = ccount;
+    a.status = status;
+    qd::launch_scan_fastscan(a, static_cast<int>(nq), st);
+    check_launch("k_scan_fastscan");
+    if (filtered && M < nlist) {
+        std::vector<int> hs(nq);
+        check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int v : hs)
+            if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
+    }
+    if (refine) {
+        qd::launch_rerank(d_q, static_cast<int>(nq), h->dim, ip, h->rows, cl, ccount, static_cast<int>(kp),
+                          static_cast<int>(k), h->labels, d_ids, d_dists, st);
+        check_launch("k_rerank");
+    } else {
+        hipLaunchKernelGGL(k_copy_results, dim3(static_cast<unsigned>((nq * k + 255) / 256)), dim3(256), 0, st, cl, cd,
+                           static_cast<int>(kp), static_cast<int>(k), static_cast<int>(nq), d_ids, d_dists);
+        check_launch("k_copy_results");
+    }
+}
+
+Scratch* checkout(qann_hip_index* h) {
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->free_scratch.empty()) {
+        Scratch* s = h->free_scratch.back();
+        h->free_scratch.pop_back();
+        return s;
+    }
+    auto* s = new Scratch();
+    check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    s->own_stream = true;
+    return s;
+}
+
+void checkin(qann_hip_index* h, Scratch* s) {
+    std::lock_guard<std::mutex> g(h->mu);
+    h->free_scratch.push_back(s);
+}
+
+} // namespace
+
+extern "C" {
+
+const char* qann_hip_last_error(void) { return g_err.c_str(); }
+
+int qann_hip_device_count(int* count) {
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
+    return guarded([&] {
+        const qann_hip_ivf_desc& dsc = *desc;
+        if (dsc.dim == 0) invalid("dim must be >= 1");
+        if (dsc.nlist < 100 || dsc.nlist > 65535) invalid("nlist must be in [100, 65535]");
+        if (dsc.metric > 2) invalid("unknown metric");
+        if (dsc.m == 0 || dsc.m % 2 != 0 || dsc.m > 1024 || dsc.dim % dsc.m != 0)
+            invalid("fast_scan needs an even num_subspaces <= 1024 dividing dim");
+        if (!dsc.centroids || !dsc.pq_centroids) invalid("centroids and PQ codebook are required");
+        auto h = std::make_unique<qann_hip_index>();
+        check(hipGetDevice(&h->device), "hipGetDevice");
+        h->kind = 2;
+        h->dim = dsc.dim;
+        h->metric = dsc.metric;
+        qd::DevIvf& ix = h->ivf;
+        ix.dim = static_cast<int>(dsc.dim);
+        ix.nlist = static_cast<int>(dsc.nlist);
+        ix.metric = static_cast<int>(dsc.metric);
+        ix.m = static_cast<int>(dsc.m);
+        ix.sub = static_cast<int>(dsc.dim / dsc.m);
+        ix.cosine_ip = dsc.cosine_ip ? 1 : 0;
+        const size_t d = dsc.dim, nl = dsc.nlist, m = dsc.m, sub = d / m;
+        hipStream_t st = nullptr;
+        float* cent = h->upload(dsc.centroids, nl * d);
+        ix.centroids = cent;
+        float* csq = h->alloc<float>(nl);
+        qd::launch_sqnorms(cent, static_cast<int>(nl), static_cast<int>(d), csq, st);
+        ix.centroid_sqnorms = csq;
+        if (dsc.residual_centers) {
+            float* rc = h->upload(dsc.residual_centers, nl * d);
+            float* rsq = h->alloc<float>(nl);
+            qd::launch_sqnorms(rc, static_cast<int>(nl), static_cast<int>(d), rsq, st);
+            ix.residual_centers = rc;
+            ix.residual_sqnorms = rsq;
+        }
+        float* pqc = h->upload(dsc.pq_centroids, m * 16 * sub);
+        float* pqt = h->alloc<float>(m * 16 * sub);
+        qd::launch_pq_transpose(pqc, static_cast<int>(m), static_cast<int>(sub), pqt, st);
+        ix.pq_cent_t = pqt;
+        if (dsc.use_precomputed) {
+            float* B = h->alloc<float>(nl * m * 16);
+            const float* centers = ix.residual_centers ? ix.residual_centers : ix.centroids;
+            qd::launch_list_term(centers, pqt, static_cast<int>(nl), static_cast<int>(d), static_cast<int>(m),
+                                 static_cast<int>(sub), B, st);
+            ix.B = B;
+            h->use_pre = true;
+        }
+        if (dsc.rotation) ix.rotation = h->upload(dsc.rotation, d * d);
+        check_launch("index setup kernels");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        *out = h.release();
+    });
+}
+
+int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint32_t* slots, const uint8_t* codes) {
+    return guarded([&] {
+        if (!h || h->kind != 2) invalid("not an IVF index");
+        qd::DevIvf& ix = h->ivf;
+        const size_t nl = static_cast<size_t>(ix.nlist), bb = static_cast<size_t>(ix.m) * 16;
+        std::vector<uint32_t> off(nl + 1, 0);
+        std::vector<uint64_t> coff(nl, 0);
+        uint64_t total = 0, cbytes = 0, src_bytes = 0;
+        for (size_t l = 0; l < nl; ++l) {
+            off[l] = static_cast<uint32_t>(total);
+            total += sizes[l];
+            if (total > 0xFFFFFFFFull) invalid("an index holds at most 2^32 - 1 vectors");
+            coff[l] = cbytes;
+            const uint64_t blk = (sizes[l] + 31) / 32 * bb;
+            cbytes += (blk + 255) & ~uint64_t{255};
+            src_bytes += blk;
+        }
+        off[nl] = static_cast<uint32_t>(total);
+        std::vector<uint8_t> packed(std::max<uint64_t>(cbytes, 1), 0);
+        uint64_t src = 0;
+        for (size_t l = 0; l < nl; ++l) {
+            const uint64_t blk = (sizes[l] + 31) / 32 * bb;
+            if (blk) std::memcpy(packed.data() + coff[l], codes + src, blk);
+            src += blk;
+        }
+        h->free_ptr(const_cast<uint32_t*>(ix.list_off));
+        h->free_ptr(const_cast<uint64_t*>(ix.code_off));
+        h->free_ptr(const_cast<uint32_t*>(ix.list_ids));
+        h->free_ptr(const_cast<uint8_t*>(ix.codes));
+        ix.list_off = h->upload(off.data(), nl + 1);
+        ix.code_off = h->upload(coff.data(), nl);
+        ix.list_ids = total ? h->upload(slots, total) : h->alloc<uint32_t>(1);
+        ix.codes = h->upload(packed.data(), packed.size());
+        h->host_list_sizes.assign(sizes, sizes + nl);
+    });
+}
+
+int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint64_t slots, qann_hip_index** out) {
+    return guarded([&] {
+        if (dim == 0) invalid("dim must be >= 1");
+        if (metric > 2) invalid("unknown metric");
+        if (slots > 0xFFFFFFFFull) invalid("an index holds at most 2^32 - 1 vectors");
+        auto h = std::make_unique<qann_hip_index>();
+        check(hipGetDevice(&h->device), "hipGetDevice");
+        h->kind = 1;
+        h->dim = dim;
+        h->metric = metric;
+        h->rows = slots ? h->upload(rows, slots * dim) : h->alloc<float>(1);
+        h->slots = slots;
+        h->live = slots;
+        uint64_t set = 0;
+        std::vector<uint64_t> bm = to_bitmap(nullptr, slots, set);
+        h->deleted = h->upload(bm.data(), std::max<size_t>(bm.size(), 1));
+        *out = h.release();
+    });
+}
+
+int qann_hip_refine_create(qann_hip_index* base, const float* rows, uint64_t slots, qann_hip_index** out) {
+    return guarded([&] {
+        if (!base || base->kind != 2) invalid("the refine base must be an IVF fast-scan index");
+        auto h = std::make_unique<qann_hip_index>();
+        check(hipGetDevice(&h->device), "hipGetDevice");
+        h->kind = 3;
+        h->dim = base->dim;
+        h->metric = base->metric;
+        h->base = base;
+        h->rows = slots ? h->upload(rows, slots * static_cast<size_t>(base->dim)) : h->alloc<float>(1);
+        h->slots = slots;
+        h->live = slots;
+        *out = h.release();
+    });
+}
+
+int qann_hip_set_ids(qann_hip_index* h, uint64_t slots, const int64_t* labels, const uint8_t* deleted) {
+    return guarded([&] {
+        if (!h) invalid("null index");
+        if (slots > 0xFFFFFFFFull) invalid("an index holds at most 2^32 - 1 vectors");
+        h->free_ptr(h->labels);
+        h->free_ptr(h->deleted);
+        h->labels = labels ? h->upload(labels, slots) : nullptr;
+        uint64_t set = 0;
+        std::vector<uint64_t> bm = to_bitmap(deleted, slots, set);
+        h->deleted = h->upload(bm.data(), std::max<size_t>(bm.size(), 1));
+        h->slots = slots;
+        h->live = slots - set;
+        if (h->kind == 2) {
+            h->ivf.labels = h->labels;
+            h->ivf.deleted = h->deleted;
+        }
+    });
+}
+
+int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const qann_hip_search_params* params,
+                    int64_t* out_ids, float* out_dists) {
+    return guarded([&] {
+        if (!h || !params) invalid("null argument");
+        const uint32_t k = params->k;
+        if (nq == 0 || k == 0) return;
+        if (params->allowed_words) {
+            uint64_t set = 0;
+            for (uint64_t w = 0; w < (h->slots + 63) / 64; ++w) set += static_cast<uint64_t>(__builtin_popcountll(params->allowed_words[w]));
+            if (set != params->allowed_count) invalid("allowed_count does not match the allowed bitmap");
+        }
+        check(hipSetDevice(h->device), "hipSetDevice");
+        Scratch* io = checkout(h);
+        try {
+            // inputs and outputs in a second buffer so run_search owns the scratch
+            const size_t qb = pad256(nq * h->dim * 4), ob = pad256(nq * k * 8), db = pad256(nq * k * 4);
+            const size_t fb = params->allowed_words ? pad256((h->slots + 63) / 64 * 8) : 0;
+            char* buf = io->reserve_io(qb + ob + db + fb + 256);
+            float* dq = reinterpret_cast<float*>(buf);
+            int64_t* dids = reinterpret_cast<int64_t*>(buf + qb);
+            float* dd = reinterpret_cast<float*>(buf + qb + ob);
+            uint64_t* df = fb ? reinterpret_cast<uint64_t*>(buf + qb + ob + db) : nullptr;
+            check(hipMemcpyAsync(dq, queries, nq * h->dim * 4, hipMemcpyHostToDevice, io->stream), "hipMemcpyAsync");
+            if (df)
+                check(hipMemcpyAsync(df, params->allowed_words, (h->slots + 63) / 64 * 8, hipMemcpyHostToDevice,
+                                     io->stream),
+                      "hipMemcpyAsync");
+            run_search(h, dq, nq, *params, df, dids, dd, *io);
+            check(hipMemcpyAsync(out_ids, dids, nq * k * 8, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
+            check(hipMemcpyAsync(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
+            check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
+        } catch (...) {
+            (void)hipStreamSynchronize(io->stream);
+            checkin(h, io);
+            throw;
+        }
+        checkin(h, io);
+    });
+}
+
+int qann_hip_search_async(qann_hip_index* h, const float* d_queries, uint64_t nq, const qann_hip_search_params* params,
+                          int64_t* d_out_ids, float* d_out_dists, void* stream) {
+    return guarded([&] {
+        if (!h || !params) invalid("null argument");
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        Scratch* sc;
+        {
+            std::lock_guard<std::mutex> g(h->mu);
+            auto it = h->stream_scratch.find(st);
+            if (it == h->stream_scratch.end()) {
+                sc = new Scratch();
+                sc->stream = st;
+                h->stream_scratch[st] = sc;
+            } else {
+                sc = it->second;
+            }
+        }
+        run_search(h, d_queries, nq, *params, params->allowed_words, d_out_ids, d_out_dists, *sc);
+    });
+}
+
+int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const float* centroids, uint32_t k,
+                            uint32_t* out) {
+    return guarded([&] {
+        if (dim == 0 || k == 0) invalid("dim and k must be >= 1");
+        if (k > 65535) invalid("at most 65535 centroids");
+        if (n == 0) return;
+        hipStream_t st = nullptr;
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        struct Cleanup {
+            std::vector<void*> ptrs;
+            hipStream_t st;
+            ~Cleanup() {
+                (void)hipStreamSynchronize(st);
+                for (void* p : ptrs) (void)hipFree(p);
+                (void)hipStreamDestroy(st);
+            }
+        } cl{{}, st};
+        // rows in blocks so the (block x k) distance matrix stays near 1 GiB
+        const uint64_t block = std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t{1} << 28) / k));
+        float *dc = nullptr, *dq = nullptr, *D = nullptr, *val = nullptr;
+        uint16_t* idx = nullptr;
+        check(hipMalloc(&dc, static_cast<size_t>(k) * dim * 4), "hipMalloc");
+        cl.ptrs.push_back(dc);
+        check(hipMalloc(&dq, block * dim * 4), "hipMalloc");
+        cl.ptrs.push_back(dq);
+        check(hipMalloc(&D, block * k * 4), "hipMalloc");
+        cl.ptrs.push_back(D);
+        check(hipMalloc(&idx, block * 2), "hipMalloc");
+        cl.ptrs.push_back(idx);
+        check(hipMalloc(&val, block * 4), "hipMalloc");
+        cl.ptrs.push_back(val);
+        check(hipMemcpyAsync(dc, centroids, static_cast<size_t>(k) * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        std::vector<uint16_t> hidx(block);
+        for (uint64_t b0 = 0; b0 < n; b0 += block) {
+            const uint64_t b = std::min(block, n - b0);
+            check(hipMemcpyAsync(dq, rows + b0 * dim, b * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+            qd::launch_coarse_dist(dq, dc, static_cast<int>(b), static_cast<int>(k), static_cast<int>(dim), 0, D, st);
+            check_launch("k_coarse_dist");
+            qd::launch_select_sorted(D, static_cast<int>(b), static_cast<int>(k), 1, idx, val, st);
+            check_launch("k_select_sorted");
+            check(hipMemcpyAsync(hidx.data(), idx, b * 2, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            for (uint64_t i = 0; i < b; ++i) out[b0 + i] = hidx[i];
+        }
+    });
+}
+
+int qann_hip_index_bytes(qann_hip_index* h, uint64_t* bytes) {
+    return guarded([&] { *bytes = h ? h->bytes : 0; });
+}
+
+void qann_hip_index_destroy(qann_hip_index* h) { delete h; }
+
+} // extern "C"

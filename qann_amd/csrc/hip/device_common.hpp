@@ -1,0 +1,154 @@
+// Device helpers shared by the qann_amd search kernels (gfx950, wave64).
+//
+// All float arithmetic here is written out operation by operation: the
+// library is compiled with -ffp-contract=off, and every fused multiply-add
+// is an explicit fmaf, so each result has the bits of the reference's CPU
+// build (g++ -O3 -mavx2 -mfma, AVX2 kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qd {
+
+constexpr int kWave = 64;
+constexpr int kFsBlock = 32;   // vectors per fast-scan block (src/fast_scan.hpp:19)
+constexpr float kLutMax = 63.0f; // largest LUT byte (src/fast_scan.hpp:39)
+
+// Orderable key of a float (src/topk.hpp:80-83): unsigned order = float order,
+// with -0.0 below +0.0.
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return b ^ (static_cast<uint32_t>(static_cast<int32_t>(b) >> 31) | 0x80000000u);
+}
+
+// Inverse of fkey (src/topk.hpp:86-88).
+__device__ __forceinline__ float from_fkey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+}
+
+// Key for orders that compare floats with operator< (std::pair / heap
+// orders of the reference): -0.0 and +0.0 tie.
+__device__ __forceinline__ uint32_t ckey(float f) { return fkey(f == 0.0f ? 0.0f : f); }
+
+// Lanes of this wave below the current one with their bit set in mask.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u))); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Orders the LDS traffic of a kernel whose workgroup is exactly one wave.
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Exact distances in the reference's AVX2 order (src/distance.cpp:69-153 as
+// g++ compiles them): 4 x 8 lane accumulators over chunks of 32, 8-wide steps
+// into the first, the fixed horizontal sum, then the tail of n % 8 elements
+// (first 4 unfused when at least 4 remain, the rest fused). One thread
+// computes one distance.
+// ---------------------------------------------------------------------------
+
+template <bool IP>
+__device__ __forceinline__ float exact_avx2(const float* __restrict__ a, const float* __restrict__ b, int n) {
+    float acc[32];
+#pragma unroll
+    for (int t = 0; t < 32; ++t) acc[t] = 0.0f;
+    int i = 0;
+    for (; i + 32 <= n; i += 32) {
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+            const float x = a[i + t], y = b[i + t];
+            if (IP) {
+                acc[t] = fmaf(x, y, acc[t]);
+            } else {
+                const float d = x - y;
+                acc[t] = fmaf(d, d, acc[t]);
+            }
+        }
+    }
+    for (; i + 8 <= n; i += 8) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float x = a[i + t], y = b[i + t];
+            if (IP) {
+                acc[t] = fmaf(x, y, acc[t]);
+            } else {
+                const float d = x - y;
+                acc[t] = fmaf(d, d, acc[t]);
+            }
+        }
+    }
+    float v[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) v[l] = (acc[l] + acc[8 + l]) + (acc[16 + l] + acc[24 + l]);
+    float res = ((v[0] + v[4]) + (v[1] + v[5])) + ((v[2] + v[6]) + (v[3] + v[7]));
+    const int r = n - i;
+    int j = 0;
+    if (r >= 4) {
+        for (; j < 4; ++j) {
+            const float x = a[i + j], y = b[i + j];
+            float p;
+            if (IP) {
+                p = x * y;
+            } else {
+                const float d = x - y;
+                p = d * d;
+            }
+            res = res + p;
+        }
+    }
+    for (; j < r; ++j) {
+        const float x = a[i + j], y = b[i + j];
+        if (IP) {
+            res = fmaf(x, y, res);
+        } else {
+            const float d = x - y;
+            res = fmaf(d, d, res);
+        }
+    }
+    return res;
+}
+
+// DenseIndex::exact_score (include/vecengine/index.hpp:269-271): squared L2,
+// or -(a . b) for inner product.
+__device__ __forceinline__ float exact_score(bool ip, const float* a, const float* b, int n) {
+    return ip ? -exact_avx2<true>(a, b, n) : exact_avx2<false>(a, b, n);
+}
+
+// Filter bit of a slot (detail::SlotFilter::test, id_map.hpp:27).
+__device__ __forceinline__ bool bit_test(const uint64_t* words, uint32_t s) {
+    return (words[s >> 6] >> (s & 63)) & 1ull;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup bitonic sort of n (power of two) 64-bit keys in LDS, ascending.
+// ---------------------------------------------------------------------------
+__device__ inline void block_bitonic_sort(uint64_t* keys, int n) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
+                const int i = 2 * t - (t & (stride - 1));
+                const int j = i + stride;
+                const bool up = (i & size) == 0;
+                const uint64_t a = keys[i], b = keys[j];
+                if ((a > b) == up) {
+                    keys[i] = b;
+                    keys[j] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+} // namespace qd

@@ -1,0 +1,784 @@
+// IVF fast-scan search kernels for MI355X (gfx950).
+//
+// One batch of prepared queries runs as:
+//   k_rotate          q * R for rotated indexes           (src/rotation.cpp:165-185)
+//   k_coarse_dist     exact A for every (query, list)     (IVFIndex::coarse / coarse_all, src/ivf_index.cpp:305-383)
+//   k_select_sorted   the M nearest lists, sorted by (A, list)
+//   k_probe_base      per-probe A (residual centres) and the IP/cosine base term
+//   k_query_tables    C = -2<q_s, y_sj> (PQCodebook::compute_inner_table, src/pq.cpp:186-204),
+//                     or the inner-product LUT quantized once per query (scan, :448-456)
+//   k_scan_fastscan   one wave per query: per probe the u8 LUT in LDS (build_quantize_lut,
+//                     src/fast_scan.cpp:243-254), the 4-bit block sums (fast_scan_block_mask,
+//                     :256-262), and the reference's TopK replayed push by push
+//                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543)
+//   k_rerank          exact FP32 re-rank of the candidates (RefineIndex::rerank, src/refine_index.cpp:112-140)
+//   k_block_select    exact top-k over a slot set (FlatIndex::query_impl, RefineIndex::direct_scan)
+#include "device_common.hpp"
+#include "ivf_kernels.hpp"
+
+namespace qd {
+
+// ---------------------------------------------------------------------------
+// Setup kernels (run once at index creation).
+// ---------------------------------------------------------------------------
+
+// ||c||^2 per row with the reference's inner_product order.
+__global__ void k_sqnorms(const float* rows, int n, int dim, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rows + static_cast<size_t>(i) * dim;
+    out[i] = exact_avx2<true>(r, r, dim);
+}
+
+// centroids_t [s][sub][c] from the saved [s][c][sub] (PQCodebook::build_tables).
+__global__ void k_pq_transpose(const float* cent, int m, int K, int sub, float* cent_t) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * K * sub) return;
+    const int s = idx / (K * sub), rem = idx % (K * sub), c = rem / sub, j = rem % sub;
+    cent_t[(static_cast<size_t>(s) * sub + j) * K + c] = cent[idx];
+}
+
+// B[l][s][c] = sum_j y_scj * (y_scj + 2 c_l,sj), fused (compute_list_term, src/pq.cpp:206-223).
+__global__ void k_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B) {
+    const int K = 16;
+    const int tiles = (m * K + blockDim.x - 1) / blockDim.x;
+    const int e = (blockIdx.x % tiles) * blockDim.x + threadIdx.x; // (s, c) entry
+    const int l = blockIdx.x / tiles;
+    if (e >= m * K || l >= nlist) return;
+    const int s = e / K, c = e % K;
+    const float* cl = centers + static_cast<size_t>(l) * dim + s * sub;
+    float acc = 0.0f;
+    for (int j = 0; j < sub; ++j) {
+        const float y = cent_t[(static_cast<size_t>(s) * sub + j) * K + c];
+        acc = fmaf(y, y + 2.0f * cl[j], acc);
+    }
+    B[static_cast<size_t>(l) * m * K + e] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Query-side kernels.
+// ---------------------------------------------------------------------------
+
+// out[i][j] = fma chain over k of q[i][k] * R[k][j] (detail::rotate).
+__global__ void k_rotate(const float* q, const float* R, int nq, int dim, float* out) {
+    const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<size_t>(nq) * dim) return;
+    const int i = static_cast<int>(t / dim), j = static_cast<int>(t % dim);
+    const float* x = q + static_cast<size_t>(i) * dim;
+    float acc = 0.0f;
+    for (int k = 0; k < dim; ++k) acc = fmaf(x[k], R[static_cast<size_t>(k) * dim + j], acc);
+    out[static_cast<size_t>(i) * dim + j] = acc;
+}
+
+// D[i][l] = exact_score(q_i, c_l): one thread per (query, list), the query in LDS.
+__global__ void __launch_bounds__(256) k_coarse_dist(const float* q, const float* cents, int nlist, int dim, int ip, float* D) {
+    extern __shared__ float sq[];
+    const int tiles = (nlist + blockDim.x - 1) / blockDim.x;
+    const int i = blockIdx.x / tiles;
+    for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    __syncthreads();
+    const int l = (blockIdx.x % tiles) * blockDim.x + threadIdx.x;
+    if (l >= nlist) return;
+    D[static_cast<size_t>(i) * nlist + l] = exact_score(ip != 0, sq, cents + static_cast<size_t>(l) * dim, dim);
+}
+
+// Per row of D (n values): the M smallest by (value, index) -- float order with
+// +-0 tied, as Probe::operator< (ivf_index.hpp:220-226) -- sorted ascending.
+// Radix select on the 64-bit key (ckey << 32 | index), then a bitonic sort of
+// the M selected in LDS. M <= kSelectMax.
+__global__ void k_select_sorted(const float* D, int n, int M, uint16_t* out_idx, float* out_val) {
+    extern __shared__ uint64_t sel[];           // pow2 >= M keys
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t prefix_s;
+    __shared__ uint32_t remain_s, count_s;
+    const int i = blockIdx.x;
+    const float* row = D + static_cast<size_t>(i) * n;
+    int Mp = 1;
+    while (Mp < M) Mp <<= 1;
+
+    uint64_t prefix = 0;
+    uint32_t remain = static_cast<uint32_t>(M);
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int t = threadIdx.x; t < 256; t += blockDim.x) hist[t] = 0;
+        __syncthreads();
+        const uint64_t hi_mask = shift == 56 ? 0ull : (~0ull << (shift + 8));
+        for (int l = threadIdx.x; l < n; l += blockDim.x) {
+            const uint64_t key = (static_cast<uint64_t>(ckey(row[l])) << 32) | static_cast<uint32_t>(l);
+            if ((key & hi_mask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            int dgt = 0;
+            for (; dgt < 256; ++dgt) {
+                if (acc + hist[dgt] >= remain) break;
+                acc += hist[dgt];
+            }
+            prefix_s = prefix | (static_cast<uint64_t>(dgt) << shift);
+            remain_s = remain - acc;
+        }
+        __syncthreads();
+        prefix = prefix_s;
+        remain = remain_s;
+    }
+    // prefix is now the M-th smallest key
+    if (threadIdx.x == 0) count_s = 0;
+    for (int t = threadIdx.x; t < Mp; t += blockDim.x) sel[t] = ~0ull;
+    __syncthreads();
+    for (int l = threadIdx.x; l < n; l += blockDim.x) {
+        const uint64_t key = (static_cast<uint64_t>(ckey(row[l])) << 32) | static_cast<uint32_t>(l);
+        if (key <= prefix) sel[atomicAdd(&count_s, 1u)] = key;
+    }
+    __syncthreads();
+    block_bitonic_sort(sel, Mp);
+    for (int t = threadIdx.x; t < M; t += blockDim.x) {
+        const uint32_t l = static_cast<uint32_t>(sel[t] & 0xFFFFFFFFu);
+        out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(l);
+        out_val[static_cast<size_t>(i) * M + t] = row[l];
+    }
+}
+
+// A per probe: with residual centres the list's own exact term
+// (scan, src/ivf_index.cpp:526-529), else the coarse distance. For the
+// inner-product tables, base = A, or for Cosine (A + 1) - |c|^2 (:567-570).
+__global__ void k_probe_base(const float* q, const DevIvf ix, int nq, int M, const uint16_t* probe_idx,
+                             const float* probe_val, float* A_out, float* base_out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * M) return;
+    const int i = t / M;
+    const int l = probe_idx[t];
+    float A = probe_val[t];
+    if (ix.residual_centers)
+        A = exact_score(ix.metric == 2, q + static_cast<size_t>(i) * ix.dim,
+                        ix.residual_centers + static_cast<size_t>(l) * ix.dim, ix.dim);
+    A_out[t] = A;
+    float base = A;
+    if (ix.metric == 1 && ix.cosine_ip) {
+        const float sq = ix.residual_centers ? ix.residual_sqnorms[l] : ix.centroid_sqnorms[l];
+        base = (A + 1.0f) - sq;
+    }
+    base_out[t] = base;
+}
+
+// Row minimum / maximum / widest / offset and the bytes of one m x 16 table
+// held in LDS (quantize_lut_scalar, src/fast_scan.cpp:22-46), by one workgroup.
+__device__ void block_quantize(const float* table, int m, float* row_min, uint8_t* lut, float* scale_out,
+                               float* offset_out, float* red) {
+    float wide = 0.0f;
+    for (int r = threadIdx.x; r < m; r += blockDim.x) {
+        const float* row = table + r * 16;
+        float lo = row[0], hi = row[0];
+        for (int j = 1; j < 16; ++j) {
+            lo = fminf(lo, row[j]);
+            hi = fmaxf(hi, row[j]);
+        }
+        row_min[r] = lo;
+        wide = fmaxf(wide, hi - lo);
+    }
+    wide = wave_max(wide);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = wide;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        float p = 0.0f;
+        for (int r = threadIdx.x; r < m; r += 4) p += row_min[r];
+        red[8 + threadIdx.x] = p;
+    }
+    __syncthreads();
+    float widest = 0.0f;
+    for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) widest = fmaxf(widest, red[w]);
+    const float scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
+    for (int e = threadIdx.x; e < m * 16; e += blockDim.x)
+        lut[e] = static_cast<uint8_t>(__builtin_rintf((table[e] - row_min[e >> 4]) * scale));
+    if (threadIdx.x == 0) {
+        *scale_out = scale;
+        *offset_out = (red[8] + red[9]) + (red[10] + red[11]);
+    }
+    __syncthreads();
+}
+
+// Per query: C = compute_inner_table(q) for precomputed L2 tables, or for
+// inner-product / cosine-IP scoring the same table (halved for Metric::IP)
+// quantized once (scan, src/ivf_index.cpp:448-456).
+__global__ void k_query_tables(const float* q, const DevIvf ix, int want_lut, float* C_out, uint8_t* lut_out,
+                               float* scale_out, float* offset_out) {
+    extern __shared__ float smem[];
+    const int m = ix.m, sub = ix.sub, K = 16;
+    float* table = smem;                  // m * 16
+    float* row_min = table + m * K;       // m
+    float* red = row_min + m;             // 16
+    const int i = blockIdx.x;
+    const float* qi = q + static_cast<size_t>(i) * ix.dim;
+    const bool half = ix.metric == 2;
+    for (int e = threadIdx.x; e < m * K; e += blockDim.x) {
+        const int s = e / K, c = e % K;
+        float acc = 0.0f;
+        for (int j = 0; j < sub; ++j) {
+            const float qd = qi[s * sub + j];
+            acc = fmaf(-2.0f * qd, ix.pq_cent_t[(static_cast<size_t>(s) * sub + j) * K + c], acc);
+        }
+        if (want_lut && half) acc *= 0.5f;
+        table[e] = acc;
+        if (C_out) C_out[static_cast<size_t>(i) * m * K + e] = acc;
+    }
+    __syncthreads();
+    if (!want_lut) return;
+    block_quantize(table, m, row_min, lut_out + static_cast<size_t>(i) * m * K, scale_out + i, offset_out + i, red);
+}
+
+// ---------------------------------------------------------------------------
+// The fast-scan scan with the reference's TopK, one wave per query.
+// ---------------------------------------------------------------------------
+
+struct WaveTopK {
+    float* dist;    // LDS, 2k
+    int64_t* label; // LDS, 2k
+    int size;       // wave-uniform
+    int k;
+    float thr;      // threshold(): +inf until the first trim
+};
+
+// count of buffered keys <= t, by the whole wave
+__device__ __forceinline__ int count_le(const WaveTopK& tk, int n, uint32_t t) {
+    int c = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int i = base + lane_id();
+        const bool p = i < n && fkey(tk.dist[i]) <= t;
+        c += __popcll(ballot(p));
+    }
+    return c;
+}
+
+// TopK::trim (src/topk.hpp:94-131): keep the k smallest keys, ties at the
+// k-th in push order; the threshold becomes the k-th smallest distance.
+__device__ void wave_trim(WaveTopK& tk) {
+    const int n = tk.size;
+    uint32_t t = 0;
+    for (int b = 31; b >= 0; --b) {
+        const uint32_t below = b == 0 ? 0u : ((1u << b) - 1u);
+        if (count_le(tk, n, t | below) < tk.k) t |= 1u << b;
+    }
+    int ties = tk.k - (t == 0 ? 0 : count_le(tk, n, t - 1));
+    int w = 0;
+    for (int base = 0; base < n; base += kWave) {
+        const int i = base + lane_id();
+        const bool valid = i < n;
+        const float d = valid ? tk.dist[i] : 0.0f;
+        const int64_t lab = valid ? tk.label[i] : 0;
+        const uint32_t key = valid ? fkey(d) : 0xFFFFFFFFu;
+        const bool lt = valid && key < t;
+        const bool eq = valid && key == t;
+        const uint64_t eqm = ballot(eq);
+        const bool tie_keep = eq && static_cast<int>(lanes_below(eqm)) < ties;
+        const bool keep = lt || tie_keep;
+        const uint64_t km = ballot(keep);
+        const int pos = w + static_cast<int>(lanes_below(km));
+        wave_sync();
+        if (keep) {
+            tk.dist[pos] = d;
+            tk.label[pos] = lab;
+        }
+        wave_sync();
+        w += __popcll(km);
+        ties -= __popcll(ballot(tie_keep));
+    }
+    tk.size = tk.k;
+    tk.thr = from_fkey(t);
+}
+
+__global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const DevIvf& ix = a.ix;
+    const int m = ix.m;
+    const int bb = m * 16;
+    const int kp = a.kp;
+    // LDS layout (ScanArgs::lds_bytes computes the same)
+    int64_t* tk_label = reinterpret_cast<int64_t*>(lds);                 // 2k'
+    float* tk_dist = reinterpret_cast<float*>(tk_label + 2 * kp);       // 2k'
+    float* row_min = tk_dist + 2 * kp;                                   // m
+    float* resid = row_min + m;                                          // dim
+    uint16_t* sums = reinterpret_cast<uint16_t*>(resid + ix.dim);        // kChunk * 32
+    uint8_t* lut = reinterpret_cast<uint8_t*>(sums + kChunkBlocks * kFsBlock); // m * 16
+
+    const int qi = blockIdx.x;
+    const int lane = lane_id();
+    const float* q = a.q + static_cast<size_t>(qi) * ix.dim;
+    const bool ip_mode = a.ip_lut != nullptr;
+    const bool filtered = a.filter != nullptr;
+
+    WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY};
+
+    float ip_scale = 1.0f, ip_offset = 0.0f;
+    if (ip_mode) {
+        const uint8_t* src = a.ip_lut + static_cast<size_t>(qi) * bb;
+        for (int e = lane; e < bb; e += kWave) lut[e] = src[e];
+        ip_scale = a.ip_scale[qi];
+        ip_offset = a.ip_offset[qi];
+    }
+    int status = 0;
+    const uint16_t* plist = a.probe_idx + static_cast<size_t>(qi) * a.M;
+    for (int p = 0;; ++p) {
+        if (p >= a.nprobe) {
+            if (p >= a.min_probes && min(tk.size, kp) >= a.wanted) break;
+        }
+        if (p >= a.M) {
+            if (p < ix.nlist) status = 1; // more probes needed than were selected
+            break;
+        }
+        const int list = plist[p];
+        const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+        if (n == 0) continue;
+        const float A = a.probe_A[static_cast<size_t>(qi) * a.M + p];
+        float scale, offset;
+        if (ip_mode) {
+            scale = ip_scale;
+            offset = ip_offset + a.probe_base[static_cast<size_t>(qi) * a.M + p];
+        } else {
+            // this list's table, rounded to bytes (build_quantize_lut / compute_adc_table + quantize_lut)
+            const float* B = ix.B ? ix.B + static_cast<size_t>(list) * bb : nullptr;
+            const float* C = a.C ? a.C + static_cast<size_t>(qi) * bb : nullptr;
+            if (!B) {
+                const float* c = ix.residual_centers ? ix.residual_centers : ix.centroids;
+                c += static_cast<size_t>(list) * ix.dim;
+                for (int e = lane; e < ix.dim; e += kWave) resid[e] = q[e] - c[e];
+                wave_sync();
+            }
+            auto entry = [&](int r, int j) -> float {
+                if (B) {
+                    float t = B[r * 16 + j] + C[r * 16 + j];
+                    if (r == 0) t = t + A;
+                    return t;
+                }
+                float acc = 0.0f;
+                for (int s = 0; s < ix.sub; ++s) {
+                    const float d = resid[r * ix.sub + s] - ix.pq_cent_t[(static_cast<size_t>(r) * ix.sub + s) * 16 + j];
+                    acc = fmaf(d, d, acc);
+                }
+                return acc;
+            };
+            float wide = 0.0f;
+            for (int r = lane; r < m; r += kWave) {
+                float lo = entry(r, 0), hi = lo;
+                for (int j = 1; j < 16; ++j) {
+                    const float t = entry(r, j);
+                    lo = fminf(lo, t);
+                    hi = fmaxf(hi, t);
+                }
+                row_min[r] = lo;
+                wide = fmaxf(wide, hi - lo);
+            }
+            const float widest = wave_max(wide);
+            wave_sync();
+            float part = 0.0f;
+            if (lane < 4)
+                for (int r = lane; r < m; r += 4) part += row_min[r];
+            const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 1, kWave);
+            const float p2 = __shfl(part, 2, kWave), p3 = __shfl(part, 3, kWave);
+            offset = (p0 + p1) + (p2 + p3);
+            scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
+            for (int r = lane; r < m; r += kWave) {
+                const float mn = row_min[r];
+                for (int j = 0; j < 16; ++j) lut[r * 16 + j] = static_cast<uint8_t>(__builtin_rintf((entry(r, j) - mn) * scale));
+            }
+            wave_sync();
+        }
+
+        const uint8_t* codes = ix.codes + ix.code_off[list];
+        const uint32_t* ids = ix.list_ids + ix.list_off[list];
+        const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
+        for (uint32_t cb = 0; cb < nb; cb += kChunkBlocks) {
+            // block sums of up to kChunkBlocks blocks: lane -> (block, 4-byte column)
+            {
+                const uint32_t blk = cb + (lane >> 2);
+                const int g = lane & 3;
+                if (blk < nb) {
+                    const uint8_t* bp = codes + static_cast<size_t>(blk) * bb + 4 * g;
+                    uint32_t lo0 = 0, lo1 = 0, lo2 = 0, lo3 = 0, hi0 = 0, hi1 = 0, hi2 = 0, hi3 = 0;
+                    for (int s = 0; s < m; ++s) {
+                        const uint32_t w = *reinterpret_cast<const uint32_t*>(bp + s * 16);
+                        const uint8_t* row = lut + s * 16;
+                        lo0 += row[w & 15];
+                        hi0 += row[(w >> 4) & 15];
+                        lo1 += row[(w >> 8) & 15];
+                        hi1 += row[(w >> 12) & 15];
+                        lo2 += row[(w >> 16) & 15];
+                        hi2 += row[(w >> 20) & 15];
+                        lo3 += row[(w >> 24) & 15];
+                        hi3 += row[(w >> 28) & 15];
+                    }
+                    uint16_t* o = sums + (lane >> 2) * kFsBlock;
+                    o[4 * g + 0] = static_cast<uint16_t>(lo0);
+                    o[4 * g + 1] = static_cast<uint16_t>(lo1);
+                    o[4 * g + 2] = static_cast<uint16_t>(lo2);
+                    o[4 * g + 3] = static_cast<uint16_t>(lo3);
+                    o[16 + 4 * g + 0] = static_cast<uint16_t>(hi0);
+                    o[16 + 4 * g + 1] = static_cast<uint16_t>(hi1);
+                    o[16 + 4 * g + 2] = static_cast<uint16_t>(hi2);
+                    o[16 + 4 * g + 3] = static_cast<uint16_t>(hi3);
+                }
+            }
+            wave_sync();
+            const uint32_t cend = min(nb, cb + static_cast<uint32_t>(kChunkBlocks));
+            for (uint32_t blk = cb; blk < cend; ++blk) {
+                const uint32_t start = blk * kFsBlock;
+                const uint32_t count = min(static_cast<uint32_t>(kFsBlock), n - start);
+                const int v = lane;
+                const bool valid = v < static_cast<int>(count);
+                const uint32_t slot = valid ? ids[start + v] : 0u;
+                bool allowed = valid;
+                if (a.block_filter) {
+                    allowed = valid && bit_test(a.filter, slot);
+                    if (ballot(allowed) == 0) continue;
+                }
+                // limit16 from the threshold as it stands at the block (scan_fast, :482-485)
+                const float limit = (tk.thr - offset) * scale;
+                const uint32_t limit16 = limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
+                const uint32_t sum = valid ? sums[(blk - cb) * kFsBlock + v] : 0u;
+                const bool surv = allowed && sum < limit16;
+                uint64_t rem = ballot(surv);
+                if (rem == 0) continue;
+                const float dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                const bool ok = surv && (filtered ? bit_test(a.filter, slot) : !bit_test(ix.deleted, slot));
+                const int64_t lab = ix.labels ? ix.labels[slot] : static_cast<int64_t>(slot);
+                while (rem) {
+                    const bool cond = ((rem >> v) & 1ull) && ok && dist < tk.thr;
+                    const uint64_t cm = ballot(cond);
+                    if (cm == 0) break;
+                    const int space = 2 * kp - tk.size;
+                    const int cnt = __popcll(cm);
+                    if (cnt < space) {
+                        if (cond) {
+                            const int pos = tk.size + static_cast<int>(lanes_below(cm));
+                            tk.dist[pos] = dist;
+                            tk.label[pos] = lab;
+                        }
+                        tk.size += cnt;
+                        wave_sync();
+                        break;
+                    }
+                    // the space-th candidate fills the buffer and triggers a trim
+                    const uint32_t rank = lanes_below(cm);
+                    const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
+                    const int bpos = __builtin_ctzll(at);
+                    if (cond && v <= bpos) {
+                        const int pos = tk.size + static_cast<int>(rank);
+                        tk.dist[pos] = dist;
+                        tk.label[pos] = lab;
+                    }
+                    tk.size = 2 * kp;
+                    wave_sync();
+                    wave_trim(tk);
+                    rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
+                }
+            }
+            wave_sync();
+        }
+    }
+
+    // take_sorted (src/topk.hpp:53-76)
+    if (tk.size > kp) wave_trim(tk);
+    const int n = tk.size;
+    // (key, id) order when every id fits 32 bits, else (float, id) order
+    bool big = false;
+    for (int base = 0; base < n; base += kWave) {
+        const int i = base + lane;
+        big |= __popcll(ballot(i < n && static_cast<uint64_t>(tk.label[i]) > 0xFFFFFFFFull)) != 0;
+    }
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int i = n + lane; i < np; i += kWave) {
+        tk.dist[i] = __uint_as_float(0x7FC00000u);
+        tk.label[i] = -1;
+    }
+    wave_sync();
+    // padding entries (label -1) sort last wherever the network moves them
+    auto sort_key = [&](int i, uint32_t& k32, uint64_t& id) {
+        if (tk.label[i] < 0) {
+            k32 = 0xFFFFFFFFu;
+            id = ~0ull;
+            return;
+        }
+        const float d = tk.dist[i];
+        k32 = big ? ckey(d) : fkey(d);
+        id = static_cast<uint64_t>(tk.label[i]);
+    };
+    for (int size = 2; size <= np; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = lane; t < (np >> 1); t += kWave) {
+                const int x = 2 * t - (t & (stride - 1));
+                const int y = x + stride;
+                const bool up = (x & size) == 0;
+                uint32_t kx, ky;
+                uint64_t ix_, iy;
+                sort_key(x, kx, ix_);
+                sort_key(y, ky, iy);
+                const bool gt = kx != ky ? kx > ky : ix_ > iy;
+                if (gt == up) {
+                    const float dx = tk.dist[x], dy = tk.dist[y];
+                    const int64_t lx = tk.label[x], ly = tk.label[y];
+                    tk.dist[x] = dy;
+                    tk.dist[y] = dx;
+                    tk.label[x] = ly;
+                    tk.label[y] = lx;
+                }
+            }
+            wave_sync();
+        }
+    }
+    float* od = a.out_dist + static_cast<size_t>(qi) * kp;
+    int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
+    for (int i = lane; i < kp; i += kWave) {
+        od[i] = i < n ? tk.dist[i] : INFINITY;
+        ol[i] = i < n ? tk.label[i] : -1;
+    }
+    if (lane == 0) {
+        a.out_count[qi] = n;
+        if (a.status) a.status[qi] = status;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact re-rank (RefineIndex::rerank): the candidates' exact distances, the k
+// smallest by (distance, slot) in pair order, mapped to the refine labels.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip, const float* rows, const int64_t* cand, const int* cand_count,
+                         int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float* sq = reinterpret_cast<float*>(lds);
+    int np = 1;
+    const int i = blockIdx.x;
+    const int n = cand_count[i];
+    while (np < n) np <<= 1;
+    uint64_t* keys = reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
+    for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    __syncthreads();
+    for (int c = threadIdx.x; c < np; c += blockDim.x) {
+        if (c < n) {
+            const uint32_t slot = static_cast<uint32_t>(cand[static_cast<size_t>(i) * kc + c]);
+            const float d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
+            keys[c] = (static_cast<uint64_t>(ckey(d)) << 32) | slot;
+        } else {
+            keys[c] = ~0ull;
+        }
+    }
+    block_bitonic_sort(keys, np);
+    const int keep = min(k, n);
+    for (int t = threadIdx.x; t < k; t += blockDim.x) {
+        int64_t id = -1;
+        float d = INFINITY;
+        if (t < keep) {
+            const uint32_t slot = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
+            id = labels ? labels[slot] : static_cast<int64_t>(slot);
+            d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
+        }
+        out_ids[static_cast<size_t>(i) * k + t] = id;
+        out_dists[static_cast<size_t>(i) * k + t] = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact top-k over a set of slots, by one workgroup per query: the k smallest
+// 64-bit keys (key32(dist) << 32 | slot), which is FlatIndex::query_impl's
+// max-heap result (ckey: float order) and TopK's result for pushes in slot
+// order (fkey). Candidates come from a slot list, or all slots minus the
+// deleted ones. Output sorted by (key32, label) -- the final order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, int ip, const float* rows, const uint32_t* slot_list,
+                               uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
+                               int cap, const int64_t* labels, int64_t* out_ids, float* out_dists) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float* sq = reinterpret_cast<float*>(lds);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1)); // cap (pow2) keys
+    __shared__ int count_s;
+    __shared__ uint64_t thr_s;
+    __shared__ int big_s;
+    const int i = blockIdx.x;
+    for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    if (threadIdx.x == 0) {
+        count_s = 0;
+        thr_s = ~0ull;
+        big_s = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < n_slots; base += blockDim.x) {
+        // room for a full round: keep the k smallest so far
+        if (count_s + static_cast<int>(blockDim.x) > cap) {
+            for (int t = count_s + threadIdx.x; t < cap; t += blockDim.x) keys[t] = ~0ull;
+            block_bitonic_sort(keys, cap);
+            if (threadIdx.x == 0) {
+                count_s = k;
+                thr_s = keys[k - 1];
+            }
+            __syncthreads();
+        }
+        const uint32_t c = base + threadIdx.x;
+        if (c < n_slots) {
+            const uint32_t slot = slot_list ? slot_list[c] : c;
+            if (!deleted || !bit_test(deleted, slot)) {
+                const float d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
+                const uint64_t key = (static_cast<uint64_t>(use_fkey ? fkey(d) : ckey(d)) << 32) | slot;
+                if (key < thr_s) keys[atomicAdd(&count_s, 1)] = key;
+            }
+        }
+        __syncthreads();
+    }
+    const int n = count_s;
+    for (int t = n + threadIdx.x; t < cap; t += blockDim.x) keys[t] = ~0ull;
+    block_bitonic_sort(keys, cap);
+    const int keep = min(k, n);
+    // TopK::take_sorted order: (fkey, id) when every id fits 32 bits, else (float, id)
+    if (final_by_label && labels) {
+        int kp2 = 1;
+        while (kp2 < keep) kp2 <<= 1;
+        uint32_t* k32 = reinterpret_cast<uint32_t*>(keys + cap);         // kp2
+        int64_t* lab = reinterpret_cast<int64_t*>(k32 + ((kp2 + 1) & ~1)); // kp2
+        uint32_t* slt = reinterpret_cast<uint32_t*>(lab + kp2);           // kp2
+        for (int t = threadIdx.x; t < keep; t += blockDim.x)
+            if (static_cast<uint64_t>(labels[static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu)]) > 0xFFFFFFFFull) big_s = 1;
+        __syncthreads();
+        const bool big = big_s != 0;
+        for (int t = threadIdx.x; t < kp2; t += blockDim.x) {
+            if (t < keep) {
+                const uint32_t slot = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
+                const float d = from_fkey(static_cast<uint32_t>(keys[t] >> 32)); // fkey mode: exact inverse
+                k32[t] = big ? ckey(d) : static_cast<uint32_t>(keys[t] >> 32);
+                lab[t] = labels[slot];
+                slt[t] = slot;
+            } else {
+                k32[t] = 0xFFFFFFFFu;
+                lab[t] = -1;
+                slt[t] = 0xFFFFFFFFu;
+            }
+        }
+        for (int size = 2; size <= kp2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (int t = threadIdx.x; t < (kp2 >> 1); t += blockDim.x) {
+                    const int x = 2 * t - (t & (stride - 1));
+                    const int y = x + stride;
+                    const bool up = (x & size) == 0;
+                    const bool gt = k32[x] != k32[y] ? k32[x] > k32[y]
+                                                     : static_cast<uint64_t>(lab[x]) > static_cast<uint64_t>(lab[y]);
+                    if (gt == up) {
+                        const uint32_t tk = k32[x];
+                        k32[x] = k32[y];
+                        k32[y] = tk;
+                        const int64_t tl = lab[x];
+                        lab[x] = lab[y];
+                        lab[y] = tl;
+                        const uint32_t ts = slt[x];
+                        slt[x] = slt[y];
+                        slt[y] = ts;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < k; t += blockDim.x) {
+            int64_t id = -1;
+            float d = INFINITY;
+            if (t < keep) {
+                id = lab[t];
+                d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slt[t]) * dim, dim);
+            }
+            out_ids[static_cast<size_t>(i) * k + t] = id;
+            out_dists[static_cast<size_t>(i) * k + t] = d;
+        }
+        return;
+    }
+    for (int t = threadIdx.x; t < k; t += blockDim.x) {
+        int64_t id = -1;
+        float d = INFINITY;
+        if (t < keep) {
+            const uint32_t slot = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
+            id = labels ? labels[slot] : static_cast<int64_t>(slot);
+            d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
+        }
+        out_ids[static_cast<size_t>(i) * k + t] = id;
+        out_dists[static_cast<size_t>(i) * k + t] = d;
+    }
+}
+
+} // namespace qd
+
+// ---------------------------------------------------------------------------
+// Launch wrappers.
+// ---------------------------------------------------------------------------
+namespace qd {
+
+static inline unsigned cdiv(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+void launch_sqnorms(const float* rows, int n, int dim, float* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sqnorms, dim3(cdiv(n, 256)), dim3(256), 0, s, rows, n, dim, out);
+}
+
+void launch_pq_transpose(const float* cent, int m, int sub, float* cent_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_pq_transpose, dim3(cdiv(static_cast<size_t>(m) * 16 * sub, 256)), dim3(256), 0, s, cent, m,
+                       16, sub, cent_t);
+}
+
+void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_list_term, dim3(cdiv(static_cast<size_t>(m) * 16, 256) * static_cast<unsigned>(nlist)),
+                       dim3(256), 0, s, centers, cent_t, nlist, dim, m, sub, B);
+}
+
+void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_rotate, dim3(cdiv(static_cast<size_t>(nq) * dim, 256)), dim3(256), 0, s, q, R, nq, dim, out);
+}
+
+void launch_coarse_dist(const float* q, const float* cents, int nq, int nlist, int dim, int ip, float* D,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_coarse_dist, dim3(cdiv(nlist, 256) * static_cast<unsigned>(nq)), dim3(256),
+                       static_cast<size_t>(dim) * 4, s, q, cents, nlist, dim, ip, D);
+}
+
+void launch_select_sorted(const float* D, int nq, int n, int M, uint16_t* out_idx, float* out_val, hipStream_t s) {
+    int mp = 1;
+    while (mp < M) mp <<= 1;
+    hipLaunchKernelGGL(k_select_sorted, dim3(nq), dim3(256), static_cast<size_t>(mp) * 8, s, D, n, M, out_idx,
+                       out_val);
+}
+
+void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
+                       float* A, float* base, hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_base, dim3(cdiv(static_cast<size_t>(nq) * M, 256)), dim3(256), 0, s, q, ix, nq, M, idx,
+                       val, A, base);
+}
+
+void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
+                         float* offset, hipStream_t s) {
+    const size_t lds = (static_cast<size_t>(ix.m) * 16 + ix.m + 16) * 4;
+    hipLaunchKernelGGL(k_query_tables, dim3(nq), dim3(256), lds, s, q, ix, want_lut, C, lut, scale, offset);
+}
+
+void launch_scan_fastscan(const ScanArgs& a, int nq, hipStream_t s) {
+    const size_t lds = scan_lds_bytes(a.kp, a.ix.m, a.ix.dim);
+    hipLaunchKernelGGL(k_scan_fastscan, dim3(nq), dim3(kWave), lds, s, a);
+}
+
+void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
+                   const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
+                   hipStream_t s) {
+    int np = 1;
+    while (np < kc) np <<= 1;
+    const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + static_cast<size_t>(np) * 8;
+    hipLaunchKernelGGL(k_rerank, dim3(nq), dim3(256), lds, s, q, dim, ip, rows, cand, cand_count, kc, k, labels,
+                       out_ids, out_dists);
+}
+
+void launch_block_select(const float* q, int nq, int dim, int ip, const float* rows, const uint32_t* slot_list,
+                         uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
+                         const int64_t* labels, int64_t* out_ids, float* out_dists, hipStream_t s) {
+    const int threads = 256;
+    int cap = 1;
+    while (cap < k + threads) cap <<= 1;
+    int kp2 = 1;
+    while (kp2 < k) kp2 <<= 1;
+    size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + static_cast<size_t>(cap) * 8;
+    if (final_by_label && labels) lds += static_cast<size_t>((kp2 + 1) & ~1) * 4 + static_cast<size_t>(kp2) * 12;
+    hipLaunchKernelGGL(k_block_select, dim3(nq), dim3(threads), lds, s, q, dim, ip, rows, slot_list, n_slots, deleted,
+                       use_fkey, final_by_label, k, cap, labels, out_ids, out_dists);
+}
+
+} // namespace qd

@@ -1,0 +1,88 @@
+// Device-side views of an index and the launch wrappers of the search kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qd {
+
+// Blocks of 32 vectors whose sums one wave computes before replaying their pushes.
+constexpr int kChunkBlocks = 16;
+// Most sorted probes one query can be given (k_select_sorted keeps them in LDS).
+constexpr int kSelectMax = 4096;
+
+// An IVF fast-scan index in HBM. Lists are stored back to back: list l's
+// slots at list_ids[list_off[l] .. list_off[l + 1]), its code blocks at
+// codes + code_off[l] (256-byte aligned), in the reference's block layout.
+struct DevIvf {
+    int dim, nlist, metric, m, sub, cosine_ip;
+    const float* centroids;        // nlist * dim
+    const float* centroid_sqnorms; // nlist
+    const float* residual_centers; // nlist * dim or null
+    const float* residual_sqnorms; // nlist or null
+    const float* pq_cent_t;        // m * sub * 16
+    const float* B;                // nlist * m * 16, or null (no precomputed tables)
+    const float* rotation;         // dim * dim or null
+    const uint32_t* list_off;      // nlist + 1
+    const uint64_t* code_off;      // nlist
+    const uint32_t* list_ids;      // slots of every list entry
+    const uint8_t* codes;
+    const int64_t* labels;         // slot -> id, or null for ids == slots
+    const uint64_t* deleted;       // bitmap over slots
+};
+
+struct ScanArgs {
+    DevIvf ix;
+    int kp;          // results kept per query (the TopK's k)
+    int nprobe, M, min_probes, wanted;
+    int block_filter;
+    const float* q;            // nq * dim, prepared (and rotated)
+    const uint16_t* probe_idx; // nq * M, sorted nearest first
+    const float* probe_A;      // nq * M
+    const float* probe_base;   // nq * M
+    const float* C;            // nq * m * 16 or null
+    const uint8_t* ip_lut;     // nq * m * 16 or null
+    const float* ip_scale;
+    const float* ip_offset;
+    const uint64_t* filter;    // allowed slots bitmap or null
+    float* out_dist;           // nq * kp
+    int64_t* out_label;        // nq * kp
+    int* out_count;            // nq
+    int* status;               // nq: 1 when the query needed more probes than M
+};
+
+// LDS bytes k_scan_fastscan needs.
+inline size_t scan_lds_bytes(int kp, int m, int dim) {
+    size_t b = static_cast<size_t>(2 * kp) * (8 + 4); // TopK buffer (labels, distances)
+    b += static_cast<size_t>(m) * 4 + static_cast<size_t>(dim) * 4;
+    b += static_cast<size_t>(kChunkBlocks) * 32 * 2;
+    b += static_cast<size_t>(m) * 16;
+    int np = 1;
+    while (np < kp) np <<= 1;
+    const size_t sort_need = static_cast<size_t>(np) * 12; // take_sorted pads to a power of two
+    const size_t have = static_cast<size_t>(2 * kp) * 12;
+    if (sort_need > have) b += sort_need - have;
+    return (b + 15) & ~size_t{15};
+}
+
+void launch_sqnorms(const float* rows, int n, int dim, float* out, hipStream_t s);
+void launch_pq_transpose(const float* cent, int m, int sub, float* cent_t, hipStream_t s);
+void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B,
+                      hipStream_t s);
+void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, hipStream_t s);
+void launch_coarse_dist(const float* q, const float* cents, int nq, int nlist, int dim, int ip, float* D,
+                        hipStream_t s);
+void launch_select_sorted(const float* D, int nq, int n, int M, uint16_t* out_idx, float* out_val, hipStream_t s);
+void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
+                       float* A, float* base, hipStream_t s);
+void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
+                         float* offset, hipStream_t s);
+void launch_scan_fastscan(const ScanArgs& a, int nq, hipStream_t s);
+void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
+                   const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
+                   hipStream_t s);
+void launch_block_select(const float* q, int nq, int dim, int ip, const float* rows, const uint32_t* slot_list,
+                         uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
+                         const int64_t* labels, int64_t* out_ids, float* out_dists, hipStream_t s);
+
+} // namespace qd

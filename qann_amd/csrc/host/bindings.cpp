@@ -1,0 +1,267 @@
+// The qann_amd._qann extension: the reference's Python API for the dense
+// indexes on this path (bindings/python_bindings.cpp:803-1199), rebuilt on
+// pybind11 over the host index objects. Queries release the GIL and hold the
+// index (and a RefineIndex's base) shared; writes hold them exclusively
+// (read_locked / write_locked, python_bindings.cpp:97-131).
+#include "indexes.hpp"
+
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <shared_mutex>
+
+namespace py = pybind11;
+using namespace qh;
+
+namespace {
+
+using FloatArray = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+// lock_chain (python_bindings.cpp:97-113): the index, then the base it wraps.
+template <class Lock>
+std::vector<Lock> lock_chain(const DenseIndex& idx) {
+    std::vector<Lock> locks;
+    locks.emplace_back(idx.mu);
+    if (auto* r = dynamic_cast<const RefineIndex*>(&idx)) locks.emplace_back(r->base()->mu);
+    return locks;
+}
+
+template <class F>
+auto read_locked(const DenseIndex& idx, F&& f) {
+    py::gil_scoped_release release;
+    auto locks = lock_chain<std::shared_lock<std::shared_mutex>>(idx);
+    return f();
+}
+
+template <class F>
+auto write_locked(const DenseIndex& idx, F&& f) {
+    py::gil_scoped_release release;
+    auto locks = lock_chain<std::unique_lock<std::shared_mutex>>(idx);
+    return f();
+}
+
+// as_ids (python_bindings.cpp:163-172): any integer sequence as contiguous int64.
+py::array_t<int64_t> as_ids(py::handle obj) {
+    py::module_ np = py::module_::import("numpy");
+    py::object arr = np.attr("asarray")(obj);
+    if (arr.attr("size").cast<size_t>() > 0) {
+        const std::string kind = arr.attr("dtype").attr("kind").cast<std::string>();
+        if (kind != "i" && kind != "u") throw py::type_error("ids must be integers");
+    }
+    if (arr.attr("ndim").cast<size_t>() != 1) throw py::type_error("ids must be one-dimensional");
+    return np.attr("ascontiguousarray")(arr, py::arg("dtype") = "int64").cast<py::array_t<int64_t>>();
+}
+
+void refuse_if_wrapped(const DenseIndex& idx, const char* what) {
+    if (idx.wrapped)
+        throw std::invalid_argument(std::string("cannot ") + what +
+                                    " on an index wrapped by a RefineIndex; use the RefineIndex instead");
+}
+
+void refuse_where(const py::object& where) {
+    if (!where.is_none())
+        throw py::type_error("where= metadata filters are not evaluated by qann_amd; pass allowed= ids instead");
+}
+
+py::tuple batch_query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where) {
+    refuse_where(where);
+    if (q.ndim() != 2) throw py::type_error("query must be a 2-D array of shape (num_queries, dim)");
+    if (static_cast<uint64_t>(q.shape(1)) != self.dim())
+        throw std::invalid_argument("query dimension does not match index dimension");
+    const uint64_t nq = static_cast<uint64_t>(q.shape(0));
+    py::array_t<int64_t> ids({static_cast<py::ssize_t>(nq), static_cast<py::ssize_t>(k)});
+    py::array_t<float> dists({static_cast<py::ssize_t>(nq), static_cast<py::ssize_t>(k)});
+    py::array_t<int64_t> al;
+    const bool filtered = !allowed.is_none();
+    if (filtered) al = as_ids(allowed);
+    const float* qp = q.data();
+    int64_t* ip = ids.mutable_data();
+    float* dp = dists.mutable_data();
+    const int64_t* ap = filtered ? al.data() : nullptr;
+    const size_t an = filtered ? static_cast<size_t>(al.shape(0)) : 0;
+    read_locked(self, [&] {
+        self.batch_query(qp, nq, k, ap, an, filtered, ip, dp);
+        return 0;
+    });
+    return py::make_tuple(ids, dists);
+}
+
+py::tuple query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where) {
+    refuse_where(where);
+    if (q.ndim() != 1) throw py::type_error("query must be a 1-D array of shape (dim,)");
+    if (static_cast<uint64_t>(q.shape(0)) != self.dim())
+        throw std::invalid_argument("query dimension does not match index dimension");
+    std::vector<int64_t> ids(k);
+    std::vector<float> dists(k);
+    py::array_t<int64_t> al;
+    const bool filtered = !allowed.is_none();
+    if (filtered) al = as_ids(allowed);
+    const float* qp = q.data();
+    const int64_t* ap = filtered ? al.data() : nullptr;
+    const size_t an = filtered ? static_cast<size_t>(al.shape(0)) : 0;
+    read_locked(self, [&] {
+        self.batch_query(qp, 1, k, ap, an, filtered, ids.data(), dists.data());
+        return 0;
+    });
+    size_t n = 0;
+    while (n < k && ids[n] >= 0) ++n;
+    py::array_t<int64_t> oi(static_cast<py::ssize_t>(n));
+    py::array_t<float> od(static_cast<py::ssize_t>(n));
+    std::copy_n(ids.data(), n, oi.mutable_data());
+    std::copy_n(dists.data(), n, od.mutable_data());
+    return py::make_tuple(oi, od);
+}
+
+py::array_t<int64_t> add(DenseIndex& self, FloatArray data, const py::object& ids, const py::object& metadata) {
+    if (!metadata.is_none()) throw py::type_error("metadata= is not supported by qann_amd");
+    if (data.ndim() != 2) throw py::type_error("data must be a 2-D array of shape (n, dim)");
+    if (static_cast<uint64_t>(data.shape(1)) != self.dim())
+        throw std::invalid_argument("data dimension does not match index dimension");
+    refuse_if_wrapped(self, "add");
+    py::array_t<int64_t> idv;
+    const bool has_ids = !ids.is_none();
+    if (has_ids) {
+        idv = as_ids(ids);
+        if (idv.shape(0) != data.shape(0)) throw std::invalid_argument("ids must hold one id per row");
+    }
+    const float* dp = data.data();
+    const uint64_t n = static_cast<uint64_t>(data.shape(0));
+    const int64_t* ip = has_ids ? idv.data() : nullptr;
+    std::vector<int64_t> out = write_locked(self, [&] { return self.add(dp, n, ip); });
+    py::array_t<int64_t> res(static_cast<py::ssize_t>(out.size()));
+    std::copy(out.begin(), out.end(), res.mutable_data());
+    return res;
+}
+
+} // namespace
+
+PYBIND11_MODULE(_qann, m) {
+    m.doc() = "qann_amd: IVF 4-bit PQ fast-scan search with exact re-ranking on AMD MI355X";
+
+    py::enum_<Metric>(m, "Metric")
+        .value("L2", Metric::L2)
+        .value("Cosine", Metric::Cosine)
+        .value("InnerProduct", Metric::InnerProduct);
+
+    m.def("set_num_threads", &set_num_threads, py::arg("n"), "Host threads for adds and training (0 = all cores)");
+    m.def("get_num_threads", &num_threads);
+
+    py::class_<DenseIndex, std::shared_ptr<DenseIndex>>(m, "DenseIndex", "An index of dense float vectors")
+        .def("add", &add, py::arg("data"), py::arg("ids") = py::none(), py::kw_only(),
+             py::arg("metadata") = py::none(), "Add an (n, dim) array of vectors; returns their ids")
+        .def("query", &query, py::arg("query"), py::arg("k"), py::kw_only(), py::arg("allowed") = py::none(),
+             py::arg("where") = py::none(), "Query k-NN for a (dim,) vector; returns 1D (indices, distances)")
+        .def("batch_query", &batch_query, py::arg("query"), py::arg("k"), py::kw_only(),
+             py::arg("allowed") = py::none(), py::arg("where") = py::none(),
+             "Query k-NN for each row; returns (indices, distances) of shape (num_queries, k), padded with -1 / inf")
+        .def("remove",
+             [](DenseIndex& self, const py::object& ids) {
+                 refuse_if_wrapped(self, "remove");
+                 py::array_t<int64_t> idv = as_ids(ids);
+                 const int64_t* p = idv.data();
+                 const size_t n = static_cast<size_t>(idv.shape(0));
+                 return write_locked(self, [&] { return self.remove(p, n); });
+             },
+             py::arg("ids"), "Delete the vectors with these ids; returns how many were deleted")
+        .def("save",
+             [](const DenseIndex& self, const std::string& path) {
+                 read_locked(self, [&] {
+                     self.save(path);
+                     return 0;
+                 });
+             },
+             py::arg("path"), "Save the index in the qann file format")
+        .def("size", [](const DenseIndex& self) { return read_locked(self, [&] { return self.size(); }); })
+        .def("dim", &DenseIndex::dim)
+        .def("ids",
+             [](const DenseIndex& self) {
+                 std::vector<int64_t> v = read_locked(self, [&] { return self.live_ids(); });
+                 py::array_t<int64_t> out(static_cast<py::ssize_t>(v.size()));
+                 std::copy(v.begin(), v.end(), out.mutable_data());
+                 return out;
+             })
+        .def("get_vectors",
+             [](const DenseIndex& self, const py::object& ids) {
+                 py::array_t<int64_t> idv = as_ids(ids);
+                 const int64_t* p = idv.data();
+                 const size_t n = static_cast<size_t>(idv.shape(0));
+                 std::vector<float> v = read_locked(self, [&] { return self.get_vectors(p, n); });
+                 py::array_t<float> out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(self.dim())});
+                 std::copy(v.begin(), v.end(), out.mutable_data());
+                 return out;
+             },
+             py::arg("ids"))
+        .def_property_readonly("metric", &DenseIndex::metric)
+        .def_property_readonly("stores_vectors", &DenseIndex::stores_vectors);
+
+    py::class_<FlatIndex, DenseIndex, std::shared_ptr<FlatIndex>>(m, "FlatIndex")
+        .def(py::init([](uint64_t dim, bool custom_ids, Metric metric) {
+                 return std::make_shared<FlatIndex>(dim, custom_ids, metric);
+             }),
+             py::arg("dim"), py::kw_only(), py::arg("custom_ids") = false, py::arg("metric") = Metric::L2);
+
+    py::class_<IVFIndex, DenseIndex, std::shared_ptr<IVFIndex>>(m, "IVFIndex")
+        .def(py::init([](uint64_t dim, uint64_t nlist, uint64_t nprobe, bool custom_ids, Metric metric) {
+                 return std::make_shared<IVFIndex>(dim, nlist, nprobe, custom_ids, metric);
+             }),
+             py::arg("dim"), py::arg("nlist"), py::arg("nprobe") = 10, py::kw_only(), py::arg("custom_ids") = false,
+             py::arg("metric") = Metric::L2)
+        .def("enable_pq",
+             [](IVFIndex& self, uint64_t m, const py::object& K, bool fast_scan, bool rotate) {
+                 const uint64_t k = K.is_none() ? (fast_scan ? 16 : 256) : K.cast<uint64_t>();
+                 write_locked(self, [&] {
+                     self.enable_pq(m, k, fast_scan, rotate);
+                     return 0;
+                 });
+             },
+             py::arg("num_subspaces"), py::arg("centroids_per_subspace") = py::none(), py::kw_only(),
+             py::arg("fast_scan") = false, py::arg("rotate") = false)
+        .def("train",
+             [](IVFIndex& self, FloatArray data, uint64_t max_iters, const py::object& seed) {
+                 if (data.ndim() != 2) throw py::type_error("data must be a 2-D array of shape (n, dim)");
+                 if (static_cast<uint64_t>(data.shape(1)) != self.dim())
+                     throw std::invalid_argument("data dimension does not match index dimension");
+                 std::optional<uint32_t> s;
+                 if (!seed.is_none()) s = seed.cast<uint32_t>();
+                 const float* p = data.data();
+                 const uint64_t n = static_cast<uint64_t>(data.shape(0));
+                 write_locked(self, [&] {
+                     self.train(p, n, max_iters, s);
+                     return 0;
+                 });
+             },
+             py::arg("data"), py::arg("max_iters") = 10, py::arg("seed") = py::none())
+        .def_property("nprobe", [](const IVFIndex& self) { return self.nprobe(); },
+                      [](IVFIndex& self, uint64_t v) { write_locked(self, [&] { self.set_nprobe(v); return 0; }); })
+        .def_property_readonly("nlist", &IVFIndex::nlist)
+        .def_property_readonly("fast_scan", &IVFIndex::fast_scan)
+        .def_property_readonly("pq_rotate", &IVFIndex::pq_rotate)
+        .def_property_readonly("sq_bits", &IVFIndex::sq_bits)
+        .def_property_readonly("mapped", [](const IVFIndex&) { return false; })
+        .def_property("precomputed_tables", &IVFIndex::precomputed_tables,
+                      [](IVFIndex& self, bool v) { write_locked(self, [&] { self.set_precomputed_tables(v); return 0; }); })
+        .def_property("cosine_inner_product", &IVFIndex::cosine_inner_product,
+                      [](IVFIndex& self, bool v) { write_locked(self, [&] { self.set_cosine_inner_product(v); return 0; }); })
+        .def_property("filter_probe_scaling", &IVFIndex::filter_probe_scaling,
+                      [](IVFIndex& self, bool v) { write_locked(self, [&] { self.set_filter_probe_scaling(v); return 0; }); });
+
+    py::class_<RefineIndex, DenseIndex, std::shared_ptr<RefineIndex>>(m, "RefineIndex")
+        .def(py::init([](std::shared_ptr<DenseIndex> base, uint64_t k_factor, bool custom_ids) {
+                 std::unique_lock<std::shared_mutex> g(base->mu);
+                 return std::make_shared<RefineIndex>(base, k_factor, custom_ids);
+             }),
+             py::arg("base"), py::arg("k_factor") = 10, py::kw_only(), py::arg("custom_ids") = false)
+        .def_property("k_factor", [](const RefineIndex& self) { return self.k_factor(); },
+                      [](RefineIndex& self, uint64_t v) { write_locked(self, [&] { self.set_k_factor(v); return 0; }); })
+        .def_property_readonly("base", &RefineIndex::base)
+        .def_property_readonly("mapped", [](const RefineIndex&) { return false; });
+
+    m.def("load",
+          [](const std::string& path, bool mmap) {
+              (void)mmap; // the device keeps its own copy in HBM either way
+              py::gil_scoped_release release;
+              return load_index(path);
+          },
+          py::arg("path"), py::arg("mmap") = false, "Load an index saved by qann or qann_amd");
+}

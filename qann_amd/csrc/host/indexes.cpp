@@ -1,0 +1,667 @@
+#include "indexes.hpp"
+
+#include "../../../include/qann_hip.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <thread>
+#include <unordered_set>
+
+namespace qh {
+
+namespace {
+
+std::atomic<size_t> g_threads{0};
+
+// Runs fn(i) for i in [0, n) on num_threads() threads (detail::parallel_for, src/parallel.hpp:19-64).
+template <class F>
+void parallel_for(size_t n, F&& fn) {
+    const size_t t = std::min(n, num_threads());
+    if (t <= 1) {
+        for (size_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex m;
+    auto work = [&] {
+        for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n) return;
+            try {
+                fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(m);
+                if (!err) err = std::current_exception();
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t i = 1; i < t; ++i) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
+// IdMap::slots_of (src/id_map.cpp:93-112)
+std::vector<uint32_t> slots_of(const IdMap& m, const int64_t* ids, size_t n) {
+    std::vector<uint32_t> out;
+    out.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int64_t id = ids[i];
+        uint32_t slot = 0;
+        bool found;
+        if (m.custom) {
+            auto it = m.slot_of.find(id);
+            found = it != m.slot_of.end();
+            if (found) slot = it->second;
+        } else if (m.labels.empty()) {
+            found = id >= 0 && static_cast<uint64_t>(id) < m.slots;
+            if (found) slot = static_cast<uint32_t>(id);
+        } else {
+            auto it = std::lower_bound(m.labels.begin(), m.labels.end(), id);
+            found = it != m.labels.end() && *it == id;
+            if (found) slot = static_cast<uint32_t>(it - m.labels.begin());
+        }
+        if (!found || m.deleted[slot]) throw std::invalid_argument("id " + std::to_string(id) + " is not in the index");
+        out.push_back(slot);
+    }
+    std::vector<uint32_t> sorted = out;
+    std::sort(sorted.begin(), sorted.end());
+    auto dup = std::adjacent_find(sorted.begin(), sorted.end());
+    if (dup != sorted.end()) throw std::invalid_argument("id " + std::to_string(m.label(*dup)) + " appears twice");
+    return out;
+}
+
+// detail::rotate (src/rotation.cpp:165-185): an fma chain per output in k order.
+std::vector<float> rotate_rows(const std::vector<float>& R, const float* in, uint64_t n, uint64_t d) {
+    std::vector<float> out(n * d);
+    parallel_for(n, [&](size_t i) {
+        const float* x = in + i * d;
+        for (uint64_t j = 0; j < d; ++j) {
+            float acc = 0.0f;
+            for (uint64_t k = 0; k < d; ++k) acc = std::fma(x[k], R[k * d + j], acc);
+            out[i * d + j] = acc;
+        }
+    });
+    return out;
+}
+
+// PQCodebook::encode with 16 centroids (src/pq.cpp:96-130): per subspace the
+// squared distance to each centroid (the first term rounded, the rest fused),
+// and the first index of the minimum.
+void pq_encode(const std::vector<float>& cent_t, uint64_t m, uint64_t sub, uint64_t K, const float* r, uint8_t* out) {
+    float dist[256];
+    for (uint64_t s = 0; s < m; ++s) {
+        const float* c = cent_t.data() + s * sub * K;
+        for (uint64_t k = 0; k < K; ++k) {
+            const float d0 = r[s * sub] - c[k];
+            dist[k] = d0 * d0;
+        }
+        for (uint64_t j = 1; j < sub; ++j) {
+            const float q = r[s * sub + j];
+            for (uint64_t k = 0; k < K; ++k) {
+                const float d = q - c[j * K + k];
+                dist[k] = std::fma(d, d, dist[k]);
+            }
+        }
+        uint64_t best = 0;
+        for (uint64_t k = 1; k < K; ++k)
+            if (dist[k] < dist[best]) best = k;
+        out[s] = static_cast<uint8_t>(best);
+    }
+}
+
+void assign_gpu(const float* rows, uint64_t n, uint64_t dim, const float* cents, uint64_t k, uint32_t* out) {
+    check_status(qann_hip_assign_nearest(rows, n, static_cast<uint32_t>(dim), cents, static_cast<uint32_t>(k), out));
+}
+
+// Lloyd's k-means with this library's own seeding and empty-cluster rule.
+std::vector<float> kmeans(const float* x, uint64_t n, uint64_t d, uint64_t k, uint64_t iters, std::mt19937& rng,
+                          bool spherical, std::vector<uint32_t>& assign, bool gpu) {
+    std::vector<uint64_t> pick(n);
+    std::iota(pick.begin(), pick.end(), 0);
+    for (uint64_t i = 0; i < k; ++i) std::swap(pick[i], pick[i + rng() % (n - i)]);
+    std::vector<float> c(k * d);
+    for (uint64_t i = 0; i < k; ++i) std::copy_n(x + pick[i] * d, d, c.data() + i * d);
+    assign.assign(n, 0);
+    auto do_assign = [&] {
+        if (gpu) {
+            assign_gpu(x, n, d, c.data(), k, assign.data());
+            return;
+        }
+        parallel_for(n, [&](size_t i) {
+            float best = INFINITY;
+            uint32_t bi = 0;
+            for (uint64_t j = 0; j < k; ++j) {
+                float s = 0.0f;
+                for (uint64_t t = 0; t < d; ++t) {
+                    const float diff = x[i * d + t] - c[j * d + t];
+                    s = std::fma(diff, diff, s);
+                }
+                if (s < best) {
+                    best = s;
+                    bi = static_cast<uint32_t>(j);
+                }
+            }
+            assign[i] = bi;
+        });
+    };
+    for (uint64_t it = 0; it < iters; ++it) {
+        do_assign();
+        std::vector<double> sum(k * d, 0.0);
+        std::vector<uint64_t> cnt(k, 0);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t a = assign[i];
+            ++cnt[a];
+            for (uint64_t t = 0; t < d; ++t) sum[a * d + t] += x[i * d + t];
+        }
+        for (uint64_t j = 0; j < k; ++j) {
+            if (cnt[j] == 0) {
+                // an empty cluster restarts at a random row
+                std::copy_n(x + (rng() % n) * d, d, c.data() + j * d);
+                continue;
+            }
+            double norm = 0.0;
+            for (uint64_t t = 0; t < d; ++t) {
+                const double v = sum[j * d + t] / double(cnt[j]);
+                c[j * d + t] = static_cast<float>(v);
+                norm += v * v;
+            }
+            if (spherical && norm > 0) {
+                const double inv = 1.0 / std::sqrt(norm);
+                for (uint64_t t = 0; t < d; ++t) c[j * d + t] = static_cast<float>(c[j * d + t] * inv);
+            }
+        }
+    }
+    do_assign();
+    return c;
+}
+
+} // namespace
+
+void check_status(int rc) {
+    if (rc == QANN_OK) return;
+    const std::string msg = qann_hip_last_error();
+    if (rc == QANN_EINVAL) throw std::invalid_argument(msg);
+    if (rc == QANN_ENOMEM) throw std::bad_alloc();
+    throw std::runtime_error(msg);
+}
+
+void set_num_threads(size_t n) { g_threads.store(n); }
+
+size_t num_threads() {
+    const size_t n = g_threads.load();
+    return n ? n : std::max<size_t>(std::thread::hardware_concurrency(), 1);
+}
+
+// ---------------------------------------------------------------------------
+// DenseIndex
+// ---------------------------------------------------------------------------
+
+DenseIndex::~DenseIndex() = default;
+
+// DenseIndex::normalize_rows (src/index.cpp:193-218): the squared length summed
+// in double, one float scale 1 / sqrt per row.
+std::vector<float> DenseIndex::prepare(const float* q, uint64_t n) const {
+    const uint64_t d = dim();
+    std::vector<float> out(q, q + n * d);
+    if (metric() != Cosine) return out;
+    for (uint64_t i = 0; i < n; ++i) {
+        double sq = 0;
+        for (uint64_t j = 0; j < d; ++j) {
+            const float c = q[i * d + j];
+            sq += double(c) * c;
+        }
+        const double len = std::sqrt(sq);
+        if (len == 0) throw std::invalid_argument("cosine metric needs non-zero vectors");
+        const float scale = static_cast<float>(1.0 / len);
+        for (uint64_t j = 0; j < d; ++j) out[i * d + j] = q[i * d + j] * scale;
+    }
+    return out;
+}
+
+void DenseIndex::batch_query(const float* queries, uint64_t nq, uint32_t k, const int64_t* allowed, size_t n_allowed,
+                             bool filtered, int64_t* out_ids, float* out_dists) {
+    const size_t total = static_cast<size_t>(nq) * k;
+    std::fill_n(out_ids, total, int64_t{-1});
+    std::fill_n(out_dists, total, INFINITY);
+    if (nq == 0 || k == 0) return;
+    SlotFilter f;
+    if (filtered) {
+        f = ids().filter(allowed, n_allowed);
+        if (f.count == 0) return; // query_prepared: nothing may be returned
+    }
+    const std::vector<float> q = prepare(queries, nq);
+    qann_hip_search_params p{};
+    fill_params(&p);
+    p.k = k;
+    if (filtered) {
+        p.allowed_words = f.words.data();
+        p.allowed_count = f.count;
+    }
+    qann_hip_index* dev = device();
+    check_status(qann_hip_search(dev, q.data(), nq, &p, out_ids, out_dists));
+    // l2_to_cosine (src/index.cpp:23-25)
+    if (metric() == Cosine)
+        for (size_t i = 0; i < total; ++i) out_dists[i] *= 0.5f;
+}
+
+std::vector<int64_t> DenseIndex::add(const float* vecs, uint64_t n, const int64_t* idv) {
+    const uint64_t first = slots();
+    const std::vector<float> prepared = prepare(vecs, n);
+    add_prepared(prepared.data(), n, idv);
+    invalidate();
+    std::vector<int64_t> out(n);
+    for (uint64_t i = 0; i < n; ++i) out[i] = ids().label(static_cast<uint32_t>(first + i));
+    return out;
+}
+
+std::vector<int64_t> DenseIndex::live_ids() const {
+    const IdMap& m = ids();
+    std::vector<int64_t> out;
+    out.reserve(m.live());
+    for (uint64_t s = 0; s < m.slots; ++s)
+        if (!m.deleted[s]) out.push_back(m.label(static_cast<uint32_t>(s)));
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+std::vector<float> DenseIndex::get_vectors(const int64_t* idv, size_t n) const {
+    if (!stores_vectors())
+        throw std::invalid_argument("this index stores PQ or SQ codes, not the vectors; a RefineIndex over it keeps them");
+    const std::vector<uint32_t> s = slots_of(ids(), idv, n);
+    const uint64_t d = dim();
+    std::vector<float> out(s.size() * d);
+    for (size_t i = 0; i < s.size(); ++i) std::copy_n(stored_row(s[i]), d, out.data() + i * d);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// FlatIndex (src/flat_index.cpp)
+// ---------------------------------------------------------------------------
+
+FlatIndex::FlatIndex(uint64_t dim, bool custom_ids, Metric metric) : d_(std::make_unique<FlatData>()) {
+    if (dim == 0) throw std::invalid_argument("dim must be >= 1");
+    d_->dim = dim;
+    d_->metric = metric;
+    d_->ids.custom = custom_ids;
+}
+
+FlatIndex::FlatIndex(std::unique_ptr<FlatData> data) : d_(std::move(data)) {}
+
+FlatIndex::~FlatIndex() { invalidate(); }
+
+void FlatIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
+    d_->ids.check(n, idv);
+    d_->rows.insert(d_->rows.end(), vecs, vecs + n * d_->dim);
+    d_->ids.add(n, idv);
+    d_->meta.append_nulls(n);
+}
+
+size_t FlatIndex::remove(const int64_t* idv, size_t n) {
+    const size_t r = d_->ids.remove(idv, n, nullptr);
+    invalidate();
+    return r;
+}
+
+void FlatIndex::save(const std::string& path) const { save_file(path, d_.get(), nullptr, nullptr); }
+
+qann_hip_index* FlatIndex::device() const {
+    std::lock_guard<std::mutex> g(dev_mu);
+    if (dev_) return dev_;
+    qann_hip_index* h = nullptr;
+    check_status(qann_hip_flat_create(static_cast<uint32_t>(d_->dim), d_->metric, d_->rows.data(), d_->ids.slots, &h));
+    const int rc = qann_hip_set_ids(h, d_->ids.slots, d_->ids.labels.empty() ? nullptr : d_->ids.labels.data(),
+                                    d_->ids.deleted.data());
+    if (rc != QANN_OK) {
+        qann_hip_index_destroy(h);
+        check_status(rc);
+    }
+    dev_ = h;
+    return dev_;
+}
+
+void FlatIndex::fill_params(void*) const {}
+
+void FlatIndex::invalidate() const {
+    std::lock_guard<std::mutex> g(dev_mu);
+    if (dev_) qann_hip_index_destroy(dev_);
+    dev_ = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// IVFIndex (src/ivf_index.cpp)
+// ---------------------------------------------------------------------------
+
+IVFIndex::IVFIndex(uint64_t dim, uint64_t nlist, uint64_t nprobe, bool custom_ids, Metric metric)
+    : d_(std::make_unique<IvfData>()) {
+    if (nlist < 100 || nlist > 65535) throw std::invalid_argument("nlist must be in [100, 65535]");
+    if (dim == 0) throw std::invalid_argument("dim must be >= 1");
+    d_->dim = dim;
+    d_->nlist = nlist;
+    d_->metric = metric;
+    d_->ids.custom = custom_ids;
+    d_->lists.resize(nlist);
+    set_nprobe(nprobe);
+}
+
+IVFIndex::IVFIndex(std::unique_ptr<IvfData> data) : d_(std::move(data)) {}
+
+IVFIndex::~IVFIndex() { invalidate(); }
+
+void IVFIndex::set_nprobe(uint64_t n) {
+    if (n == 0 || n > d_->nlist) throw std::invalid_argument("nprobe must be in [1, nlist]");
+    d_->nprobe = n;
+}
+
+void IVFIndex::set_precomputed_tables(bool on) {
+    d_->use_precomputed = on;
+    invalidate();
+}
+
+void IVFIndex::set_cosine_inner_product(bool on) {
+    d_->cosine_ip = on;
+    invalidate();
+}
+
+void IVFIndex::enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate) {
+    if (d_->ids.slots > 0 || d_->trained) throw std::logic_error("enable_pq must be called before train()/add()");
+    if (d_->sq_bits) throw std::logic_error("enable_pq cannot be combined with enable_sq");
+    if (fast_scan && (K != 16 || m % 2 != 0 || m > 1024))
+        throw std::invalid_argument("fast_scan needs 16 centroids per subspace and an even num_subspaces <= 1024");
+    if (d_->dim == 0 || m == 0) throw std::invalid_argument("dim and num_subspaces must be >= 1");
+    if (d_->dim % m != 0) throw std::invalid_argument("dim must be divisible by num_subspaces");
+    if (K == 0 || K > 256) throw std::invalid_argument("centroids_per_subspace must be in [1, 256]");
+    d_->has_pq = true;
+    d_->fast_scan = fast_scan;
+    d_->rotate = rotate;
+    d_->pq.dim = d_->dim;
+    d_->pq.m = m;
+    d_->pq.K = K;
+    d_->pq.sub = d_->dim / m;
+    d_->pq.centroids.assign(m * K * d_->pq.sub, 0.0f);
+}
+
+void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed) {
+    if (d_->ids.slots > 0) throw std::logic_error("train() cannot be called after add()");
+    if (d_->trained) throw std::logic_error("train() has already been called");
+    if (n < d_->nlist) throw std::invalid_argument("training set must have at least nlist vectors");
+    if (d_->rotate) throw std::runtime_error("training a rotated index is not supported by qann_amd; train it with qann and load it");
+    if (d_->sq_bits || (d_->has_pq && !d_->fast_scan))
+        throw std::runtime_error("qann_amd trains IVF fast-scan indexes only");
+    const uint64_t d = d_->dim;
+    const std::vector<float> x = prepare(vecs, n);
+    std::mt19937 rng(seed ? *seed : std::random_device{}());
+    std::vector<uint32_t> assign;
+    d_->centroids = kmeans(x.data(), n, d, d_->nlist, std::max<uint64_t>(iters, 1), rng, d_->metric == Cosine, assign, true);
+    if (d_->has_pq && d_->metric == Cosine) {
+        // each list's plain mean as its residual centre (ivf_index.cpp:769-785)
+        d_->residual_centers.assign(d_->nlist * d, 0.0f);
+        std::vector<uint64_t> cnt(d_->nlist, 0);
+        for (uint64_t i = 0; i < n; ++i) {
+            float* e = d_->residual_centers.data() + size_t{assign[i]} * d;
+            for (uint64_t t = 0; t < d; ++t) e[t] += x[i * d + t];
+            ++cnt[assign[i]];
+        }
+        for (uint64_t l = 0; l < d_->nlist; ++l) {
+            float* e = d_->residual_centers.data() + l * d;
+            if (cnt[l] == 0) std::copy_n(d_->centroids.data() + l * d, d, e);
+            else
+                for (uint64_t t = 0; t < d; ++t) e[t] /= static_cast<float>(cnt[l]);
+        }
+        d_->cosine_ip = true;
+    }
+    if (d_->has_pq) {
+        PQ& pq = d_->pq;
+        const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
+        // a capped sample of residuals, 256 per centroid (pq.cpp:46-60)
+        const uint64_t cap = 256 * pq.K;
+        std::vector<uint64_t> rows(n);
+        std::iota(rows.begin(), rows.end(), 0);
+        const uint64_t ns = std::min(n, cap);
+        for (uint64_t i = 0; i < ns; ++i) std::swap(rows[i], rows[i + rng() % (n - i)]);
+        for (uint64_t s = 0; s < pq.m; ++s) {
+            std::vector<float> sx(ns * pq.sub);
+            for (uint64_t i = 0; i < ns; ++i) {
+                const uint64_t r = rows[i];
+                for (uint64_t j = 0; j < pq.sub; ++j)
+                    sx[i * pq.sub + j] = x[r * d + s * pq.sub + j] - rc[size_t{assign[r]} * d + s * pq.sub + j];
+            }
+            std::vector<uint32_t> sa;
+            std::mt19937 srng(static_cast<uint32_t>(rng() + s));
+            const std::vector<float> sc = kmeans(sx.data(), ns, pq.sub, pq.K, 25, srng, false, sa, false);
+            std::copy(sc.begin(), sc.end(), pq.centroids.begin() + s * pq.K * pq.sub);
+        }
+    }
+    d_->trained = true;
+    invalidate();
+}
+
+void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
+    if (!d_->trained) throw std::logic_error("train() has not be run yet");
+    if (!d_->has_pq || !d_->fast_scan)
+        throw std::runtime_error("qann_amd adds to IVF fast-scan indexes only");
+    d_->ids.check(n, idv);
+    const uint64_t d = d_->dim;
+    std::vector<float> rot;
+    if (d_->rotate) {
+        rot = rotate_rows(d_->rotation, vecs, n, d);
+        vecs = rot.data();
+    }
+    std::vector<uint32_t> assign(n);
+    assign_gpu(vecs, n, d, d_->centroids.data(), d_->nlist, assign.data());
+    const PQ& pq = d_->pq;
+    const std::vector<float> cent_t = pq_transpose(pq);
+    const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
+    std::vector<uint8_t> codes(n * pq.m);
+    parallel_for(n, [&](size_t i) {
+        std::vector<float> r(d);
+        const float* c = rc + size_t{assign[i]} * d;
+        for (uint64_t t = 0; t < d; ++t) r[t] = vecs[i * d + t] - c[t];
+        pq_encode(cent_t, pq.m, pq.sub, pq.K, r.data(), codes.data() + i * pq.m);
+    });
+    // each list filled in input order (ivf_index.cpp:160-194), codes packed with fs_set
+    const uint32_t first = static_cast<uint32_t>(d_->ids.slots);
+    std::vector<std::vector<uint32_t>> members(d_->nlist);
+    for (uint64_t i = 0; i < n; ++i) members[assign[i]].push_back(static_cast<uint32_t>(i));
+    const uint64_t bb = pq.m * 16;
+    parallel_for(d_->nlist, [&](size_t l) {
+        IvfList& list = d_->lists[l];
+        for (uint32_t i : members[l]) {
+            const uint64_t pos = list.ids.size();
+            list.ids.push_back(first + i);
+            if (pos % 32 == 0) list.codes.resize(list.codes.size() + bb, 0);
+            uint8_t* block = list.codes.data() + (pos / 32) * bb;
+            const uint64_t v = pos % 32;
+            for (uint64_t s = 0; s < pq.m; ++s) {
+                uint8_t& b = block[s * 16 + (v % 16)];
+                const uint8_t code = codes[i * pq.m + s];
+                b = v < 16 ? static_cast<uint8_t>((b & 0xF0) | code) : static_cast<uint8_t>((b & 0x0F) | (code << 4));
+            }
+        }
+    });
+    d_->ids.add(n, idv);
+    d_->meta.append_nulls(n);
+}
+
+size_t IVFIndex::remove(const int64_t* idv, size_t n) {
+    const size_t r = d_->ids.remove(idv, n, nullptr);
+    invalidate();
+    return r;
+}
+
+void IVFIndex::save(const std::string& path) const { save_file(path, nullptr, d_.get(), nullptr); }
+
+qann_hip_index* IVFIndex::device() const {
+    std::lock_guard<std::mutex> g(dev_mu);
+    if (dev_) return dev_;
+    if (!d_->trained) throw std::logic_error("train() has not be run yet");
+    if (!d_->has_pq || !d_->fast_scan || d_->sq_bits)
+        throw std::runtime_error("qann_amd searches IVF indexes with fast-scan PQ codes; this index stores " +
+                                 std::string(d_->sq_bits ? "SQ codes" : d_->has_pq ? "8-bit PQ codes" : "float vectors"));
+    qann_hip_ivf_desc desc{};
+    desc.dim = static_cast<uint32_t>(d_->dim);
+    desc.nlist = static_cast<uint32_t>(d_->nlist);
+    desc.metric = d_->metric;
+    desc.m = static_cast<uint32_t>(d_->pq.m);
+    desc.cosine_ip = d_->cosine_ip;
+    desc.use_precomputed = wants_precomputed(*d_);
+    desc.centroids = d_->centroids.data();
+    desc.residual_centers = d_->residual_centers.empty() ? nullptr : d_->residual_centers.data();
+    desc.pq_centroids = d_->pq.centroids.data();
+    desc.rotation = d_->rotate && !d_->rotation.empty() ? d_->rotation.data() : nullptr;
+    qann_hip_index* h = nullptr;
+    check_status(qann_hip_ivf_create(&desc, &h));
+    std::vector<uint32_t> sizes(d_->nlist);
+    uint64_t total = 0, cb = 0;
+    for (uint64_t l = 0; l < d_->nlist; ++l) {
+        sizes[l] = static_cast<uint32_t>(d_->lists[l].ids.size());
+        total += sizes[l];
+        cb += d_->lists[l].codes.size();
+    }
+    std::vector<uint32_t> slots;
+    std::vector<uint8_t> codes;
+    slots.reserve(total);
+    codes.reserve(cb);
+    for (const IvfList& list : d_->lists) {
+        slots.insert(slots.end(), list.ids.begin(), list.ids.end());
+        codes.insert(codes.end(), list.codes.begin(), list.codes.end());
+    }
+    int rc = qann_hip_ivf_set_lists(h, sizes.data(), slots.data(), codes.data());
+    if (rc == QANN_OK)
+        rc = qann_hip_set_ids(h, d_->ids.slots, d_->ids.labels.empty() ? nullptr : d_->ids.labels.data(),
+                              d_->ids.deleted.data());
+    if (rc != QANN_OK) {
+        qann_hip_index_destroy(h);
+        check_status(rc);
+    }
+    dev_ = h;
+    return dev_;
+}
+
+void IVFIndex::fill_params(void* params) const {
+    auto* p = static_cast<qann_hip_search_params*>(params);
+    p->nprobe = static_cast<uint32_t>(d_->nprobe);
+    p->filter_probe_scaling = filter_probe_scaling_;
+}
+
+void IVFIndex::invalidate() const {
+    if (wrapper) wrapper->invalidate();
+    std::lock_guard<std::mutex> g(dev_mu);
+    if (dev_) qann_hip_index_destroy(dev_);
+    dev_ = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// RefineIndex (src/refine_index.cpp)
+// ---------------------------------------------------------------------------
+
+RefineIndex::RefineIndex(std::shared_ptr<DenseIndex> base, uint64_t k_factor, bool custom_ids)
+    : base_(std::move(base)), d_(std::make_unique<RefineData>()) {
+    if (base_->slots() != 0) throw std::invalid_argument("base index must be empty");
+    if (k_factor == 0) throw std::invalid_argument("k_factor must be >= 1");
+    if (!base_->ids().ids_are_slots())
+        throw std::invalid_argument("base index must not use custom ids or have been compacted");
+    if (base_->wrapped) throw std::invalid_argument("base index is already wrapped by another index");
+    if (dynamic_cast<RefineIndex*>(base_.get()))
+        throw std::invalid_argument("a RefineIndex's base must not be a RefineIndex; wrap the approximate index directly");
+    d_->k_factor = k_factor;
+    d_->ids.custom = custom_ids;
+    base_->wrapped = true;
+    base_->wrapper = this;
+}
+
+RefineIndex::RefineIndex(std::shared_ptr<DenseIndex> base, std::unique_ptr<RefineData> data)
+    : base_(std::move(base)), d_(std::move(data)) {
+    base_->wrapped = true;
+    base_->wrapper = this;
+}
+
+RefineIndex::~RefineIndex() {
+    invalidate();
+    base_->wrapped = false;
+    base_->wrapper = nullptr;
+}
+
+void RefineIndex::set_k_factor(uint64_t kf) {
+    if (kf == 0) throw std::invalid_argument("k_factor must be >= 1");
+    d_->k_factor = kf;
+}
+
+void RefineIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
+    d_->ids.check(n, idv);
+    base_->add_prepared(vecs, n, nullptr);
+    base_->invalidate();
+    d_->ids.add(n, idv);
+    d_->rows.insert(d_->rows.end(), vecs, vecs + n * dim());
+    d_->meta.append_nulls(n);
+}
+
+size_t RefineIndex::remove(const int64_t* idv, size_t n) {
+    std::vector<uint32_t> slots;
+    const size_t removed = d_->ids.remove(idv, n, &slots);
+    const std::vector<int64_t> base_ids(slots.begin(), slots.end());
+    base_->remove(base_ids.data(), base_ids.size());
+    invalidate();
+    return removed;
+}
+
+void RefineIndex::save(const std::string& path) const {
+    const auto* flat = dynamic_cast<const FlatIndex*>(base_.get());
+    const auto* ivf = dynamic_cast<const IVFIndex*>(base_.get());
+    save_file(path, flat ? &flat->data() : nullptr, ivf ? &ivf->data() : nullptr, d_.get());
+}
+
+qann_hip_index* RefineIndex::device() const {
+    std::lock_guard<std::mutex> g(dev_mu);
+    if (dev_) return dev_;
+    qann_hip_index* h = nullptr;
+    if (auto* ivf = dynamic_cast<IVFIndex*>(base_.get())) {
+        qann_hip_index* b = ivf->device();
+        check_status(qann_hip_refine_create(b, d_->rows.data(), d_->ids.slots, &h));
+    } else {
+        // over a Flat base the candidates are already exact: the k nearest of
+        // the raw rows by (distance, slot) are the re-ranked result
+        check_status(qann_hip_flat_create(static_cast<uint32_t>(dim()), metric(), d_->rows.data(), d_->ids.slots, &h));
+    }
+    const int rc = qann_hip_set_ids(h, d_->ids.slots, d_->ids.labels.empty() ? nullptr : d_->ids.labels.data(),
+                                    d_->ids.deleted.data());
+    if (rc != QANN_OK) {
+        qann_hip_index_destroy(h);
+        check_status(rc);
+    }
+    dev_ = h;
+    return dev_;
+}
+
+void RefineIndex::fill_params(void* params) const {
+    base_->fill_params(params);
+    static_cast<qann_hip_search_params*>(params)->k_factor = static_cast<uint32_t>(d_->k_factor);
+}
+
+void RefineIndex::invalidate() const {
+    std::lock_guard<std::mutex> g(dev_mu);
+    if (dev_) qann_hip_index_destroy(dev_);
+    dev_ = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+
+std::shared_ptr<DenseIndex> load_index(const std::string& path) {
+    LoadedIndex li = load_file(path);
+    std::shared_ptr<DenseIndex> base;
+    if (li.flat) base = std::make_shared<FlatIndex>(std::move(li.flat));
+    else base = std::make_shared<IVFIndex>(std::move(li.ivf));
+    if (!li.refine) return base;
+    return std::make_shared<RefineIndex>(base, std::move(li.refine));
+}
+
+} // namespace qh

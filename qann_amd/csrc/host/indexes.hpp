@@ -1,0 +1,180 @@
+// Host-side index objects with the reference's API (include/vecengine/
+// index.hpp, flat_index.hpp, ivf_index.hpp, refine_index.hpp): they own the
+// index contents as the saved format holds them, keep a device copy in HBM
+// through the C ABI of include/qann_hip.h, and run every query there.
+#pragma once
+
+#include "qfile.hpp"
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <shared_mutex>
+#include <string>
+#include <vector>
+
+struct qann_hip_index;
+
+namespace qh {
+
+// Throws the exception the reference would for a C-ABI status: ValueError
+// (std::invalid_argument), MemoryError (std::bad_alloc) or RuntimeError.
+void check_status(int rc);
+
+class DenseIndex {
+public:
+    virtual ~DenseIndex();
+    virtual uint64_t dim() const = 0;
+    virtual Metric metric() const = 0;
+    virtual const IdMap& ids() const = 0;
+    virtual IdMap& ids() = 0;
+    uint64_t size() const { return ids().live(); }
+    uint64_t slots() const { return ids().slots; }
+
+    // DenseIndex::query_batch (src/index.cpp:117-159, query_prepared :170-184):
+    // padded nq x k results. allowed / n_allowed: the id filter, or null.
+    void batch_query(const float* queries, uint64_t nq, uint32_t k, const int64_t* allowed, size_t n_allowed,
+                     bool filtered, int64_t* out_ids, float* out_dists);
+
+    // add_batch (src/index.cpp:33-49): returns the added ids.
+    std::vector<int64_t> add(const float* vecs, uint64_t n, const int64_t* ids);
+    virtual size_t remove(const int64_t* ids, size_t n) = 0;
+    std::vector<int64_t> live_ids() const;
+    virtual bool stores_vectors() const { return false; }
+    std::vector<float> get_vectors(const int64_t* ids, size_t n) const;
+    virtual void save(const std::string& path) const = 0;
+
+    // readers take it shared, writers exclusive (python_bindings.cpp:97-131)
+    mutable std::shared_mutex mu;
+    bool wrapped = false;
+    // the RefineIndex wrapping this index, whose device copy points into ours
+    const DenseIndex* wrapper = nullptr;
+
+    // frees the device copy after a change (called with mu held exclusively);
+    // a wrapped index also frees its wrapper's copy, which points into its own
+    virtual void invalidate() const = 0;
+
+protected:
+    friend class RefineIndex;
+    // the vectors already prepared (unit length for Cosine)
+    virtual void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) = 0;
+    virtual const float* stored_row(uint32_t) const { return nullptr; }
+    // builds or refreshes the device copy; called with mu held shared
+    virtual qann_hip_index* device() const = 0;
+    virtual void fill_params(void* params) const = 0;
+    std::vector<float> prepare(const float* q, uint64_t n) const;
+    mutable std::mutex dev_mu;
+};
+
+class FlatIndex : public DenseIndex {
+public:
+    FlatIndex(uint64_t dim, bool custom_ids, Metric metric);
+    explicit FlatIndex(std::unique_ptr<FlatData> data);
+    ~FlatIndex() override;
+    uint64_t dim() const override { return d_->dim; }
+    Metric metric() const override { return d_->metric; }
+    const IdMap& ids() const override { return d_->ids; }
+    IdMap& ids() override { return d_->ids; }
+    size_t remove(const int64_t* ids, size_t n) override;
+    bool stores_vectors() const override { return true; }
+    void save(const std::string& path) const override;
+    const FlatData& data() const { return *d_; }
+
+protected:
+    void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) override;
+    const float* stored_row(uint32_t s) const override { return d_->rows.data() + static_cast<size_t>(s) * d_->dim; }
+    qann_hip_index* device() const override;
+    void fill_params(void* params) const override;
+    void invalidate() const override;
+
+private:
+    std::unique_ptr<FlatData> d_;
+    mutable qann_hip_index* dev_ = nullptr;
+};
+
+class IVFIndex : public DenseIndex {
+public:
+    IVFIndex(uint64_t dim, uint64_t nlist, uint64_t nprobe, bool custom_ids, Metric metric);
+    explicit IVFIndex(std::unique_ptr<IvfData> data);
+    ~IVFIndex() override;
+    uint64_t dim() const override { return d_->dim; }
+    Metric metric() const override { return d_->metric; }
+    const IdMap& ids() const override { return d_->ids; }
+    IdMap& ids() override { return d_->ids; }
+    size_t remove(const int64_t* ids, size_t n) override;
+    void save(const std::string& path) const override;
+
+    uint64_t nlist() const { return d_->nlist; }
+    uint64_t nprobe() const { return d_->nprobe; }
+    void set_nprobe(uint64_t n);
+    bool fast_scan() const { return d_->fast_scan; }
+    bool pq_rotate() const { return d_->rotate; }
+    int sq_bits() const { return d_->sq_bits; }
+    bool trained() const { return d_->trained; }
+    bool precomputed_tables() const { return wants_precomputed(*d_); }
+    void set_precomputed_tables(bool on);
+    bool cosine_inner_product() const { return d_->cosine_ip; }
+    void set_cosine_inner_product(bool on);
+    bool filter_probe_scaling() const { return filter_probe_scaling_; }
+    void set_filter_probe_scaling(bool on) { filter_probe_scaling_ = on; }
+    uint8_t pq_distance() const { return d_->pq_distance; }
+
+    // IVFIndex::enable_pq (src/ivf_index.cpp:83-94)
+    void enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate);
+    // IVFIndex::train (src/ivf_index.cpp:725-806), with this library's own
+    // k-means (the clustering is not restated; the index it makes is in the
+    // reference's format and searches identically in both)
+    void train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed);
+    const IvfData& data() const { return *d_; }
+
+protected:
+    void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) override;
+    qann_hip_index* device() const override;
+    void fill_params(void* params) const override;
+    void invalidate() const override;
+
+private:
+    friend class RefineIndex;
+    std::unique_ptr<IvfData> d_;
+    bool filter_probe_scaling_ = true;
+    mutable qann_hip_index* dev_ = nullptr;
+};
+
+class RefineIndex : public DenseIndex {
+public:
+    RefineIndex(std::shared_ptr<DenseIndex> base, uint64_t k_factor, bool custom_ids);
+    RefineIndex(std::shared_ptr<DenseIndex> base, std::unique_ptr<RefineData> data);
+    ~RefineIndex() override;
+    uint64_t dim() const override { return base_->dim(); }
+    Metric metric() const override { return base_->metric(); }
+    const IdMap& ids() const override { return d_->ids; }
+    IdMap& ids() override { return d_->ids; }
+    size_t remove(const int64_t* ids, size_t n) override;
+    bool stores_vectors() const override { return true; }
+    void save(const std::string& path) const override;
+    uint64_t k_factor() const { return d_->k_factor; }
+    void set_k_factor(uint64_t kf);
+    std::shared_ptr<DenseIndex> base() const { return base_; }
+
+protected:
+    void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) override;
+    const float* stored_row(uint32_t s) const override { return d_->rows.data() + static_cast<size_t>(s) * dim(); }
+    qann_hip_index* device() const override;
+    void fill_params(void* params) const override;
+    void invalidate() const override;
+
+private:
+    std::shared_ptr<DenseIndex> base_;
+    std::unique_ptr<RefineData> d_;
+    mutable qann_hip_index* dev_ = nullptr;
+};
+
+// load_index (src/index_factory.cpp:120-139)
+std::shared_ptr<DenseIndex> load_index(const std::string& path);
+
+// set_num_threads / num_threads (src/threads.cpp): host threads for adds and training.
+void set_num_threads(size_t n);
+size_t num_threads();
+
+} // namespace qh

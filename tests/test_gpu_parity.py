@@ -1,0 +1,135 @@
+"""Parity of the HIP search path with the reference.
+
+Every search here runs on the GPU through libqann_hip.so (the C ABI of
+include/qann_hip.h, called by the qann_amd host objects). Results are
+compared with
+  * the results the reference library itself produced for the synthetic
+    golden indexes (tests/golden/synth): ids and distance bits exact;
+  * the results the reference recorded for its own shipped indexes: ids exact,
+    distances within 1e-5 (its tests' bar), and bit-exact with the oracle;
+  * the CPU oracle on indexes built here, at sizes it finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import qann_amd
+from cases import REF_FIXTURES, SYNTH_CASES, ref_fixture, synth_case, synth_queries
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(got, want, msg=""):
+    np.testing.assert_array_equal(got[0], want[0], err_msg=msg)
+    np.testing.assert_array_equal(got[1].view(np.uint32), want[1].view(np.uint32), err_msg=msg)
+
+
+@pytest.mark.parametrize("name", SYNTH_CASES)
+def test_gpu_bit_exact_on_reference_goldens(name):
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    for npb, k, fname, allowed, stem in synth_queries(exp, params):
+        if npb is not None:
+            ivf.nprobe = npb
+        got = idx.batch_query(q, k, allowed=allowed)
+        assert_same(got, (exp[stem + "_ids"], exp[stem + "_dists"]), f"{name} {stem}")
+
+
+@pytest.mark.parametrize("name,queries", REF_FIXTURES)
+def test_gpu_reproduces_reference_fixtures(oracle, name, queries):
+    path, q, ids, dists = ref_fixture(name, queries)
+    idx = qann_amd.load(str(path))
+    got_ids, got_d = idx.batch_query(q, ids.shape[1])
+    np.testing.assert_array_equal(got_ids, ids)
+    np.testing.assert_allclose(got_d, dists, rtol=1e-5, atol=1e-5)
+    assert_same((got_ids, got_d), oracle.load(path).batch_query(q, ids.shape[1]), name)
+
+
+def test_gpu_single_query_matches_batch_row():
+    path, q, _, _ = synth_case("l2_fs_refine")
+    idx = qann_amd.load(str(path))
+    ids, d = idx.batch_query(q, 10)
+    for i in range(5):
+        si, sd = idx.query(q[i], 10)
+        np.testing.assert_array_equal(si, ids[i])
+        np.testing.assert_array_equal(sd, d[i])
+    with pytest.raises(TypeError):
+        idx.query(q[:2], 10)
+    with pytest.raises(ValueError):
+        idx.batch_query(q[:, :5], 10)
+
+
+def test_gpu_edge_cases():
+    path, q, exp, _ = synth_case("l2_fs")
+    idx = qann_amd.load(str(path))
+    # k above the number of vectors: padded with -1 / inf
+    ids, d = idx.batch_query(q[:3], idx.size() + 10)
+    assert (ids[:, -10:] == -1).all() and np.isinf(d[:, -10:]).all()
+    # an empty or all-unknown filter returns nothing
+    ids, d = idx.batch_query(q[:3], 5, allowed=[])
+    assert (ids == -1).all()
+    ids, d = idx.batch_query(q[:3], 5, allowed=[10**9, -5])
+    assert (ids == -1).all()
+    # k = 0 and no queries
+    assert idx.batch_query(q[:3], 0)[0].shape == (3, 0)
+    assert idx.batch_query(q[:0], 5)[0].shape == (0, 5)
+    # every list probed
+    idx.nprobe = 100
+    ids, _ = idx.batch_query(q, 10)
+    assert (ids >= 0).all()
+
+
+def mixture(rng, n, d, centers=50, sigma=0.08):
+    c = rng.random((centers, d)).astype(np.float32)
+    lab = rng.integers(0, centers, n)
+    return (c[lab] + sigma * rng.standard_normal((n, d))).astype(np.float32)
+
+
+BUILDS = [
+    # metric, dim, m, nlist, k_factor (0: plain IVF)
+    (qann_amd.Metric.L2, 48, 24, 128, 0),
+    (qann_amd.Metric.L2, 48, 24, 128, 5),
+    (qann_amd.Metric.InnerProduct, 32, 16, 100, 0),
+    (qann_amd.Metric.InnerProduct, 32, 16, 100, 3),
+    (qann_amd.Metric.Cosine, 100, 50, 150, 10),
+    (qann_amd.Metric.Cosine, 64, 32, 100, 0),
+]
+
+
+@pytest.mark.parametrize("metric,dim,m,nlist,kf", BUILDS)
+def test_gpu_built_index_matches_oracle(tmp_path, oracle, metric, dim, m, nlist, kf):
+    rng = np.random.default_rng(dim * 7 + m + kf)
+    x = mixture(rng, 12000, dim)
+    q = mixture(rng, 200, dim)
+    ivf = qann_amd.IVFIndex(dim, nlist, 12, metric=metric)
+    ivf.enable_pq(m, fast_scan=True)
+    ivf.train(x, seed=3)
+    idx = qann_amd.RefineIndex(ivf, kf) if kf else ivf
+    idx.add(x)
+    idx.remove(rng.choice(12000, 300, replace=False))
+    path = tmp_path / "built.qann"
+    idx.save(str(path))
+    orc = oracle.load(path)
+    for npb in (1, 12, 40):
+        ivf.nprobe = npb
+        orc.set_nprobe(npb)
+        for k in (1, 10, 100):
+            assert_same(idx.batch_query(q, k), orc.batch_query(q, k), f"nprobe {npb} k {k}")
+    allowed = rng.choice(12000, 1200, replace=False)
+    ivf.nprobe = 12
+    orc.set_nprobe(12)
+    assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "filtered")
+
+
+def test_gpu_flat_matches_oracle(tmp_path, oracle):
+    rng = np.random.default_rng(11)
+    for metric in (qann_amd.Metric.L2, qann_amd.Metric.InnerProduct, qann_amd.Metric.Cosine):
+        x = mixture(rng, 5000, 37)
+        q = mixture(rng, 64, 37)
+        flat = qann_amd.FlatIndex(37, metric=metric, custom_ids=True)
+        flat.add(x, ids=rng.choice(10**7, 5000, replace=False))
+        path = tmp_path / "flat.qann"
+        flat.save(str(path))
+        orc = oracle.load(path)
+        for k in (1, 16, 300):
+            assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"{metric} k {k}")
