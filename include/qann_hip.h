@@ -150,6 +150,29 @@ int qann_hip_search_async(qann_hip_index* index, const float* d_queries, uint64_
 int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const float* centroids, uint32_t k,
                             uint32_t* out);
 
+/*
+ * Per-stage kernel timing, the device counterpart of the reference's
+ * compile-time section timers (src/profile.hpp: coarse, table, scan, rerank).
+ * While enabled, every search synchronizes its stream after each stage and
+ * adds the stage's HIP-event time to a process-wide total; read returns the
+ * totals in milliseconds and launch counts for QANN_STAGE_COUNT stages, in the
+ * order of qann_hip_stage_name. Timing serializes the stages: use it for a
+ * measurement pass, not the throughput pass.
+ */
+enum {
+    QANN_STAGE_ROTATE = 0,
+    QANN_STAGE_COARSE = 1,
+    QANN_STAGE_SELECT = 2,
+    QANN_STAGE_TABLES = 3,
+    QANN_STAGE_SCAN = 4,
+    QANN_STAGE_RERANK = 5,
+    QANN_STAGE_FLAT = 6,
+    QANN_STAGE_COUNT = 7,
+};
+int qann_hip_profile_enable(int on);
+int qann_hip_profile_read(double* ms, uint64_t* launches, int reset);
+const char* qann_hip_stage_name(int stage);
+
 /* Device bytes the index holds. */
 int qann_hip_index_bytes(qann_hip_index* index, uint64_t* bytes);
 

@@ -5,6 +5,7 @@
 #include "ivf_kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -93,6 +94,42 @@ struct Scratch {
 };
 
 size_t pad256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+// Stage timing (qann_hip_profile_*).
+std::atomic<int> g_prof_on{0};
+std::mutex g_prof_mu;
+double g_prof_ms[QANN_STAGE_COUNT] = {};
+uint64_t g_prof_n[QANN_STAGE_COUNT] = {};
+
+// Times one stage when profiling is on: events around the launches, then a sync.
+struct StageTimer {
+    int stage;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(int s, hipStream_t stream) : stage(s), st(stream) {
+        if (!g_prof_on.load()) return;
+        check(hipEventCreate(&a), "hipEventCreate");
+        check(hipEventCreate(&b), "hipEventCreate");
+        check(hipEventRecord(a, st), "hipEventRecord");
+    }
+    void done() {
+        if (!a) return;
+        check(hipEventRecord(b, st), "hipEventRecord");
+        check(hipEventSynchronize(b), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        a = b = nullptr;
+        std::lock_guard<std::mutex> g(g_prof_mu);
+        g_prof_ms[stage] += ms;
+        g_prof_n[stage] += 1;
+    }
+    ~StageTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
 
 } // namespace
 
@@ -238,10 +275,15 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const int ip = h->metric == QANN_METRIC_IP;
     const uint64_t nwords = (h->slots + 63) / 64;
 
+    const size_t qlds = static_cast<size_t>((h->dim + 1) & ~1u) * 4;
+    if (qlds > qd::kLdsMax / 2) invalid("dim too large for the on-chip query copy");
     if (h->kind == 1) { // FlatIndex::query_impl
         sc.used = 0;
         const bool f = filtered;
-        sc.reserve(f ? pad256(h->slots * 4) + 256 : 256);
+        const size_t selb = qd::block_select_bytes(static_cast<int>(k), false);
+        const bool gsel = qlds + selb > qd::kLdsMax;
+        sc.reserve((f ? pad256(h->slots * 4) + 256 : 256) + (gsel ? pad256(nq * selb) : 0));
+        unsigned char* gbuf = gsel ? sc.take<unsigned char>(nq * selb) : nullptr;
         uint32_t* slot_list = nullptr;
         uint32_t n_slots = static_cast<uint32_t>(h->slots);
         if (f) {
@@ -250,9 +292,12 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             n_slots = static_cast<uint32_t>(std::min<uint64_t>(p.allowed_count, h->slots));
             bitmap_slots(d_filter, nwords, slot_list, cnt, n_slots, st);
         }
+        StageTimer t(QANN_STAGE_FLAT, st);
         qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, h->rows, slot_list, n_slots,
-                                f ? nullptr : h->deleted, 0, 0, static_cast<int>(k), h->labels, d_ids, d_dists, st);
+                                f ? nullptr : h->deleted, 0, 0, static_cast<int>(k), h->labels, d_ids, d_dists, gbuf,
+                                st);
         check_launch("k_block_select");
+        t.done();
         return;
     }
 
@@ -270,14 +315,19 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         // RefineIndex::use_direct (src/refine_index.cpp:59-69): score the allowed rows directly
         if (filtered && p.allowed_count <= 6ull * ix.nlist) {
             sc.used = 0;
-            sc.reserve(pad256(p.allowed_count * 4) + 256);
+            const size_t selb = qd::block_select_bytes(static_cast<int>(k), h->labels != nullptr);
+            const bool gsel = qlds + selb > qd::kLdsMax;
+            sc.reserve(pad256(p.allowed_count * 4) + 256 + (gsel ? pad256(nq * selb) : 0));
+            unsigned char* gbuf = gsel ? sc.take<unsigned char>(nq * selb) : nullptr;
             uint32_t* slot_list = sc.take<uint32_t>(p.allowed_count);
             uint32_t* cnt = sc.take<uint32_t>(1);
             bitmap_slots(d_filter, nwords, slot_list, cnt, static_cast<uint32_t>(p.allowed_count), st);
+            StageTimer t(QANN_STAGE_RERANK, st);
             qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, h->rows, slot_list,
                                     static_cast<uint32_t>(p.allowed_count), nullptr, 1, 1, static_cast<int>(k),
-                                    h->labels, d_ids, d_dists, st);
+                                    h->labels, d_ids, d_dists, gbuf, st);
             check_launch("k_block_select");
+            t.done();
             return;
         }
     }
@@ -294,8 +344,12 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const bool ip_pq = ix.metric == QANN_METRIC_IP || cos_pq;
     const bool use_pre = !ip_pq && ix.B != nullptr;
     const uint32_t m = static_cast<uint32_t>(ix.m);
-    const size_t lds = qd::scan_lds_bytes(static_cast<int>(kp), ix.m, ix.dim);
-    if (lds > 64 * 1024) invalid("k * k_factor too large for the on-chip top-k buffer");
+    // the TopK buffers go to HBM when they do not fit LDS
+    const bool gtopk = qd::scan_lds_bytes(static_cast<int>(kp), ix.m, ix.dim, true) > qd::kLdsMax;
+    if (qd::scan_lds_bytes(static_cast<int>(kp), ix.m, ix.dim, false) > qd::kLdsMax)
+        invalid("num_subspaces and dim too large for the on-chip tables");
+    const size_t rkeys = qd::rerank_keys(static_cast<int>(kp));
+    const bool grerank = refine && qlds + rkeys * 8 > qd::kLdsMax;
 
     // IVFIndex::scan's filter parameters (src/ivf_index.cpp:462, 502, 510-514)
     int min_probes = static_cast<int>(nprobe), wanted = 0, block_filter = 0;
@@ -317,29 +371,38 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += use_pre ? pad256(nq * m * 16 * 4) : 0;
     need += ip_pq ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
     need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + 2 * pad256(nq * 4);
+    need += gtopk ? pad256(nq * 2 * kp * 4) + pad256(nq * 2 * kp * 8) : 0;
+    need += grerank ? pad256(nq * rkeys * 8) : 0;
     sc.used = 0;
     sc.reserve(need);
     const float* qr = d_q;
     if (ix.rotation) {
         float* r = sc.take<float>(nq * d);
+        StageTimer t(QANN_STAGE_ROTATE, st);
         qd::launch_rotate(d_q, ix.rotation, static_cast<int>(nq), ix.dim, r, st);
         check_launch("k_rotate");
+        t.done();
         qr = r;
     }
     float* D = sc.take<float>(nq * nlist);
+    StageTimer t_coarse(QANN_STAGE_COARSE, st);
     qd::launch_coarse_dist(qr, ix.centroids, static_cast<int>(nq), ix.nlist, ix.dim, ip, D, st);
     check_launch("k_coarse_dist");
+    t_coarse.done();
     uint16_t* pidx = sc.take<uint16_t>(nq * M);
     float* pval = sc.take<float>(nq * M);
+    StageTimer t_sel(QANN_STAGE_SELECT, st);
     qd::launch_select_sorted(D, static_cast<int>(nq), ix.nlist, static_cast<int>(M), pidx, pval, st);
     check_launch("k_select_sorted");
     float* pA = sc.take<float>(nq * M);
     float* pbase = sc.take<float>(nq * M);
     qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
     check_launch("k_probe_base");
+    t_sel.done();
     float* C = nullptr;
     uint8_t* lut = nullptr;
     float *lscale = nullptr, *loff = nullptr;
+    StageTimer t_tab(QANN_STAGE_TABLES, st);
     if (use_pre) {
         C = sc.take<float>(nq * m * 16);
         qd::launch_query_tables(qr, ix, static_cast<int>(nq), 0, C, nullptr, nullptr, nullptr, st);
@@ -351,6 +414,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         qd::launch_query_tables(qr, ix, static_cast<int>(nq), 1, nullptr, lut, lscale, loff, st);
         check_launch("k_query_tables");
     }
+    t_tab.done();
     qd::ScanArgs a{};
     a.ix = ix;
     if (!use_pre) a.ix.B = nullptr;
@@ -377,8 +441,12 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     a.out_label = cl;
     a.out_count = ccount;
     a.status = status;
+    a.g_dist = gtopk ? sc.take<float>(nq * 2 * kp) : nullptr;
+    a.g_label = gtopk ? sc.take<int64_t>(nq * 2 * kp) : nullptr;
+    StageTimer t_scan(QANN_STAGE_SCAN, st);
     qd::launch_scan_fastscan(a, static_cast<int>(nq), st);
     check_launch("k_scan_fastscan");
+    t_scan.done();
     if (filtered && M < nlist) {
         std::vector<int> hs(nq);
         check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
@@ -387,9 +455,12 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
     }
     if (refine) {
+        uint64_t* gkeys = grerank ? sc.take<uint64_t>(nq * rkeys) : nullptr;
+        StageTimer t_re(QANN_STAGE_RERANK, st);
         qd::launch_rerank(d_q, static_cast<int>(nq), h->dim, ip, h->rows, cl, ccount, static_cast<int>(kp),
-                          static_cast<int>(k), h->labels, d_ids, d_dists, st);
+                          static_cast<int>(k), h->labels, d_ids, d_dists, gkeys, st);
         check_launch("k_rerank");
+        t_re.done();
     } else {
         hipLaunchKernelGGL(k_copy_results, dim3(static_cast<unsigned>((nq * k + 255) / 256)), dim3(256), 0, st, cl, cd,
                            static_cast<int>(kp), static_cast<int>(k), static_cast<int>(nq), d_ids, d_dists);
@@ -681,6 +752,29 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
             for (uint64_t i = 0; i < b; ++i) out[b0 + i] = hidx[i];
         }
     });
+}
+
+int qann_hip_profile_enable(int on) {
+    g_prof_on.store(on ? 1 : 0);
+    return QANN_OK;
+}
+
+int qann_hip_profile_read(double* ms, uint64_t* launches, int reset) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (int i = 0; i < QANN_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = g_prof_ms[i];
+        if (launches) launches[i] = g_prof_n[i];
+        if (reset) {
+            g_prof_ms[i] = 0.0;
+            g_prof_n[i] = 0;
+        }
+    }
+    return QANN_OK;
+}
+
+const char* qann_hip_stage_name(int stage) {
+    static const char* names[QANN_STAGE_COUNT] = {"rotate", "coarse", "select", "tables", "scan", "rerank", "flat"};
+    return stage >= 0 && stage < QANN_STAGE_COUNT ? names[stage] : "";
 }
 
 int qann_hip_index_bytes(qann_hip_index* h, uint64_t* bytes) {

@@ -291,15 +291,16 @@ __global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
     const int m = ix.m;
     const int bb = m * 16;
     const int kp = a.kp;
-    // LDS layout (ScanArgs::lds_bytes computes the same)
-    int64_t* tk_label = reinterpret_cast<int64_t*>(lds);                 // 2k'
-    float* tk_dist = reinterpret_cast<float*>(tk_label + 2 * kp);       // 2k'
-    float* row_min = tk_dist + 2 * kp;                                   // m
+    const int qi = blockIdx.x;
+    // LDS layout (scan_lds_bytes computes the same); the TopK buffer may live in HBM
+    const bool tk_lds = a.g_dist == nullptr;
+    int64_t* tk_label = tk_lds ? reinterpret_cast<int64_t*>(lds) : a.g_label + static_cast<size_t>(qi) * 2 * kp;
+    float* tk_dist = tk_lds ? reinterpret_cast<float*>(tk_label + 2 * kp) : a.g_dist + static_cast<size_t>(qi) * 2 * kp;
+    float* row_min = tk_lds ? tk_dist + 2 * kp : reinterpret_cast<float*>(lds); // m
     float* resid = row_min + m;                                          // dim
     uint16_t* sums = reinterpret_cast<uint16_t*>(resid + ix.dim);        // kChunk * 32
     uint8_t* lut = reinterpret_cast<uint8_t*>(sums + kChunkBlocks * kFsBlock); // m * 16
 
-    const int qi = blockIdx.x;
     const int lane = lane_id();
     const float* q = a.q + static_cast<size_t>(qi) * ix.dim;
     const bool ip_mode = a.ip_lut != nullptr;
@@ -541,14 +542,15 @@ __global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
 // smallest by (distance, slot) in pair order, mapped to the refine labels.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip, const float* rows, const int64_t* cand, const int* cand_count,
-                         int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists) {
+                         int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists, uint64_t* gkeys,
+                         int gstride) {
     extern __shared__ __align__(16) unsigned char lds[];
     float* sq = reinterpret_cast<float*>(lds);
     int np = 1;
     const int i = blockIdx.x;
     const int n = cand_count[i];
     while (np < n) np <<= 1;
-    uint64_t* keys = reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
+    uint64_t* keys = gkeys ? gkeys + static_cast<size_t>(i) * gstride : reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
     for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
     __syncthreads();
     for (int c = threadIdx.x; c < np; c += blockDim.x) {
@@ -584,10 +586,13 @@ __global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, int ip, const float* rows, const uint32_t* slot_list,
                                uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
-                               int cap, const int64_t* labels, int64_t* out_ids, float* out_dists) {
+                               int cap, const int64_t* labels, int64_t* out_ids, float* out_dists,
+                               unsigned char* gbuf, size_t gstride) {
     extern __shared__ __align__(16) unsigned char lds[];
     float* sq = reinterpret_cast<float*>(lds);
-    uint64_t* keys = reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1)); // cap (pow2) keys
+    // cap (pow2) keys, then the final sort's arrays: in LDS after the query, or in HBM
+    uint64_t* keys = gbuf ? reinterpret_cast<uint64_t*>(gbuf + blockIdx.x * gstride)
+                          : reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
     __shared__ int count_s;
     __shared__ uint64_t thr_s;
     __shared__ int big_s;
@@ -753,32 +758,47 @@ void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut,
 }
 
 void launch_scan_fastscan(const ScanArgs& a, int nq, hipStream_t s) {
-    const size_t lds = scan_lds_bytes(a.kp, a.ix.m, a.ix.dim);
+    const size_t lds = scan_lds_bytes(a.kp, a.ix.m, a.ix.dim, a.g_dist == nullptr);
     hipLaunchKernelGGL(k_scan_fastscan, dim3(nq), dim3(kWave), lds, s, a);
+}
+
+size_t rerank_keys(int kc) {
+    size_t np = 1;
+    while (np < static_cast<size_t>(kc)) np <<= 1;
+    return np;
 }
 
 void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
                    const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
-                   hipStream_t s) {
-    int np = 1;
-    while (np < kc) np <<= 1;
-    const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + static_cast<size_t>(np) * 8;
+                   uint64_t* gkeys, hipStream_t s) {
+    const size_t np = rerank_keys(kc);
+    const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + (gkeys ? 0 : np * 8);
     hipLaunchKernelGGL(k_rerank, dim3(nq), dim3(256), lds, s, q, dim, ip, rows, cand, cand_count, kc, k, labels,
-                       out_ids, out_dists);
+                       out_ids, out_dists, gkeys, static_cast<int>(np));
+}
+
+size_t block_select_bytes(int k, bool by_label) {
+    const int threads = 256;
+    size_t cap = 1;
+    while (cap < static_cast<size_t>(k) + threads) cap <<= 1;
+    size_t kp2 = 1;
+    while (kp2 < static_cast<size_t>(k)) kp2 <<= 1;
+    size_t b = cap * 8;
+    if (by_label) b += ((kp2 + 1) & ~size_t{1}) * 4 + kp2 * 12;
+    return (b + 255) & ~size_t{255};
 }
 
 void launch_block_select(const float* q, int nq, int dim, int ip, const float* rows, const uint32_t* slot_list,
                          uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
-                         const int64_t* labels, int64_t* out_ids, float* out_dists, hipStream_t s) {
+                         const int64_t* labels, int64_t* out_ids, float* out_dists, unsigned char* gbuf,
+                         hipStream_t s) {
     const int threads = 256;
     int cap = 1;
     while (cap < k + threads) cap <<= 1;
-    int kp2 = 1;
-    while (kp2 < k) kp2 <<= 1;
-    size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + static_cast<size_t>(cap) * 8;
-    if (final_by_label && labels) lds += static_cast<size_t>((kp2 + 1) & ~1) * 4 + static_cast<size_t>(kp2) * 12;
+    const size_t buf = block_select_bytes(k, final_by_label && labels);
+    const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + (gbuf ? 0 : buf);
     hipLaunchKernelGGL(k_block_select, dim3(nq), dim3(threads), lds, s, q, dim, ip, rows, slot_list, n_slots, deleted,
-                       use_fkey, final_by_label, k, cap, labels, out_ids, out_dists);
+                       use_fkey, final_by_label, k, cap, labels, out_ids, out_dists, gbuf, buf);
 }
 
 } // namespace qd

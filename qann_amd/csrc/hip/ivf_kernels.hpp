@@ -49,19 +49,22 @@ struct ScanArgs {
     int64_t* out_label;        // nq * kp
     int* out_count;            // nq
     int* status;               // nq: 1 when the query needed more probes than M
+    float* g_dist;             // nq * 2kp TopK buffers in HBM when they do not fit LDS, else null
+    int64_t* g_label;
 };
 
+// Largest LDS a kernel here asks for (the default dynamic limit).
+constexpr size_t kLdsMax = 64 * 1024;
+
 // LDS bytes k_scan_fastscan needs.
-inline size_t scan_lds_bytes(int kp, int m, int dim) {
-    size_t b = static_cast<size_t>(2 * kp) * (8 + 4); // TopK buffer (labels, distances)
+inline size_t scan_lds_bytes(int kp, int m, int dim, bool topk_in_lds = true) {
+    size_t b = topk_in_lds ? static_cast<size_t>(2 * kp) * (8 + 4) : 0; // TopK buffer (labels, distances)
     b += static_cast<size_t>(m) * 4 + static_cast<size_t>(dim) * 4;
     b += static_cast<size_t>(kChunkBlocks) * 32 * 2;
     b += static_cast<size_t>(m) * 16;
     int np = 1;
     while (np < kp) np <<= 1;
-    const size_t sort_need = static_cast<size_t>(np) * 12; // take_sorted pads to a power of two
-    const size_t have = static_cast<size_t>(2 * kp) * 12;
-    if (sort_need > have) b += sort_need - have;
+    (void)np; // take_sorted pads to a power of two below 2k
     return (b + 15) & ~size_t{15};
 }
 
@@ -78,11 +81,16 @@ void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const ui
 void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
                          float* offset, hipStream_t s);
 void launch_scan_fastscan(const ScanArgs& a, int nq, hipStream_t s);
+// Keys of one query's sort: in LDS, or gkeys (nq * rerank_keys(kc)) in HBM when too many.
+size_t rerank_keys(int kc);
 void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
                    const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
-                   hipStream_t s);
+                   uint64_t* gkeys, hipStream_t s);
+// Bytes of one query's selection buffers; in HBM (gbuf, nq of them) when above kLdsMax.
+size_t block_select_bytes(int k, bool by_label);
 void launch_block_select(const float* q, int nq, int dim, int ip, const float* rows, const uint32_t* slot_list,
                          uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
-                         const int64_t* labels, int64_t* out_ids, float* out_dists, hipStream_t s);
+                         const int64_t* labels, int64_t* out_ids, float* out_dists, unsigned char* gbuf,
+                         hipStream_t s);
 
 } // namespace qd

@@ -192,6 +192,19 @@ PYBIND11_MODULE(_qann, m) {
                  return out;
              },
              py::arg("ids"))
+        .def("search_device",
+             [](DenseIndex& self, uintptr_t q, uint64_t nq, uint32_t k, uintptr_t ids, uintptr_t dists,
+                uintptr_t stream) {
+                 read_locked(self, [&] {
+                     self.search_device(reinterpret_cast<const float*>(q), nq, k, reinterpret_cast<int64_t*>(ids),
+                                        reinterpret_cast<float*>(dists), reinterpret_cast<void*>(stream));
+                     return 0;
+                 });
+             },
+             py::arg("queries_ptr"), py::arg("nq"), py::arg("k"), py::arg("ids_ptr"), py::arg("dists_ptr"),
+             py::arg("stream") = 0,
+             "batch_query on prepared queries already in device memory (pointers as ints); results stay on the "
+             "device, enqueued on the given HIP stream. Distances are the internal form (not halved for Cosine)")
         .def_property_readonly("metric", &DenseIndex::metric)
         .def_property_readonly("stores_vectors", &DenseIndex::stores_vectors);
 
@@ -234,6 +247,22 @@ PYBIND11_MODULE(_qann, m) {
              py::arg("data"), py::arg("max_iters") = 10, py::arg("seed") = py::none())
         .def_property("nprobe", [](const IVFIndex& self) { return self.nprobe(); },
                       [](IVFIndex& self, uint64_t v) { write_locked(self, [&] { self.set_nprobe(v); return 0; }); })
+        .def("list_sizes",
+             [](const IVFIndex& self) {
+                 const auto& lists = self.data().lists;
+                 py::array_t<int64_t> out(static_cast<py::ssize_t>(lists.size()));
+                 for (size_t l = 0; l < lists.size(); ++l) out.mutable_data()[l] = static_cast<int64_t>(lists[l].ids.size());
+                 return out;
+             },
+             "Entries stored in each inverted list")
+        .def("centroids",
+             [](const IVFIndex& self) {
+                 const auto& c = self.data().centroids;
+                 py::array_t<float> out({static_cast<py::ssize_t>(self.nlist()), static_cast<py::ssize_t>(self.dim())});
+                 std::copy(c.begin(), c.end(), out.mutable_data());
+                 return out;
+             },
+             "The coarse centroids, (nlist, dim)")
         .def_property_readonly("nlist", &IVFIndex::nlist)
         .def_property_readonly("fast_scan", &IVFIndex::fast_scan)
         .def_property_readonly("pq_rotate", &IVFIndex::pq_rotate)

@@ -254,6 +254,14 @@ void DenseIndex::batch_query(const float* queries, uint64_t nq, uint32_t k, cons
         for (size_t i = 0; i < total; ++i) out_dists[i] *= 0.5f;
 }
 
+void DenseIndex::search_device(const float* d_queries, uint64_t nq, uint32_t k, int64_t* d_ids, float* d_dists,
+                               void* stream) {
+    qann_hip_search_params p{};
+    fill_params(&p);
+    p.k = k;
+    check_status(qann_hip_search_async(device(), d_queries, nq, &p, d_ids, d_dists, stream));
+}
+
 std::vector<int64_t> DenseIndex::add(const float* vecs, uint64_t n, const int64_t* idv) {
     const uint64_t first = slots();
     const std::vector<float> prepared = prepare(vecs, n);

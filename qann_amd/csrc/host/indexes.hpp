@@ -37,6 +37,10 @@ public:
     void batch_query(const float* queries, uint64_t nq, uint32_t k, const int64_t* allowed, size_t n_allowed,
                      bool filtered, int64_t* out_ids, float* out_dists);
 
+    // batch_query on prepared queries already in HBM, results left in HBM,
+    // enqueued on stream (qann_hip_search_async). No filter.
+    void search_device(const float* d_queries, uint64_t nq, uint32_t k, int64_t* d_ids, float* d_dists, void* stream);
+
     // add_batch (src/index.cpp:33-49): returns the added ids.
     std::vector<int64_t> add(const float* vecs, uint64_t n, const int64_t* ids);
     virtual size_t remove(const int64_t* ids, size_t n) = 0;
