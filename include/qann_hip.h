@@ -152,7 +152,9 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
 
 /*
  * Per-stage kernel timing, the device counterpart of the reference's
- * compile-time section timers (src/profile.hpp: coarse, table, scan, rerank).
+ * compile-time section timers (src/profile.hpp: coarse, table, scan, drain,
+ * rerank). QANN_STAGE_SCAN is the fast-scan sums kernel, QANN_STAGE_REPLAY the
+ * kernel replaying the TopK pushes over them.
  * While enabled, every search synchronizes its stream after each stage and
  * adds the stage's HIP-event time to a process-wide total; read returns the
  * totals in milliseconds and launch counts for QANN_STAGE_COUNT stages, in the
@@ -167,7 +169,8 @@ enum {
     QANN_STAGE_SCAN = 4,
     QANN_STAGE_RERANK = 5,
     QANN_STAGE_FLAT = 6,
-    QANN_STAGE_COUNT = 7,
+    QANN_STAGE_REPLAY = 7,
+    QANN_STAGE_COUNT = 8,
 };
 int qann_hip_profile_enable(int on);
 int qann_hip_profile_read(double* ms, uint64_t* launches, int reset);

@@ -99,6 +99,8 @@ size_t pad256(size_t b) { return (b + 255) & ~size_t{255}; }
 std::atomic<int> g_prof_on{0};
 std::mutex g_prof_mu;
 double g_prof_ms[QANN_STAGE_COUNT] = {};
+// bytes of u16 sums one search keeps at once (queries are scanned in chunks within it)
+uint64_t g_sums_budget = uint64_t{2} << 30;
 uint64_t g_prof_n[QANN_STAGE_COUNT] = {};
 
 // Times one stage when profiling is on: events around the launches, then a sync.
@@ -141,6 +143,8 @@ struct qann_hip_index {
     qd::DevIvf ivf{};
     bool use_pre = false;
     std::vector<uint32_t> host_list_sizes;
+    // blocks_desc_prefix[j]: blocks of the j largest lists together
+    std::vector<uint64_t> blocks_desc_prefix{0};
     // ids
     uint64_t slots = 0, live = 0;
     int64_t* labels = nullptr;
@@ -345,8 +349,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const bool use_pre = !ip_pq && ix.B != nullptr;
     const uint32_t m = static_cast<uint32_t>(ix.m);
     // the TopK buffers go to HBM when they do not fit LDS
-    const bool gtopk = qd::scan_lds_bytes(static_cast<int>(kp), ix.m, ix.dim, true) > qd::kLdsMax;
-    if (qd::scan_lds_bytes(static_cast<int>(kp), ix.m, ix.dim, false) > qd::kLdsMax)
+    const bool gtopk = qd::replay_lds_bytes(static_cast<int>(kp)) > qd::kLdsMax;
+    if ((static_cast<size_t>(m) * 16 + 15) / 16 * 16 + (static_cast<size_t>(m) + ix.dim) * 4 > qd::kLdsMax)
         invalid("num_subspaces and dim too large for the on-chip tables");
     const size_t rkeys = qd::rerank_keys(static_cast<int>(kp));
     const bool grerank = refine && qlds + rkeys * 8 > qd::kLdsMax;
@@ -361,16 +365,25 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             min_probes = static_cast<int>(std::clamp<size_t>(static_cast<size_t>(std::min(want, double(nlist))), nprobe, nlist));
         }
     }
+    // sums of one query take at most the blocks of its M largest probed lists;
+    // queries go through the two scan phases in chunks that bound that buffer
+    const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];
+    const uint64_t stride = std::max<uint64_t>(wblocks, 1) * qd::kFsBlock;
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * 2)));
 
     // scratch plan
     const size_t d = ix.dim;
     size_t need = 0;
     need += ix.rotation ? pad256(nq * d * 4) : 0;
-    need += pad256(nq * nlist * 4);       // coarse distances
+    need += pad256(nq * nlist * 4); // coarse distances
     need += pad256(nq * M * 2) + 3 * pad256(nq * M * 4);
     need += use_pre ? pad256(nq * m * 16 * 4) : 0;
     need += ip_pq ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
-    need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + 2 * pad256(nq * 4);
+    need += filtered ? pad256(nlist * 4) : 0;
+    need += 2 * pad256(nq * 4) + pad256(nq * M * 4);    // plan
+    need += 3 * pad256(nq * M * 4);                     // per-probe scale, offset, min
+    need += pad256(chunk * stride * 2);                 // sums
+    need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + pad256(nq * 4);
     need += gtopk ? pad256(nq * 2 * kp * 4) + pad256(nq * 2 * kp * 8) : 0;
     need += grerank ? pad256(nq * rkeys * 8) : 0;
     sc.used = 0;
@@ -398,7 +411,26 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     float* pbase = sc.take<float>(nq * M);
     qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
     check_launch("k_probe_base");
+    uint32_t* list_allowed = nullptr;
+    if (filtered) {
+        list_allowed = sc.take<uint32_t>(nlist);
+        qd::launch_list_allowed(ix, d_filter, list_allowed, st);
+        check_launch("k_list_allowed");
+    }
+    int* nprobe_q = sc.take<int>(nq);
+    int* status = sc.take<int>(nq);
+    uint32_t* item_blk = sc.take<uint32_t>(nq * M);
+    qd::launch_probe_plan(ix, static_cast<int>(nq), static_cast<int>(M), static_cast<int>(nprobe), min_probes, wanted,
+                          pidx, list_allowed, nprobe_q, item_blk, status, st);
+    check_launch("k_probe_plan");
     t_sel.done();
+    if (filtered && M < nlist) {
+        std::vector<int> hs(nq);
+        check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int v : hs)
+            if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
+    }
     float* C = nullptr;
     uint8_t* lut = nullptr;
     float *lscale = nullptr, *loff = nullptr;
@@ -415,44 +447,65 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         check_launch("k_query_tables");
     }
     t_tab.done();
-    qd::ScanArgs a{};
-    a.ix = ix;
-    if (!use_pre) a.ix.B = nullptr;
-    a.kp = static_cast<int>(kp);
-    a.nprobe = static_cast<int>(nprobe);
-    a.M = static_cast<int>(M);
-    a.min_probes = min_probes;
-    a.wanted = wanted;
-    a.block_filter = block_filter;
-    a.q = qr;
-    a.probe_idx = pidx;
-    a.probe_A = pA;
-    a.probe_base = pbase;
-    a.C = C;
-    a.ip_lut = lut;
-    a.ip_scale = lscale;
-    a.ip_offset = loff;
-    a.filter = filtered ? d_filter : nullptr;
+    float* item_scale = sc.take<float>(nq * M);
+    float* item_offset = sc.take<float>(nq * M);
+    uint32_t* item_min = sc.take<uint32_t>(nq * M);
+    uint16_t* sums = sc.take<uint16_t>(chunk * stride);
     float* cd = sc.take<float>(nq * kp);
     int64_t* cl = sc.take<int64_t>(nq * kp);
     int* ccount = sc.take<int>(nq);
-    int* status = sc.take<int>(nq);
-    a.out_dist = cd;
-    a.out_label = cl;
-    a.out_count = ccount;
-    a.status = status;
-    a.g_dist = gtopk ? sc.take<float>(nq * 2 * kp) : nullptr;
-    a.g_label = gtopk ? sc.take<int64_t>(nq * 2 * kp) : nullptr;
-    StageTimer t_scan(QANN_STAGE_SCAN, st);
-    qd::launch_scan_fastscan(a, static_cast<int>(nq), st);
-    check_launch("k_scan_fastscan");
-    t_scan.done();
-    if (filtered && M < nlist) {
-        std::vector<int> hs(nq);
-        check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        for (int v : hs)
-            if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
+    float* gd = gtopk ? sc.take<float>(nq * 2 * kp) : nullptr;
+    int64_t* gl = gtopk ? sc.take<int64_t>(nq * 2 * kp) : nullptr;
+    qd::DevIvf ixs = ix;
+    if (!use_pre) ixs.B = nullptr;
+    const size_t bb = static_cast<size_t>(m) * 16;
+    for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+        const uint64_t nc = std::min<uint64_t>(chunk, nq - q0);
+        qd::SumsArgs a{};
+        a.ix = ixs;
+        a.M = static_cast<int>(M);
+        a.q = qr + q0 * d;
+        a.probe_idx = pidx + q0 * M;
+        a.probe_A = pA + q0 * M;
+        a.probe_base = pbase + q0 * M;
+        a.nprobe_q = nprobe_q + q0;
+        a.item_blk = item_blk + q0 * M;
+        a.C = C ? C + q0 * bb : nullptr;
+        a.ip_lut = lut ? lut + q0 * bb : nullptr;
+        a.ip_scale = lscale ? lscale + q0 : nullptr;
+        a.ip_offset = loff ? loff + q0 : nullptr;
+        a.sums = sums;
+        a.stride_slots = stride;
+        a.item_scale = item_scale + q0 * M;
+        a.item_offset = item_offset + q0 * M;
+        a.item_min = item_min + q0 * M;
+        StageTimer t_scan(QANN_STAGE_SCAN, st);
+        qd::launch_fs_sums(a, static_cast<int>(nc), st);
+        check_launch("k_fs_sums");
+        t_scan.done();
+        qd::ReplayArgs r{};
+        r.ix = ixs;
+        r.kp = static_cast<int>(kp);
+        r.M = static_cast<int>(M);
+        r.block_filter = block_filter;
+        r.probe_idx = a.probe_idx;
+        r.nprobe_q = a.nprobe_q;
+        r.item_blk = a.item_blk;
+        r.sums = sums;
+        r.stride_slots = stride;
+        r.item_scale = a.item_scale;
+        r.item_offset = a.item_offset;
+        r.item_min = a.item_min;
+        r.filter = filtered ? d_filter : nullptr;
+        r.out_dist = cd + q0 * kp;
+        r.out_label = cl + q0 * kp;
+        r.out_count = ccount + q0;
+        r.g_dist = gd ? gd + q0 * 2 * kp : nullptr;
+        r.g_label = gl ? gl + q0 * 2 * kp : nullptr;
+        StageTimer t_rep(QANN_STAGE_REPLAY, st);
+        qd::launch_fs_replay(r, static_cast<int>(nc), st);
+        check_launch("k_fs_replay");
+        t_rep.done();
     }
     if (refine) {
         uint64_t* gkeys = grerank ? sc.take<uint64_t>(nq * rkeys) : nullptr;
@@ -588,6 +641,11 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
         ix.list_ids = total ? h->upload(slots, total) : h->alloc<uint32_t>(1);
         ix.codes = h->upload(packed.data(), packed.size());
         h->host_list_sizes.assign(sizes, sizes + nl);
+        std::vector<uint64_t> nb(nl);
+        for (size_t l = 0; l < nl; ++l) nb[l] = (sizes[l] + 31) / 32;
+        std::sort(nb.begin(), nb.end(), std::greater<uint64_t>());
+        h->blocks_desc_prefix.assign(nl + 1, 0);
+        for (size_t l = 0; l < nl; ++l) h->blocks_desc_prefix[l + 1] = h->blocks_desc_prefix[l] + nb[l];
     });
 }
 
@@ -773,7 +831,8 @@ int qann_hip_profile_read(double* ms, uint64_t* launches, int reset) {
 }
 
 const char* qann_hip_stage_name(int stage) {
-    static const char* names[QANN_STAGE_COUNT] = {"rotate", "coarse", "select", "tables", "scan", "rerank", "flat"};
+    static const char* names[QANN_STAGE_COUNT] = {"rotate", "coarse", "select", "tables", "scan", "rerank", "flat",
+                                                  "replay"};
     return stage >= 0 && stage < QANN_STAGE_COUNT ? names[stage] : "";
 }
 

@@ -7,9 +7,12 @@
 //   k_probe_base      per-probe A (residual centres) and the IP/cosine base term
 //   k_query_tables    C = -2<q_s, y_sj> (PQCodebook::compute_inner_table, src/pq.cpp:186-204),
 //                     or the inner-product LUT quantized once per query (scan, :448-456)
-//   k_scan_fastscan   one wave per query: per probe the u8 LUT in LDS (build_quantize_lut,
-//                     src/fast_scan.cpp:243-254), the 4-bit block sums (fast_scan_block_mask,
-//                     :256-262), and the reference's TopK replayed push by push
+//   k_probe_plan      probes each query scans (filters: min_probes / wanted) and sum offsets
+//   k_fs_sums         one wave per (query, probe): the u8 LUT in LDS (build_quantize_lut,
+//                     src/fast_scan.cpp:243-254) and the u16 sums of every 4-bit code of
+//                     the list (fast_scan_block, :256-262) with v_perm lookups
+//   k_fs_replay       one wave per query: the reference's scan replayed push by push over
+//                     those sums -- limit16, survivors in bit order, TopK trims
 //                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543)
 //   k_rerank          exact FP32 re-rank of the candidates (RefineIndex::rerank, src/refine_index.cpp:112-140)
 //   k_block_select    exact top-k over a slot set (FlatIndex::query_impl, RefineIndex::direct_scan)
@@ -285,163 +288,258 @@ __device__ void wave_trim(WaveTopK& tk) {
     tk.thr = from_fkey(t);
 }
 
-__global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
+// The table of one (query, probed list), rounded to bytes in LDS: the
+// inner-product table of the query, or the list's L2 table from B + C + A
+// (build_quantize_lut, src/fast_scan.cpp:243-254) or from the residual
+// (compute_adc_table + quantize_lut, src/pq.cpp:165-184). Returns scale and
+// offset as quantize_lut sets them, plus the list's base for inner product.
+__device__ void wave_build_lut(const DevIvf& ix, const float* q, int qi, int list, float A, float base,
+                               const float* Cq, const uint8_t* ip_lut, const float* ip_scale, const float* ip_offset,
+                               uint8_t* lut, float* row_min, float* resid, float& scale, float& offset) {
+    const int m = ix.m, bb = m * 16, lane = lane_id();
+    if (ip_lut) {
+        const uint8_t* src = ip_lut + static_cast<size_t>(qi) * bb;
+        for (int e = 4 * lane; e < bb; e += 4 * kWave) *reinterpret_cast<uint32_t*>(lut + e) = *reinterpret_cast<const uint32_t*>(src + e);
+        scale = ip_scale[qi];
+        offset = ip_offset[qi] + base; // scan_fast(list, ip_scale, ip_offset + base) (ivf_index.cpp:571-572)
+        wave_sync();
+        return;
+    }
+    const float* B = ix.B ? ix.B + static_cast<size_t>(list) * bb : nullptr;
+    if (!B) {
+        const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * ix.dim;
+        for (int e = lane; e < ix.dim; e += kWave) resid[e] = q[e] - c[e];
+        wave_sync();
+    }
+    auto entry = [&](int r, int j) -> float {
+        if (B) {
+            float t = B[r * 16 + j] + Cq[r * 16 + j];
+            if (r == 0) t = t + A;
+            return t;
+        }
+        float acc = 0.0f;
+        for (int s = 0; s < ix.sub; ++s) {
+            const float d = resid[r * ix.sub + s] - ix.pq_cent_t[(static_cast<size_t>(r) * ix.sub + s) * 16 + j];
+            acc = fmaf(d, d, acc);
+        }
+        return acc;
+    };
+    float wide = 0.0f;
+    for (int r = lane; r < m; r += kWave) {
+        float lo = entry(r, 0), hi = lo;
+        for (int j = 1; j < 16; ++j) {
+            const float t = entry(r, j);
+            lo = fminf(lo, t);
+            hi = fmaxf(hi, t);
+        }
+        row_min[r] = lo;
+        wide = fmaxf(wide, hi - lo);
+    }
+    const float widest = wave_max(wide);
+    wave_sync();
+    float part = 0.0f;
+    if (lane < 4)
+        for (int r = lane; r < m; r += 4) part += row_min[r];
+    const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 1, kWave);
+    const float p2 = __shfl(part, 2, kWave), p3 = __shfl(part, 3, kWave);
+    offset = (p0 + p1) + (p2 + p3);
+    scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
+    for (int r = lane; r < m; r += kWave) {
+        const float mn = row_min[r];
+        for (int j = 0; j < 16; ++j) lut[r * 16 + j] = static_cast<uint8_t>(__builtin_rintf((entry(r, j) - mn) * scale));
+    }
+    wave_sync();
+}
+
+// Four LUT lookups at once: byte i of the result is row[idx byte i] for a
+// 16-byte LUT row held in four registers (idx bytes in 0..15). v_perm_b32
+// picks bytes from an 8-byte pair, so the low and high halves of the row are
+// looked up separately and merged on bit 3 of each index.
+__device__ __forceinline__ uint32_t lut16x4(uint32_t idx, uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3) {
+    const uint32_t sel = idx & 0x07070707u;
+    const uint32_t lo = __builtin_amdgcn_perm(r1, r0, sel);
+    const uint32_t hi = __builtin_amdgcn_perm(r3, r2, sel);
+    const uint32_t m = ((idx >> 3) & 0x01010101u) * 0xFFu;
+    return (hi & m) | (lo & ~m);
+}
+
+// Phase 1: one wave per (query, probe). The LUT of the pair, then the u16 sums
+// of every vector of the list (fast_scan_block, src/fast_scan.cpp:256-262):
+// lane (block, column g) takes byte column 4g..4g+3 of each 16-byte subspace
+// row, i.e. vectors 4g..4g+3 (low nibbles) and 16+4g..16+4g+3 (high nibbles),
+// adds four subspaces' bytes in bytes (at most 4 * 63) and widens to u16.
+__global__ void __launch_bounds__(64) k_fs_sums(const SumsArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevIvf& ix = a.ix;
-    const int m = ix.m;
-    const int bb = m * 16;
+    const int m = ix.m, bb = m * 16;
+    uint8_t* lut = lds;                                                    // m * 16
+    float* row_min = reinterpret_cast<float*>(lds + ((bb + 15) & ~15));   // m
+    float* resid = row_min + m;                                            // dim
+    const int item = blockIdx.x;
+    const int qi = item / a.M, p = item % a.M;
+    if (p >= a.nprobe_q[qi]) return;
+    const int list = a.probe_idx[item];
+    const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+    if (n == 0) return;
+    const int lane = lane_id();
+    const float* q = a.q + static_cast<size_t>(qi) * ix.dim;
+    float scale, offset;
+    wave_build_lut(ix, q, qi, list, a.probe_A[item], a.probe_base[item], a.C ? a.C + static_cast<size_t>(qi) * bb : nullptr,
+                   a.ip_lut, a.ip_scale, a.ip_offset, lut, row_min, resid, scale, offset);
+    if (lane == 0) {
+        a.item_scale[item] = scale;
+        a.item_offset[item] = offset;
+    }
+    const uint8_t* codes = ix.codes + ix.code_off[list];
+    uint16_t* out = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(a.item_blk[item]) * kFsBlock;
+    const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
+    const int g = lane & 3;
+    uint32_t local_min = 0xFFFFFFFFu;
+    for (uint32_t cb = 0; cb < nb; cb += kChunkBlocks) {
+        const uint32_t blk = cb + (lane >> 2);
+        if (blk < nb) {
+            const uint8_t* bp = codes + static_cast<size_t>(blk) * bb + 4 * g;
+            uint32_t lo_a = 0, lo_b = 0, hi_a = 0, hi_b = 0; // u16 pairs: vectors (4g, 4g+2), (4g+1, 4g+3), +16
+            int s = 0;
+            while (s < m) {
+                uint32_t lo8 = 0, hi8 = 0;
+                const int s_end = min(m, s + 4);
+                for (; s < s_end; ++s) {
+                    const uint4 row = *reinterpret_cast<const uint4*>(lut + s * 16);
+                    const uint32_t w = *reinterpret_cast<const uint32_t*>(bp + s * 16);
+                    lo8 += lut16x4(w & 0x0F0F0F0Fu, row.x, row.y, row.z, row.w);
+                    hi8 += lut16x4((w >> 4) & 0x0F0F0F0Fu, row.x, row.y, row.z, row.w);
+                }
+                lo_a += lo8 & 0x00FF00FFu;
+                lo_b += (lo8 >> 8) & 0x00FF00FFu;
+                hi_a += hi8 & 0x00FF00FFu;
+                hi_b += (hi8 >> 8) & 0x00FF00FFu;
+            }
+            // vectors 4g+0..3 are lo_a.lo, lo_b.lo, lo_a.hi, lo_b.hi
+            const uint32_t l01 = (lo_a & 0xFFFFu) | (lo_b << 16), l23 = (lo_a >> 16) | (lo_b & 0xFFFF0000u);
+            const uint32_t h01 = (hi_a & 0xFFFFu) | (hi_b << 16), h23 = (hi_a >> 16) | (hi_b & 0xFFFF0000u);
+            uint16_t* o = out + static_cast<size_t>(blk) * kFsBlock;
+            *reinterpret_cast<uint2*>(o + 4 * g) = make_uint2(l01, l23);
+            *reinterpret_cast<uint2*>(o + 16 + 4 * g) = make_uint2(h01, h23);
+            const uint32_t base_v = blk * kFsBlock;
+            const uint32_t vals[8] = {l01 & 0xFFFFu, l01 >> 16, l23 & 0xFFFFu, l23 >> 16,
+                                      h01 & 0xFFFFu, h01 >> 16, h23 & 0xFFFFu, h23 >> 16};
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t v = base_v + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
+                if (v < n) local_min = min(local_min, vals[i]);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) local_min = min(local_min, static_cast<uint32_t>(__shfl_xor(static_cast<int>(local_min), off, kWave)));
+    if (lane == 0) a.item_min[item] = local_min;
+}
+
+// Probes each query scans and where each probe's sums start: nprobe lists
+// unfiltered; with a filter, at least min_probes and then until the lists
+// scanned hold `wanted` allowed vectors (scan, ivf_index.cpp:510-521: before
+// the TopK first trims, every allowed vector is pushed, so top.size() >=
+// wanted exactly when that many allowed vectors were scanned).
+__global__ void k_probe_plan(const DevIvf ix, int nq, int M, int nprobe, int min_probes, int wanted,
+                             const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
+                             int* status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint16_t* pl = probe_idx + static_cast<size_t>(i) * M;
+    uint64_t seen = 0;
+    uint32_t blk = 0;
+    int p = 0, st = 0;
+    for (;; ++p) {
+        if (p >= nprobe && p >= min_probes && seen >= static_cast<uint64_t>(wanted)) break;
+        if (p >= M) {
+            if (p < ix.nlist) st = 1; // needs probes beyond those selected
+            break;
+        }
+        const int list = pl[p];
+        const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+        item_blk[static_cast<size_t>(i) * M + p] = blk;
+        blk += (n + kFsBlock - 1) / kFsBlock;
+        if (list_allowed) seen += list_allowed[list];
+    }
+    nprobe_q[i] = p;
+    status[i] = st;
+}
+
+// Allowed entries of each list under a filter.
+__global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t* out) {
+    const int l = blockIdx.x;
+    const uint32_t b = ix.list_off[l], e = ix.list_off[l + 1];
+    uint32_t c = 0;
+    for (uint32_t j = b + threadIdx.x; j < e; j += blockDim.x) c += bit_test(filter, ix.list_ids[j]) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += static_cast<uint32_t>(__shfl_xor(static_cast<int>(c), off, kWave));
+    __shared__ uint32_t part[16];
+    if (lane_id() == 0) part[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < static_cast<int>(blockDim.x) / kWave; ++w) t += part[w];
+        out[l] = t;
+    }
+}
+
+// Phase 2: one wave per query replays the reference's scan over the sums in
+// probe order -- limit16 from the threshold at each block (scan_fast,
+// ivf_index.cpp:482-485), survivors in bit order through push (:539-543) into
+// the TopK (src/topk.hpp), trims when its buffer fills. A list whose smallest
+// sum cannot pass the current limit is skipped whole (the limit only falls),
+// and so is a pair of blocks with no sum below it.
+__global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const DevIvf& ix = a.ix;
     const int kp = a.kp;
     const int qi = blockIdx.x;
-    // LDS layout (scan_lds_bytes computes the same); the TopK buffer may live in HBM
     const bool tk_lds = a.g_dist == nullptr;
     int64_t* tk_label = tk_lds ? reinterpret_cast<int64_t*>(lds) : a.g_label + static_cast<size_t>(qi) * 2 * kp;
     float* tk_dist = tk_lds ? reinterpret_cast<float*>(tk_label + 2 * kp) : a.g_dist + static_cast<size_t>(qi) * 2 * kp;
-    float* row_min = tk_lds ? tk_dist + 2 * kp : reinterpret_cast<float*>(lds); // m
-    float* resid = row_min + m;                                          // dim
-    uint16_t* sums = reinterpret_cast<uint16_t*>(resid + ix.dim);        // kChunk * 32
-    uint8_t* lut = reinterpret_cast<uint8_t*>(sums + kChunkBlocks * kFsBlock); // m * 16
-
     const int lane = lane_id();
-    const float* q = a.q + static_cast<size_t>(qi) * ix.dim;
-    const bool ip_mode = a.ip_lut != nullptr;
     const bool filtered = a.filter != nullptr;
-
     WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY};
-
-    float ip_scale = 1.0f, ip_offset = 0.0f;
-    if (ip_mode) {
-        const uint8_t* src = a.ip_lut + static_cast<size_t>(qi) * bb;
-        for (int e = lane; e < bb; e += kWave) lut[e] = src[e];
-        ip_scale = a.ip_scale[qi];
-        ip_offset = a.ip_offset[qi];
-    }
-    int status = 0;
-    const uint16_t* plist = a.probe_idx + static_cast<size_t>(qi) * a.M;
-    for (int p = 0;; ++p) {
-        if (p >= a.nprobe) {
-            if (p >= a.min_probes && min(tk.size, kp) >= a.wanted) break;
-        }
-        if (p >= a.M) {
-            if (p < ix.nlist) status = 1; // more probes needed than were selected
-            break;
-        }
-        const int list = plist[p];
+    const int P = a.nprobe_q[qi];
+    for (int p = 0; p < P; ++p) {
+        const size_t item = static_cast<size_t>(qi) * a.M + p;
+        const int list = a.probe_idx[item];
         const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
         if (n == 0) continue;
-        const float A = a.probe_A[static_cast<size_t>(qi) * a.M + p];
-        float scale, offset;
-        if (ip_mode) {
-            scale = ip_scale;
-            offset = ip_offset + a.probe_base[static_cast<size_t>(qi) * a.M + p];
-        } else {
-            // this list's table, rounded to bytes (build_quantize_lut / compute_adc_table + quantize_lut)
-            const float* B = ix.B ? ix.B + static_cast<size_t>(list) * bb : nullptr;
-            const float* C = a.C ? a.C + static_cast<size_t>(qi) * bb : nullptr;
-            if (!B) {
-                const float* c = ix.residual_centers ? ix.residual_centers : ix.centroids;
-                c += static_cast<size_t>(list) * ix.dim;
-                for (int e = lane; e < ix.dim; e += kWave) resid[e] = q[e] - c[e];
-                wave_sync();
-            }
-            auto entry = [&](int r, int j) -> float {
-                if (B) {
-                    float t = B[r * 16 + j] + C[r * 16 + j];
-                    if (r == 0) t = t + A;
-                    return t;
-                }
-                float acc = 0.0f;
-                for (int s = 0; s < ix.sub; ++s) {
-                    const float d = resid[r * ix.sub + s] - ix.pq_cent_t[(static_cast<size_t>(r) * ix.sub + s) * 16 + j];
-                    acc = fmaf(d, d, acc);
-                }
-                return acc;
-            };
-            float wide = 0.0f;
-            for (int r = lane; r < m; r += kWave) {
-                float lo = entry(r, 0), hi = lo;
-                for (int j = 1; j < 16; ++j) {
-                    const float t = entry(r, j);
-                    lo = fminf(lo, t);
-                    hi = fmaxf(hi, t);
-                }
-                row_min[r] = lo;
-                wide = fmaxf(wide, hi - lo);
-            }
-            const float widest = wave_max(wide);
-            wave_sync();
-            float part = 0.0f;
-            if (lane < 4)
-                for (int r = lane; r < m; r += 4) part += row_min[r];
-            const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 1, kWave);
-            const float p2 = __shfl(part, 2, kWave), p3 = __shfl(part, 3, kWave);
-            offset = (p0 + p1) + (p2 + p3);
-            scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
-            for (int r = lane; r < m; r += kWave) {
-                const float mn = row_min[r];
-                for (int j = 0; j < 16; ++j) lut[r * 16 + j] = static_cast<uint8_t>(__builtin_rintf((entry(r, j) - mn) * scale));
-            }
-            wave_sync();
-        }
-
-        const uint8_t* codes = ix.codes + ix.code_off[list];
+        const float scale = a.item_scale[item], offset = a.item_offset[item];
+        auto limit_now = [&]() -> uint32_t {
+            const float limit = (tk.thr - offset) * scale;
+            return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
+        };
+        if (a.item_min[item] >= limit_now()) continue;
         const uint32_t* ids = ix.list_ids + ix.list_off[list];
+        const uint16_t* sums = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(a.item_blk[item]) * kFsBlock;
         const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
-        for (uint32_t cb = 0; cb < nb; cb += kChunkBlocks) {
-            // block sums of up to kChunkBlocks blocks: lane -> (block, 4-byte column)
-            {
-                const uint32_t blk = cb + (lane >> 2);
-                const int g = lane & 3;
-                if (blk < nb) {
-                    const uint8_t* bp = codes + static_cast<size_t>(blk) * bb + 4 * g;
-                    uint32_t lo0 = 0, lo1 = 0, lo2 = 0, lo3 = 0, hi0 = 0, hi1 = 0, hi2 = 0, hi3 = 0;
-                    for (int s = 0; s < m; ++s) {
-                        const uint32_t w = *reinterpret_cast<const uint32_t*>(bp + s * 16);
-                        const uint8_t* row = lut + s * 16;
-                        lo0 += row[w & 15];
-                        hi0 += row[(w >> 4) & 15];
-                        lo1 += row[(w >> 8) & 15];
-                        hi1 += row[(w >> 12) & 15];
-                        lo2 += row[(w >> 16) & 15];
-                        hi2 += row[(w >> 20) & 15];
-                        lo3 += row[(w >> 24) & 15];
-                        hi3 += row[(w >> 28) & 15];
-                    }
-                    uint16_t* o = sums + (lane >> 2) * kFsBlock;
-                    o[4 * g + 0] = static_cast<uint16_t>(lo0);
-                    o[4 * g + 1] = static_cast<uint16_t>(lo1);
-                    o[4 * g + 2] = static_cast<uint16_t>(lo2);
-                    o[4 * g + 3] = static_cast<uint16_t>(lo3);
-                    o[16 + 4 * g + 0] = static_cast<uint16_t>(hi0);
-                    o[16 + 4 * g + 1] = static_cast<uint16_t>(hi1);
-                    o[16 + 4 * g + 2] = static_cast<uint16_t>(hi2);
-                    o[16 + 4 * g + 3] = static_cast<uint16_t>(hi3);
-                }
-            }
-            wave_sync();
-            const uint32_t cend = min(nb, cb + static_cast<uint32_t>(kChunkBlocks));
-            for (uint32_t blk = cb; blk < cend; ++blk) {
-                const uint32_t start = blk * kFsBlock;
-                const uint32_t count = min(static_cast<uint32_t>(kFsBlock), n - start);
-                const int v = lane;
-                const bool valid = v < static_cast<int>(count);
-                const uint32_t slot = valid ? ids[start + v] : 0u;
+        for (uint32_t blk = 0; blk < nb; blk += 2) {
+            const uint32_t b = blk + (lane >> 5);
+            const int v = lane & 31;
+            const uint32_t e = b * kFsBlock + v;
+            const bool in = b < nb && e < n;
+            const uint32_t sum = in ? sums[e] : 0xFFFFu;
+            if (ballot(in && sum < limit_now()) == 0) continue;
+            for (int h = 0; h < 2 && blk + h < nb; ++h) {
+                const bool mine = (lane >> 5) == h;
+                const bool valid = mine && in;
+                uint32_t slot = 0;
                 bool allowed = valid;
                 if (a.block_filter) {
+                    slot = valid ? ids[e] : 0u;
                     allowed = valid && bit_test(a.filter, slot);
                     if (ballot(allowed) == 0) continue;
                 }
-                // limit16 from the threshold as it stands at the block (scan_fast, :482-485)
-                const float limit = (tk.thr - offset) * scale;
-                const uint32_t limit16 = limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
-                const uint32_t sum = valid ? sums[(blk - cb) * kFsBlock + v] : 0u;
-                const bool surv = allowed && sum < limit16;
+                const bool surv = allowed && sum < limit_now();
                 uint64_t rem = ballot(surv);
                 if (rem == 0) continue;
+                if (!a.block_filter && surv) slot = ids[e];
                 const float dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
                 const bool ok = surv && (filtered ? bit_test(a.filter, slot) : !bit_test(ix.deleted, slot));
-                const int64_t lab = ix.labels ? ix.labels[slot] : static_cast<int64_t>(slot);
+                const int64_t lab = ok ? (ix.labels ? ix.labels[slot] : static_cast<int64_t>(slot)) : 0;
                 while (rem) {
-                    const bool cond = ((rem >> v) & 1ull) && ok && dist < tk.thr;
+                    const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
                     const uint64_t cm = ballot(cond);
                     if (cm == 0) break;
                     const int space = 2 * kp - tk.size;
@@ -460,7 +558,7 @@ __global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
                     const uint32_t rank = lanes_below(cm);
                     const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
                     const int bpos = __builtin_ctzll(at);
-                    if (cond && v <= bpos) {
+                    if (cond && lane <= bpos) {
                         const int pos = tk.size + static_cast<int>(rank);
                         tk.dist[pos] = dist;
                         tk.label[pos] = lab;
@@ -471,7 +569,6 @@ __global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
                     rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
                 }
             }
-            wave_sync();
         }
     }
 
@@ -531,10 +628,7 @@ __global__ void __launch_bounds__(64) k_scan_fastscan(const ScanArgs a) {
         od[i] = i < n ? tk.dist[i] : INFINITY;
         ol[i] = i < n ? tk.label[i] : -1;
     }
-    if (lane == 0) {
-        a.out_count[qi] = n;
-        if (a.status) a.status[qi] = status;
-    }
+    if (lane == 0) a.out_count[qi] = n;
 }
 
 // ---------------------------------------------------------------------------
@@ -757,9 +851,25 @@ void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut,
     hipLaunchKernelGGL(k_query_tables, dim3(nq), dim3(256), lds, s, q, ix, want_lut, C, lut, scale, offset);
 }
 
-void launch_scan_fastscan(const ScanArgs& a, int nq, hipStream_t s) {
-    const size_t lds = scan_lds_bytes(a.kp, a.ix.m, a.ix.dim, a.g_dist == nullptr);
-    hipLaunchKernelGGL(k_scan_fastscan, dim3(nq), dim3(kWave), lds, s, a);
+void launch_fs_sums(const SumsArgs& a, int nq, hipStream_t s) {
+    const size_t lds = static_cast<size_t>((a.ix.m * 16 + 15) & ~15) + static_cast<size_t>(a.ix.m + a.ix.dim) * 4;
+    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(static_cast<size_t>(nq) * a.M)), dim3(kWave), lds, s, a);
+}
+
+void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
+                       const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
+                       int* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_plan, dim3(cdiv(nq, 128)), dim3(128), 0, s, ix, nq, M, nprobe, min_probes, wanted,
+                       probe_idx, list_allowed, nprobe_q, item_blk, status);
+}
+
+void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_list_allowed, dim3(ix.nlist), dim3(256), 0, s, ix, filter, out);
+}
+
+void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
+    const size_t lds = a.g_dist ? 0 : static_cast<size_t>(2 * a.kp) * 12;
+    hipLaunchKernelGGL(k_fs_replay, dim3(nq), dim3(kWave), lds, s, a);
 }
 
 size_t rerank_keys(int kc) {

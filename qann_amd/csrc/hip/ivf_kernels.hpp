@@ -31,24 +31,43 @@ struct DevIvf {
     const uint64_t* deleted;       // bitmap over slots
 };
 
-struct ScanArgs {
+// Phase 1 of the fast-scan: the LUT and the u16 sums of every (query, probe).
+struct SumsArgs {
     DevIvf ix;
-    int kp;          // results kept per query (the TopK's k)
-    int nprobe, M, min_probes, wanted;
-    int block_filter;
+    int M;                     // probe slots per query in the probe arrays
     const float* q;            // nq * dim, prepared (and rotated)
     const uint16_t* probe_idx; // nq * M, sorted nearest first
     const float* probe_A;      // nq * M
     const float* probe_base;   // nq * M
+    const int* nprobe_q;       // probes each query scans
+    const uint32_t* item_blk;  // nq * M: first block of each probe's sums in its query's region
     const float* C;            // nq * m * 16 or null
     const uint8_t* ip_lut;     // nq * m * 16 or null
     const float* ip_scale;
     const float* ip_offset;
+    uint16_t* sums;            // query i's region starts at i * stride_slots
+    uint64_t stride_slots;
+    float* item_scale;         // nq * M
+    float* item_offset;
+    uint32_t* item_min;        // smallest sum of the probe's list
+};
+
+// Phase 2: the reference's TopK replayed over the sums.
+struct ReplayArgs {
+    DevIvf ix;
+    int kp, M, block_filter;
+    const uint16_t* probe_idx;
+    const int* nprobe_q;
+    const uint32_t* item_blk;
+    const uint16_t* sums;
+    uint64_t stride_slots;
+    const float* item_scale;
+    const float* item_offset;
+    const uint32_t* item_min;
     const uint64_t* filter;    // allowed slots bitmap or null
     float* out_dist;           // nq * kp
     int64_t* out_label;        // nq * kp
     int* out_count;            // nq
-    int* status;               // nq: 1 when the query needed more probes than M
     float* g_dist;             // nq * 2kp TopK buffers in HBM when they do not fit LDS, else null
     int64_t* g_label;
 };
@@ -56,17 +75,8 @@ struct ScanArgs {
 // Largest LDS a kernel here asks for (the default dynamic limit).
 constexpr size_t kLdsMax = 64 * 1024;
 
-// LDS bytes k_scan_fastscan needs.
-inline size_t scan_lds_bytes(int kp, int m, int dim, bool topk_in_lds = true) {
-    size_t b = topk_in_lds ? static_cast<size_t>(2 * kp) * (8 + 4) : 0; // TopK buffer (labels, distances)
-    b += static_cast<size_t>(m) * 4 + static_cast<size_t>(dim) * 4;
-    b += static_cast<size_t>(kChunkBlocks) * 32 * 2;
-    b += static_cast<size_t>(m) * 16;
-    int np = 1;
-    while (np < kp) np <<= 1;
-    (void)np; // take_sorted pads to a power of two below 2k
-    return (b + 15) & ~size_t{15};
-}
+// LDS bytes of k_fs_replay's TopK buffer.
+inline size_t replay_lds_bytes(int kp) { return static_cast<size_t>(2 * kp) * 12; }
 
 void launch_sqnorms(const float* rows, int n, int dim, float* out, hipStream_t s);
 void launch_pq_transpose(const float* cent, int m, int sub, float* cent_t, hipStream_t s);
@@ -80,7 +90,12 @@ void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const ui
                        float* A, float* base, hipStream_t s);
 void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
                          float* offset, hipStream_t s);
-void launch_scan_fastscan(const ScanArgs& a, int nq, hipStream_t s);
+void launch_fs_sums(const SumsArgs& a, int nq, hipStream_t s);
+void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
+                       const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
+                       int* status, hipStream_t s);
+void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s);
+void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s);
 // Keys of one query's sort: in LDS, or gkeys (nq * rerank_keys(kc)) in HBM when too many.
 size_t rerank_keys(int kc);
 void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,

@@ -425,7 +425,6 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
             else
                 for (uint64_t t = 0; t < d; ++t) e[t] /= static_cast<float>(cnt[l]);
         }
-        d_->cosine_ip = true;
     }
     if (d_->has_pq) {
         PQ& pq = d_->pq;
@@ -449,8 +448,73 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
             std::copy(sc.begin(), sc.end(), pq.centroids.begin() + s * pq.K * pq.sub);
         }
     }
+    if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = pick_cosine_ip(x.data(), n, assign, rng);
     d_->trained = true;
     invalidate();
+}
+
+// Whether Cosine PQ ranks the training sample better by inner product than by
+// L2 tables, as IVFIndex::pick_cosine_inner_product (ivf_index.cpp:808-902)
+// decides: up to 200 sample rows query up to 10,000 others and each way's share
+// of the exact top 10 in its own top 20 is compared; inner product wins ties.
+// The codes are scored with float tables here (the reference also rounds them
+// to fast-scan bytes); the choice is a training heuristic, not search semantics.
+bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign, std::mt19937& rng) const {
+    const uint64_t d = d_->dim;
+    const PQ& pq = d_->pq;
+    std::vector<uint64_t> rows(n);
+    std::iota(rows.begin(), rows.end(), 0);
+    for (uint64_t i = 0; i + 1 < n; ++i) std::swap(rows[i], rows[i + rng() % (n - i)]);
+    const uint64_t nq = std::min<uint64_t>(200, n / 2), ndb = std::min<uint64_t>(10000, n - nq);
+    const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
+    const std::vector<float> cent_t = pq_transpose(pq);
+    // each database row decoded: its list's residual centre plus the chosen centroids
+    std::vector<float> dec(ndb * d);
+    parallel_for(ndb, [&](size_t j) {
+        const uint64_t r = rows[nq + j];
+        const float* c = rc + size_t{assign[r]} * d;
+        std::vector<float> res(d);
+        for (uint64_t t = 0; t < d; ++t) res[t] = x[r * d + t] - c[t];
+        std::vector<uint8_t> code(pq.m);
+        pq_encode(cent_t, pq.m, pq.sub, pq.K, res.data(), code.data());
+        for (uint64_t s = 0; s < pq.m; ++s)
+            for (uint64_t t = 0; t < pq.sub; ++t) {
+                const float y = pq.centroids[(s * pq.K + code[s]) * pq.sub + t];
+                dec[j * d + s * pq.sub + t] = c[s * pq.sub + t] + y;
+            }
+    });
+    std::atomic<uint64_t> hit_ip{0}, hit_l2{0};
+    parallel_for(nq, [&](size_t i) {
+        const float* q = x + rows[i] * d;
+        std::vector<std::pair<float, uint32_t>> ex(ndb), ip(ndb), l2(ndb);
+        for (uint64_t j = 0; j < ndb; ++j) {
+            const float* v = x + rows[nq + j] * d;
+            float e = 0, a = 0, b = 0;
+            for (uint64_t t = 0; t < d; ++t) {
+                const float de = q[t] - v[t], dl = q[t] - dec[j * d + t];
+                e += de * de;
+                b += dl * dl;
+                a -= q[t] * dec[j * d + t];
+            }
+            ex[j] = {e, static_cast<uint32_t>(j)};
+            ip[j] = {a, static_cast<uint32_t>(j)};
+            l2[j] = {b, static_cast<uint32_t>(j)};
+        }
+        const size_t t10 = std::min<size_t>(10, ndb), t20 = std::min<size_t>(20, ndb);
+        std::partial_sort(ex.begin(), ex.begin() + t10, ex.end());
+        std::partial_sort(ip.begin(), ip.begin() + t20, ip.end());
+        std::partial_sort(l2.begin(), l2.begin() + t20, l2.end());
+        std::unordered_set<uint32_t> truth;
+        for (size_t j = 0; j < t10; ++j) truth.insert(ex[j].second);
+        uint64_t hi = 0, hl = 0;
+        for (size_t j = 0; j < t20; ++j) {
+            hi += truth.count(ip[j].second);
+            hl += truth.count(l2[j].second);
+        }
+        hit_ip += hi;
+        hit_l2 += hl;
+    });
+    return hit_ip.load() >= hit_l2.load();
 }
 
 void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
