@@ -124,9 +124,11 @@ def cpu_baseline(path, q, nprobe, gpu_ids, gpu_d, budget_s):
     t = time.perf_counter()
     idx.batch_query(q[:200], K)
     per_q = (time.perf_counter() - t) / 200
-    n = int(min(q.shape[0], max(200, budget_s / max(per_q, 1e-9))))
+    # best of 3 passes (benchmarks/bench_indexes.py:85-93), the sample cut so that
+    # the three passes stay within the budget
+    n = int(min(q.shape[0], max(200, budget_s / 3 / max(per_q, 1e-9))))
     best = None
-    for _ in range(2 if n * per_q < budget_s / 2 else 1):
+    for _ in range(3):
         t = time.perf_counter()
         ids, d = idx.batch_query(q[:n], K)
         dt = time.perf_counter() - t
@@ -139,7 +141,7 @@ def cpu_baseline(path, q, nprobe, gpu_ids, gpu_d, budget_s):
         "cores": threads,
         "kind": kind,
         "sample": f"first {n} of the {q.shape[0]} queries, nprobe {nprobe}, k {K}, k_factor {K_FACTOR}; "
-                  f"best of passes; {kind} library QANN_DISABLE_AVX512=1",
+                  f"best of 3 passes; {kind} library built from its sources, QANN_DISABLE_AVX512=1",
         "parity_rows_identical": same_ids,
         "parity_rows_bit_exact": same_bits,
     }
@@ -173,6 +175,12 @@ def main():
     ivf.enable_pq(M, fast_scan=True)
     rng = np.random.default_rng(SEED)
     ivf.train(x[rng.choice(N_BASE, TRAIN_ROWS, replace=False)], max_iters=10, seed=SEED)
+    # train() keeps inner-product tables here (they put as many of the exact top 10
+    # in their top 20 of ~2,000 candidates); after the k_factor 10 re-rank, per-list
+    # L2 tables reach recall@10 0.95 and inner product stays below 0.89 at every
+    # nprobe, so the bench overrides the choice as the property allows.
+    picked_ip = bool(ivf.cosine_inner_product)
+    ivf.cosine_inner_product = False
     index = qann_amd.RefineIndex(ivf, K_FACTOR)
     index.add(x)
     sizes = ivf.list_sizes()
@@ -245,6 +253,7 @@ def main():
     lib.qann_hip_profile_enable(0)
     st = stage_times(lib, reset=True)
     scan_ms = st["scan"][0] / max(st["scan"][1], 1)
+    launches = max(st["scan"][1], 1) / prof_steps
     # algorithmic bytes of one scan launch: the code blocks of every probed list
     # of every query, B_scan(q) = sum over probes ceil(n_l / 32) * M * 16
     cents = torch.from_numpy(ivf.centroids()).cuda()
@@ -255,7 +264,7 @@ def main():
         d2 = (cents * cents).sum(1)[None, :] - 2.0 * (qs[s:s + 1024] @ cents.T)
         lists = d2.topk(chosen, dim=1, largest=False).indices
         total_blocks += int(blocks[lists].sum().item())
-    scan_bytes = total_blocks * M * 16
+    scan_bytes = total_blocks * M * 16 / launches
     achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
     log("stages (ms per launch): " + ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}" for k, v in st.items() if v[1]))
     log(f"scan: {scan_bytes / 1e9:.3f} GB of code blocks per launch, {achieved:,.0f} GB/s algorithmic")
@@ -289,6 +298,7 @@ def main():
             "workload": "IVF4096 + PQ50x4 fast-scan + RefineIndex(k_factor=10), 1.2M x 100 cosine, Zipf(1.1) "
                         "2048-centre mixture (i^-0.8 noise spectrum), 10k-query batch, k=10",
             "cosine_inner_product": bool(ivf.cosine_inner_product),
+            "cosine_inner_product_picked_by_train": picked_ip,
             "nprobe": chosen,
             "recall@10": round(rec, 4),
             "p50_ms_per_batch": round(float(np.percentile(ms, 50)), 3),
@@ -304,7 +314,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": None,
-            "algorithmic_bytes_per_launch": scan_bytes,
+            "algorithmic_bytes_per_launch": int(scan_bytes),
+            "launches_per_step": launches,
             "launch_ms": round(scan_ms, 4),
         },
         "cpu_baseline": cpu,

@@ -479,6 +479,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.item_scale = item_scale + q0 * M;
         a.item_offset = item_offset + q0 * M;
         a.item_min = item_min + q0 * M;
+        if (qd::fs_sums_lds(a) == 0) throw Error(QANN_EINVAL, "the fast-scan tables of this index do not fit in LDS");
         StageTimer t_scan(QANN_STAGE_SCAN, st);
         qd::launch_fs_sums(a, static_cast<int>(nc), st);
         check_launch("k_fs_sums");
@@ -572,6 +573,7 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
         ix.nlist = static_cast<int>(dsc.nlist);
         ix.metric = static_cast<int>(dsc.metric);
         ix.m = static_cast<int>(dsc.m);
+        ix.m4 = (ix.m + 3) & ~3;
         ix.sub = static_cast<int>(dsc.dim / dsc.m);
         ix.cosine_ip = dsc.cosine_ip ? 1 : 0;
         const size_t d = dsc.dim, nl = dsc.nlist, m = dsc.m, sub = d / m;
@@ -611,26 +613,33 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
     return guarded([&] {
         if (!h || h->kind != 2) invalid("not an IVF index");
         qd::DevIvf& ix = h->ivf;
-        const size_t nl = static_cast<size_t>(ix.nlist), bb = static_cast<size_t>(ix.m) * 16;
+        const size_t nl = static_cast<size_t>(ix.nlist), m = static_cast<size_t>(ix.m);
+        const size_t bb = m * 16, lb = static_cast<size_t>(ix.m4) * 16;
         std::vector<uint32_t> off(nl + 1, 0);
-        std::vector<uint64_t> coff(nl, 0);
+        std::vector<uint64_t> coff(nl, 0), src_off(nl, 0);
         uint64_t total = 0, cbytes = 0, src_bytes = 0;
         for (size_t l = 0; l < nl; ++l) {
             off[l] = static_cast<uint32_t>(total);
             total += sizes[l];
             if (total > 0xFFFFFFFFull) invalid("an index holds at most 2^32 - 1 vectors");
             coff[l] = cbytes;
-            const uint64_t blk = (sizes[l] + 31) / 32 * bb;
-            cbytes += (blk + 255) & ~uint64_t{255};
-            src_bytes += blk;
+            src_off[l] = src_bytes;
+            const uint64_t nblk = (sizes[l] + 31) / 32;
+            cbytes += (nblk * lb + 255) & ~uint64_t{255};
+            src_bytes += nblk * bb;
         }
         off[nl] = static_cast<uint32_t>(total);
+        // the device block layout: word (subspace s, column g) of a block at
+        // (s / 4) * 64 + g * 16 + (s % 4) * 4; subspaces m .. m4 - 1 stay zero
         std::vector<uint8_t> packed(std::max<uint64_t>(cbytes, 1), 0);
-        uint64_t src = 0;
         for (size_t l = 0; l < nl; ++l) {
-            const uint64_t blk = (sizes[l] + 31) / 32 * bb;
-            if (blk) std::memcpy(packed.data() + coff[l], codes + src, blk);
-            src += blk;
+            const uint64_t nblk = (sizes[l] + 31) / 32;
+            for (uint64_t b = 0; b < nblk; ++b) {
+                const uint8_t* sb = codes + src_off[l] + b * bb;
+                uint8_t* db = packed.data() + coff[l] + b * lb;
+                for (size_t sp = 0; sp < m; ++sp)
+                    for (size_t g = 0; g < 4; ++g) std::memcpy(db + (sp / 4) * 64 + g * 16 + (sp % 4) * 4, sb + sp * 16 + 4 * g, 4);
+            }
         }
         h->free_ptr(const_cast<uint32_t*>(ix.list_off));
         h->free_ptr(const_cast<uint64_t*>(ix.code_off));

@@ -16,6 +16,8 @@
 //                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543)
 //   k_rerank          exact FP32 re-rank of the candidates (RefineIndex::rerank, src/refine_index.cpp:112-140)
 //   k_block_select    exact top-k over a slot set (FlatIndex::query_impl, RefineIndex::direct_scan)
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "ivf_kernels.hpp"
 
@@ -288,151 +290,253 @@ __device__ void wave_trim(WaveTopK& tk) {
     tk.thr = from_fkey(t);
 }
 
-// The table of one (query, probed list), rounded to bytes in LDS: the
-// inner-product table of the query, or the list's L2 table from B + C + A
-// (build_quantize_lut, src/fast_scan.cpp:243-254) or from the residual
-// (compute_adc_table + quantize_lut, src/pq.cpp:165-184). Returns scale and
-// offset as quantize_lut sets them, plus the list's base for inner product.
-__device__ void wave_build_lut(const DevIvf& ix, const float* q, int qi, int list, float A, float base,
-                               const float* Cq, const uint8_t* ip_lut, const float* ip_scale, const float* ip_offset,
-                               uint8_t* lut, float* row_min, float* resid, float& scale, float& offset) {
-    const int m = ix.m, bb = m * 16, lane = lane_id();
-    if (ip_lut) {
-        const uint8_t* src = ip_lut + static_cast<size_t>(qi) * bb;
-        for (int e = 4 * lane; e < bb; e += 4 * kWave) *reinterpret_cast<uint32_t*>(lut + e) = *reinterpret_cast<const uint32_t*>(src + e);
-        scale = ip_scale[qi];
-        offset = ip_offset[qi] + base; // scan_fast(list, ip_scale, ip_offset + base) (ivf_index.cpp:571-572)
-        wave_sync();
-        return;
-    }
-    const float* B = ix.B ? ix.B + static_cast<size_t>(list) * bb : nullptr;
-    if (!B) {
-        const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * ix.dim;
+// Cross-lane moves within 16-lane DPP rows (CDNA data-parallel primitives).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, 0xF, 0xF, false));
+}
+
+// Minimum / maximum over each row of 16 lanes, left in every lane of the row:
+// quad swaps (quad_perm [1,0,3,2] then [2,3,0,1]), half-row mirror, row mirror.
+__device__ __forceinline__ float row16_min(float v) {
+    v = fminf(v, dpp_f<0xB1>(v));
+    v = fminf(v, dpp_f<0x4E>(v));
+    v = fminf(v, dpp_f<0x141>(v));
+    return fminf(v, dpp_f<0x140>(v));
+}
+
+__device__ __forceinline__ float row16_max(float v) {
+    v = fmaxf(v, dpp_f<0xB1>(v));
+    v = fmaxf(v, dpp_f<0x4E>(v));
+    v = fmaxf(v, dpp_f<0x141>(v));
+    return fmaxf(v, dpp_f<0x140>(v));
+}
+
+// Orders LDS writes and reads between the lanes of one wave (LDS operations
+// of a wave complete in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The L2 table of one (query, probed list), rounded to bytes, by one wave:
+// B + C (+ A on row 0) from the precomputed list terms (build_quantize_lut,
+// src/fast_scan.cpp:243-254), or the table of the residual q - c
+// (compute_adc_table, src/pq.cpp:165-184), then quantize_lut
+// (src/fast_scan.cpp:22-46): row minimums and the widest row through DPP row
+// reductions (lane e % 64 holds entry e, so a DPP row is a table row), one
+// scale 63 / widest, offset (p0 + p1) + (p2 + p3) where lane 16j sums the
+// minimums of rows j, j + 4, j + 8, ... in order, bytes rounded half to even.
+// Rows m .. m4 - 1 of the LUT are zero.
+__device__ void wave_lut_l2(const DevIvf& ix, const float* q, int list, float A, const float* Cq, float* rowmin,
+                            float* resid, uint8_t* lut, float& scale_out, float& offset_out) {
+    const int m = ix.m, n = m * 16, lane = lane_id();
+    const float* B = ix.B ? ix.B + static_cast<size_t>(list) * n : nullptr;
+    const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * ix.dim;
+    if (!B && resid) {
         for (int e = lane; e < ix.dim; e += kWave) resid[e] = q[e] - c[e];
-        wave_sync();
+        wave_lds_sync();
     }
-    auto entry = [&](int r, int j) -> float {
+    auto entry = [&](int e) -> float {
         if (B) {
-            float t = B[r * 16 + j] + Cq[r * 16 + j];
-            if (r == 0) t = t + A;
+            float t = B[e] + Cq[e];
+            if (e < 16) t = t + A;
             return t;
         }
+        const int r = e >> 4, j = e & 15;
         float acc = 0.0f;
         for (int s = 0; s < ix.sub; ++s) {
-            const float d = resid[r * ix.sub + s] - ix.pq_cent_t[(static_cast<size_t>(r) * ix.sub + s) * 16 + j];
+            const int x = r * ix.sub + s;
+            const float rs = resid ? resid[x] : q[x] - c[x];
+            const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * 16 + j];
             acc = fmaf(d, d, acc);
         }
         return acc;
     };
-    float wide = 0.0f;
-    for (int r = lane; r < m; r += kWave) {
-        float lo = entry(r, 0), hi = lo;
-        for (int j = 1; j < 16; ++j) {
-            const float t = entry(r, j);
-            lo = fminf(lo, t);
-            hi = fmaxf(hi, t);
+    float wide = 0.0f, part = 0.0f;
+    for (int base = 0; base < n; base += kWave) {
+        const int e = base + lane;
+        const bool valid = e < n; // n is a multiple of 16: a DPP row is all valid or all not
+        const float t = valid ? entry(e) : 0.0f;
+        const float lo = row16_min(t), hi = row16_max(t);
+        if (valid) {
+            if ((lane & 15) == 0) {
+                rowmin[e >> 4] = lo;
+                part += lo;
+            }
+            wide = fmaxf(wide, hi - lo);
         }
-        row_min[r] = lo;
-        wide = fmaxf(wide, hi - lo);
     }
     const float widest = wave_max(wide);
-    wave_sync();
-    float part = 0.0f;
-    if (lane < 4)
-        for (int r = lane; r < m; r += 4) part += row_min[r];
-    const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 1, kWave);
-    const float p2 = __shfl(part, 2, kWave), p3 = __shfl(part, 3, kWave);
-    offset = (p0 + p1) + (p2 + p3);
-    scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
-    for (int r = lane; r < m; r += kWave) {
-        const float mn = row_min[r];
-        for (int j = 0; j < 16; ++j) lut[r * 16 + j] = static_cast<uint8_t>(__builtin_rintf((entry(r, j) - mn) * scale));
-    }
-    wave_sync();
+    const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 16, kWave);
+    const float p2 = __shfl(part, 32, kWave), p3 = __shfl(part, 48, kWave);
+    const float scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
+    wave_lds_sync();
+    for (int e = lane; e < n; e += kWave)
+        lut[e] = static_cast<uint8_t>(__builtin_rintf((entry(e) - rowmin[e >> 4]) * scale));
+    for (int e = n + lane; e < ix.m4 * 16; e += kWave) lut[e] = 0;
+    scale_out = scale;
+    offset_out = (p0 + p1) + (p2 + p3);
 }
 
-// Four LUT lookups at once: byte i of the result is row[idx byte i] for a
-// 16-byte LUT row held in four registers (idx bytes in 0..15). v_perm_b32
-// picks bytes from an 8-byte pair, so the low and high halves of the row are
-// looked up separately and merged on bit 3 of each index.
-__device__ __forceinline__ uint32_t lut16x4(uint32_t idx, uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3) {
-    const uint32_t sel = idx & 0x07070707u;
-    const uint32_t lo = __builtin_amdgcn_perm(r1, r0, sel);
-    const uint32_t hi = __builtin_amdgcn_perm(r3, r2, sel);
-    const uint32_t m = ((idx >> 3) & 0x01010101u) * 0xFFu;
-    return (hi & m) | (lo & ~m);
+// Byte masks of bit 3 of each nibble of a code word, 0xFF where set: v_perm's
+// selectors 8..11 give the sign of bytes 1, 3 of its second source and bytes
+// 1, 3 of its first; with the word (shifted so the wanted bit is each byte's
+// bit 7) as the second source and the same shifted by 8 as the first, selector
+// 0x090B080A puts the sign of byte b of the word in byte b.
+__device__ __forceinline__ uint32_t bit3_mask_lo(uint32_t w) {
+    return __builtin_amdgcn_perm(w << 12, w << 4, 0x090B080Au);
+}
+__device__ __forceinline__ uint32_t bit3_mask_hi(uint32_t w) {
+    return __builtin_amdgcn_perm(w << 8, w, 0x090B080Au);
 }
 
-// Phase 1: one wave per (query, probe). The LUT of the pair, then the u16 sums
-// of every vector of the list (fast_scan_block, src/fast_scan.cpp:256-262):
-// lane (block, column g) takes byte column 4g..4g+3 of each 16-byte subspace
-// row, i.e. vectors 4g..4g+3 (low nibbles) and 16+4g..16+4g+3 (high nibbles),
-// adds four subspaces' bytes in bytes (at most 4 * 63) and widens to u16.
-__global__ void __launch_bounds__(64) k_fs_sums(const SumsArgs a) {
+// Eight LUT lookups of one code word (4 bytes = vectors 4g..4g+3 in the low
+// nibbles, 16+4g.. in the high): each 16-byte LUT row is held in four
+// registers; v_perm picks from entries 0..7 and 8..15 with the nibble's low 3
+// bits and bit 3 chooses between them. The results are added bytewise.
+__device__ __forceinline__ void lookup8(uint32_t w, const uint4 r, uint32_t& lo8, uint32_t& hi8) {
+    const uint32_t sl = w & 0x07070707u;
+    const uint32_t sh = (w >> 4) & 0x07070707u;
+    const uint32_t ml = bit3_mask_lo(w), mh = bit3_mask_hi(w);
+    const uint32_t a0 = __builtin_amdgcn_perm(r.y, r.x, sl), a1 = __builtin_amdgcn_perm(r.w, r.z, sl);
+    const uint32_t b0 = __builtin_amdgcn_perm(r.y, r.x, sh), b1 = __builtin_amdgcn_perm(r.w, r.z, sh);
+    lo8 += (a1 & ml) | (a0 & ~ml);
+    hi8 += (b1 & mh) | (b0 & ~mh);
+}
+
+__device__ __forceinline__ uint4 load_code16(const uint8_t* p) { return *reinterpret_cast<const uint4*>(p); }
+
+// Phase 1: one workgroup of kSumsWaves waves per (query, window of up to
+// kSumsWin consecutive probes). The waves build the window's LUTs in LDS (one
+// per probe for L2 tables; the query's one inner-product LUT otherwise), then
+// the window's code blocks are taken as one sequence, four lanes per block:
+// lane g of a block reads 16 bytes per 4 subspaces (the device layout keeps
+// column g's words of subspaces 4i..4i+3 together) and writes the u16 sums of
+// vectors 4g..4g+3 and 16+4g..16+4g+3 (fast_scan_block, src/fast_scan.cpp:256-262).
+// Each probe's smallest sum goes to item_min for the replay's list skip.
+__global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t w_start[kSumsWin + 1];
+    __shared__ int w_list[kSumsWin];
+    __shared__ uint32_t w_min[kSumsWin];
     const DevIvf& ix = a.ix;
-    const int m = ix.m, bb = m * 16;
-    uint8_t* lut = lds;                                                    // m * 16
-    float* row_min = reinterpret_cast<float*>(lds + ((bb + 15) & ~15));   // m
-    float* resid = row_min + m;                                            // dim
-    const int item = blockIdx.x;
-    const int qi = item / a.M, p = item % a.M;
-    if (p >= a.nprobe_q[qi]) return;
-    const int list = a.probe_idx[item];
-    const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
-    if (n == 0) return;
-    const int lane = lane_id();
+    const int m = ix.m, lb = ix.m4 * 16, win = a.win;
+    const int qi = blockIdx.x / a.nwin, p0 = (blockIdx.x % a.nwin) * win;
+    const int P = a.nprobe_q[qi];
+    if (p0 >= P) return;
+    const int np = min(win, P - p0);
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const bool ip = a.ip_lut != nullptr;
+    const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
     const float* q = a.q + static_cast<size_t>(qi) * ix.dim;
-    float scale, offset;
-    wave_build_lut(ix, q, qi, list, a.probe_A[item], a.probe_base[item], a.C ? a.C + static_cast<size_t>(qi) * bb : nullptr,
-                   a.ip_lut, a.ip_scale, a.ip_offset, lut, row_min, resid, scale, offset);
-    if (lane == 0) {
-        a.item_scale[item] = scale;
-        a.item_offset[item] = offset;
+    uint8_t* luts = lds;
+    float* Cs = reinterpret_cast<float*>(lds + static_cast<size_t>(ip ? 1 : win) * lb);
+    float* rowmin = Cs + (a.c_in_lds ? m * 16 : 0);
+    float* resid = rowmin + kSumsWaves * m;
+    const uint32_t blk0 = a.item_blk[item0];
+    if (tid < np) {
+        const int list = a.probe_idx[item0 + tid];
+        w_list[tid] = list;
+        w_min[tid] = 0xFFFFFFFFu;
+        const uint32_t rel = a.item_blk[item0 + tid] - blk0;
+        w_start[tid] = rel;
+        if (tid == np - 1) w_start[np] = rel + (ix.list_off[list + 1] - ix.list_off[list] + kFsBlock - 1) / kFsBlock;
     }
-    const uint8_t* codes = ix.codes + ix.code_off[list];
-    uint16_t* out = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(a.item_blk[item]) * kFsBlock;
-    const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
-    const int g = lane & 3;
-    uint32_t local_min = 0xFFFFFFFFu;
-    for (uint32_t cb = 0; cb < nb; cb += kChunkBlocks) {
-        const uint32_t blk = cb + (lane >> 2);
-        if (blk < nb) {
-            const uint8_t* bp = codes + static_cast<size_t>(blk) * bb + 4 * g;
-            uint32_t lo_a = 0, lo_b = 0, hi_a = 0, hi_b = 0; // u16 pairs: vectors (4g, 4g+2), (4g+1, 4g+3), +16
-            int s = 0;
-            while (s < m) {
-                uint32_t lo8 = 0, hi8 = 0;
-                const int s_end = min(m, s + 4);
-                for (; s < s_end; ++s) {
-                    const uint4 row = *reinterpret_cast<const uint4*>(lut + s * 16);
-                    const uint32_t w = *reinterpret_cast<const uint32_t*>(bp + s * 16);
-                    lo8 += lut16x4(w & 0x0F0F0F0Fu, row.x, row.y, row.z, row.w);
-                    hi8 += lut16x4((w >> 4) & 0x0F0F0F0Fu, row.x, row.y, row.z, row.w);
-                }
-                lo_a += lo8 & 0x00FF00FFu;
-                lo_b += (lo8 >> 8) & 0x00FF00FFu;
-                hi_a += hi8 & 0x00FF00FFu;
-                hi_b += (hi8 >> 8) & 0x00FF00FFu;
+    if (ip) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ip_lut + static_cast<size_t>(qi) * m * 16);
+        for (int e = tid; e < lb / 4; e += blockDim.x) reinterpret_cast<uint32_t*>(luts)[e] = e < m * 4 ? src[e] : 0u;
+        if (tid < np) {
+            a.item_scale[item0 + tid] = a.ip_scale[qi];
+            a.item_offset[item0 + tid] = a.ip_offset[qi] + a.probe_base[item0 + tid];
+        }
+    } else if (a.c_in_lds) {
+        for (int e = tid; e < m * 16; e += blockDim.x) Cs[e] = a.C[static_cast<size_t>(qi) * m * 16 + e];
+    }
+    __syncthreads();
+    if (!ip) {
+        const float* Cq = a.c_in_lds ? Cs : (a.C ? a.C + static_cast<size_t>(qi) * m * 16 : nullptr);
+        for (int p = wave; p < np; p += kSumsWaves) {
+            if (w_start[p + 1] == w_start[p]) continue; // an empty list: the replay skips it
+            const size_t item = item0 + p;
+            float scale, offset;
+            wave_lut_l2(ix, q, w_list[p], a.probe_A[item], Cq, rowmin + wave * m,
+                        a.resid_in_lds ? resid + wave * ix.dim : nullptr,
+                        luts + static_cast<size_t>(p) * lb, scale, offset);
+            if (lane == 0) {
+                a.item_scale[item] = scale;
+                a.item_offset[item] = offset;
             }
-            // vectors 4g+0..3 are lo_a.lo, lo_b.lo, lo_a.hi, lo_b.hi
+        }
+    }
+    __syncthreads();
+
+    const uint32_t T = w_start[np];
+    const int g = tid & 3, n4 = ix.m4 >> 2;
+    uint16_t* qsums = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
+    for (uint32_t base = 0; base < T; base += blockDim.x / 4) {
+        const uint32_t t = base + (tid >> 2);
+        uint32_t vmin = 0xFFFFFFFFu;
+        int p = 0;
+        if (t < T) {
+            // the probe of block t: the last p with w_start[p] <= t (empty lists share a start)
+            for (int step = kSumsWin / 2; step > 0; step >>= 1)
+                if (p + step < np && w_start[p + step] <= t) p += step;
+            const int list = w_list[p];
+            const uint32_t b = t - w_start[p];
+            const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+            const uint8_t* cp = ix.codes + ix.code_off[list] + static_cast<size_t>(b) * lb + g * 16;
+            const uint8_t* lut = luts + (ip ? 0 : static_cast<size_t>(p) * lb);
+            uint32_t lo_a = 0, lo_b = 0, hi_a = 0, hi_b = 0; // u16 pairs: vectors (4g, 4g+2), (4g+1, 4g+3), +16
+            // 16 subspaces per step of the main loop, their four 16-byte loads in flight together
+            auto step4 = [&](const uint4 w, int s4) {
+                const uint8_t* rp = lut + s4 * 64;
+                uint32_t lo8 = 0, hi8 = 0; // four subspaces' bytes add to at most 4 * 63
+                lookup8(w.x, *reinterpret_cast<const uint4*>(rp), lo8, hi8);
+                lookup8(w.y, *reinterpret_cast<const uint4*>(rp + 16), lo8, hi8);
+                lookup8(w.z, *reinterpret_cast<const uint4*>(rp + 32), lo8, hi8);
+                lookup8(w.w, *reinterpret_cast<const uint4*>(rp + 48), lo8, hi8);
+                lo_a += lo8 & 0x00FF00FFu;
+                lo_b += __builtin_amdgcn_perm(0u, lo8, 0x0C030C01u);
+                hi_a += hi8 & 0x00FF00FFu;
+                hi_b += __builtin_amdgcn_perm(0u, hi8, 0x0C030C01u);
+            };
+            int s4 = 0;
+            for (; s4 + 4 <= n4; s4 += 4) {
+                const uint4 w0 = load_code16(cp + 64 * s4), w1 = load_code16(cp + 64 * (s4 + 1));
+                const uint4 w2 = load_code16(cp + 64 * (s4 + 2)), w3 = load_code16(cp + 64 * (s4 + 3));
+                step4(w0, s4);
+                step4(w1, s4 + 1);
+                step4(w2, s4 + 2);
+                step4(w3, s4 + 3);
+            }
+            for (; s4 < n4; ++s4) step4(load_code16(cp + 64 * s4), s4);
             const uint32_t l01 = (lo_a & 0xFFFFu) | (lo_b << 16), l23 = (lo_a >> 16) | (lo_b & 0xFFFF0000u);
             const uint32_t h01 = (hi_a & 0xFFFFu) | (hi_b << 16), h23 = (hi_a >> 16) | (hi_b & 0xFFFF0000u);
-            uint16_t* o = out + static_cast<size_t>(blk) * kFsBlock;
+            uint16_t* o = qsums + static_cast<size_t>(t) * kFsBlock;
             *reinterpret_cast<uint2*>(o + 4 * g) = make_uint2(l01, l23);
             *reinterpret_cast<uint2*>(o + 16 + 4 * g) = make_uint2(h01, h23);
-            const uint32_t base_v = blk * kFsBlock;
+            const uint32_t v0 = b * kFsBlock;
             const uint32_t vals[8] = {l01 & 0xFFFFu, l01 >> 16, l23 & 0xFFFFu, l23 >> 16,
                                       h01 & 0xFFFFu, h01 >> 16, h23 & 0xFFFFu, h23 >> 16};
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const uint32_t v = base_v + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
-                if (v < n) local_min = min(local_min, vals[i]);
+                const uint32_t v = v0 + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
+                if (v < n) vmin = min(vmin, vals[i]);
             }
         }
+        // the block's four lanes hold one probe: their minimum, then one LDS atomic
+        vmin = min(vmin, dpp_u<0xB1>(vmin));
+        vmin = min(vmin, dpp_u<0x4E>(vmin));
+        if (t < T && g == 0) atomicMin(&w_min[p], vmin);
     }
-    for (int off = 32; off > 0; off >>= 1) local_min = min(local_min, static_cast<uint32_t>(__shfl_xor(static_cast<int>(local_min), off, kWave)));
-    if (lane == 0) a.item_min[item] = local_min;
+    __syncthreads();
+    if (tid < np) a.item_min[item0 + tid] = w_min[tid];
 }
 
 // Probes each query scans and where each probe's sums start: nprobe lists
@@ -851,9 +955,34 @@ void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut,
     hipLaunchKernelGGL(k_query_tables, dim3(nq), dim3(256), lds, s, q, ix, want_lut, C, lut, scale, offset);
 }
 
-void launch_fs_sums(const SumsArgs& a, int nq, hipStream_t s) {
-    const size_t lds = static_cast<size_t>((a.ix.m * 16 + 15) & ~15) + static_cast<size_t>(a.ix.m + a.ix.dim) * 4;
-    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(static_cast<size_t>(nq) * a.M)), dim3(kWave), lds, s, a);
+size_t fs_sums_lds(SumsArgs& a) {
+    const DevIvf& ix = a.ix;
+    const size_t lb = static_cast<size_t>(ix.m4) * 16;
+    const bool ip = a.ip_lut != nullptr;
+    size_t fixed = 0;
+    a.c_in_lds = 0;
+    a.resid_in_lds = 0;
+    if (!ip) {
+        fixed += static_cast<size_t>(kSumsWaves) * ix.m * 4;
+        if (a.C && static_cast<size_t>(ix.m) * 64 <= 16384) {
+            a.c_in_lds = 1;
+            fixed += static_cast<size_t>(ix.m) * 64;
+        }
+        if (!a.C && static_cast<size_t>(kSumsWaves) * ix.dim * 4 + fixed + lb <= kLdsMax) {
+            a.resid_in_lds = 1;
+            fixed += static_cast<size_t>(kSumsWaves) * ix.dim * 4;
+        }
+    }
+    if (fixed + lb > kLdsMax) return 0;
+    a.win = ip ? kSumsWin : static_cast<int>(std::min<size_t>(kSumsWin, (kLdsMax - fixed) / lb));
+    a.nwin = (a.M + a.win - 1) / a.win;
+    return (ip ? lb : static_cast<size_t>(a.win) * lb) + fixed;
+}
+
+void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
+    const size_t lds = fs_sums_lds(a);
+    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(static_cast<size_t>(nq) * a.nwin)),
+                       dim3(kSumsWaves * kWave), lds, s, a);
 }
 
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,

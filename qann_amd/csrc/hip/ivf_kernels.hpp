@@ -6,16 +6,20 @@
 
 namespace qd {
 
-// Blocks of 32 vectors whose sums one wave computes before replaying their pushes.
-constexpr int kChunkBlocks = 16;
+// Probes per k_fs_sums workgroup (a window of one query's probe order) and its waves.
+constexpr int kSumsWin = 16;
+constexpr int kSumsWaves = 4;
 // Most sorted probes one query can be given (k_select_sorted keeps them in LDS).
 constexpr int kSelectMax = 4096;
 
 // An IVF fast-scan index in HBM. Lists are stored back to back: list l's
 // slots at list_ids[list_off[l] .. list_off[l + 1]), its code blocks at
-// codes + code_off[l] (256-byte aligned), in the reference's block layout.
+// codes + code_off[l] (256-byte aligned). A block of 32 vectors takes m4 * 16
+// bytes (m4 = m rounded up to a multiple of 4, the extra subspaces zero): for
+// each group of 4 subspaces, 4 columns of 16 bytes, column g holding bytes
+// 4g..4g+3 of the reference's 16-byte rows of those subspaces (src/fast_scan.hpp:19-28).
 struct DevIvf {
-    int dim, nlist, metric, m, sub, cosine_ip;
+    int dim, nlist, metric, m, m4, sub, cosine_ip;
     const float* centroids;        // nlist * dim
     const float* centroid_sqnorms; // nlist
     const float* residual_centers; // nlist * dim or null
@@ -50,6 +54,8 @@ struct SumsArgs {
     float* item_scale;         // nq * M
     float* item_offset;
     uint32_t* item_min;        // smallest sum of the probe's list
+    int win, nwin;             // probes per workgroup and workgroups per query (set by the launcher)
+    int c_in_lds, resid_in_lds;
 };
 
 // Phase 2: the reference's TopK replayed over the sums.
@@ -90,7 +96,9 @@ void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const ui
                        float* A, float* base, hipStream_t s);
 void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
                          float* offset, hipStream_t s);
-void launch_fs_sums(const SumsArgs& a, int nq, hipStream_t s);
+// Dynamic LDS of k_fs_sums (0 when a table does not fit); sets win, nwin and the LDS flags.
+size_t fs_sums_lds(SumsArgs& a);
+void launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
                        const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
                        int* status, hipStream_t s);

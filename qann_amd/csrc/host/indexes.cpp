@@ -448,73 +448,130 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
             std::copy(sc.begin(), sc.end(), pq.centroids.begin() + s * pq.K * pq.sub);
         }
     }
-    if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = pick_cosine_ip(x.data(), n, assign, rng);
+    if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = pick_cosine_ip(x.data(), n, assign);
     d_->trained = true;
     invalidate();
 }
 
-// Whether Cosine PQ ranks the training sample better by inner product than by
-// L2 tables, as IVFIndex::pick_cosine_inner_product (ivf_index.cpp:808-902)
-// decides: up to 200 sample rows query up to 10,000 others and each way's share
-// of the exact top 10 in its own top 20 is compared; inner product wins ties.
-// The codes are scored with float tables here (the reference also rounds them
-// to fast-scan bytes); the choice is a training heuristic, not search semantics.
-bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign, std::mt19937& rng) const {
-    const uint64_t d = d_->dim;
+// Fast-scan table rounding as the search uses it (fast_scan.cpp quantize_lut):
+// one scale for the widest row, bytes rounded half to even from each row's
+// minimum, offset the sum of the row minimums.
+static void round_table(const float* table, uint64_t m, uint8_t* lut, float& scale, float& offset) {
+    std::vector<float> lo(m);
+    float widest = 0.0f, part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint64_t s = 0; s < m; ++s) {
+        const float* row = table + s * 16;
+        float a = row[0], b = row[0];
+        for (int j = 1; j < 16; ++j) {
+            a = std::min(a, row[j]);
+            b = std::max(b, row[j]);
+        }
+        lo[s] = a;
+        part[s % 4] += a;
+        widest = std::max(widest, b - a);
+    }
+    offset = (part[0] + part[1]) + (part[2] + part[3]);
+    scale = widest > 0.0f ? 63.0f / widest : 1.0f;
+    for (uint64_t s = 0; s < m; ++s)
+        for (int j = 0; j < 16; ++j) lut[s * 16 + j] = static_cast<uint8_t>(std::nearbyint((table[s * 16 + j] - lo[s]) * scale));
+}
+
+// Whether Cosine fast-scan PQ ranks its own training sample better with the
+// one inner-product table than with per-list L2 tables, decided as
+// IVFIndex::pick_cosine_inner_product does (ivf_index.cpp:808-902): the last
+// 200 sample rows query the first 20,000; each query probes lists nearest
+// first until about 2,000 rows; both ways score those rows with their rounded
+// byte tables, and each way's share of the exact top 10 in its own top 20 is
+// counted. L2 tables are chosen only when they rank strictly better.
+bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign) const {
+    constexpr uint64_t kQueries = 200, kRows = 20000, kCandidates = 2000, kTop = 10, kKept = 20;
+    if (n < kQueries + kCandidates) return true;
+    const uint64_t d = d_->dim, nl = d_->nlist, nq = kQueries, nd = std::min(kRows, n - nq);
     const PQ& pq = d_->pq;
-    std::vector<uint64_t> rows(n);
-    std::iota(rows.begin(), rows.end(), 0);
-    for (uint64_t i = 0; i + 1 < n; ++i) std::swap(rows[i], rows[i + rng() % (n - i)]);
-    const uint64_t nq = std::min<uint64_t>(200, n / 2), ndb = std::min<uint64_t>(10000, n - nq);
+    const uint64_t m = pq.m, T = m * 16;
     const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
+    auto dot = [d](const float* a, const float* b) {
+        float s = 0.0f;
+        for (uint64_t t = 0; t < d; ++t) s += a[t] * b[t];
+        return s;
+    };
     const std::vector<float> cent_t = pq_transpose(pq);
-    // each database row decoded: its list's residual centre plus the chosen centroids
-    std::vector<float> dec(ndb * d);
-    parallel_for(ndb, [&](size_t j) {
-        const uint64_t r = rows[nq + j];
-        const float* c = rc + size_t{assign[r]} * d;
-        std::vector<float> res(d);
-        for (uint64_t t = 0; t < d; ++t) res[t] = x[r * d + t] - c[t];
-        std::vector<uint8_t> code(pq.m);
-        pq_encode(cent_t, pq.m, pq.sub, pq.K, res.data(), code.data());
-        for (uint64_t s = 0; s < pq.m; ++s)
-            for (uint64_t t = 0; t < pq.sub; ++t) {
-                const float y = pq.centroids[(s * pq.K + code[s]) * pq.sub + t];
-                dec[j * d + s * pq.sub + t] = c[s * pq.sub + t] + y;
+    std::vector<uint8_t> codes(nd * m);
+    std::vector<std::vector<uint32_t>> by_list(nl);
+    for (uint64_t i = 0; i < nd; ++i) by_list[assign[i]].push_back(static_cast<uint32_t>(i));
+    parallel_for(nd, [&](size_t i) {
+        std::vector<float> r(d);
+        const float* c = rc + size_t{assign[i]} * d;
+        for (uint64_t t = 0; t < d; ++t) r[t] = x[i * d + t] - c[t];
+        pq_encode(cent_t, m, pq.sub, pq.K, r.data(), codes.data() + i * m);
+    });
+    // table[s][c] = sum over the subspace of f(q_t, y_t) for centroid y = pq[s][c]
+    auto build = [&](const float* q, bool ip, float* table) {
+        for (uint64_t s = 0; s < m; ++s)
+            for (uint64_t c = 0; c < 16; ++c) {
+                const float* y = pq.centroids.data() + (s * 16 + c) * pq.sub;
+                float acc = 0.0f;
+                for (uint64_t t = 0; t < pq.sub; ++t) {
+                    const float qt = q[s * pq.sub + t];
+                    acc += ip ? -2.0f * qt * y[t] : (qt - y[t]) * (qt - y[t]);
+                }
+                table[s * 16 + c] = acc;
+            }
+    };
+    std::vector<uint64_t> hits_ip(nq, 0), hits_l2(nq, 0);
+    parallel_for(nq, [&](size_t qi) {
+        const float* q = x + (n - nq + qi) * d;
+        std::vector<std::pair<float, uint32_t>> lists(nl);
+        for (uint64_t l = 0; l < nl; ++l) lists[l] = {-dot(q, d_->centroids.data() + l * d), static_cast<uint32_t>(l)};
+        std::sort(lists.begin(), lists.end());
+        std::vector<uint32_t> probed;
+        uint64_t rows = 0;
+        for (uint64_t p = 0; p < nl && rows < kCandidates; ++p) {
+            if (by_list[lists[p].second].empty()) continue;
+            probed.push_back(lists[p].second);
+            rows += by_list[lists[p].second].size();
+        }
+        std::vector<std::pair<float, uint32_t>> exact, ip, l2;
+        std::vector<float> table(T), resid(d);
+        std::vector<uint8_t> lut(T), list_lut(T);
+        float ip_scale = 1.0f, ip_offset = 0.0f;
+        build(q, true, table.data());
+        round_table(table.data(), m, lut.data(), ip_scale, ip_offset);
+        for (uint32_t l : probed) {
+            const float* c = rc + size_t{l} * d;
+            const float base = 2.0f - 2.0f * dot(q, c);
+            for (uint64_t t = 0; t < d; ++t) resid[t] = q[t] - c[t];
+            build(resid.data(), false, table.data());
+            float l2_scale = 1.0f, l2_offset = 0.0f;
+            round_table(table.data(), m, list_lut.data(), l2_scale, l2_offset);
+            for (uint32_t i : by_list[l]) {
+                const uint8_t* code = codes.data() + size_t{i} * m;
+                uint32_t a = 0, b = 0;
+                for (uint64_t s = 0; s < m; ++s) {
+                    a += lut[s * 16 + code[s]];
+                    b += list_lut[s * 16 + code[s]];
+                }
+                exact.emplace_back(-dot(q, x + size_t{i} * d), i);
+                ip.emplace_back(base + (static_cast<float>(a) / ip_scale + ip_offset), i);
+                l2.emplace_back(static_cast<float>(b) / l2_scale + l2_offset, i);
+            }
+        }
+        const size_t top = std::min<size_t>(kTop, exact.size()), kept = std::min<size_t>(kKept, exact.size());
+        std::partial_sort(exact.begin(), exact.begin() + top, exact.end());
+        std::partial_sort(ip.begin(), ip.begin() + kept, ip.end());
+        std::partial_sort(l2.begin(), l2.begin() + kept, l2.end());
+        for (size_t t = 0; t < top; ++t)
+            for (size_t j = 0; j < kept; ++j) {
+                hits_ip[qi] += ip[j].second == exact[t].second;
+                hits_l2[qi] += l2[j].second == exact[t].second;
             }
     });
-    std::atomic<uint64_t> hit_ip{0}, hit_l2{0};
-    parallel_for(nq, [&](size_t i) {
-        const float* q = x + rows[i] * d;
-        std::vector<std::pair<float, uint32_t>> ex(ndb), ip(ndb), l2(ndb);
-        for (uint64_t j = 0; j < ndb; ++j) {
-            const float* v = x + rows[nq + j] * d;
-            float e = 0, a = 0, b = 0;
-            for (uint64_t t = 0; t < d; ++t) {
-                const float de = q[t] - v[t], dl = q[t] - dec[j * d + t];
-                e += de * de;
-                b += dl * dl;
-                a -= q[t] * dec[j * d + t];
-            }
-            ex[j] = {e, static_cast<uint32_t>(j)};
-            ip[j] = {a, static_cast<uint32_t>(j)};
-            l2[j] = {b, static_cast<uint32_t>(j)};
-        }
-        const size_t t10 = std::min<size_t>(10, ndb), t20 = std::min<size_t>(20, ndb);
-        std::partial_sort(ex.begin(), ex.begin() + t10, ex.end());
-        std::partial_sort(ip.begin(), ip.begin() + t20, ip.end());
-        std::partial_sort(l2.begin(), l2.begin() + t20, l2.end());
-        std::unordered_set<uint32_t> truth;
-        for (size_t j = 0; j < t10; ++j) truth.insert(ex[j].second);
-        uint64_t hi = 0, hl = 0;
-        for (size_t j = 0; j < t20; ++j) {
-            hi += truth.count(ip[j].second);
-            hl += truth.count(l2[j].second);
-        }
-        hit_ip += hi;
-        hit_l2 += hl;
-    });
-    return hit_ip.load() >= hit_l2.load();
+    uint64_t total_ip = 0, total_l2 = 0;
+    for (uint64_t qi = 0; qi < nq; ++qi) {
+        total_ip += hits_ip[qi];
+        total_l2 += hits_l2[qi];
+    }
+    return total_ip >= total_l2;
 }
 
 void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {

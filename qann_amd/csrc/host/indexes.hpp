@@ -141,7 +141,7 @@ protected:
 
 private:
     friend class RefineIndex;
-    bool pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign, std::mt19937& rng) const;
+    bool pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign) const;
     std::unique_ptr<IvfData> d_;
     bool filter_probe_scaling_ = true;
     mutable qann_hip_index* dev_ = nullptr;
