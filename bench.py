@@ -156,6 +156,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nprobe", type=int, default=0, help="skip the sweep and use this nprobe")
+    ap.add_argument("--copy-check-mib", type=int, default=0,
+                    help="after timing, copy this many MiB device to device (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -268,6 +270,17 @@ def main():
     achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
     log("stages (ms per launch): " + ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}" for k, v in st.items() if v[1]))
     log(f"scan: {scan_bytes / 1e9:.3f} GB of code blocks per launch, {achieved:,.0f} GB/s algorithmic")
+    if args.copy_check_mib:
+        # a plain device-to-device copy of known size, so that HBM counters read
+        # under a profiler can be scaled to bytes
+        nbytes = args.copy_check_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy check: 3 device copies of {nbytes} bytes")
+        del src, dst
 
     cpu = None
     if not args.no_cpu:

@@ -324,62 +324,110 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The L2 table of one (query, probed list), rounded to bytes, by one wave:
-// B + C (+ A on row 0) from the precomputed list terms (build_quantize_lut,
-// src/fast_scan.cpp:243-254), or the table of the residual q - c
-// (compute_adc_table, src/pq.cpp:165-184), then quantize_lut
-// (src/fast_scan.cpp:22-46): row minimums and the widest row through DPP row
-// reductions (lane e % 64 holds entry e, so a DPP row is a table row), one
-// scale 63 / widest, offset (p0 + p1) + (p2 + p3) where lane 16j sums the
-// minimums of rows j, j + 4, j + 8, ... in order, bytes rounded half to even.
-// Rows m .. m4 - 1 of the LUT are zero.
+// The 16 entries of row r of one (query, list) L2 table: B + C (+ A on row 0)
+// from the precomputed list terms (build_quantize_lut, src/fast_scan.cpp:243-254),
+// or the table of the residual q - c (compute_adc_table, src/pq.cpp:165-184:
+// out[j] += (r_d - y_jd)^2 over the subspace's dimensions, fused).
+__device__ __forceinline__ void l2_row(const DevIvf& ix, const float* B, const float* Cq, float A, const float* q,
+                                       const float* c, const float* resid, int r, float t[16]) {
+    if (B) {
+        const float4* b4 = reinterpret_cast<const float4*>(B + r * 16);
+        const float4* c4 = reinterpret_cast<const float4*>(Cq + r * 16);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float4 x = b4[i], y = c4[i];
+            t[4 * i + 0] = x.x + y.x;
+            t[4 * i + 1] = x.y + y.y;
+            t[4 * i + 2] = x.z + y.z;
+            t[4 * i + 3] = x.w + y.w;
+        }
+        if (r == 0) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) t[j] = t[j] + A;
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t[j] = 0.0f;
+    for (int s = 0; s < ix.sub; ++s) {
+        const int x = r * ix.sub + s;
+        const float rs = resid ? resid[x] : q[x] - c[x];
+        const float4* y4 = reinterpret_cast<const float4*>(ix.pq_cent_t + static_cast<size_t>(x) * 16);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float4 y = y4[i];
+            float d;
+            d = rs - y.x; t[4 * i + 0] = fmaf(d, d, t[4 * i + 0]);
+            d = rs - y.y; t[4 * i + 1] = fmaf(d, d, t[4 * i + 1]);
+            d = rs - y.z; t[4 * i + 2] = fmaf(d, d, t[4 * i + 2]);
+            d = rs - y.w; t[4 * i + 3] = fmaf(d, d, t[4 * i + 3]);
+        }
+    }
+}
+
+// The L2 table of one (query, probed list), rounded to bytes (quantize_lut,
+// src/fast_scan.cpp:22-46), by one wave, one table row per lane: the row's
+// minimum and maximum taken in entry order as std::min / std::max do, one
+// scale 63 / widest over all rows, offset (p0 + p1) + (p2 + p3) with p[i % 4]
+// summing row minimums in row order, bytes rounded half to even. Rows m ..
+// m4 - 1 of the LUT are zero.
 __device__ void wave_lut_l2(const DevIvf& ix, const float* q, int list, float A, const float* Cq, float* rowmin,
                             float* resid, uint8_t* lut, float& scale_out, float& offset_out) {
-    const int m = ix.m, n = m * 16, lane = lane_id();
-    const float* B = ix.B ? ix.B + static_cast<size_t>(list) * n : nullptr;
+    const int m = ix.m, lane = lane_id();
+    const float* B = ix.B ? ix.B + static_cast<size_t>(list) * m * 16 : nullptr;
     const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * ix.dim;
     if (!B && resid) {
         for (int e = lane; e < ix.dim; e += kWave) resid[e] = q[e] - c[e];
         wave_lds_sync();
     }
-    auto entry = [&](int e) -> float {
-        if (B) {
-            float t = B[e] + Cq[e];
-            if (e < 16) t = t + A;
-            return t;
-        }
-        const int r = e >> 4, j = e & 15;
-        float acc = 0.0f;
-        for (int s = 0; s < ix.sub; ++s) {
-            const int x = r * ix.sub + s;
-            const float rs = resid ? resid[x] : q[x] - c[x];
-            const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * 16 + j];
-            acc = fmaf(d, d, acc);
-        }
-        return acc;
-    };
-    float wide = 0.0f, part = 0.0f;
-    for (int base = 0; base < n; base += kWave) {
-        const int e = base + lane;
-        const bool valid = e < n; // n is a multiple of 16: a DPP row is all valid or all not
-        const float t = valid ? entry(e) : 0.0f;
-        const float lo = row16_min(t), hi = row16_max(t);
-        if (valid) {
-            if ((lane & 15) == 0) {
-                rowmin[e >> 4] = lo;
-                part += lo;
+    float t[16];
+    float wide = 0.0f, lo_keep = 0.0f;
+    for (int r0 = 0; r0 < m; r0 += kWave) {
+        const int r = r0 + lane;
+        if (r < m) {
+            l2_row(ix, B, Cq, A, q, c, resid, r, t);
+            float lo = t[0], hi = t[0];
+#pragma unroll
+            for (int j = 1; j < 16; ++j) {
+                lo = t[j] < lo ? t[j] : lo;
+                hi = hi < t[j] ? t[j] : hi;
             }
+            rowmin[r] = lo;
+            lo_keep = lo;
             wide = fmaxf(wide, hi - lo);
         }
     }
     const float widest = wave_max(wide);
-    const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 16, kWave);
-    const float p2 = __shfl(part, 32, kWave), p3 = __shfl(part, 48, kWave);
     const float scale = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
     wave_lds_sync();
-    for (int e = lane; e < n; e += kWave)
-        lut[e] = static_cast<uint8_t>(__builtin_rintf((entry(e) - rowmin[e >> 4]) * scale));
-    for (int e = n + lane; e < ix.m4 * 16; e += kWave) lut[e] = 0;
+    float part = 0.0f;
+    if (lane < 4)
+        for (int r = lane; r < m; r += 4) part += rowmin[r];
+    const float p0 = __shfl(part, 0, kWave), p1 = __shfl(part, 1, kWave);
+    const float p2 = __shfl(part, 2, kWave), p3 = __shfl(part, 3, kWave);
+    for (int r0 = 0; r0 < ix.m4; r0 += kWave) {
+        const int r = r0 + lane;
+        if (r < m) {
+            float lo = lo_keep;
+            if (m > kWave) {
+                // a row of an earlier group: its entries again
+                l2_row(ix, B, Cq, A, q, c, resid, r, t);
+                lo = rowmin[r];
+            }
+            uint32_t w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    v |= static_cast<uint32_t>(__builtin_rintf((t[4 * i + j] - lo) * scale)) << (8 * j);
+                w[i] = v;
+            }
+            *reinterpret_cast<uint4*>(lut + r * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else if (r < ix.m4) {
+            *reinterpret_cast<uint4*>(lut + r * 16) = make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
     scale_out = scale;
     offset_out = (p0 + p1) + (p2 + p3);
 }
@@ -436,8 +484,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
     const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
     const float* q = a.q + static_cast<size_t>(qi) * ix.dim;
     uint8_t* luts = lds;
-    float* Cs = reinterpret_cast<float*>(lds + static_cast<size_t>(ip ? 1 : win) * lb);
-    float* rowmin = Cs + (a.c_in_lds ? m * 16 : 0);
+    float* rowmin = reinterpret_cast<float*>(lds + static_cast<size_t>(ip ? 1 : win) * lb);
     float* resid = rowmin + kSumsWaves * m;
     const uint32_t blk0 = a.item_blk[item0];
     if (tid < np) {
@@ -455,12 +502,10 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
             a.item_scale[item0 + tid] = a.ip_scale[qi];
             a.item_offset[item0 + tid] = a.ip_offset[qi] + a.probe_base[item0 + tid];
         }
-    } else if (a.c_in_lds) {
-        for (int e = tid; e < m * 16; e += blockDim.x) Cs[e] = a.C[static_cast<size_t>(qi) * m * 16 + e];
     }
     __syncthreads();
     if (!ip) {
-        const float* Cq = a.c_in_lds ? Cs : (a.C ? a.C + static_cast<size_t>(qi) * m * 16 : nullptr);
+        const float* Cq = a.C ? a.C + static_cast<size_t>(qi) * m * 16 : nullptr;
         for (int p = wave; p < np; p += kSumsWaves) {
             if (w_start[p + 1] == w_start[p]) continue; // an empty list: the replay skips it
             const size_t item = item0 + p;
@@ -485,8 +530,11 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
         int p = 0;
         if (t < T) {
             // the probe of block t: the last p with w_start[p] <= t (empty lists share a start)
-            for (int step = kSumsWin / 2; step > 0; step >>= 1)
-                if (p + step < np && w_start[p + step] <= t) p += step;
+#pragma unroll
+            for (int step = kSumsWin / 2; step > 0; step >>= 1) {
+                const int c = min(p + step, np - 1);
+                p = w_start[c] <= t ? c : p;
+            }
             const int list = w_list[p];
             const uint32_t b = t - w_start[p];
             const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
@@ -960,14 +1008,9 @@ size_t fs_sums_lds(SumsArgs& a) {
     const size_t lb = static_cast<size_t>(ix.m4) * 16;
     const bool ip = a.ip_lut != nullptr;
     size_t fixed = 0;
-    a.c_in_lds = 0;
     a.resid_in_lds = 0;
     if (!ip) {
         fixed += static_cast<size_t>(kSumsWaves) * ix.m * 4;
-        if (a.C && static_cast<size_t>(ix.m) * 64 <= 16384) {
-            a.c_in_lds = 1;
-            fixed += static_cast<size_t>(ix.m) * 64;
-        }
         if (!a.C && static_cast<size_t>(kSumsWaves) * ix.dim * 4 + fixed + lb <= kLdsMax) {
             a.resid_in_lds = 1;
             fixed += static_cast<size_t>(kSumsWaves) * ix.dim * 4;

@@ -55,7 +55,7 @@ struct SumsArgs {
     float* item_offset;
     uint32_t* item_min;        // smallest sum of the probe's list
     int win, nwin;             // probes per workgroup and workgroups per query (set by the launcher)
-    int c_in_lds, resid_in_lds;
+    int resid_in_lds;
 };
 
 // Phase 2: the reference's TopK replayed over the sums.
