@@ -383,6 +383,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += 2 * pad256(nq * 4) + pad256(nq * M * 4);    // plan
     need += 3 * pad256(nq * M * 4);                     // per-probe scale, offset, min
     need += pad256(chunk * stride * 2);                 // sums
+    need += pad256(chunk * (stride / qd::kFsBlock) * 2); // block minimums
     need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + pad256(nq * 4);
     need += gtopk ? pad256(nq * 2 * kp * 4) + pad256(nq * 2 * kp * 8) : 0;
     need += grerank ? pad256(nq * rkeys * 8) : 0;
@@ -399,14 +400,15 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     }
     float* D = sc.take<float>(nq * nlist);
     StageTimer t_coarse(QANN_STAGE_COARSE, st);
-    qd::launch_coarse_dist(qr, ix.centroids, static_cast<int>(nq), ix.nlist, ix.dim, ip, D, st);
-    check_launch("k_coarse_dist");
+    qd::launch_coarse_gemm(qr, static_cast<int>(nq), ix.centroids, ix.centroid_sqnorms, ix.nlist, ix.dim, ip, D, st);
+    check_launch("k_coarse_gemm");
     t_coarse.done();
     uint16_t* pidx = sc.take<uint16_t>(nq * M);
     float* pval = sc.take<float>(nq * M);
     StageTimer t_sel(QANN_STAGE_SELECT, st);
-    qd::launch_select_sorted(D, static_cast<int>(nq), ix.nlist, static_cast<int>(M), pidx, pval, st);
-    check_launch("k_select_sorted");
+    qd::launch_coarse_select(D, qr, static_cast<int>(nq), ix.centroids, ix.cnorm_max, ix.nlist, ix.dim, ip,
+                             static_cast<int>(M), pidx, pval, st);
+    check_launch("k_coarse_select");
     float* pA = sc.take<float>(nq * M);
     float* pbase = sc.take<float>(nq * M);
     qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
@@ -451,6 +453,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     float* item_offset = sc.take<float>(nq * M);
     uint32_t* item_min = sc.take<uint32_t>(nq * M);
     uint16_t* sums = sc.take<uint16_t>(chunk * stride);
+    uint16_t* bmin = sc.take<uint16_t>(chunk * (stride / qd::kFsBlock));
     float* cd = sc.take<float>(nq * kp);
     int64_t* cl = sc.take<int64_t>(nq * kp);
     int* ccount = sc.take<int>(nq);
@@ -476,6 +479,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.ip_offset = loff ? loff + q0 : nullptr;
         a.sums = sums;
         a.stride_slots = stride;
+        a.bmin = bmin;
         a.item_scale = item_scale + q0 * M;
         a.item_offset = item_offset + q0 * M;
         a.item_min = item_min + q0 * M;
@@ -494,6 +498,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.item_blk = a.item_blk;
         r.sums = sums;
         r.stride_slots = stride;
+        r.bmin = bmin;
         r.item_scale = a.item_scale;
         r.item_offset = a.item_offset;
         r.item_min = a.item_min;
@@ -554,6 +559,17 @@ int qann_hip_device_count(int* count) {
     });
 }
 
+// An upper bound on the norms of n rows (the largest, from double sums, rounded up).
+static float norm_bound(const float* rows, uint64_t n, uint32_t dim) {
+    double mx = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (uint32_t t = 0; t < dim; ++t) acc += double(rows[i * dim + t]) * double(rows[i * dim + t]);
+        mx = std::max(mx, acc);
+    }
+    return static_cast<float>(std::sqrt(mx) * (1.0 + 1e-6)) + 1e-30f;
+}
+
 int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
     return guarded([&] {
         const qann_hip_ivf_desc& dsc = *desc;
@@ -583,6 +599,7 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
         float* csq = h->alloc<float>(nl);
         qd::launch_sqnorms(cent, static_cast<int>(nl), static_cast<int>(d), csq, st);
         ix.centroid_sqnorms = csq;
+        ix.cnorm_max = norm_bound(dsc.centroids, nl, static_cast<uint32_t>(d));
         if (dsc.residual_centers) {
             float* rc = h->upload(dsc.residual_centers, nl * d);
             float* rsq = h->alloc<float>(nl);
@@ -708,7 +725,7 @@ int qann_hip_set_ids(qann_hip_index* h, uint64_t slots, const int64_t* labels, c
         h->live = slots - set;
         if (h->kind == 2) {
             h->ivf.labels = h->labels;
-            h->ivf.deleted = h->deleted;
+            h->ivf.deleted = set ? h->deleted : nullptr;
         }
     });
 }
@@ -806,14 +823,21 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
         check(hipMalloc(&val, block * 4), "hipMalloc");
         cl.ptrs.push_back(val);
         check(hipMemcpyAsync(dc, centroids, static_cast<size_t>(k) * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        float* csq = nullptr;
+        check(hipMalloc(&csq, static_cast<size_t>(k) * 4), "hipMalloc");
+        cl.ptrs.push_back(csq);
+        qd::launch_sqnorms(dc, static_cast<int>(k), static_cast<int>(dim), csq, st);
+        check_launch("k_sqnorms");
+        const float cmax = norm_bound(centroids, k, dim);
         std::vector<uint16_t> hidx(block);
         for (uint64_t b0 = 0; b0 < n; b0 += block) {
             const uint64_t b = std::min(block, n - b0);
             check(hipMemcpyAsync(dq, rows + b0 * dim, b * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-            qd::launch_coarse_dist(dq, dc, static_cast<int>(b), static_cast<int>(k), static_cast<int>(dim), 0, D, st);
-            check_launch("k_coarse_dist");
-            qd::launch_select_sorted(D, static_cast<int>(b), static_cast<int>(k), 1, idx, val, st);
-            check_launch("k_select_sorted");
+            qd::launch_coarse_gemm(dq, static_cast<int>(b), dc, csq, static_cast<int>(k), static_cast<int>(dim), 0, D, st);
+            check_launch("k_coarse_gemm");
+            qd::launch_coarse_select(D, dq, static_cast<int>(b), dc, cmax, static_cast<int>(k), static_cast<int>(dim), 0, 1,
+                                     idx, val, st);
+            check_launch("k_coarse_select");
             check(hipMemcpyAsync(hidx.data(), idx, b * 2, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
             check(hipStreamSynchronize(st), "hipStreamSynchronize");
             for (uint64_t i = 0; i < b; ++i) out[b0 + i] = hidx[i];

@@ -75,69 +75,233 @@ __global__ void k_rotate(const float* q, const float* R, int nq, int dim, float*
     out[static_cast<size_t>(i) * dim + j] = acc;
 }
 
-// D[i][l] = exact_score(q_i, c_l): one thread per (query, list), the query in LDS.
-__global__ void __launch_bounds__(256) k_coarse_dist(const float* q, const float* cents, int nlist, int dim, int ip, float* D) {
-    extern __shared__ float sq[];
-    const int tiles = (nlist + blockDim.x - 1) / blockDim.x;
-    const int i = blockIdx.x / tiles;
-    for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
-    __syncthreads();
-    const int l = (blockIdx.x % tiles) * blockDim.x + threadIdx.x;
-    if (l >= nlist) return;
-    D[static_cast<size_t>(i) * nlist + l] = exact_score(ip != 0, sq, cents + static_cast<size_t>(l) * dim, dim);
+// ---------------------------------------------------------------------------
+// Coarse quantizer (IVFIndex::coarse / coarse_all, src/ivf_index.cpp:305-383):
+// the M nearest lists of each query by (exact A, list), A = exact_score(q, c)
+// in the reference's AVX2 order. An FP32 MFMA GEMM scores every (query, list)
+// approximately; k_coarse_select then recomputes A exactly only for the lists
+// whose approximate score is within a proven rounding bound of the M-th
+// smallest, a set that holds every one of the M nearest.
+// ---------------------------------------------------------------------------
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kGemmTile = 64; // queries x lists per workgroup
+constexpr int kGemmK = 16;    // K staged through LDS per step
+
+// D[i][l] = |c_l|^2 - 2 q_i . c_l, or -q_i . c_l for inner product, by
+// v_mfma_f32_16x16x4f32: four waves of 32 x 32 (2 x 2 tiles of 16 x 16). Lane
+// t gives A[t % 16][t / 16] and B[t / 16][t % 16] of a 16 x 4 x 16 step and
+// holds D[4 (t / 16) + v][t % 16] in accumulator v.
+__global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, const float* cents, const float* csq,
+                                                     int nlist, int dim, int ip, float* D) {
+    __shared__ float qs[kGemmTile][kGemmK + 1];
+    __shared__ float cs[kGemmTile][kGemmK + 1];
+    const int l0 = blockIdx.x * kGemmTile, q0 = blockIdx.y * kGemmTile;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const int wq = (wave >> 1) * 32, wl = (wave & 1) * 32;
+    v4f acc00 = {0.0f, 0.0f, 0.0f, 0.0f}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+    for (int k0 = 0; k0 < dim; k0 += kGemmK) {
+        for (int e = tid; e < kGemmTile * kGemmK; e += blockDim.x) {
+            const int r = e / kGemmK, k = e % kGemmK, gk = k0 + k;
+            const int gq = q0 + r, gl = l0 + r;
+            qs[r][k] = (gq < nq && gk < dim) ? q[static_cast<size_t>(gq) * dim + gk] : 0.0f;
+            cs[r][k] = (gl < nlist && gk < dim) ? cents[static_cast<size_t>(gl) * dim + gk] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGemmK; kk += 4) {
+            const int kr = kk + lane / 16, rr = lane % 16;
+            const float a0 = qs[wq + rr][kr], a1 = qs[wq + 16 + rr][kr];
+            const float b0 = cs[wl + rr][kr], b1 = cs[wl + 16 + rr][kr];
+            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc11, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    auto put = [&](const v4f& acc, int ti, int tj) {
+        const int l = l0 + wl + 16 * tj + lane % 16;
+        if (l >= nlist) return;
+        const float c2 = ip ? 0.0f : csq[l];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = q0 + wq + 16 * ti + 4 * (lane / 16) + v;
+            if (i < nq) D[static_cast<size_t>(i) * nlist + l] = ip ? -acc[v] : c2 - 2.0f * acc[v];
+        }
+    };
+    put(acc00, 0, 0);
+    put(acc01, 0, 1);
+    put(acc10, 1, 0);
+    put(acc11, 1, 1);
 }
 
-// Per row of D (n values): the M smallest by (value, index) -- float order with
-// +-0 tied, as Probe::operator< (ivf_index.hpp:220-226) -- sorted ascending.
-// Radix select on the 64-bit key (ckey << 32 | index), then a bitonic sort of
-// the M selected in LDS. M <= kSelectMax.
-__global__ void k_select_sorted(const float* D, int n, int M, uint16_t* out_idx, float* out_val) {
-    extern __shared__ uint64_t sel[];           // pow2 >= M keys
+// The digit d of a 256-bin histogram (one bin per thread of a 256-thread
+// block) whose running count first reaches `remain`, and the count below it:
+// an inclusive scan of the bins within each wave, then across the four waves.
+__device__ void block_find_digit(const uint32_t* hist, uint32_t remain, uint32_t* s_wsum, uint32_t* s_digit,
+                                 uint32_t* s_below) {
+    const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
+    const uint32_t h = hist[t];
+    uint32_t x = h;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), off, kWave));
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) s_wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int v = 0; v < w; ++v) before += s_wsum[v];
+    const uint32_t incl = before + x, excl = incl - h;
+    if (excl < remain && incl >= remain) {
+        *s_digit = static_cast<uint32_t>(t);
+        *s_below = excl;
+    }
+    __syncthreads();
+}
+
+// Radix select over a row of n values by a 32- or 64-bit key (8 bits a pass):
+// the key of rank `want` (1-based) among those key(l) gives. 256 threads;
+// with CACHE, the row's keys (n <= 16 * 256) stay in registers across passes.
+template <typename Key, bool CACHE, typename KeyOf>
+__device__ Key block_radix_select(int n, uint32_t want, KeyOf key, uint32_t* hist, uint32_t* s_wsum, uint32_t* s_digit,
+                                  uint32_t* s_below) {
+    Key cached[CACHE ? 16 : 1];
+    if (CACHE) {
+#pragma unroll
+        for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
+            const int l = threadIdx.x + j * 256;
+            cached[j] = l < n ? key(l) : ~Key(0);
+        }
+    }
+    Key prefix = 0;
+    uint32_t remain = want;
+    constexpr int kBits = 8 * static_cast<int>(sizeof(Key));
+    for (int shift = kBits - 8; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        const Key hi_mask = shift == kBits - 8 ? Key(0) : static_cast<Key>(~Key(0) << (shift + 8));
+        if (CACHE) {
+#pragma unroll
+            for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
+                const int l = threadIdx.x + j * 256;
+                if (l < n && (cached[j] & hi_mask) == prefix) atomicAdd(&hist[(cached[j] >> shift) & 255], 1u);
+            }
+        } else {
+            for (int l = threadIdx.x; l < n; l += 256) {
+                const Key k = key(l);
+                if ((k & hi_mask) == prefix) atomicAdd(&hist[(k >> shift) & 255], 1u);
+            }
+        }
+        __syncthreads();
+        block_find_digit(hist, remain, s_wsum, s_digit, s_below);
+        prefix |= static_cast<Key>(*s_digit) << shift;
+        remain -= *s_below;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// Per query (one workgroup): the M smallest (A, list), A exact, sorted. The
+// candidates are the lists whose approximate score is at most tau + 3 delta,
+// tau the M-th smallest approximate score and delta twice the bound on
+// |A - approx - const| (exact side: (n + 3) u (|q| + |c|)^2; GEMM side with
+// |c|^2 rounded: (n + 2) u (|q| + |c|)^2; inner product: 2 (n + 1) u |q| |c|;
+// plus n 2^-125 for flushed denormals). If more than `cap` lists qualify (or
+// fewer than M, with NaN scores), every list is scored exactly and selected by
+// the full 64-bit key.
+__global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
+                                                       int dim, int ip, float cnorm_max, int M, int cap,
+                                                       uint16_t* out_idx, float* out_val) {
+    extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
-    __shared__ uint64_t prefix_s;
-    __shared__ uint32_t remain_s, count_s;
-    const int i = blockIdx.x;
-    const float* row = D + static_cast<size_t>(i) * n;
+    __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
+    __shared__ float s_red[4];
+    float* qsh = reinterpret_cast<float*>(lds);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
+    const int i = blockIdx.x, tid = threadIdx.x;
+    float* row = D + static_cast<size_t>(i) * nlist;
+    float ss = 0.0f;
+    for (int t = tid; t < dim; t += blockDim.x) {
+        const float v = q[static_cast<size_t>(i) * dim + t];
+        qsh[t] = v;
+        ss += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, kWave);
+    if (lane_id() == 0) s_red[tid / kWave] = ss;
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    ss = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+
+    auto key32 = [&](int l) { return ckey(row[l]); };
+    const uint32_t tau_key =
+        nlist <= 16 * 256
+            ? block_radix_select<uint32_t, true>(nlist, static_cast<uint32_t>(M), key32, hist, s_wsum, &s_digit, &s_below)
+            : block_radix_select<uint32_t, false>(nlist, static_cast<uint32_t>(M), key32, hist, s_wsum, &s_digit, &s_below);
+    const float tau = from_fkey(tau_key);
+    const float u = 5.9604645e-8f, n = static_cast<float>(dim);
+    const float qn = sqrtf(ss * (1.0f + 2.0f * (n + 2.0f) * u)) * 1.0001f;
+    const float delta = 2.0f * (ip ? 2.0f * (n + 1.0f) * u * qn * cnorm_max
+                                   : (2.0f * n + 5.0f) * u * (qn + cnorm_max) * (qn + cnorm_max)) +
+                        n * 2.3509887e-38f;
+    const float limit = tau + 3.0f * delta;
+    for (int l0 = 0; l0 < nlist; l0 += 16 * 256) {
+        float v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int l = l0 + tid + 256 * j;
+            v[j] = l < nlist ? row[l] : INFINITY;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (v[j] <= limit) {
+                const uint32_t pos = atomicAdd(&s_count, 1u);
+                if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l0 + tid + 256 * j);
+            }
+        }
+    }
+    __syncthreads();
+    const int cnt = static_cast<int>(s_count);
+    if (cnt >= M && cnt <= cap) { // fewer than M only when scores are NaN
+        int np = 1;
+        while (np < cnt) np <<= 1;
+        for (int t = tid; t < np; t += blockDim.x) {
+            if (t < cnt) {
+                const uint32_t l = static_cast<uint32_t>(keys[t]);
+                const float A = exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim);
+                keys[t] = (static_cast<uint64_t>(ckey(A)) << 32) | l;
+            } else {
+                keys[t] = ~0ull;
+            }
+        }
+        block_bitonic_sort(keys, np);
+        for (int t = tid; t < M; t += blockDim.x) {
+            const uint32_t l = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
+            out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(l);
+            out_val[static_cast<size_t>(i) * M + t] = from_fkey(static_cast<uint32_t>(keys[t] >> 32));
+        }
+        return;
+    }
+    // every list exactly, then the M smallest (ckey(A), list) keys
+    for (int l = tid; l < nlist; l += blockDim.x)
+        row[l] = exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim);
+    __syncthreads();
+    auto key64 = [&](int l) { return (static_cast<uint64_t>(ckey(row[l])) << 32) | static_cast<uint32_t>(l); };
+    const uint64_t kth =
+        block_radix_select<uint64_t, false>(nlist, static_cast<uint32_t>(M), key64, hist, s_wsum, &s_digit, &s_below);
     int Mp = 1;
     while (Mp < M) Mp <<= 1;
-
-    uint64_t prefix = 0;
-    uint32_t remain = static_cast<uint32_t>(M);
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        for (int t = threadIdx.x; t < 256; t += blockDim.x) hist[t] = 0;
-        __syncthreads();
-        const uint64_t hi_mask = shift == 56 ? 0ull : (~0ull << (shift + 8));
-        for (int l = threadIdx.x; l < n; l += blockDim.x) {
-            const uint64_t key = (static_cast<uint64_t>(ckey(row[l])) << 32) | static_cast<uint32_t>(l);
-            if ((key & hi_mask) == prefix) atomicAdd(&hist[(key >> shift) & 255], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            int dgt = 0;
-            for (; dgt < 256; ++dgt) {
-                if (acc + hist[dgt] >= remain) break;
-                acc += hist[dgt];
-            }
-            prefix_s = prefix | (static_cast<uint64_t>(dgt) << shift);
-            remain_s = remain - acc;
-        }
-        __syncthreads();
-        prefix = prefix_s;
-        remain = remain_s;
-    }
-    // prefix is now the M-th smallest key
-    if (threadIdx.x == 0) count_s = 0;
-    for (int t = threadIdx.x; t < Mp; t += blockDim.x) sel[t] = ~0ull;
+    if (tid == 0) s_count = 0;
+    for (int t = tid; t < Mp; t += blockDim.x) keys[t] = ~0ull;
     __syncthreads();
-    for (int l = threadIdx.x; l < n; l += blockDim.x) {
-        const uint64_t key = (static_cast<uint64_t>(ckey(row[l])) << 32) | static_cast<uint32_t>(l);
-        if (key <= prefix) sel[atomicAdd(&count_s, 1u)] = key;
+    for (int l = tid; l < nlist; l += blockDim.x) {
+        const uint64_t k = key64(l);
+        if (k <= kth) keys[atomicAdd(&s_count, 1u)] = k;
     }
-    __syncthreads();
-    block_bitonic_sort(sel, Mp);
-    for (int t = threadIdx.x; t < M; t += blockDim.x) {
-        const uint32_t l = static_cast<uint32_t>(sel[t] & 0xFFFFFFFFu);
+    block_bitonic_sort(keys, Mp);
+    for (int t = tid; t < M; t += blockDim.x) {
+        const uint32_t l = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
         out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(l);
         out_val[static_cast<size_t>(i) * M + t] = row[l];
     }
@@ -524,6 +688,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
     const uint32_t T = w_start[np];
     const int g = tid & 3, n4 = ix.m4 >> 2;
     uint16_t* qsums = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
+    uint16_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock) + blk0;
     for (uint32_t base = 0; base < T; base += blockDim.x / 4) {
         const uint32_t t = base + (tid >> 2);
         uint32_t vmin = 0xFFFFFFFFu;
@@ -581,7 +746,10 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
         // the block's four lanes hold one probe: their minimum, then one LDS atomic
         vmin = min(vmin, dpp_u<0xB1>(vmin));
         vmin = min(vmin, dpp_u<0x4E>(vmin));
-        if (t < T && g == 0) atomicMin(&w_min[p], vmin);
+        if (t < T && g == 0) {
+            atomicMin(&w_min[p], vmin);
+            qbmin[t] = static_cast<uint16_t>(min(vmin, 0xFFFFu));
+        }
     }
     __syncthreads();
     if (tid < np) a.item_min[item0 + tid] = w_min[tid];
@@ -637,9 +805,11 @@ __global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t
 // Phase 2: one wave per query replays the reference's scan over the sums in
 // probe order -- limit16 from the threshold at each block (scan_fast,
 // ivf_index.cpp:482-485), survivors in bit order through push (:539-543) into
-// the TopK (src/topk.hpp), trims when its buffer fills. A list whose smallest
-// sum cannot pass the current limit is skipped whole (the limit only falls),
-// and so is a pair of blocks with no sum below it.
+// the TopK (src/topk.hpp), trims when its buffer fills. The probes' metadata
+// is read 64 at a time, one probe per lane. A list whose smallest sum cannot
+// pass the current limit is skipped whole, and within a list only blocks whose
+// smallest sum can pass it are read (the limit only falls). The TopK holds
+// slots; they become ids (labels) for the final sort.
 __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevIvf& ix = a.ix;
@@ -652,73 +822,105 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
     const bool filtered = a.filter != nullptr;
     WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY};
     const int P = a.nprobe_q[qi];
-    for (int p = 0; p < P; ++p) {
-        const size_t item = static_cast<size_t>(qi) * a.M + p;
-        const int list = a.probe_idx[item];
-        const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
-        if (n == 0) continue;
-        const float scale = a.item_scale[item], offset = a.item_offset[item];
-        auto limit_now = [&]() -> uint32_t {
-            const float limit = (tk.thr - offset) * scale;
-            return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
-        };
-        if (a.item_min[item] >= limit_now()) continue;
-        const uint32_t* ids = ix.list_ids + ix.list_off[list];
-        const uint16_t* sums = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(a.item_blk[item]) * kFsBlock;
-        const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
-        for (uint32_t blk = 0; blk < nb; blk += 2) {
-            const uint32_t b = blk + (lane >> 5);
-            const int v = lane & 31;
-            const uint32_t e = b * kFsBlock + v;
-            const bool in = b < nb && e < n;
-            const uint32_t sum = in ? sums[e] : 0xFFFFu;
-            if (ballot(in && sum < limit_now()) == 0) continue;
-            for (int h = 0; h < 2 && blk + h < nb; ++h) {
-                const bool mine = (lane >> 5) == h;
-                const bool valid = mine && in;
-                uint32_t slot = 0;
-                bool allowed = valid;
-                if (a.block_filter) {
-                    slot = valid ? ids[e] : 0u;
-                    allowed = valid && bit_test(a.filter, slot);
-                    if (ballot(allowed) == 0) continue;
-                }
-                const bool surv = allowed && sum < limit_now();
-                uint64_t rem = ballot(surv);
-                if (rem == 0) continue;
-                if (!a.block_filter && surv) slot = ids[e];
-                const float dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
-                const bool ok = surv && (filtered ? bit_test(a.filter, slot) : !bit_test(ix.deleted, slot));
-                const int64_t lab = ok ? (ix.labels ? ix.labels[slot] : static_cast<int64_t>(slot)) : 0;
-                while (rem) {
-                    const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
-                    const uint64_t cm = ballot(cond);
-                    if (cm == 0) break;
-                    const int space = 2 * kp - tk.size;
-                    const int cnt = __popcll(cm);
-                    if (cnt < space) {
-                        if (cond) {
-                            const int pos = tk.size + static_cast<int>(lanes_below(cm));
-                            tk.dist[pos] = dist;
-                            tk.label[pos] = lab;
+    const uint16_t* qsums = a.sums + static_cast<size_t>(qi) * a.stride_slots;
+    const uint16_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock);
+    for (int c0 = 0; c0 < P; c0 += kWave) {
+        int m_list = 0;
+        uint32_t m_n = 0, m_blk = 0, m_min = 0xFFFFFFFFu;
+        float m_scale = 1.0f, m_off = 0.0f;
+        if (c0 + lane < P) {
+            const size_t item = static_cast<size_t>(qi) * a.M + c0 + lane;
+            m_list = a.probe_idx[item];
+            m_n = ix.list_off[m_list + 1] - ix.list_off[m_list];
+            m_blk = a.item_blk[item];
+            m_min = a.item_min[item];
+            m_scale = a.item_scale[item];
+            m_off = a.item_offset[item];
+        }
+        const int cn = min(kWave, P - c0);
+        for (int j = 0; j < cn; ++j) {
+            const uint32_t n = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_n), j));
+            if (n == 0) continue;
+            const float scale = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_scale), j));
+            const float offset = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_off), j));
+            auto limit_now = [&]() -> uint32_t {
+                const float limit = (tk.thr - offset) * scale;
+                return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
+            };
+            if (static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_min), j)) >= limit_now()) continue;
+            const int list = __builtin_amdgcn_readlane(m_list, j);
+            const uint32_t blk0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_blk), j));
+            const uint32_t* ids = ix.list_ids + ix.list_off[list];
+            const uint16_t* sums = qsums + static_cast<size_t>(blk0) * kFsBlock;
+            const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
+            for (uint32_t bc = 0; bc < nb; bc += kWave) {
+                const uint32_t bl = bc + lane;
+                const uint32_t bm = bl < nb ? qbmin[blk0 + bl] : 0xFFFFu;
+                uint64_t cand = ballot(bm < limit_now());
+                while (cand) {
+                    // the next two candidate blocks: lanes 0-31 and 32-63
+                    const int b1 = __builtin_ctzll(cand);
+                    cand &= cand - 1;
+                    int b2 = -1;
+                    if (cand) {
+                        b2 = __builtin_ctzll(cand);
+                        cand &= cand - 1;
+                    }
+                    const int mb = (lane >> 5) == 0 ? b1 : b2;
+                    const int v = lane & 31;
+                    const uint32_t e = (bc + static_cast<uint32_t>(mb)) * kFsBlock + v;
+                    const bool in = mb >= 0 && e < n;
+                    const uint32_t sum = in ? sums[e] : 0xFFFFu;
+                    if (ballot(in && sum < limit_now()) == 0) continue;
+                    for (int h = 0; h < 2; ++h) {
+                        if (h == 1 && b2 < 0) break;
+                        const bool mine = (lane >> 5) == h;
+                        const bool valid = mine && in;
+                        uint32_t slot = 0;
+                        bool allowed = valid;
+                        if (a.block_filter) {
+                            slot = valid ? ids[e] : 0u;
+                            allowed = valid && bit_test(a.filter, slot);
+                            if (ballot(allowed) == 0) continue;
                         }
-                        tk.size += cnt;
-                        wave_sync();
-                        break;
+                        const bool surv = allowed && sum < limit_now();
+                        uint64_t rem = ballot(surv);
+                        if (rem == 0) continue;
+                        if (!a.block_filter && surv) slot = ids[e];
+                        const float dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                        const bool ok = surv && (filtered ? bit_test(a.filter, slot)
+                                                          : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                        while (rem) {
+                            const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
+                            const uint64_t cm = ballot(cond);
+                            if (cm == 0) break;
+                            const int space = 2 * kp - tk.size;
+                            const int cnt = __popcll(cm);
+                            if (cnt < space) {
+                                if (cond) {
+                                    const int pos = tk.size + static_cast<int>(lanes_below(cm));
+                                    tk.dist[pos] = dist;
+                                    tk.label[pos] = slot;
+                                }
+                                tk.size += cnt;
+                                wave_sync();
+                                break;
+                            }
+                            // the space-th candidate fills the buffer and triggers a trim
+                            const uint32_t rank = lanes_below(cm);
+                            const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
+                            const int bpos = __builtin_ctzll(at);
+                            if (cond && lane <= bpos) {
+                                const int pos = tk.size + static_cast<int>(rank);
+                                tk.dist[pos] = dist;
+                                tk.label[pos] = slot;
+                            }
+                            tk.size = 2 * kp;
+                            wave_sync();
+                            wave_trim(tk);
+                            rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
+                        }
                     }
-                    // the space-th candidate fills the buffer and triggers a trim
-                    const uint32_t rank = lanes_below(cm);
-                    const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
-                    const int bpos = __builtin_ctzll(at);
-                    if (cond && lane <= bpos) {
-                        const int pos = tk.size + static_cast<int>(rank);
-                        tk.dist[pos] = dist;
-                        tk.label[pos] = lab;
-                    }
-                    tk.size = 2 * kp;
-                    wave_sync();
-                    wave_trim(tk);
-                    rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
                 }
             }
         }
@@ -727,10 +929,11 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
     // take_sorted (src/topk.hpp:53-76)
     if (tk.size > kp) wave_trim(tk);
     const int n = tk.size;
-    // (key, id) order when every id fits 32 bits, else (float, id) order
+    // slots become ids; (key, id) order when every id fits 32 bits, else (float, id) order
     bool big = false;
     for (int base = 0; base < n; base += kWave) {
         const int i = base + lane;
+        if (i < n && ix.labels) tk.label[i] = ix.labels[static_cast<uint32_t>(tk.label[i])];
         big |= __popcll(ballot(i < n && static_cast<uint64_t>(tk.label[i]) > 0xFFFFFFFFull)) != 0;
     }
     int np = 1;
@@ -978,17 +1181,30 @@ void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, 
     hipLaunchKernelGGL(k_rotate, dim3(cdiv(static_cast<size_t>(nq) * dim, 256)), dim3(256), 0, s, q, R, nq, dim, out);
 }
 
-void launch_coarse_dist(const float* q, const float* cents, int nq, int nlist, int dim, int ip, float* D,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_coarse_dist, dim3(cdiv(nlist, 256) * static_cast<unsigned>(nq)), dim3(256),
-                       static_cast<size_t>(dim) * 4, s, q, cents, nlist, dim, ip, D);
+size_t coarse_select_cap(int M) {
+    size_t cap = 1024;
+    while (cap < static_cast<size_t>(M)) cap <<= 1;
+    return cap;
 }
 
-void launch_select_sorted(const float* D, int nq, int n, int M, uint16_t* out_idx, float* out_val, hipStream_t s) {
-    int mp = 1;
-    while (mp < M) mp <<= 1;
-    hipLaunchKernelGGL(k_select_sorted, dim3(nq), dim3(256), static_cast<size_t>(mp) * 8, s, D, n, M, out_idx,
-                       out_val);
+void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
+                        float* D, hipStream_t s) {
+    const int rows = 65535 * kGemmTile; // grid y limit
+    for (int q0 = 0; q0 < nq; q0 += rows) {
+        const int n = std::min(rows, nq - q0);
+        hipLaunchKernelGGL(k_coarse_gemm, dim3(cdiv(nlist, kGemmTile), cdiv(n, kGemmTile)), dim3(256), 0, s,
+                           q + static_cast<size_t>(q0) * dim, n, cents, csq, nlist, dim, ip,
+                           D + static_cast<size_t>(q0) * nlist);
+    }
+}
+
+void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
+                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s) {
+    if (nq <= 0) return;
+    const size_t cap = coarse_select_cap(M);
+    const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
+    hipLaunchKernelGGL(k_coarse_select, dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip, cnorm_max, M,
+                       static_cast<int>(cap), out_idx, out_val);
 }
 
 void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,

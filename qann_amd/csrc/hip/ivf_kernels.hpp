@@ -22,6 +22,7 @@ struct DevIvf {
     int dim, nlist, metric, m, m4, sub, cosine_ip;
     const float* centroids;        // nlist * dim
     const float* centroid_sqnorms; // nlist
+    float cnorm_max;               // an upper bound on every centroid's norm
     const float* residual_centers; // nlist * dim or null
     const float* residual_sqnorms; // nlist or null
     const float* pq_cent_t;        // m * sub * 16
@@ -32,7 +33,7 @@ struct DevIvf {
     const uint32_t* list_ids;      // slots of every list entry
     const uint8_t* codes;
     const int64_t* labels;         // slot -> id, or null for ids == slots
-    const uint64_t* deleted;       // bitmap over slots
+    const uint64_t* deleted;       // bitmap over slots, or null when none is deleted
 };
 
 // Phase 1 of the fast-scan: the LUT and the u16 sums of every (query, probe).
@@ -51,6 +52,7 @@ struct SumsArgs {
     const float* ip_offset;
     uint16_t* sums;            // query i's region starts at i * stride_slots
     uint64_t stride_slots;
+    uint16_t* bmin;            // smallest sum of each block: query i's region at i * stride_slots / 32
     float* item_scale;         // nq * M
     float* item_offset;
     uint32_t* item_min;        // smallest sum of the probe's list
@@ -67,6 +69,7 @@ struct ReplayArgs {
     const uint32_t* item_blk;
     const uint16_t* sums;
     uint64_t stride_slots;
+    const uint16_t* bmin;
     const float* item_scale;
     const float* item_offset;
     const uint32_t* item_min;
@@ -89,9 +92,15 @@ void launch_pq_transpose(const float* cent, int m, int sub, float* cent_t, hipSt
 void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B,
                       hipStream_t s);
 void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, hipStream_t s);
-void launch_coarse_dist(const float* q, const float* cents, int nq, int nlist, int dim, int ip, float* D,
-                        hipStream_t s);
-void launch_select_sorted(const float* D, int nq, int n, int M, uint16_t* out_idx, float* out_val, hipStream_t s);
+// Coarse quantizer in two launches: approximate scores of every (query, list)
+// into D (nq * nlist), then per query the M nearest lists by (exact A, list),
+// sorted (D is overwritten when a query needs every list scored exactly).
+// The select's LDS: dim floats + coarse_select_cap(M) keys.
+size_t coarse_select_cap(int M);
+void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
+                        float* D, hipStream_t s);
+void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
+                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s);
 void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
                        float* A, float* base, hipStream_t s);
 void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
