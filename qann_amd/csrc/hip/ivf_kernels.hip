@@ -203,9 +203,114 @@ __device__ Key block_radix_select(int n, uint32_t want, KeyOf key, uint32_t* his
     return prefix;
 }
 
+// Minimum / maximum over a 256-thread block (every thread gets the result).
+__device__ float block_min256(float v, float* s_red) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+    __syncthreads();
+    if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    return fminf(fminf(s_red[0], s_red[1]), fminf(s_red[2], s_red[3]));
+}
+
+__device__ float block_max256(float v, float* s_red) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    __syncthreads();
+    if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    return fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+}
+
+// A value tau of the row with at least M values <= tau and, as a rule, few
+// more than M: values are bucketed linearly over [min, max] (256 buckets, the
+// bucket a monotone function of the value), the bucket where the running count
+// reaches M is split the same way once more when it holds many, and tau is the
+// largest value in the buckets up to the one found. With CACHE the row
+// (n <= 16 * 256) stays in registers.
+template <bool CACHE>
+__device__ float select_threshold(const float* row, int n, int M, int target, uint32_t* hist, uint32_t* s_wsum,
+                                  uint32_t* s_digit, uint32_t* s_below, float* s_red) {
+    const int tid = threadIdx.x;
+    float cached[CACHE ? 16 : 1];
+    if (CACHE) {
+#pragma unroll
+        for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
+            const int l = tid + 256 * j;
+            cached[j] = l < n ? row[l] : 0.0f;
+        }
+    }
+    auto each = [&](auto&& fn) {
+        if (CACHE) {
+#pragma unroll
+            for (int j = 0; j < (CACHE ? 16 : 1); ++j)
+                if (tid + 256 * j < n) fn(cached[j]);
+        } else {
+            for (int l = tid; l < n; l += 256) fn(row[l]);
+        }
+    };
+    float lo = INFINITY, hi = -INFINITY;
+    each([&](float v) {
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    });
+    lo = block_min256(lo, s_red);
+    hi = block_max256(hi, s_red + 4);
+    if (!(hi > lo) || !isfinite(hi - lo)) return hi;
+    const float s1 = 256.0f / (hi - lo);
+    auto b1 = [&](float v) { return min(255, static_cast<int>((v - lo) * s1)); };
+    hist[tid] = 0;
+    __syncthreads();
+    each([&](float v) { atomicAdd(&hist[b1(v)], 1u); });
+    __syncthreads();
+    block_find_digit(hist, static_cast<uint32_t>(M), s_wsum, s_digit, s_below);
+    const int d1 = static_cast<int>(*s_digit);
+    const uint32_t below1 = *s_below, c1 = hist[d1];
+    __syncthreads();
+    if (below1 + c1 <= static_cast<uint32_t>(target)) {
+        float m = -INFINITY;
+        each([&](float v) {
+            if (b1(v) <= d1) m = fmaxf(m, v);
+        });
+        return block_max256(m, s_red);
+    }
+    float lo2 = INFINITY, hi2 = -INFINITY;
+    each([&](float v) {
+        if (b1(v) == d1) {
+            lo2 = fminf(lo2, v);
+            hi2 = fmaxf(hi2, v);
+        }
+    });
+    lo2 = block_min256(lo2, s_red);
+    hi2 = block_max256(hi2, s_red + 4);
+    if (!(hi2 > lo2)) {
+        float m = -INFINITY;
+        each([&](float v) {
+            if (b1(v) <= d1) m = fmaxf(m, v);
+        });
+        return block_max256(m, s_red);
+    }
+    const float s2 = 256.0f / (hi2 - lo2);
+    auto b2 = [&](float v) { return min(255, max(0, static_cast<int>((v - lo2) * s2))); };
+    hist[tid] = 0;
+    __syncthreads();
+    each([&](float v) {
+        if (b1(v) == d1) atomicAdd(&hist[b2(v)], 1u);
+    });
+    __syncthreads();
+    block_find_digit(hist, static_cast<uint32_t>(M) - below1, s_wsum, s_digit, s_below);
+    const int d2 = static_cast<int>(*s_digit);
+    __syncthreads();
+    float m = -INFINITY;
+    each([&](float v) {
+        const int b = b1(v);
+        if (b < d1 || (b == d1 && b2(v) <= d2)) m = fmaxf(m, v);
+    });
+    return block_max256(m, s_red);
+}
+
 // Per query (one workgroup): the M smallest (A, list), A exact, sorted. The
 // candidates are the lists whose approximate score is at most tau + 3 delta,
-// tau the M-th smallest approximate score and delta twice the bound on
+// tau an approximate score with at least M at or below it (so every list of
+// the M nearest is a candidate) and delta twice the bound on
 // |A - approx - const| (exact side: (n + 3) u (|q| + |c|)^2; GEMM side with
 // |c|^2 rounded: (n + 2) u (|q| + |c|)^2; inner product: 2 (n + 1) u |q| |c|;
 // plus n 2^-125 for flushed denormals). If more than `cap` lists qualify (or
@@ -217,7 +322,7 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
-    __shared__ float s_red[4];
+    __shared__ float s_red[4], s_red8[8];
     float* qsh = reinterpret_cast<float*>(lds);
     uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
     const int i = blockIdx.x, tid = threadIdx.x;
@@ -234,12 +339,10 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
     __syncthreads();
     ss = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
 
-    auto key32 = [&](int l) { return ckey(row[l]); };
-    const uint32_t tau_key =
-        nlist <= 16 * 256
-            ? block_radix_select<uint32_t, true>(nlist, static_cast<uint32_t>(M), key32, hist, s_wsum, &s_digit, &s_below)
-            : block_radix_select<uint32_t, false>(nlist, static_cast<uint32_t>(M), key32, hist, s_wsum, &s_digit, &s_below);
-    const float tau = from_fkey(tau_key);
+    const int target = std::min(cap, M + std::max(64, M / 4));
+    const float tau = nlist <= 16 * 256
+                          ? select_threshold<true>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
+                          : select_threshold<false>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
     const float u = 5.9604645e-8f, n = static_cast<float>(dim);
     const float qn = sqrtf(ss * (1.0f + 2.0f * (n + 2.0f) * u)) * 1.0001f;
     const float delta = 2.0f * (ip ? 2.0f * (n + 1.0f) * u * qn * cnorm_max
