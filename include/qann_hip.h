@@ -58,12 +58,17 @@ const char* qann_hip_last_error(void);
 /* Number of visible GPUs (0 without a usable device). */
 int qann_hip_device_count(int* count);
 
+/* How an IVF index stores its lists (include/vecengine/ivf_index.hpp:32-50):
+ * 4-bit PQ fast-scan blocks (enable_pq(m, 16, fast_scan=true)), PQ codes of
+ * one byte per subspace scored by ADC (enable_pq without fast-scan), SQ codes
+ * (enable_sq, src/sq.cpp), or the raw float vectors. */
+enum { QANN_IVF_FASTSCAN = 0, QANN_IVF_PQ = 1, QANN_IVF_SQ = 2, QANN_IVF_FLOAT = 3 };
+
 /*
- * An IVF index with 4-bit PQ fast-scan codes: what IVFIndex::load_body builds
- * (src/ivf_index.cpp:976-1122) after enable_pq(m, 16, fast_scan=true), moved
- * into HBM. The device derives what IVFIndex derives on load: centroid and
- * residual-centre squared norms (update_centroid_sqnorms, :297-303), the PQ
- * transpose (PQCodebook::build_tables, src/pq.cpp:266-275) and, when
+ * An IVF index: what IVFIndex::load_body builds (src/ivf_index.cpp:976-1122),
+ * moved into HBM. The device derives what IVFIndex derives on load: centroid
+ * and residual-centre squared norms (update_centroid_sqnorms, :297-303), the
+ * PQ transpose (PQCodebook::build_tables, src/pq.cpp:266-275) and, when
  * use_precomputed is set, the per-list term B (build_precomputed, :66-81 with
  * PQCodebook::compute_list_term, src/pq.cpp:206-223).
  */
@@ -71,13 +76,18 @@ typedef struct {
     uint32_t dim;
     uint32_t nlist;           /* 100 .. 65535 */
     uint32_t metric;          /* QANN_METRIC_* */
-    uint32_t m;               /* PQ subspaces: even, <= 1024, dim % m == 0 */
+    uint32_t m;               /* PQ subspaces (fast-scan: even, <= 1024); dim % m == 0; 0 without PQ */
     uint32_t cosine_ip;       /* Metric::Cosine scored by inner product (ivf_index.hpp:116) */
     uint32_t use_precomputed; /* build B (the caller applies the reference's rule and 256 MiB cap) */
     const float* centroids;        /* nlist * dim, coarse centroids */
     const float* residual_centers; /* nlist * dim, or NULL (format 13, Cosine PQ) */
-    const float* pq_centroids;     /* m * 16 * (dim / m), the saved [s][c][sub] layout */
+    const float* pq_centroids;     /* m * pq_k * (dim / m), the saved [s][c][sub] layout; NULL without PQ */
     const float* rotation;         /* dim * dim row-major R (x * R), or NULL */
+    uint32_t storage;              /* QANN_IVF_*; 0 (fast-scan) for zero-initialised descriptors */
+    uint32_t pq_k;                 /* PQ centroids per subspace, 1 .. 256 (fast-scan: 16; 0 reads as 16) */
+    uint32_t sq_bits;              /* QANN_IVF_SQ: 4 or 8 */
+    const float* sq_vmin;          /* QANN_IVF_SQ: dim floats, detail::SQ::vmin */
+    const float* sq_step;          /* QANN_IVF_SQ: dim floats, detail::SQ::step */
 } qann_hip_ivf_desc;
 
 /* Creates the device index on the current GPU (replaces IVFIndex::load_body's
@@ -85,11 +95,13 @@ typedef struct {
 int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out);
 
 /*
- * Uploads every inverted list at once (InvertedList::ids / codes,
+ * Uploads every inverted list at once (InvertedList::ids / codes / vecs,
  * ivf_index.hpp:32-50): sizes[l] entries for list l; slots holds the lists'
- * slot ids back to back (sum(sizes) of them); codes holds each list's
- * ceil(sizes[l] / 32) * m * 16 fast-scan bytes back to back, in the
- * reference's block layout (src/fast_scan.hpp:19-34).
+ * slot ids back to back (sum(sizes) of them); codes holds each list's payload
+ * back to back: ceil(sizes[l] / 32) * m * 16 fast-scan bytes in the
+ * reference's block layout (src/fast_scan.hpp:19-34), sizes[l] * m PQ code
+ * bytes, sizes[l] * SQ code bytes (dim, or ceil(dim / 2) for 4 bits), or
+ * sizes[l] * dim floats, by the index's storage.
  */
 int qann_hip_ivf_set_lists(qann_hip_index* index, const uint32_t* sizes, const uint32_t* slots, const uint8_t* codes);
 
@@ -97,7 +109,7 @@ int qann_hip_ivf_set_lists(qann_hip_index* index, const uint32_t* sizes, const u
 int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint64_t slots, qann_hip_index** out);
 
 /*
- * A RefineIndex over an IVF fast-scan device index (src/refine_index.cpp):
+ * A RefineIndex over an IVF device index (src/refine_index.cpp):
  * holds the raw rows it re-ranks with (VectorStore, one row per slot). base
  * must outlive the refine handle.
  */

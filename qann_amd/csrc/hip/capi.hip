@@ -141,6 +141,7 @@ struct qann_hip_index {
     uint32_t dim = 0, metric = 0;
     // IVF
     qd::DevIvf ivf{};
+    std::vector<uint32_t> entry_of_slot_host; // host copy of ivf.entry_of_slot
     bool use_pre = false;
     std::vector<uint32_t> host_list_sizes;
     // blocks_desc_prefix[j]: blocks of the j largest lists together
@@ -297,7 +298,9 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             bitmap_slots(d_filter, nwords, slot_list, cnt, n_slots, st);
         }
         StageTimer t(QANN_STAGE_FLAT, st);
-        qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, h->rows, slot_list, n_slots,
+        qd::SlotScorer scorer{};
+        scorer.rows = h->rows;
+        qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, scorer, slot_list, n_slots,
                                 f ? nullptr : h->deleted, 0, 0, static_cast<int>(k), h->labels, d_ids, d_dists, gbuf,
                                 st);
         check_launch("k_block_select");
@@ -327,7 +330,9 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             uint32_t* cnt = sc.take<uint32_t>(1);
             bitmap_slots(d_filter, nwords, slot_list, cnt, static_cast<uint32_t>(p.allowed_count), st);
             StageTimer t(QANN_STAGE_RERANK, st);
-            qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, h->rows, slot_list,
+            qd::SlotScorer scorer{};
+            scorer.rows = h->rows;
+            qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, scorer, slot_list,
                                     static_cast<uint32_t>(p.allowed_count), nullptr, 1, 1, static_cast<int>(k),
                                     h->labels, d_ids, d_dists, gbuf, st);
             check_launch("k_block_select");
@@ -339,19 +344,51 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         fill_empty(d_ids, d_dists, nq * k, st);
         return;
     }
+    const bool flat_lists = ix.storage == QANN_IVF_SQ || ix.storage == QANN_IVF_FLOAT;
+    // IVFIndex::direct_scan (src/ivf_index.cpp:248-256): without PQ, a filter of at
+    // most 3 nlist slots is scored slot by slot, pushed in slot order
+    if (!refine && flat_lists && filtered && p.allowed_count <= 3ull * ix.nlist) {
+        sc.used = 0;
+        const size_t selb = qd::block_select_bytes(static_cast<int>(k), h->labels != nullptr);
+        const bool gsel = qlds + selb > qd::kLdsMax;
+        sc.reserve(pad256(p.allowed_count * 4) + 256 + (gsel ? pad256(nq * selb) : 0));
+        unsigned char* gbuf = gsel ? sc.take<unsigned char>(nq * selb) : nullptr;
+        uint32_t* slot_list = sc.take<uint32_t>(p.allowed_count);
+        uint32_t* cnt = sc.take<uint32_t>(1);
+        bitmap_slots(d_filter, nwords, slot_list, cnt, static_cast<uint32_t>(p.allowed_count), st);
+        qd::SlotScorer scorer{};
+        scorer.entry_of_slot = ix.entry_of_slot;
+        if (ix.storage == QANN_IVF_SQ) {
+            scorer.sq_codes = ix.codes;
+            scorer.sq_vmin = ix.sq_vmin;
+            scorer.sq_step = ix.sq_step;
+            scorer.sq_bits = ix.sq_bits;
+            scorer.vbytes = ix.vbytes;
+        } else {
+            scorer.rows = reinterpret_cast<const float*>(ix.codes);
+        }
+        StageTimer t(QANN_STAGE_FLAT, st);
+        qd::launch_block_select(d_q, static_cast<int>(nq), h->dim, ip, scorer, slot_list,
+                                static_cast<uint32_t>(p.allowed_count), nullptr, 1, 1, static_cast<int>(k), h->labels,
+                                d_ids, d_dists, gbuf, st);
+        check_launch("k_block_select");
+        t.done();
+        return;
+    }
     const uint32_t nlist = static_cast<uint32_t>(ix.nlist);
     const uint32_t nprobe = p.nprobe;
     if (nprobe == 0 || nprobe > nlist) invalid("nprobe must be in [1, nlist]");
     const uint32_t M = filtered ? std::min<uint32_t>(nlist, qd::kSelectMax) : nprobe;
     if (M > static_cast<uint32_t>(qd::kSelectMax)) invalid("nprobe above 4096 is not supported");
-    const bool cos_pq = ix.metric == QANN_METRIC_COSINE && ix.cosine_ip;
-    const bool ip_pq = ix.metric == QANN_METRIC_IP || cos_pq;
-    const bool use_pre = !ip_pq && ix.B != nullptr;
+    const bool fs = ix.storage == QANN_IVF_FASTSCAN;
+    const bool has_pq = fs || ix.storage == QANN_IVF_PQ;
+    const bool cos_pq = has_pq && ix.metric == QANN_METRIC_COSINE && ix.cosine_ip;
+    const bool ip_pq = has_pq && (ix.metric == QANN_METRIC_IP || cos_pq);
+    const bool use_pre = has_pq && !ip_pq && ix.B != nullptr;
     const uint32_t m = static_cast<uint32_t>(ix.m);
+    const size_t tsz = static_cast<size_t>(m) * static_cast<size_t>(ix.K); // table floats
     // the TopK buffers go to HBM when they do not fit LDS
     const bool gtopk = qd::replay_lds_bytes(static_cast<int>(kp)) > qd::kLdsMax;
-    if ((static_cast<size_t>(m) * 16 + 15) / 16 * 16 + (static_cast<size_t>(m) + ix.dim) * 4 > qd::kLdsMax)
-        invalid("num_subspaces and dim too large for the on-chip tables");
     const size_t rkeys = qd::rerank_keys(static_cast<int>(kp));
     const bool grerank = refine && qlds + rkeys * 8 > qd::kLdsMax;
 
@@ -369,7 +406,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     // queries go through the two scan phases in chunks that bound that buffer
     const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];
     const uint64_t stride = std::max<uint64_t>(wblocks, 1) * qd::kFsBlock;
-    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * 2)));
+    const uint64_t vbytes = fs ? 2 : 4; // u16 sums or float distances per vector
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * vbytes)));
 
     // scratch plan
     const size_t d = ix.dim;
@@ -377,13 +415,13 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += ix.rotation ? pad256(nq * d * 4) : 0;
     need += pad256(nq * nlist * 4); // coarse distances
     need += pad256(nq * M * 2) + 3 * pad256(nq * M * 4);
-    need += use_pre ? pad256(nq * m * 16 * 4) : 0;
-    need += ip_pq ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
+    need += use_pre || (ip_pq && !fs) ? pad256(nq * tsz * 4) : 0;
+    need += ip_pq && fs ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
     need += filtered ? pad256(nlist * 4) : 0;
     need += 2 * pad256(nq * 4) + pad256(nq * M * 4);    // plan
     need += 3 * pad256(nq * M * 4);                     // per-probe scale, offset, min
-    need += pad256(chunk * stride * 2);                 // sums
-    need += pad256(chunk * (stride / qd::kFsBlock) * 2); // block minimums
+    need += pad256(chunk * stride * vbytes);                 // sums or distances
+    need += pad256(chunk * (stride / qd::kFsBlock) * vbytes); // block minimums
     need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + pad256(nq * 4);
     need += gtopk ? pad256(nq * 2 * kp * 4) + pad256(nq * 2 * kp * 8) : 0;
     need += grerank ? pad256(nq * rkeys * 8) : 0;
@@ -437,23 +475,33 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     uint8_t* lut = nullptr;
     float *lscale = nullptr, *loff = nullptr;
     StageTimer t_tab(QANN_STAGE_TABLES, st);
-    if (use_pre) {
-        C = sc.take<float>(nq * m * 16);
-        qd::launch_query_tables(qr, ix, static_cast<int>(nq), 0, C, nullptr, nullptr, nullptr, st);
+    if (use_pre || (ip_pq && !fs)) {
+        C = sc.take<float>(nq * tsz);
+        qd::launch_query_tables(qr, ix, static_cast<int>(nq), use_pre ? qd::kTableInner : qd::kTableIp, C, nullptr,
+                                nullptr, nullptr, st);
         check_launch("k_query_tables");
     } else if (ip_pq) {
         lut = sc.take<uint8_t>(nq * m * 16);
         lscale = sc.take<float>(nq);
         loff = sc.take<float>(nq);
-        qd::launch_query_tables(qr, ix, static_cast<int>(nq), 1, nullptr, lut, lscale, loff, st);
+        qd::launch_query_tables(qr, ix, static_cast<int>(nq), qd::kTableIpLut, nullptr, lut, lscale, loff, st);
         check_launch("k_query_tables");
     }
     t_tab.done();
     float* item_scale = sc.take<float>(nq * M);
     float* item_offset = sc.take<float>(nq * M);
     uint32_t* item_min = sc.take<uint32_t>(nq * M);
-    uint16_t* sums = sc.take<uint16_t>(chunk * stride);
-    uint16_t* bmin = sc.take<uint16_t>(chunk * (stride / qd::kFsBlock));
+    uint16_t* sums = nullptr;
+    uint16_t* bmin = nullptr;
+    float* fdists = nullptr;
+    uint32_t* fbmin = nullptr;
+    if (fs) {
+        sums = sc.take<uint16_t>(chunk * stride);
+        bmin = sc.take<uint16_t>(chunk * (stride / qd::kFsBlock));
+    } else {
+        fdists = sc.take<float>(chunk * stride);
+        fbmin = sc.take<uint32_t>(chunk * (stride / qd::kFsBlock));
+    }
     float* cd = sc.take<float>(nq * kp);
     int64_t* cl = sc.take<int64_t>(nq * kp);
     int* ccount = sc.take<int>(nq);
@@ -462,7 +510,58 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     qd::DevIvf ixs = ix;
     if (!use_pre) ixs.B = nullptr;
     const size_t bb = static_cast<size_t>(m) * 16;
-    for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+    const int dist_mode = ip_pq ? qd::kDistIp
+                          : use_pre ? qd::kDistL2Pre
+                          : has_pq  ? qd::kDistL2Resid
+                          : ix.storage == QANN_IVF_SQ ? qd::kDistSq
+                                                      : qd::kDistFloat;
+    for (uint64_t q0 = 0; q0 < nq && !fs; q0 += chunk) {
+        // PQ (ADC), SQ and float lists: distances, then the replay over them
+        const uint64_t nc = std::min<uint64_t>(chunk, nq - q0);
+        qd::DistArgs a{};
+        a.ix = ixs;
+        a.M = static_cast<int>(M);
+        a.mode = dist_mode;
+        a.q = qr + q0 * d;
+        a.probe_idx = pidx + q0 * M;
+        a.probe_A = pA + q0 * M;
+        a.probe_base = pbase + q0 * M;
+        a.nprobe_q = nprobe_q + q0;
+        a.item_blk = item_blk + q0 * M;
+        a.C = C ? C + q0 * tsz : nullptr;
+        a.filter = filtered ? d_filter : nullptr;
+        a.dists = fdists;
+        a.stride_slots = stride;
+        a.bmin = fbmin;
+        a.item_min = item_min + q0 * M;
+        if (qd::ivf_dists_lds(a) == 0) throw Error(QANN_EINVAL, "the PQ tables of this index do not fit in LDS");
+        StageTimer t_scan(QANN_STAGE_SCAN, st);
+        qd::launch_ivf_dists(a, static_cast<int>(nc), st);
+        check_launch("k_ivf_dists");
+        t_scan.done();
+        qd::ReplayArgs r{};
+        r.ix = ixs;
+        r.kp = static_cast<int>(kp);
+        r.M = static_cast<int>(M);
+        r.probe_idx = a.probe_idx;
+        r.nprobe_q = a.nprobe_q;
+        r.item_blk = a.item_blk;
+        r.fdists = fdists;
+        r.stride_slots = stride;
+        r.fbmin = fbmin;
+        r.item_min = a.item_min;
+        r.filter = filtered ? d_filter : nullptr;
+        r.out_dist = cd + q0 * kp;
+        r.out_label = cl + q0 * kp;
+        r.out_count = ccount + q0;
+        r.g_dist = gd ? gd + q0 * 2 * kp : nullptr;
+        r.g_label = gl ? gl + q0 * 2 * kp : nullptr;
+        StageTimer t_rep(QANN_STAGE_REPLAY, st);
+        qd::launch_fs_replay(r, static_cast<int>(nc), st);
+        check_launch("k_fl_replay");
+        t_rep.done();
+    }
+    for (uint64_t q0 = 0; q0 < nq && fs; q0 += chunk) {
         const uint64_t nc = std::min<uint64_t>(chunk, nq - q0);
         qd::SumsArgs a{};
         a.ix = ixs;
@@ -576,9 +675,16 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
         if (dsc.dim == 0) invalid("dim must be >= 1");
         if (dsc.nlist < 100 || dsc.nlist > 65535) invalid("nlist must be in [100, 65535]");
         if (dsc.metric > 2) invalid("unknown metric");
-        if (dsc.m == 0 || dsc.m % 2 != 0 || dsc.m > 1024 || dsc.dim % dsc.m != 0)
-            invalid("fast_scan needs an even num_subspaces <= 1024 dividing dim");
-        if (!dsc.centroids || !dsc.pq_centroids) invalid("centroids and PQ codebook are required");
+        if (dsc.storage > QANN_IVF_FLOAT) invalid("unknown list storage");
+        const bool pq = dsc.storage == QANN_IVF_FASTSCAN || dsc.storage == QANN_IVF_PQ;
+        const uint32_t K = dsc.pq_k ? dsc.pq_k : 16;
+        if (dsc.storage == QANN_IVF_FASTSCAN && (dsc.m == 0 || dsc.m % 2 != 0 || dsc.m > 1024 || dsc.dim % dsc.m != 0 || K != 16))
+            invalid("fast_scan needs 16 centroids per subspace and an even num_subspaces <= 1024 dividing dim");
+        if (dsc.storage == QANN_IVF_PQ && (dsc.m == 0 || dsc.dim % dsc.m != 0 || K > 256))
+            invalid("PQ needs num_subspaces dividing dim and at most 256 centroids per subspace");
+        if (dsc.storage == QANN_IVF_SQ && ((dsc.sq_bits != 4 && dsc.sq_bits != 8) || !dsc.sq_vmin || !dsc.sq_step))
+            invalid("SQ needs 4 or 8 bits and its ranges");
+        if (!dsc.centroids || (pq && !dsc.pq_centroids)) invalid("centroids (and the PQ codebook with PQ) are required");
         auto h = std::make_unique<qann_hip_index>();
         check(hipGetDevice(&h->device), "hipGetDevice");
         h->kind = 2;
@@ -588,11 +694,18 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
         ix.dim = static_cast<int>(dsc.dim);
         ix.nlist = static_cast<int>(dsc.nlist);
         ix.metric = static_cast<int>(dsc.metric);
-        ix.m = static_cast<int>(dsc.m);
+        ix.storage = static_cast<int>(dsc.storage);
+        ix.m = pq ? static_cast<int>(dsc.m) : 0;
         ix.m4 = (ix.m + 3) & ~3;
-        ix.sub = static_cast<int>(dsc.dim / dsc.m);
+        ix.K = pq ? static_cast<int>(K) : 0;
+        ix.sub = pq ? static_cast<int>(dsc.dim / dsc.m) : 0;
         ix.cosine_ip = dsc.cosine_ip ? 1 : 0;
-        const size_t d = dsc.dim, nl = dsc.nlist, m = dsc.m, sub = d / m;
+        ix.sq_bits = dsc.storage == QANN_IVF_SQ ? static_cast<int>(dsc.sq_bits) : 0;
+        ix.vbytes = dsc.storage == QANN_IVF_PQ     ? static_cast<int>(dsc.m)
+                    : dsc.storage == QANN_IVF_SQ    ? static_cast<int>(dsc.sq_bits == 8 ? dsc.dim : (dsc.dim + 1) / 2)
+                    : dsc.storage == QANN_IVF_FLOAT ? static_cast<int>(dsc.dim * 4)
+                                                    : 0;
+        const size_t d = dsc.dim, nl = dsc.nlist, m = static_cast<size_t>(ix.m), sub = static_cast<size_t>(ix.sub);
         hipStream_t st = nullptr;
         float* cent = h->upload(dsc.centroids, nl * d);
         ix.centroids = cent;
@@ -607,17 +720,23 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
             ix.residual_centers = rc;
             ix.residual_sqnorms = rsq;
         }
-        float* pqc = h->upload(dsc.pq_centroids, m * 16 * sub);
-        float* pqt = h->alloc<float>(m * 16 * sub);
-        qd::launch_pq_transpose(pqc, static_cast<int>(m), static_cast<int>(sub), pqt, st);
-        ix.pq_cent_t = pqt;
-        if (dsc.use_precomputed) {
-            float* B = h->alloc<float>(nl * m * 16);
-            const float* centers = ix.residual_centers ? ix.residual_centers : ix.centroids;
-            qd::launch_list_term(centers, pqt, static_cast<int>(nl), static_cast<int>(d), static_cast<int>(m),
-                                 static_cast<int>(sub), B, st);
-            ix.B = B;
-            h->use_pre = true;
+        if (pq) {
+            float* pqc = h->upload(dsc.pq_centroids, m * K * sub);
+            float* pqt = h->alloc<float>(m * K * sub);
+            qd::launch_pq_transpose(pqc, static_cast<int>(m), static_cast<int>(K), static_cast<int>(sub), pqt, st);
+            ix.pq_cent_t = pqt;
+            if (dsc.use_precomputed) {
+                float* B = h->alloc<float>(nl * m * K);
+                const float* centers = ix.residual_centers ? ix.residual_centers : ix.centroids;
+                qd::launch_list_term(centers, pqt, static_cast<int>(nl), static_cast<int>(d), static_cast<int>(m),
+                                     static_cast<int>(sub), static_cast<int>(K), B, st);
+                ix.B = B;
+                h->use_pre = true;
+            }
+        }
+        if (dsc.storage == QANN_IVF_SQ) {
+            ix.sq_vmin = h->upload(dsc.sq_vmin, d);
+            ix.sq_step = h->upload(dsc.sq_step, d);
         }
         if (dsc.rotation) ix.rotation = h->upload(dsc.rotation, d * d);
         check_launch("index setup kernels");
@@ -646,19 +765,28 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
             src_bytes += nblk * bb;
         }
         off[nl] = static_cast<uint32_t>(total);
-        // the device block layout: word (subspace s, column g) of a block at
-        // (s / 4) * 64 + g * 16 + (s % 4) * 4; subspaces m .. m4 - 1 stay zero
-        std::vector<uint8_t> packed(std::max<uint64_t>(cbytes, 1), 0);
-        for (size_t l = 0; l < nl; ++l) {
-            const uint64_t nblk = (sizes[l] + 31) / 32;
-            for (uint64_t b = 0; b < nblk; ++b) {
-                const uint8_t* sb = codes + src_off[l] + b * bb;
-                uint8_t* db = packed.data() + coff[l] + b * lb;
-                for (size_t sp = 0; sp < m; ++sp)
-                    for (size_t g = 0; g < 4; ++g) std::memcpy(db + (sp / 4) * 64 + g * 16 + (sp % 4) * 4, sb + sp * 16 + 4 * g, 4);
+        std::vector<uint8_t> packed;
+        if (ix.storage == QANN_IVF_FASTSCAN) {
+            // the device block layout: word (subspace s, column g) of a block at
+            // (s / 4) * 64 + g * 16 + (s % 4) * 4; subspaces m .. m4 - 1 stay zero
+            packed.assign(std::max<uint64_t>(cbytes, 1), 0);
+            for (size_t l = 0; l < nl; ++l) {
+                const uint64_t nblk = (sizes[l] + 31) / 32;
+                for (uint64_t b = 0; b < nblk; ++b) {
+                    const uint8_t* sb = codes + src_off[l] + b * bb;
+                    uint8_t* db = packed.data() + coff[l] + b * lb;
+                    for (size_t sp = 0; sp < m; ++sp)
+                        for (size_t g = 0; g < 4; ++g)
+                            std::memcpy(db + (sp / 4) * 64 + g * 16 + (sp % 4) * 4, sb + sp * 16 + 4 * g, 4);
+                }
             }
-        }
-        h->free_ptr(const_cast<uint32_t*>(ix.list_off));
+        } else {
+            // PQ / SQ codes or float rows: vbytes per entry, in entry order
+            const size_t vb = static_cast<size_t>(ix.vbytes);
+            packed.assign(std::max<uint64_t>(total * vb, 4), 0);
+            if (total) std::memcpy(packed.data(), codes, total * vb);
+            std::fill(coff.begin(), coff.end(), 0);
+        }        h->free_ptr(const_cast<uint32_t*>(ix.list_off));
         h->free_ptr(const_cast<uint64_t*>(ix.code_off));
         h->free_ptr(const_cast<uint32_t*>(ix.list_ids));
         h->free_ptr(const_cast<uint8_t*>(ix.codes));
@@ -666,6 +794,19 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
         ix.code_off = h->upload(coff.data(), nl);
         ix.list_ids = total ? h->upload(slots, total) : h->alloc<uint32_t>(1);
         ix.codes = h->upload(packed.data(), packed.size());
+        h->free_ptr(const_cast<uint32_t*>(ix.entry_of_slot));
+        ix.entry_of_slot = nullptr;
+        if (ix.storage == QANN_IVF_SQ || ix.storage == QANN_IVF_FLOAT) {
+            // slot -> entry, for scans over a filter's slots (IVFIndex::direct_scan, ivf_index.cpp:248-256)
+            uint32_t top = 0;
+            for (uint64_t e = 0; e < total; ++e)
+                if (slots[e] != 0xFFFFFFFFu) top = std::max(top, slots[e] + 1);
+            std::vector<uint32_t> eos(std::max<uint32_t>(top, 1), 0xFFFFFFFFu);
+            for (uint64_t e = 0; e < total; ++e)
+                if (slots[e] != 0xFFFFFFFFu) eos[slots[e]] = static_cast<uint32_t>(e);
+            ix.entry_of_slot = h->upload(eos.data(), eos.size());
+            h->entry_of_slot_host = std::move(eos);
+        }
         h->host_list_sizes.assign(sizes, sizes + nl);
         std::vector<uint64_t> nb(nl);
         for (size_t l = 0; l < nl; ++l) nb[l] = (sizes[l] + 31) / 32;
@@ -697,7 +838,7 @@ int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint6
 
 int qann_hip_refine_create(qann_hip_index* base, const float* rows, uint64_t slots, qann_hip_index** out) {
     return guarded([&] {
-        if (!base || base->kind != 2) invalid("the refine base must be an IVF fast-scan index");
+        if (!base || base->kind != 2) invalid("the refine base must be an IVF index");
         auto h = std::make_unique<qann_hip_index>();
         check(hipGetDevice(&h->device), "hipGetDevice");
         h->kind = 3;
@@ -726,6 +867,13 @@ int qann_hip_set_ids(qann_hip_index* h, uint64_t slots, const int64_t* labels, c
         if (h->kind == 2) {
             h->ivf.labels = h->labels;
             h->ivf.deleted = set ? h->deleted : nullptr;
+            auto& eos = h->entry_of_slot_host;
+            if (h->ivf.entry_of_slot && eos.size() < slots) {
+                // every slot gets an entry (0xFFFFFFFF: in no list)
+                eos.resize(slots, 0xFFFFFFFFu);
+                h->free_ptr(const_cast<uint32_t*>(h->ivf.entry_of_slot));
+                h->ivf.entry_of_slot = h->upload(eos.data(), eos.size());
+            }
         }
     });
 }

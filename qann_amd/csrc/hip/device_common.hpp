@@ -124,6 +124,57 @@ __device__ __forceinline__ float exact_score(bool ip, const float* a, const floa
     return ip ? -exact_avx2<true>(a, b, n) : exact_avx2<false>(a, b, n);
 }
 
+// SQ::score (src/sq.cpp:126-211) as the AVX2 build computes it: each value
+// decoded as fma(code, step, vmin), 8 lane accumulators fed by fma (the
+// difference squared, or q times the value), folded as ((l0+l4)+(l2+l6)) +
+// ((l1+l5)+(l3+l7)), then the scalar tail, fused. 4-bit codes take 16
+// dimensions per step of the vector loop.
+template <bool IP>
+__device__ __forceinline__ float sq_score(const float* q, const uint8_t* code, const float* vmin, const float* step,
+                                          int n, int bits) {
+    auto code_at = [&](int d) -> float {
+        return bits == 8 ? static_cast<float>(code[d]) : static_cast<float>((code[d >> 1] >> ((d & 1) * 4)) & 0x0F);
+    };
+    float acc[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) acc[l] = 0.0f;
+    int d = 0;
+    auto step8 = [&](int at) {
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            const float v = fmaf(code_at(at + l), step[at + l], vmin[at + l]);
+            if (IP) {
+                acc[l] = fmaf(q[at + l], v, acc[l]);
+            } else {
+                const float diff = q[at + l] - v;
+                acc[l] = fmaf(diff, diff, acc[l]);
+            }
+        }
+    };
+    if (bits == 8) {
+        for (; d + 8 <= n; d += 8) step8(d);
+    } else {
+        for (; d + 16 <= n; d += 16) {
+            step8(d);
+            step8(d + 8);
+        }
+    }
+    float h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = acc[j] + acc[j + 4];
+    float sum = (h[0] + h[2]) + (h[1] + h[3]);
+    for (; d < n; ++d) {
+        const float v = fmaf(code_at(d), step[d], vmin[d]);
+        if (IP) {
+            sum = fmaf(q[d], v, sum);
+        } else {
+            const float diff = q[d] - v;
+            sum = fmaf(diff, diff, sum);
+        }
+    }
+    return sum;
+}
+
 // Filter bit of a slot (detail::SlotFilter::test, id_map.hpp:27).
 __device__ __forceinline__ bool bit_test(const uint64_t* words, uint32_t s) {
     return (words[s >> 6] >> (s & 63)) & 1ull;

@@ -44,8 +44,8 @@ __global__ void k_pq_transpose(const float* cent, int m, int K, int sub, float* 
 }
 
 // B[l][s][c] = sum_j y_scj * (y_scj + 2 c_l,sj), fused (compute_list_term, src/pq.cpp:206-223).
-__global__ void k_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B) {
-    const int K = 16;
+__global__ void k_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, int K,
+                            float* B) {
     const int tiles = (m * K + blockDim.x - 1) / blockDim.x;
     const int e = (blockIdx.x % tiles) * blockDim.x + threadIdx.x; // (s, c) entry
     const int l = blockIdx.x / tiles;
@@ -468,19 +468,18 @@ __device__ void block_quantize(const float* table, int m, float* row_min, uint8_
     __syncthreads();
 }
 
-// Per query: C = compute_inner_table(q) for precomputed L2 tables, or for
-// inner-product / cosine-IP scoring the same table (halved for Metric::IP)
-// quantized once (scan, src/ivf_index.cpp:448-456).
-__global__ void k_query_tables(const float* q, const DevIvf ix, int want_lut, float* C_out, uint8_t* lut_out,
+// Per query: C = compute_inner_table(q) (src/pq.cpp:186-204) for precomputed
+// L2 tables, the inner-product scoring table (C, halved for
+// Metric::InnerProduct), or that table rounded to fast-scan bytes once per
+// query (scan, src/ivf_index.cpp:448-456).
+__global__ void k_query_tables(const float* q, const DevIvf ix, int mode, float* C_out, uint8_t* lut_out,
                                float* scale_out, float* offset_out) {
     extern __shared__ float smem[];
-    const int m = ix.m, sub = ix.sub, K = 16;
-    float* table = smem;                  // m * 16
-    float* row_min = table + m * K;       // m
-    float* red = row_min + m;             // 16
+    const int m = ix.m, sub = ix.sub, K = ix.K;
     const int i = blockIdx.x;
     const float* qi = q + static_cast<size_t>(i) * ix.dim;
-    const bool half = ix.metric == 2;
+    const bool half = mode != kTableInner && ix.metric == 2;
+    float* table = smem; // m * 16 (LUT mode)
     for (int e = threadIdx.x; e < m * K; e += blockDim.x) {
         const int s = e / K, c = e % K;
         float acc = 0.0f;
@@ -488,13 +487,15 @@ __global__ void k_query_tables(const float* q, const DevIvf ix, int want_lut, fl
             const float qd = qi[s * sub + j];
             acc = fmaf(-2.0f * qd, ix.pq_cent_t[(static_cast<size_t>(s) * sub + j) * K + c], acc);
         }
-        if (want_lut && half) acc *= 0.5f;
-        table[e] = acc;
-        if (C_out) C_out[static_cast<size_t>(i) * m * K + e] = acc;
+        if (half) acc *= 0.5f;
+        if (mode == kTableIpLut) table[e] = acc;
+        else C_out[static_cast<size_t>(i) * m * K + e] = acc;
     }
+    if (mode != kTableIpLut) return;
     __syncthreads();
-    if (!want_lut) return;
-    block_quantize(table, m, row_min, lut_out + static_cast<size_t>(i) * m * K, scale_out + i, offset_out + i, red);
+    float* row_min = table + m * 16; // m
+    float* red = row_min + m;        // 16
+    block_quantize(table, m, row_min, lut_out + static_cast<size_t>(i) * m * 16, scale_out + i, offset_out + i, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -858,6 +859,149 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
     if (tid < np) a.item_min[item0 + tid] = w_min[tid];
 }
 
+// Phase 1 for lists stored as PQ codes (ADC), SQ codes or raw vectors: one
+// workgroup per (query, window of consecutive probes), one thread per vector.
+// The distance of each vector is what scan pushes for it
+// (src/ivf_index.cpp:552-611): the ADC sum over subspaces of the (query, list)
+// table, in subspace order (PQCodebook::distances_adc, include/vecengine/
+// pq.hpp:75-98), plus the list's base for inner product; SQ::distance /
+// -SQ::inner_product; or exact_score on the raw row. A vector the query may
+// not return (removed, filtered out, deleted) gets +inf, which never passes
+// TopK's `dist < threshold`. Each block's and each probe's smallest distance
+// (as fkey) let the replay skip what cannot be pushed.
+__global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t w_start[kSumsWin + 1];
+    __shared__ int w_list[kSumsWin];
+    __shared__ uint32_t w_min[kSumsWin];
+    __shared__ float w_base[kSumsWin];
+    const DevIvf& ix = a.ix;
+    const int m = ix.m, K = ix.K, dim = ix.dim, win = a.win, mode = a.mode;
+    const int qi = blockIdx.x / a.nwin, p0 = (blockIdx.x % a.nwin) * win;
+    const int P = a.nprobe_q[qi];
+    if (p0 >= P) return;
+    const int np = min(win, P - p0);
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
+    const bool adc = mode <= kDistL2Resid;
+    const size_t tsz = static_cast<size_t>(m) * K; // table floats
+    float* qs = reinterpret_cast<float*>(lds);                      // dim
+    float* sqv = qs + dim;                                          // SQ: vmin, step (2 dim)
+    float* tables = sqv + (mode == kDistSq ? 2 * dim : 0);          // ADC: 1 (inner product) or win tables
+    float* resid = tables + (adc ? (mode == kDistIp ? 1 : win) * tsz : 0); // kDistL2Resid: a residual per wave
+    const float* q = a.q + static_cast<size_t>(qi) * dim;
+    for (int t = tid; t < dim; t += blockDim.x) qs[t] = q[t];
+    if (mode == kDistSq)
+        for (int t = tid; t < dim; t += blockDim.x) {
+            sqv[t] = ix.sq_vmin[t];
+            sqv[dim + t] = ix.sq_step[t];
+        }
+    if (mode == kDistIp)
+        for (size_t e = tid; e < tsz; e += blockDim.x) tables[e] = a.C[static_cast<size_t>(qi) * tsz + e];
+    const uint32_t blk0 = a.item_blk[item0];
+    if (tid < np) {
+        const int list = a.probe_idx[item0 + tid];
+        w_list[tid] = list;
+        w_min[tid] = 0xFFFFFFFFu;
+        w_base[tid] = a.probe_base[item0 + tid];
+        const uint32_t rel = a.item_blk[item0 + tid] - blk0;
+        w_start[tid] = rel;
+        if (tid == np - 1) w_start[np] = rel + (ix.list_off[list + 1] - ix.list_off[list] + kFsBlock - 1) / kFsBlock;
+    }
+    __syncthreads();
+    if (mode == kDistL2Pre || mode == kDistL2Resid) {
+        // the (query, list) L2 table: B + C (+ A on row 0), or compute_adc_table of q - c
+        for (int p = wave; p < np; p += kSumsWaves) {
+            if (w_start[p + 1] == w_start[p]) continue;
+            const int list = w_list[p];
+            float* tb = tables + static_cast<size_t>(p) * tsz;
+            if (mode == kDistL2Pre) {
+                const float* B = ix.B + static_cast<size_t>(list) * tsz;
+                const float* Cq = a.C + static_cast<size_t>(qi) * tsz;
+                const float A = a.probe_A[item0 + p];
+                for (size_t e = lane; e < tsz; e += kWave) {
+                    float t = B[e] + Cq[e];
+                    if (e < static_cast<size_t>(K)) t = t + A;
+                    tb[e] = t;
+                }
+            } else {
+                const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * dim;
+                float* r = a.resid_in_lds ? resid + wave * dim : nullptr;
+                if (r) {
+                    for (int t = lane; t < dim; t += kWave) r[t] = qs[t] - c[t];
+                    wave_lds_sync();
+                }
+                for (size_t e = lane; e < tsz; e += kWave) {
+                    const int sidx = static_cast<int>(e / K), cc = static_cast<int>(e % K);
+                    float acc = 0.0f;
+                    for (int j = 0; j < ix.sub; ++j) {
+                        const int x = sidx * ix.sub + j;
+                        const float rs = r ? r[x] : qs[x] - c[x];
+                        const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
+                        acc = fmaf(d, d, acc);
+                    }
+                    tb[e] = acc;
+                }
+                wave_lds_sync();
+            }
+        }
+        __syncthreads();
+    }
+
+    const uint32_t T = w_start[np];
+    float* qd = a.dists + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
+    uint32_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock) + blk0;
+    const int v = tid & 31;
+    for (uint32_t base = 0; base < T; base += blockDim.x / kFsBlock) {
+        const uint32_t t = base + (tid >> 5);
+        uint32_t key = 0xFFFFFFFFu;
+        int p = 0;
+        if (t < T) {
+#pragma unroll
+            for (int step = kSumsWin / 2; step > 0; step >>= 1) {
+                const int c = min(p + step, np - 1);
+                p = w_start[c] <= t ? c : p;
+            }
+            const int list = w_list[p];
+            const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+            const uint32_t j = (t - w_start[p]) * kFsBlock + v;
+            float dist = INFINITY;
+            if (j < n) {
+                const uint32_t entry = ix.list_off[list] + j;
+                const uint32_t slot = ix.list_ids[entry];
+                const bool ok = slot != 0xFFFFFFFFu &&
+                                (a.filter ? bit_test(a.filter, slot) : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                if (ok) {
+                    if (adc) {
+                        const uint8_t* code = ix.codes + static_cast<size_t>(entry) * m;
+                        const float* tb = tables + (mode == kDistIp ? 0 : static_cast<size_t>(p) * tsz);
+                        float d = 0.0f;
+                        for (int sidx = 0; sidx < m; ++sidx) d += tb[sidx * K + code[sidx]];
+                        dist = mode == kDistIp ? d + w_base[p] : d;
+                    } else if (mode == kDistSq) {
+                        const uint8_t* code = ix.codes + static_cast<size_t>(entry) * ix.vbytes;
+                        dist = ix.metric == 2 ? -sq_score<true>(qs, code, sqv, sqv + dim, dim, ix.sq_bits)
+                                              : sq_score<false>(qs, code, sqv, sqv + dim, dim, ix.sq_bits);
+                    } else {
+                        const float* row = reinterpret_cast<const float*>(ix.codes) + static_cast<size_t>(entry) * dim;
+                        dist = exact_score(ix.metric == 2, qs, row, dim);
+                    }
+                    key = fkey(dist);
+                }
+            }
+            qd[static_cast<size_t>(t) * kFsBlock + v] = dist;
+        }
+        // the block's 32 threads: their smallest key
+        for (int off = 16; off > 0; off >>= 1) key = min(key, static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), off, 32)));
+        if (t < T && v == 0) {
+            qbmin[t] = key;
+            atomicMin(&w_min[p], key);
+        }
+    }
+    __syncthreads();
+    if (tid < np) a.item_min[item0 + tid] = w_min[tid];
+}
+
 // Probes each query scans and where each probe's sums start: nprobe lists
 // unfiltered; with a filter, at least min_probes and then until the lists
 // scanned hold `wanted` allowed vectors (scan, ivf_index.cpp:510-521: before
@@ -905,15 +1049,19 @@ __global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t
     }
 }
 
-// Phase 2: one wave per query replays the reference's scan over the sums in
-// probe order -- limit16 from the threshold at each block (scan_fast,
-// ivf_index.cpp:482-485), survivors in bit order through push (:539-543) into
-// the TopK (src/topk.hpp), trims when its buffer fills. The probes' metadata
-// is read 64 at a time, one probe per lane. A list whose smallest sum cannot
-// pass the current limit is skipped whole, and within a list only blocks whose
-// smallest sum can pass it are read (the limit only falls). The TopK holds
-// slots; they become ids (labels) for the final sort.
-__global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
+// Phase 2: one wave per query replays the reference's scan in probe order.
+// Fast-scan (FL false): over the u16 sums -- limit16 from the threshold at
+// each block (scan_fast, ivf_index.cpp:482-485), survivors in bit order
+// through push (:539-543). PQ / SQ / float lists (FL true): over the float
+// distances, each pushed in list order when below the threshold (:552-611).
+// Either way into the TopK (src/topk.hpp), trimmed when its buffer fills.
+// The probes' metadata is read 64 at a time, one probe per lane. A list whose
+// smallest sum (distance) cannot pass the current limit (threshold) is
+// skipped whole, and within a list only blocks whose smallest can are read
+// (the threshold only falls). The TopK holds slots; they become ids (labels)
+// for the final sort.
+template <bool FL>
+__device__ void fs_replay(const ReplayArgs& a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevIvf& ix = a.ix;
     const int kp = a.kp;
@@ -925,8 +1073,8 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
     const bool filtered = a.filter != nullptr;
     WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY};
     const int P = a.nprobe_q[qi];
-    const uint16_t* qsums = a.sums + static_cast<size_t>(qi) * a.stride_slots;
-    const uint16_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock);
+    const size_t qoff = static_cast<size_t>(qi) * a.stride_slots;
+    const size_t qboff = static_cast<size_t>(qi) * (a.stride_slots / kFsBlock);
     for (int c0 = 0; c0 < P; c0 += kWave) {
         int m_list = 0;
         uint32_t m_n = 0, m_blk = 0, m_min = 0xFFFFFFFFu;
@@ -937,8 +1085,10 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
             m_n = ix.list_off[m_list + 1] - ix.list_off[m_list];
             m_blk = a.item_blk[item];
             m_min = a.item_min[item];
-            m_scale = a.item_scale[item];
-            m_off = a.item_offset[item];
+            if (!FL) {
+                m_scale = a.item_scale[item];
+                m_off = a.item_offset[item];
+            }
         }
         const int cn = min(kWave, P - c0);
         for (int j = 0; j < cn; ++j) {
@@ -946,7 +1096,9 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
             if (n == 0) continue;
             const float scale = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_scale), j));
             const float offset = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_off), j));
+            // the bound a block's or list's smallest must be below to hold a push
             auto limit_now = [&]() -> uint32_t {
+                if (FL) return fkey(tk.thr);
                 const float limit = (tk.thr - offset) * scale;
                 return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
             };
@@ -954,11 +1106,12 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
             const int list = __builtin_amdgcn_readlane(m_list, j);
             const uint32_t blk0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_blk), j));
             const uint32_t* ids = ix.list_ids + ix.list_off[list];
-            const uint16_t* sums = qsums + static_cast<size_t>(blk0) * kFsBlock;
+            const size_t voff = qoff + static_cast<size_t>(blk0) * kFsBlock;
             const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
             for (uint32_t bc = 0; bc < nb; bc += kWave) {
                 const uint32_t bl = bc + lane;
-                const uint32_t bm = bl < nb ? qbmin[blk0 + bl] : 0xFFFFu;
+                uint32_t bm = 0xFFFFFFFFu;
+                if (bl < nb) bm = FL ? a.fbmin[qboff + blk0 + bl] : a.bmin[qboff + blk0 + bl];
                 uint64_t cand = ballot(bm < limit_now());
                 while (cand) {
                     // the next two candidate blocks: lanes 0-31 and 32-63
@@ -973,26 +1126,39 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
                     const int v = lane & 31;
                     const uint32_t e = (bc + static_cast<uint32_t>(mb)) * kFsBlock + v;
                     const bool in = mb >= 0 && e < n;
-                    const uint32_t sum = in ? sums[e] : 0xFFFFu;
-                    if (ballot(in && sum < limit_now()) == 0) continue;
+                    uint32_t sum = 0xFFFFu;
+                    float fd = INFINITY;
+                    if (in) {
+                        if (FL) fd = a.fdists[voff + e];
+                        else sum = a.sums[voff + e];
+                    }
+                    auto passes_now = [&]() -> bool { return FL ? fd < tk.thr : sum < limit_now(); };
+                    if (ballot(in && passes_now()) == 0) continue;
                     for (int h = 0; h < 2; ++h) {
                         if (h == 1 && b2 < 0) break;
                         const bool mine = (lane >> 5) == h;
                         const bool valid = mine && in;
                         uint32_t slot = 0;
                         bool allowed = valid;
-                        if (a.block_filter) {
+                        if (!FL && a.block_filter) {
                             slot = valid ? ids[e] : 0u;
                             allowed = valid && bit_test(a.filter, slot);
                             if (ballot(allowed) == 0) continue;
                         }
-                        const bool surv = allowed && sum < limit_now();
+                        const bool surv = allowed && passes_now();
                         uint64_t rem = ballot(surv);
                         if (rem == 0) continue;
-                        if (!a.block_filter && surv) slot = ids[e];
-                        const float dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
-                        const bool ok = surv && (filtered ? bit_test(a.filter, slot)
-                                                          : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                        if ((FL || !a.block_filter) && surv) slot = ids[e];
+                        float dist;
+                        bool ok;
+                        if (FL) {
+                            dist = fd; // +inf already marks what may not be returned
+                            ok = surv;
+                        } else {
+                            dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                            ok = surv && (filtered ? bit_test(a.filter, slot)
+                                                   : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                        }
                         while (rem) {
                             const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
                             const uint64_t cm = ballot(cond);
@@ -1089,6 +1255,9 @@ __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) {
     if (lane == 0) a.out_count[qi] = n;
 }
 
+__global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) { fs_replay<false>(a); }
+__global__ void __launch_bounds__(64) k_fl_replay(const ReplayArgs a) { fs_replay<true>(a); }
+
 // ---------------------------------------------------------------------------
 // Exact re-rank (RefineIndex::rerank): the candidates' exact distances, the k
 // smallest by (distance, slot) in pair order, mapped to the refine labels.
@@ -1136,7 +1305,19 @@ __global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip,
 // order (fkey). Candidates come from a slot list, or all slots minus the
 // deleted ones. Output sorted by (key32, label) -- the final order.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, int ip, const float* rows, const uint32_t* slot_list,
+// The score of one slot under a SlotScorer (exact_score on its row, or SQ).
+__device__ __forceinline__ float slot_score(const SlotScorer& sc, bool ip, const float* q, uint32_t slot, int dim) {
+    const uint32_t e = sc.entry_of_slot ? sc.entry_of_slot[slot] : slot;
+    if (e == 0xFFFFFFFFu) return INFINITY; // a slot in no list
+    if (sc.sq_codes) {
+        const uint8_t* code = sc.sq_codes + static_cast<size_t>(e) * sc.vbytes;
+        return ip ? -sq_score<true>(q, code, sc.sq_vmin, sc.sq_step, dim, sc.sq_bits)
+                  : sq_score<false>(q, code, sc.sq_vmin, sc.sq_step, dim, sc.sq_bits);
+    }
+    return exact_score(ip, q, sc.rows + static_cast<size_t>(e) * dim, dim);
+}
+
+__global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, int ip, const SlotScorer sc, const uint32_t* slot_list,
                                uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
                                int cap, const int64_t* labels, int64_t* out_ids, float* out_dists,
                                unsigned char* gbuf, size_t gstride) {
@@ -1171,7 +1352,7 @@ __global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, i
         if (c < n_slots) {
             const uint32_t slot = slot_list ? slot_list[c] : c;
             if (!deleted || !bit_test(deleted, slot)) {
-                const float d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
+                const float d = slot_score(sc, ip != 0, sq, slot, dim);
                 const uint64_t key = (static_cast<uint64_t>(use_fkey ? fkey(d) : ckey(d)) << 32) | slot;
                 if (key < thr_s) keys[atomicAdd(&count_s, 1)] = key;
             }
@@ -1235,7 +1416,7 @@ __global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, i
             float d = INFINITY;
             if (t < keep) {
                 id = lab[t];
-                d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slt[t]) * dim, dim);
+                d = slot_score(sc, ip != 0, sq, slt[t], dim);
             }
             out_ids[static_cast<size_t>(i) * k + t] = id;
             out_dists[static_cast<size_t>(i) * k + t] = d;
@@ -1248,7 +1429,7 @@ __global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, i
         if (t < keep) {
             const uint32_t slot = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
             id = labels ? labels[slot] : static_cast<int64_t>(slot);
-            d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
+            d = slot_score(sc, ip != 0, sq, slot, dim);
         }
         out_ids[static_cast<size_t>(i) * k + t] = id;
         out_dists[static_cast<size_t>(i) * k + t] = d;
@@ -1269,15 +1450,15 @@ void launch_sqnorms(const float* rows, int n, int dim, float* out, hipStream_t s
     hipLaunchKernelGGL(k_sqnorms, dim3(cdiv(n, 256)), dim3(256), 0, s, rows, n, dim, out);
 }
 
-void launch_pq_transpose(const float* cent, int m, int sub, float* cent_t, hipStream_t s) {
-    hipLaunchKernelGGL(k_pq_transpose, dim3(cdiv(static_cast<size_t>(m) * 16 * sub, 256)), dim3(256), 0, s, cent, m,
-                       16, sub, cent_t);
+void launch_pq_transpose(const float* cent, int m, int K, int sub, float* cent_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_pq_transpose, dim3(cdiv(static_cast<size_t>(m) * K * sub, 256)), dim3(256), 0, s, cent, m,
+                       K, sub, cent_t);
 }
 
-void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B,
+void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, int K, float* B,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_list_term, dim3(cdiv(static_cast<size_t>(m) * 16, 256) * static_cast<unsigned>(nlist)),
-                       dim3(256), 0, s, centers, cent_t, nlist, dim, m, sub, B);
+    hipLaunchKernelGGL(k_list_term, dim3(cdiv(static_cast<size_t>(m) * K, 256) * static_cast<unsigned>(nlist)),
+                       dim3(256), 0, s, centers, cent_t, nlist, dim, m, sub, K, B);
 }
 
 void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, hipStream_t s) {
@@ -1316,10 +1497,10 @@ void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const ui
                        val, A, base);
 }
 
-void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
+void launch_query_tables(const float* q, const DevIvf& ix, int nq, int mode, float* C, uint8_t* lut, float* scale,
                          float* offset, hipStream_t s) {
-    const size_t lds = (static_cast<size_t>(ix.m) * 16 + ix.m + 16) * 4;
-    hipLaunchKernelGGL(k_query_tables, dim3(nq), dim3(256), lds, s, q, ix, want_lut, C, lut, scale, offset);
+    const size_t lds = mode == kTableIpLut ? (static_cast<size_t>(ix.m) * 16 + ix.m + 16) * 4 : 0;
+    hipLaunchKernelGGL(k_query_tables, dim3(nq), dim3(256), lds, s, q, ix, mode, C, lut, scale, offset);
 }
 
 size_t fs_sums_lds(SumsArgs& a) {
@@ -1347,6 +1528,34 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
                        dim3(kSumsWaves * kWave), lds, s, a);
 }
 
+size_t ivf_dists_lds(DistArgs& a) {
+    const DevIvf& ix = a.ix;
+    const size_t dim4 = static_cast<size_t>(ix.dim) * 4;
+    const size_t tbytes = static_cast<size_t>(ix.m) * ix.K * 4;
+    size_t fixed = dim4 + (a.mode == kDistSq ? 2 * dim4 : 0);
+    a.resid_in_lds = 0;
+    a.win = kSumsWin;
+    if (a.mode == kDistIp) fixed += tbytes;
+    if (a.mode == kDistL2Resid && fixed + kSumsWaves * dim4 + tbytes <= kLdsMax) {
+        a.resid_in_lds = 1;
+        fixed += kSumsWaves * dim4;
+    }
+    if (a.mode == kDistL2Pre || a.mode == kDistL2Resid) {
+        if (fixed + tbytes > kLdsMax) return 0;
+        a.win = static_cast<int>(std::min<size_t>(kSumsWin, (kLdsMax - fixed) / tbytes));
+        fixed += static_cast<size_t>(a.win) * tbytes;
+    }
+    if (fixed > kLdsMax) return 0;
+    a.nwin = (a.M + a.win - 1) / a.win;
+    return fixed;
+}
+
+void launch_ivf_dists(DistArgs a, int nq, hipStream_t s) {
+    const size_t lds = ivf_dists_lds(a);
+    hipLaunchKernelGGL(k_ivf_dists, dim3(static_cast<unsigned>(static_cast<size_t>(nq) * a.nwin)),
+                       dim3(kSumsWaves * kWave), lds, s, a);
+}
+
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
                        const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
                        int* status, hipStream_t s) {
@@ -1360,7 +1569,8 @@ void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out
 
 void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
     const size_t lds = a.g_dist ? 0 : static_cast<size_t>(2 * a.kp) * 12;
-    hipLaunchKernelGGL(k_fs_replay, dim3(nq), dim3(kWave), lds, s, a);
+    if (a.fdists) hipLaunchKernelGGL(k_fl_replay, dim3(nq), dim3(kWave), lds, s, a);
+    else hipLaunchKernelGGL(k_fs_replay, dim3(nq), dim3(kWave), lds, s, a);
 }
 
 size_t rerank_keys(int kc) {
@@ -1389,7 +1599,7 @@ size_t block_select_bytes(int k, bool by_label) {
     return (b + 255) & ~size_t{255};
 }
 
-void launch_block_select(const float* q, int nq, int dim, int ip, const float* rows, const uint32_t* slot_list,
+void launch_block_select(const float* q, int nq, int dim, int ip, const SlotScorer& sc, const uint32_t* slot_list,
                          uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
                          const int64_t* labels, int64_t* out_ids, float* out_dists, unsigned char* gbuf,
                          hipStream_t s) {
@@ -1398,7 +1608,7 @@ void launch_block_select(const float* q, int nq, int dim, int ip, const float* r
     while (cap < k + threads) cap <<= 1;
     const size_t buf = block_select_bytes(k, final_by_label && labels);
     const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + (gbuf ? 0 : buf);
-    hipLaunchKernelGGL(k_block_select, dim3(nq), dim3(threads), lds, s, q, dim, ip, rows, slot_list, n_slots, deleted,
+    hipLaunchKernelGGL(k_block_select, dim3(nq), dim3(threads), lds, s, q, dim, ip, sc, slot_list, n_slots, deleted,
                        use_fkey, final_by_label, k, cap, labels, out_ids, out_dists, gbuf, buf);
 }
 

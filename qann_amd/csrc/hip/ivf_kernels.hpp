@@ -20,18 +20,25 @@ constexpr int kSelectMax = 4096;
 // 4g..4g+3 of the reference's 16-byte rows of those subspaces (src/fast_scan.hpp:19-28).
 struct DevIvf {
     int dim, nlist, metric, m, m4, sub, cosine_ip;
+    int storage;                   // QANN_IVF_FASTSCAN / PQ / SQ / FLOAT (see qann_hip.h)
+    int K;                         // PQ centroids per subspace
+    int sq_bits;                   // SQ: 4 or 8
+    int vbytes;                    // bytes per entry in codes for PQ / SQ / FLOAT storage
+    const float* sq_vmin;          // dim, SQ only
+    const float* sq_step;          // dim, SQ only
+    const uint32_t* entry_of_slot; // slot -> list entry (list_off[l] + j), PQ-less storages only
     const float* centroids;        // nlist * dim
     const float* centroid_sqnorms; // nlist
     float cnorm_max;               // an upper bound on every centroid's norm
     const float* residual_centers; // nlist * dim or null
     const float* residual_sqnorms; // nlist or null
-    const float* pq_cent_t;        // m * sub * 16
-    const float* B;                // nlist * m * 16, or null (no precomputed tables)
+    const float* pq_cent_t;        // m * sub * K
+    const float* B;                // nlist * m * K, or null (no precomputed tables)
     const float* rotation;         // dim * dim or null
     const uint32_t* list_off;      // nlist + 1
     const uint64_t* code_off;      // nlist
     const uint32_t* list_ids;      // slots of every list entry
-    const uint8_t* codes;
+    const uint8_t* codes;          // fast-scan blocks (below), or vbytes per entry in entry order
     const int64_t* labels;         // slot -> id, or null for ids == slots
     const uint64_t* deleted;       // bitmap over slots, or null when none is deleted
 };
@@ -60,6 +67,28 @@ struct SumsArgs {
     int resid_in_lds;
 };
 
+// Phase 1 for PQ (ADC), SQ and float lists: the float distance of every
+// vector of every (query, probe), +inf for vectors a query may not return.
+struct DistArgs {
+    DevIvf ix;
+    int M, win, nwin;
+    int mode;                  // kDist* below
+    const float* q;
+    const uint16_t* probe_idx;
+    const float* probe_A;
+    const float* probe_base;
+    const int* nprobe_q;
+    const uint32_t* item_blk;
+    const float* C;            // nq * m * K: the query's inner table (kDistIp: the scoring table)
+    const uint64_t* filter;    // allowed slots or null
+    float* dists;              // query i's region starts at i * stride_slots
+    uint64_t stride_slots;
+    uint32_t* bmin;            // fkey of each block's smallest distance
+    uint32_t* item_min;        // fkey of each probe's smallest distance
+    int resid_in_lds;
+};
+enum { kDistIp = 0, kDistL2Pre = 1, kDistL2Resid = 2, kDistSq = 3, kDistFloat = 4 };
+
 // Phase 2: the reference's TopK replayed over the sums.
 struct ReplayArgs {
     DevIvf ix;
@@ -67,12 +96,14 @@ struct ReplayArgs {
     const uint16_t* probe_idx;
     const int* nprobe_q;
     const uint32_t* item_blk;
-    const uint16_t* sums;
+    const uint16_t* sums;      // fast-scan sums, or null when fdists is set
+    const float* fdists;       // float distances (PQ / SQ / float lists), or null
     uint64_t stride_slots;
-    const uint16_t* bmin;
+    const uint16_t* bmin;      // fast-scan: smallest sum per block
+    const uint32_t* fbmin;     // float: fkey of the smallest distance per block
     const float* item_scale;
     const float* item_offset;
-    const uint32_t* item_min;
+    const uint32_t* item_min;  // fast-scan: smallest sum; float: fkey of the smallest distance
     const uint64_t* filter;    // allowed slots bitmap or null
     float* out_dist;           // nq * kp
     int64_t* out_label;        // nq * kp
@@ -88,8 +119,8 @@ constexpr size_t kLdsMax = 64 * 1024;
 inline size_t replay_lds_bytes(int kp) { return static_cast<size_t>(2 * kp) * 12; }
 
 void launch_sqnorms(const float* rows, int n, int dim, float* out, hipStream_t s);
-void launch_pq_transpose(const float* cent, int m, int sub, float* cent_t, hipStream_t s);
-void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, float* B,
+void launch_pq_transpose(const float* cent, int m, int K, int sub, float* cent_t, hipStream_t s);
+void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, int K, float* B,
                       hipStream_t s);
 void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, hipStream_t s);
 // Coarse quantizer in two launches: approximate scores of every (query, list)
@@ -103,7 +134,11 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
                           int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s);
 void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
                        float* A, float* base, hipStream_t s);
-void launch_query_tables(const float* q, const DevIvf& ix, int nq, int want_lut, float* C, uint8_t* lut, float* scale,
+// Per query: kTableInner the inner table C (m * K), kTableIp the inner-product
+// scoring table (C, halved for Metric::InnerProduct), kTableIpLut that table
+// rounded to fast-scan bytes (K = 16) with its scale and offset.
+enum { kTableInner = 0, kTableIp = 1, kTableIpLut = 2 };
+void launch_query_tables(const float* q, const DevIvf& ix, int nq, int mode, float* C, uint8_t* lut, float* scale,
                          float* offset, hipStream_t s);
 // Dynamic LDS of k_fs_sums (0 when a table does not fit); sets win, nwin and the LDS flags.
 size_t fs_sums_lds(SumsArgs& a);
@@ -113,14 +148,28 @@ void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_prob
                        int* status, hipStream_t s);
 void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s);
 void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s);
+// Dynamic LDS of k_ivf_dists (0 when a table does not fit); sets win, nwin and resid_in_lds.
+size_t ivf_dists_lds(DistArgs& a);
+void launch_ivf_dists(DistArgs a, int nq, hipStream_t s);
 // Keys of one query's sort: in LDS, or gkeys (nq * rerank_keys(kc)) in HBM when too many.
 size_t rerank_keys(int kc);
 void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
                    const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
                    uint64_t* gkeys, hipStream_t s);
+// How k_block_select scores a slot: exact distance to its float row, found
+// at rows + slot * dim or through entry_of_slot (an IVF's float lists), or
+// the SQ score of its code (an IVF's SQ lists).
+struct SlotScorer {
+    const float* rows;
+    const uint32_t* entry_of_slot; // null: rows are indexed by slot
+    const uint8_t* sq_codes;       // non-null: SQ scoring of sq_codes + entry * vbytes
+    const float* sq_vmin;
+    const float* sq_step;
+    int sq_bits, vbytes;
+};
 // Bytes of one query's selection buffers; in HBM (gbuf, nq of them) when above kLdsMax.
 size_t block_select_bytes(int k, bool by_label);
-void launch_block_select(const float* q, int nq, int dim, int ip, const float* rows, const uint32_t* slot_list,
+void launch_block_select(const float* q, int nq, int dim, int ip, const SlotScorer& sc, const uint32_t* slot_list,
                          uint32_t n_slots, const uint64_t* deleted, int use_fkey, int final_by_label, int k,
                          const int64_t* labels, int64_t* out_ids, float* out_dists, unsigned char* gbuf,
                          hipStream_t s);

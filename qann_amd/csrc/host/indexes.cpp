@@ -633,28 +633,39 @@ qann_hip_index* IVFIndex::device() const {
     std::lock_guard<std::mutex> g(dev_mu);
     if (dev_) return dev_;
     if (!d_->trained) throw std::logic_error("train() has not be run yet");
-    if (!d_->has_pq || !d_->fast_scan || d_->sq_bits)
-        throw std::runtime_error("qann_amd searches IVF indexes with fast-scan PQ codes; this index stores " +
-                                 std::string(d_->sq_bits ? "SQ codes" : d_->has_pq ? "8-bit PQ codes" : "float vectors"));
+    if (d_->has_pq && d_->pq_distance != 0)
+        throw std::runtime_error("qann_amd does not search PQ indexes scored by SDC; this index uses SDC");
     qann_hip_ivf_desc desc{};
     desc.dim = static_cast<uint32_t>(d_->dim);
     desc.nlist = static_cast<uint32_t>(d_->nlist);
     desc.metric = d_->metric;
-    desc.m = static_cast<uint32_t>(d_->pq.m);
     desc.cosine_ip = d_->cosine_ip;
-    desc.use_precomputed = wants_precomputed(*d_);
     desc.centroids = d_->centroids.data();
     desc.residual_centers = d_->residual_centers.empty() ? nullptr : d_->residual_centers.data();
-    desc.pq_centroids = d_->pq.centroids.data();
     desc.rotation = d_->rotate && !d_->rotation.empty() ? d_->rotation.data() : nullptr;
+    if (d_->sq_bits) {
+        desc.storage = QANN_IVF_SQ;
+        desc.sq_bits = static_cast<uint32_t>(d_->sq_bits);
+        desc.sq_vmin = d_->sq_vmin.data();
+        desc.sq_step = d_->sq_step.data();
+    } else if (d_->has_pq) {
+        desc.storage = d_->fast_scan ? QANN_IVF_FASTSCAN : QANN_IVF_PQ;
+        desc.m = static_cast<uint32_t>(d_->pq.m);
+        desc.pq_k = static_cast<uint32_t>(d_->pq.K);
+        desc.use_precomputed = wants_precomputed(*d_);
+        desc.pq_centroids = d_->pq.centroids.data();
+    } else {
+        desc.storage = QANN_IVF_FLOAT;
+    }
     qann_hip_index* h = nullptr;
     check_status(qann_hip_ivf_create(&desc, &h));
     std::vector<uint32_t> sizes(d_->nlist);
     uint64_t total = 0, cb = 0;
+    const bool raw = desc.storage == QANN_IVF_FLOAT;
     for (uint64_t l = 0; l < d_->nlist; ++l) {
         sizes[l] = static_cast<uint32_t>(d_->lists[l].ids.size());
         total += sizes[l];
-        cb += d_->lists[l].codes.size();
+        cb += raw ? d_->lists[l].vecs.size() * sizeof(float) : d_->lists[l].codes.size();
     }
     std::vector<uint32_t> slots;
     std::vector<uint8_t> codes;
@@ -662,7 +673,12 @@ qann_hip_index* IVFIndex::device() const {
     codes.reserve(cb);
     for (const IvfList& list : d_->lists) {
         slots.insert(slots.end(), list.ids.begin(), list.ids.end());
-        codes.insert(codes.end(), list.codes.begin(), list.codes.end());
+        if (raw) {
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(list.vecs.data());
+            codes.insert(codes.end(), b, b + list.vecs.size() * sizeof(float));
+        } else {
+            codes.insert(codes.end(), list.codes.begin(), list.codes.end());
+        }
     }
     int rc = qann_hip_ivf_set_lists(h, sizes.data(), slots.data(), codes.data());
     if (rc == QANN_OK)

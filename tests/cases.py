@@ -20,7 +20,7 @@ REF_FIXTURES = [
     ("v9_ip_flat", "v9_ip_queries"),
     ("v10_refine", "v10_queries"),
 ]
-# files of other IVF storage kinds, loaded by the oracle only
+# files of other IVF storage kinds (8-bit PQ, SQ, float lists) and their refines
 REF_FIXTURES_OTHER = [
     ("v1_ivf_pq", "v1_queries"),
     ("v2_ivf_pq", "v2_queries"),

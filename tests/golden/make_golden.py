@@ -37,6 +37,23 @@ CASES = [
     ("flat_l2_custom", dict(d=24, n=2000, flat=True, metric=L2, custom=True, remove=30)),
     ("flat_ip", dict(d=16, n=2000, flat=True, metric=IP)),
     ("flat_cos", dict(d=20, n=2000, flat=True, metric=COS)),
+    # IVF lists stored as 8-bit PQ codes (ADC), SQ codes or float rows; "half"
+    # adds a filter above 3 nlist slots, so SQ / float lists go through the
+    # probes rather than IVFIndex::direct_scan
+    ("l2_pq8", dict(d=32, n=3000, nlist=100, nprobe=8, metric=L2, m=8, pq_k=256, fast=False, kf=0, remove=60,
+                    half=True)),
+    ("l2_pq8_noprecomp", dict(d=32, n=3000, nlist=100, nprobe=8, metric=L2, m=8, pq_k=256, fast=False, kf=0,
+                              precomputed=False)),
+    ("ip_pq6_custom", dict(d=24, n=3000, nlist=100, nprobe=10, metric=IP, m=6, pq_k=64, fast=False, kf=0,
+                           custom=True, remove=40)),
+    ("cos_pq8_refine", dict(d=40, n=2500, nlist=128, nprobe=12, metric=COS, m=10, pq_k=256, fast=False, kf=3)),
+    ("cos_pq8_l2", dict(d=40, n=2500, nlist=128, nprobe=12, metric=COS, m=10, pq_k=256, fast=False, kf=0,
+                        cosine_ip=False)),
+    ("l2_sq8", dict(d=32, n=3000, nlist=100, nprobe=8, metric=L2, m=0, sq=8, kf=0, remove=40, half=True)),
+    ("ip_sq4_custom", dict(d=30, n=3000, nlist=100, nprobe=8, metric=IP, m=0, sq=4, kf=0, custom=True, half=True)),
+    ("cos_sq8_refine", dict(d=20, n=2500, nlist=100, nprobe=8, metric=COS, m=0, sq=8, kf=3, half=True)),
+    ("l2_float", dict(d=19, n=3000, nlist=100, nprobe=8, metric=L2, m=0, kf=0, remove=30, half=True)),
+    ("ip_float_refine", dict(d=24, n=2500, nlist=100, nprobe=6, metric=IP, m=0, kf=2, half=True)),
 ]
 
 
@@ -67,8 +84,9 @@ def main():
         if p.get("flat"):
             index = ref.build_flat(x, metric=p["metric"], ids=ids)
         else:
-            index = ref.build_ivf(x, p["nlist"], p["nprobe"], metric=p["metric"], m=p["m"], fast_scan=True,
-                                  rotate=p.get("rotate", False), seed=7, k_factor=p["kf"], ids=ids)
+            index = ref.build_ivf(x, p["nlist"], p["nprobe"], metric=p["metric"], m=p["m"], pq_k=p.get("pq_k", 16),
+                                  fast_scan=p.get("fast", True), rotate=p.get("rotate", False), sq_bits=p.get("sq", 0),
+                                  seed=7, k_factor=p["kf"], ids=ids)
             if "cosine_ip" in p:
                 ref._check(ref.set_cosine_ip(index._h, int(p["cosine_ip"])))
             if "precomputed" in p:
@@ -87,6 +105,8 @@ def main():
             "tenth": np.sort(rng.choice(pool, size=p["n"] // 10, replace=False)),
             "few": np.sort(rng.choice(pool, size=25, replace=False)),
         }
+        if p.get("half"):
+            filters["half"] = np.sort(rng.choice(pool, size=p["n"] // 2, replace=False))
         results = {}
         nprobes = [None] if p.get("flat") else [p["nprobe"], 1, p["nlist"] // 3]
         for npb in nprobes:

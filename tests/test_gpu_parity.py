@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import qann_amd
-from cases import REF_FIXTURES, SYNTH_CASES, ref_fixture, synth_case, synth_queries
+from cases import REF_FIXTURES, REF_FIXTURES_OTHER, SYNTH_CASES, ref_fixture, synth_case, synth_queries
 
 pytestmark = pytest.mark.gpu
 
@@ -35,7 +35,7 @@ def test_gpu_bit_exact_on_reference_goldens(name):
         assert_same(got, (exp[stem + "_ids"], exp[stem + "_dists"]), f"{name} {stem}")
 
 
-@pytest.mark.parametrize("name,queries", REF_FIXTURES)
+@pytest.mark.parametrize("name,queries", REF_FIXTURES + REF_FIXTURES_OTHER)
 def test_gpu_reproduces_reference_fixtures(oracle, name, queries):
     path, q, ids, dists = ref_fixture(name, queries)
     idx = qann_amd.load(str(path))
