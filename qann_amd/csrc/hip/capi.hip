@@ -152,6 +152,8 @@ struct qann_hip_index {
     uint64_t* deleted = nullptr;
     // raw rows (flat, refine)
     float* rows = nullptr;
+    float* row_sq = nullptr; // flat: squared norms (exact order), for the GEMM path
+    float xnorm_max = 0.0f;  // flat: an upper bound on every row's norm
     qann_hip_index* base = nullptr;
     std::vector<void*> allocs;
     uint64_t bytes = 0;
@@ -283,6 +285,34 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const size_t qlds = static_cast<size_t>((h->dim + 1) & ~1u) * 4;
     if (qlds > qd::kLdsMax / 2) invalid("dim too large for the on-chip query copy");
     if (h->kind == 1) { // FlatIndex::query_impl
+        const uint64_t n = h->slots;
+        const size_t fcap = qd::coarse_select_cap(static_cast<int>(k));
+        const bool gemm = nq >= 8 && n > 0 && n <= 0x7FFFFFFFull && k <= 4096 && qlds + fcap * 8 <= qd::kLdsMax &&
+                          (!filtered || p.allowed_count * 8 >= n);
+        if (gemm) {
+            // MFMA scores of a chunk of queries against every row, then per query the
+            // exact k nearest from a provably sufficient shortlist
+            const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, (uint64_t{1} << 32) / (n * 4)));
+            sc.used = 0;
+            sc.reserve(pad256(chunk * n * 4));
+            float* D = sc.take<float>(chunk * n);
+            const uint64_t* excl = filtered ? nullptr : h->deleted;
+            const uint64_t* incl = filtered ? d_filter : nullptr;
+            StageTimer t(QANN_STAGE_FLAT, st);
+            for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+                const uint64_t nc = std::min(chunk, nq - q0);
+                const float* qc = d_q + q0 * h->dim;
+                qd::launch_coarse_gemm(qc, static_cast<int>(nc), h->rows, h->row_sq, static_cast<int>(n),
+                                       static_cast<int>(h->dim), ip, D, st, excl, incl);
+                check_launch("k_coarse_gemm");
+                qd::launch_flat_select(D, qc, static_cast<int>(nc), h->rows, static_cast<int>(n),
+                                       static_cast<int>(h->dim), ip, h->xnorm_max, static_cast<int>(k), excl, incl,
+                                       h->labels, d_ids + q0 * k, d_dists + q0 * k, st);
+                check_launch("k_flat_select");
+            }
+            t.done();
+            return;
+        }
         sc.used = 0;
         const bool f = filtered;
         const size_t selb = qd::block_select_bytes(static_cast<int>(k), false);
@@ -829,6 +859,13 @@ int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint6
         h->rows = slots ? h->upload(rows, slots * dim) : h->alloc<float>(1);
         h->slots = slots;
         h->live = slots;
+        if (slots) {
+            h->row_sq = h->alloc<float>(slots);
+            qd::launch_sqnorms(h->rows, static_cast<int>(std::min<uint64_t>(slots, 0x7FFFFFFF)), static_cast<int>(dim),
+                               h->row_sq, nullptr);
+            check_launch("k_sqnorms");
+            h->xnorm_max = norm_bound(rows, slots, dim);
+        }
         uint64_t set = 0;
         std::vector<uint64_t> bm = to_bitmap(nullptr, slots, set);
         h->deleted = h->upload(bm.data(), std::max<size_t>(bm.size(), 1));

@@ -89,12 +89,14 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kGemmTile = 64; // queries x lists per workgroup
 constexpr int kGemmK = 16;    // K staged through LDS per step
 
-// D[i][l] = |c_l|^2 - 2 q_i . c_l, or -q_i . c_l for inner product, by
+// D[i][l] = |c_l|^2 - 2 q_i . c_l, or -q_i . c_l for inner product (+inf for
+// a row excluded by the excl bitmap or missing from the incl bitmap), by
 // v_mfma_f32_16x16x4f32: four waves of 32 x 32 (2 x 2 tiles of 16 x 16). Lane
 // t gives A[t % 16][t / 16] and B[t / 16][t % 16] of a 16 x 4 x 16 step and
 // holds D[4 (t / 16) + v][t % 16] in accumulator v.
 __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, const float* cents, const float* csq,
-                                                     int nlist, int dim, int ip, float* D) {
+                                                     int nlist, int dim, int ip, float* D, const uint64_t* excl,
+                                                     const uint64_t* incl) {
     __shared__ float qs[kGemmTile][kGemmK + 1];
     __shared__ float cs[kGemmTile][kGemmK + 1];
     const int l0 = blockIdx.x * kGemmTile, q0 = blockIdx.y * kGemmTile;
@@ -125,10 +127,12 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
         const int l = l0 + wl + 16 * tj + lane % 16;
         if (l >= nlist) return;
         const float c2 = ip ? 0.0f : csq[l];
+        // a row the query may not return scores +inf
+        const bool out = (excl && bit_test(excl, static_cast<uint32_t>(l))) || (incl && !bit_test(incl, static_cast<uint32_t>(l)));
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = q0 + wq + 16 * ti + 4 * (lane / 16) + v;
-            if (i < nq) D[static_cast<size_t>(i) * nlist + l] = ip ? -acc[v] : c2 - 2.0f * acc[v];
+            if (i < nq) D[static_cast<size_t>(i) * nlist + l] = out ? INFINITY : ip ? -acc[v] : c2 - 2.0f * acc[v];
         }
     };
     put(acc00, 0, 0);
@@ -140,7 +144,7 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
 // The digit d of a 256-bin histogram (one bin per thread of a 256-thread
 // block) whose running count first reaches `remain`, and the count below it:
 // an inclusive scan of the bins within each wave, then across the four waves.
-__device__ void block_find_digit(const uint32_t* hist, uint32_t remain, uint32_t* s_wsum, uint32_t* s_digit,
+__device__ __forceinline__ void block_find_digit(const uint32_t* hist, uint32_t remain, uint32_t* s_wsum, uint32_t* s_digit,
                                  uint32_t* s_below) {
     const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
     const uint32_t h = hist[t];
@@ -165,7 +169,7 @@ __device__ void block_find_digit(const uint32_t* hist, uint32_t remain, uint32_t
 // the key of rank `want` (1-based) among those key(l) gives. 256 threads;
 // with CACHE, the row's keys (n <= 16 * 256) stay in registers across passes.
 template <typename Key, bool CACHE, typename KeyOf>
-__device__ Key block_radix_select(int n, uint32_t want, KeyOf key, uint32_t* hist, uint32_t* s_wsum, uint32_t* s_digit,
+__device__ __forceinline__ Key block_radix_select(int n, uint32_t want, KeyOf key, uint32_t* hist, uint32_t* s_wsum, uint32_t* s_digit,
                                   uint32_t* s_below) {
     Key cached[CACHE ? 16 : 1];
     if (CACHE) {
@@ -203,8 +207,21 @@ __device__ Key block_radix_select(int n, uint32_t want, KeyOf key, uint32_t* his
     return prefix;
 }
 
+// Selection keys of (distance, index): ckey(d) (float order, -0 tied with +0)
+// above the index, with the index shifted up one bit so the lowest bit can
+// record a -0 distance -- indexes are unique, so that bit never decides the
+// order, and the key gives back the distance's exact bits.
+__device__ __forceinline__ uint64_t dist_key(float d, uint32_t idx) {
+    const uint32_t negzero = (d == 0.0f && (__float_as_uint(d) >> 31)) ? 1u : 0u;
+    return (static_cast<uint64_t>(ckey(d)) << 32) | (static_cast<uint64_t>(idx) << 1) | negzero;
+}
+__device__ __forceinline__ uint32_t key_index(uint64_t key) { return static_cast<uint32_t>(key & 0xFFFFFFFFu) >> 1; }
+__device__ __forceinline__ float key_dist(uint64_t key) {
+    return (key & 1u) ? -0.0f : from_fkey(static_cast<uint32_t>(key >> 32));
+}
+
 // Minimum / maximum over a 256-thread block (every thread gets the result).
-__device__ float block_min256(float v, float* s_red) {
+__device__ __forceinline__ float block_min256(float v, float* s_red) {
     for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
     __syncthreads();
     if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
@@ -212,7 +229,7 @@ __device__ float block_min256(float v, float* s_red) {
     return fminf(fminf(s_red[0], s_red[1]), fminf(s_red[2], s_red[3]));
 }
 
-__device__ float block_max256(float v, float* s_red) {
+__device__ __forceinline__ float block_max256(float v, float* s_red) {
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
     __syncthreads();
     if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
@@ -227,7 +244,7 @@ __device__ float block_max256(float v, float* s_red) {
 // largest value in the buckets up to the one found. With CACHE the row
 // (n <= 16 * 256) stays in registers.
 template <bool CACHE>
-__device__ float select_threshold(const float* row, int n, int M, int target, uint32_t* hist, uint32_t* s_wsum,
+__device__ __forceinline__ float select_threshold(const float* row, int n, int M, int target, uint32_t* hist, uint32_t* s_wsum,
                                   uint32_t* s_digit, uint32_t* s_below, float* s_red) {
     const int tid = threadIdx.x;
     float cached[CACHE ? 16 : 1];
@@ -372,17 +389,15 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
         for (int t = tid; t < np; t += blockDim.x) {
             if (t < cnt) {
                 const uint32_t l = static_cast<uint32_t>(keys[t]);
-                const float A = exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim);
-                keys[t] = (static_cast<uint64_t>(ckey(A)) << 32) | l;
+                keys[t] = dist_key(exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim), l);
             } else {
                 keys[t] = ~0ull;
             }
         }
         block_bitonic_sort(keys, np);
         for (int t = tid; t < M; t += blockDim.x) {
-            const uint32_t l = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
-            out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(l);
-            out_val[static_cast<size_t>(i) * M + t] = from_fkey(static_cast<uint32_t>(keys[t] >> 32));
+            out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(key_index(keys[t]));
+            out_val[static_cast<size_t>(i) * M + t] = key_dist(keys[t]);
         }
         return;
     }
@@ -390,7 +405,7 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
     for (int l = tid; l < nlist; l += blockDim.x)
         row[l] = exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim);
     __syncthreads();
-    auto key64 = [&](int l) { return (static_cast<uint64_t>(ckey(row[l])) << 32) | static_cast<uint32_t>(l); };
+    auto key64 = [&](int l) { return dist_key(row[l], static_cast<uint32_t>(l)); };
     const uint64_t kth =
         block_radix_select<uint64_t, false>(nlist, static_cast<uint32_t>(M), key64, hist, s_wsum, &s_digit, &s_below);
     int Mp = 1;
@@ -404,9 +419,115 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
     }
     block_bitonic_sort(keys, Mp);
     for (int t = tid; t < M; t += blockDim.x) {
-        const uint32_t l = static_cast<uint32_t>(keys[t] & 0xFFFFFFFFu);
-        out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(l);
-        out_val[static_cast<size_t>(i) * M + t] = row[l];
+        out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(key_index(keys[t]));
+        out_val[static_cast<size_t>(i) * M + t] = key_dist(keys[t]);
+    }
+}
+
+// FlatIndex::query_impl (src/flat_index.cpp:33-64) from approximate scores:
+// per query (one workgroup), the k smallest (exact distance, slot) over the
+// rows it may return -- float order, ties by slot -- with ids and exact
+// distances. Rows scored +inf by k_coarse_gemm's masks are not eligible. The
+// shortlist is k_coarse_select's: every eligible row within tau + 3 delta of
+// the k-th approximate score, so it holds the k nearest; when it overflows
+// (or holds fewer than k) every eligible row is scored exactly.
+__global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, const float* rows, int n, int dim,
+                                                     int ip, float xnorm_max, int k, int cap, const uint64_t* excl,
+                                                     const uint64_t* incl, const int64_t* labels, int64_t* out_ids,
+                                                     float* out_dists) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
+    __shared__ float s_red[4], s_red8[8];
+    float* qsh = reinterpret_cast<float*>(lds);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
+    const int i = blockIdx.x, tid = threadIdx.x;
+    float* row = D + static_cast<size_t>(i) * n;
+    auto eligible = [&](int l) {
+        return !((excl && bit_test(excl, static_cast<uint32_t>(l))) || (incl && !bit_test(incl, static_cast<uint32_t>(l))));
+    };
+    float ss = 0.0f;
+    for (int t = tid; t < dim; t += blockDim.x) {
+        const float v = q[static_cast<size_t>(i) * dim + t];
+        qsh[t] = v;
+        ss += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, kWave);
+    if (lane_id() == 0) s_red[tid / kWave] = ss;
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    ss = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    const int M = min(k, n);
+    const int target = min(cap, M + max(64, M / 4));
+    const float tau = n <= 16 * 256 ? select_threshold<true>(row, n, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
+                                    : select_threshold<false>(row, n, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+    const float u = 5.9604645e-8f, nd = static_cast<float>(dim);
+    const float qn = sqrtf(ss * (1.0f + 2.0f * (nd + 2.0f) * u)) * 1.0001f;
+    const float delta = 2.0f * (ip ? 2.0f * (nd + 1.0f) * u * qn * xnorm_max
+                                   : (2.0f * nd + 5.0f) * u * (qn + xnorm_max) * (qn + xnorm_max)) +
+                        nd * 2.3509887e-38f;
+    const float limit = tau + 3.0f * delta;
+    for (int l0 = 0; l0 < n; l0 += 16 * 256) {
+        float v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int l = l0 + tid + 256 * j;
+            v[j] = l < n ? row[l] : INFINITY;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int l = l0 + tid + 256 * j;
+            if (v[j] <= limit && l < n && eligible(l)) {
+                const uint32_t pos = atomicAdd(&s_count, 1u);
+                if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l);
+            }
+        }
+    }
+    __syncthreads();
+    const int cnt = static_cast<int>(s_count);
+    int kept = M;
+    if (cnt >= M && cnt <= cap) {
+        int np = 1;
+        while (np < cnt) np <<= 1;
+        for (int t = tid; t < np; t += blockDim.x) {
+            if (t < cnt) {
+                const uint32_t l = static_cast<uint32_t>(keys[t]);
+                keys[t] = dist_key(exact_score(ip != 0, qsh, rows + static_cast<size_t>(l) * dim, dim), l);
+            } else {
+                keys[t] = ~0ull;
+            }
+        }
+        block_bitonic_sort(keys, np);
+    } else {
+        // every eligible row exactly, then the M smallest (ckey, slot) keys
+        for (int l = tid; l < n; l += blockDim.x)
+            if (eligible(l)) row[l] = exact_score(ip != 0, qsh, rows + static_cast<size_t>(l) * dim, dim);
+        __syncthreads();
+        auto key64 = [&](int l) { return eligible(l) ? dist_key(row[l], static_cast<uint32_t>(l)) : ~0ull; };
+        const uint64_t kth =
+            block_radix_select<uint64_t, false>(n, static_cast<uint32_t>(M), key64, hist, s_wsum, &s_digit, &s_below);
+        int Mp = 1;
+        while (Mp < M) Mp <<= 1;
+        if (tid == 0) s_count = 0;
+        for (int t = tid; t < Mp; t += blockDim.x) keys[t] = ~0ull;
+        __syncthreads();
+        for (int l = tid; l < n; l += blockDim.x) {
+            const uint64_t kk = key64(l);
+            if (kk != ~0ull && kk <= kth) keys[atomicAdd(&s_count, 1u)] = kk;
+        }
+        block_bitonic_sort(keys, Mp);
+        kept = min(M, static_cast<int>(s_count));
+    }
+    for (int t = tid; t < k; t += blockDim.x) {
+        int64_t id = -1;
+        float d = INFINITY;
+        if (t < kept && keys[t] != ~0ull) {
+            const uint32_t l = key_index(keys[t]);
+            id = labels ? labels[l] : static_cast<int64_t>(l);
+            d = key_dist(keys[t]);
+        }
+        out_ids[static_cast<size_t>(i) * k + t] = id;
+        out_dists[static_cast<size_t>(i) * k + t] = d;
     }
 }
 
@@ -1472,13 +1593,13 @@ size_t coarse_select_cap(int M) {
 }
 
 void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
-                        float* D, hipStream_t s) {
+                        float* D, hipStream_t s, const uint64_t* excl, const uint64_t* incl) {
     const int rows = 65535 * kGemmTile; // grid y limit
     for (int q0 = 0; q0 < nq; q0 += rows) {
         const int n = std::min(rows, nq - q0);
         hipLaunchKernelGGL(k_coarse_gemm, dim3(cdiv(nlist, kGemmTile), cdiv(n, kGemmTile)), dim3(256), 0, s,
                            q + static_cast<size_t>(q0) * dim, n, cents, csq, nlist, dim, ip,
-                           D + static_cast<size_t>(q0) * nlist);
+                           D + static_cast<size_t>(q0) * nlist, excl, incl);
     }
 }
 
@@ -1489,6 +1610,16 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
     hipLaunchKernelGGL(k_coarse_select, dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip, cnorm_max, M,
                        static_cast<int>(cap), out_idx, out_val);
+}
+
+void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,
+                        int k, const uint64_t* excl, const uint64_t* incl, const int64_t* labels, int64_t* out_ids,
+                        float* out_dists, hipStream_t s) {
+    if (nq <= 0) return;
+    const size_t cap = coarse_select_cap(k);
+    const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
+    hipLaunchKernelGGL(k_flat_select, dim3(nq), dim3(256), lds, s, D, q, rows, n, dim, ip, xnorm_max, k,
+                       static_cast<int>(cap), excl, incl, labels, out_ids, out_dists);
 }
 
 void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,

@@ -129,9 +129,14 @@ void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, 
 // The select's LDS: dim floats + coarse_select_cap(M) keys.
 size_t coarse_select_cap(int M);
 void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
-                        float* D, hipStream_t s);
+                        float* D, hipStream_t s, const uint64_t* excl = nullptr, const uint64_t* incl = nullptr);
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
                           int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s);
+// FlatIndex over D from launch_coarse_gemm (rows as the "centroids", masked by
+// excl / incl): the k nearest rows per query, exact, in (distance, slot) order.
+void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,
+                        int k, const uint64_t* excl, const uint64_t* incl, const int64_t* labels, int64_t* out_ids,
+                        float* out_dists, hipStream_t s);
 void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
                        float* A, float* base, hipStream_t s);
 // Per query: kTableInner the inner table C (m * K), kTableIp the inner-product

@@ -133,3 +133,43 @@ def test_gpu_flat_matches_oracle(tmp_path, oracle):
         orc = oracle.load(path)
         for k in (1, 16, 300):
             assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"{metric} k {k}")
+
+
+def test_gpu_flat_gemm_precision_shape(tmp_path, oracle):
+    # the precision configuration's shape at a size the oracle scores in seconds:
+    # 768-dim unit rows with an i^-0.8 spectrum, inner product, k = 100 -- scored
+    # by the MFMA path (8 or more queries), with deletions and a large filter
+    rng = np.random.default_rng(768)
+    lam = np.arange(1, 769, dtype=np.float64) ** -0.8
+    x = (rng.standard_normal((6000, 768)) * np.sqrt(lam)).astype(np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    q = (rng.standard_normal((48, 768)) * np.sqrt(lam)).astype(np.float32)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    for metric in (qann_amd.Metric.InnerProduct, qann_amd.Metric.L2):
+        flat = qann_amd.FlatIndex(768, metric=metric)
+        flat.add(x)
+        flat.remove(rng.choice(6000, 200, replace=False))
+        path = tmp_path / "flat768.qann"
+        flat.save(str(path))
+        orc = oracle.load(path)
+        for k in (1, 100):
+            assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"{metric} k {k}")
+        allowed = rng.choice(6000, 3500, replace=False)
+        assert_same(flat.batch_query(q, 100, allowed=allowed), orc.batch_query(q, 100, allowed=allowed),
+                    f"{metric} filtered")
+
+
+def test_gpu_flat_gemm_many_ties(tmp_path, oracle):
+    # 1500 copies of one row nearest to every query: more rows within the
+    # shortlist bound than it holds, so every row is scored exactly; ties go by slot
+    rng = np.random.default_rng(5)
+    base = rng.random(32).astype(np.float32)
+    x = np.concatenate([np.repeat(base[None], 1500, axis=0), rng.random((2500, 32)).astype(np.float32) + 0.5])
+    q = (base[None] + 0.01 * rng.standard_normal((16, 32))).astype(np.float32)
+    flat = qann_amd.FlatIndex(32, metric=qann_amd.Metric.L2, custom_ids=True)
+    flat.add(x, ids=rng.choice(10**6, 4000, replace=False))
+    path = tmp_path / "ties.qann"
+    flat.save(str(path))
+    orc = oracle.load(path)
+    for k in (10, 1200):
+        assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"k {k}")
