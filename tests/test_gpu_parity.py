@@ -173,3 +173,31 @@ def test_gpu_flat_gemm_many_ties(tmp_path, oracle):
     orc = oracle.load(path)
     for k in (10, 1200):
         assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"k {k}")
+
+
+def test_gpu_config_100k_x_128_matches_oracle(tmp_path, oracle):
+    # the CPU-runnable configuration at its full size: 100,000 x 128 L2 from a
+    # 64-centre mixture (centres U[0,1)^128, sigma 0.05), seed 1, IVF nlist 256
+    # nprobe 16, PQ32 fast-scan, RefineIndex k_factor 10, k = 10, 1,000 queries
+    rng = np.random.Generator(np.random.PCG64(1))
+    centres = rng.random((64, 128), dtype=np.float32)
+    x = (centres[rng.integers(0, 64, 100_000)] + 0.05 * rng.standard_normal((100_000, 128), dtype=np.float32))
+    q = (centres[rng.integers(0, 64, 1000)] + 0.05 * rng.standard_normal((1000, 128), dtype=np.float32))
+    ivf = qann_amd.IVFIndex(128, 256, 16)
+    ivf.enable_pq(32, fast_scan=True)
+    ivf.train(x, seed=1)
+    idx = qann_amd.RefineIndex(ivf, 10)
+    idx.add(x)
+    path = tmp_path / "config1.qann"
+    idx.save(str(path))
+    got = idx.batch_query(q, 10)
+    assert_same(got, oracle.load(path).batch_query(q, 10), "100k x 128")
+    # recall@10 against exact search (float64): 0.7635 for this index -- the
+    # 128-dim clusters' distances concentrate, so PQ32x4 and a k_factor 10
+    # re-rank keep about three quarters of the exact 10; a guard against
+    # regressions in training, not a target
+    d2 = ((q.astype(np.float64) ** 2).sum(1)[:, None] - 2 * q.astype(np.float64) @ x.T.astype(np.float64)
+          + (x.astype(np.float64) ** 2).sum(1)[None, :])
+    truth = np.argpartition(d2, 10, axis=1)[:, :10]
+    recall = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(got[0], truth)])
+    assert recall > 0.74, recall
