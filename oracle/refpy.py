@@ -163,6 +163,11 @@ class Reference(_Lib):
             raise RuntimeError(self.last_error().decode())
         return _Index(self, h)
 
+    def add(self, index: _Index, data, ids=None):
+        x = np.ascontiguousarray(data, dtype=np.float32)
+        idv = None if ids is None else np.ascontiguousarray(ids, dtype=np.int64)
+        self._check(self.add_(index._h, _fp(x), x.shape[0], _ip(idv)))
+
     def remove(self, index: _Index, ids):
         idv = np.ascontiguousarray(ids, dtype=np.int64)
         out = C.c_uint64(0)

@@ -201,3 +201,26 @@ def test_gpu_config_100k_x_128_matches_oracle(tmp_path, oracle):
     truth = np.argpartition(d2, 10, axis=1)[:, :10]
     recall = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(got[0], truth)])
     assert recall > 0.74, recall
+
+
+@pytest.mark.parametrize("name", ["l2_fs", "ip_fs_custom", "cos_fs_refine", "cos_l2_refine", "rot_refine"])
+def test_gpu_add_writes_the_reference_bytes(tmp_path, reference, name):
+    # IVFIndex::add_batch_impl (src/ivf_index.cpp:111-195): the same vectors added
+    # to the same reference-built index by qann_amd (nearest list on the GPU, PQ
+    # encoding and fast-scan packing on the host) and by the reference library
+    # save to the same bytes
+    path, q, _, params = synth_case(name)
+    rng = np.random.default_rng(len(name))
+    new = (q[rng.integers(0, len(q), 400)] + 0.03 * rng.standard_normal((400, q.shape[1]))).astype(np.float32)
+    ids = None
+    if params.get("custom"):
+        ids = (np.int64(1) << 40) + np.arange(400, dtype=np.int64) * 3
+    ours = qann_amd.load(str(path))
+    ours.add(new, ids=ids) if ids is not None else ours.add(new)
+    ours.save(str(tmp_path / "ours.qann"))
+    ref = reference.load(path)
+    reference.add(ref, new, ids)
+    reference.save(ref, tmp_path / "ref.qann")
+    a = (tmp_path / "ours.qann").read_bytes()
+    b = (tmp_path / "ref.qann").read_bytes()
+    assert len(a) == len(b) and a == b, f"{name}: files differ ({len(a)} vs {len(b)} bytes)"
