@@ -230,6 +230,14 @@ PYBIND11_MODULE(_qann, m) {
              },
              py::arg("num_subspaces"), py::arg("centroids_per_subspace") = py::none(), py::kw_only(),
              py::arg("fast_scan") = false, py::arg("rotate") = false)
+        .def("enable_sq",
+             [](IVFIndex& self, int bits) {
+                 write_locked(self, [&] {
+                     self.enable_sq(bits);
+                     return 0;
+                 });
+             },
+             py::arg("bits") = 8)
         .def("train",
              [](IVFIndex& self, FloatArray data, uint64_t max_iters, const py::object& seed) {
                  if (data.ndim() != 2) throw py::type_error("data must be a 2-D array of shape (n, dim)");

@@ -398,13 +398,19 @@ void IVFIndex::enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate) {
     d_->pq.centroids.assign(m * K * d_->pq.sub, 0.0f);
 }
 
+// IVFIndex::enable_sq (src/ivf_index.cpp:103-109).
+void IVFIndex::enable_sq(int bits) {
+    if (bits != 4 && bits != 8) throw std::invalid_argument("bits must be 4 or 8");
+    if (d_->ids.slots > 0 || d_->trained) throw std::logic_error("enable_sq must be called before train()/add()");
+    if (d_->has_pq) throw std::logic_error("enable_sq cannot be combined with enable_pq");
+    d_->sq_bits = bits;
+}
+
 void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed) {
     if (d_->ids.slots > 0) throw std::logic_error("train() cannot be called after add()");
     if (d_->trained) throw std::logic_error("train() has already been called");
     if (n < d_->nlist) throw std::invalid_argument("training set must have at least nlist vectors");
     if (d_->rotate) throw std::runtime_error("training a rotated index is not supported by qann_amd; train it with qann and load it");
-    if (d_->sq_bits || (d_->has_pq && !d_->fast_scan))
-        throw std::runtime_error("qann_amd trains IVF fast-scan indexes only");
     const uint64_t d = d_->dim;
     const std::vector<float> x = prepare(vecs, n);
     std::mt19937 rng(seed ? *seed : std::random_device{}());
@@ -449,6 +455,19 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
         }
     }
     if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = pick_cosine_ip(x.data(), n, assign);
+    if (d_->sq_bits) {
+        // detail::SQ::train (src/sq.cpp): each dimension's range over the sample
+        d_->sq_vmin.assign(x.begin(), x.begin() + d);
+        std::vector<float> hi(d_->sq_vmin);
+        for (uint64_t i = 1; i < n; ++i)
+            for (uint64_t t = 0; t < d; ++t) {
+                d_->sq_vmin[t] = std::min(d_->sq_vmin[t], x[i * d + t]);
+                hi[t] = std::max(hi[t], x[i * d + t]);
+            }
+        const float top = static_cast<float>((1 << d_->sq_bits) - 1);
+        d_->sq_step.resize(d);
+        for (uint64_t t = 0; t < d; ++t) d_->sq_step[t] = (hi[t] - d_->sq_vmin[t]) / top;
+    }
     d_->trained = true;
     invalidate();
 }
@@ -488,7 +507,8 @@ bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint
     if (n < kQueries + kCandidates) return true;
     const uint64_t d = d_->dim, nl = d_->nlist, nq = kQueries, nd = std::min(kRows, n - nq);
     const PQ& pq = d_->pq;
-    const uint64_t m = pq.m, T = m * 16;
+    const uint64_t m = pq.m, K = pq.K, T = m * K;
+    const bool fs = d_->fast_scan;
     const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
     auto dot = [d](const float* a, const float* b) {
         float s = 0.0f;
@@ -508,14 +528,14 @@ bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint
     // table[s][c] = sum over the subspace of f(q_t, y_t) for centroid y = pq[s][c]
     auto build = [&](const float* q, bool ip, float* table) {
         for (uint64_t s = 0; s < m; ++s)
-            for (uint64_t c = 0; c < 16; ++c) {
-                const float* y = pq.centroids.data() + (s * 16 + c) * pq.sub;
+            for (uint64_t c = 0; c < K; ++c) {
+                const float* y = pq.centroids.data() + (s * K + c) * pq.sub;
                 float acc = 0.0f;
                 for (uint64_t t = 0; t < pq.sub; ++t) {
                     const float qt = q[s * pq.sub + t];
                     acc += ip ? -2.0f * qt * y[t] : (qt - y[t]) * (qt - y[t]);
                 }
-                table[s * 16 + c] = acc;
+                table[s * K + c] = acc;
             }
     };
     std::vector<uint64_t> hits_ip(nq, 0), hits_l2(nq, 0);
@@ -532,28 +552,38 @@ bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint
             rows += by_list[lists[p].second].size();
         }
         std::vector<std::pair<float, uint32_t>> exact, ip, l2;
-        std::vector<float> table(T), resid(d);
-        std::vector<uint8_t> lut(T), list_lut(T);
+        std::vector<float> table(T), ip_table(T), resid(d);
+        std::vector<uint8_t> lut(fs ? T : 0), list_lut(fs ? T : 0);
         float ip_scale = 1.0f, ip_offset = 0.0f;
-        build(q, true, table.data());
-        round_table(table.data(), m, lut.data(), ip_scale, ip_offset);
+        build(q, true, ip_table.data());
+        if (fs) round_table(ip_table.data(), m, lut.data(), ip_scale, ip_offset);
         for (uint32_t l : probed) {
             const float* c = rc + size_t{l} * d;
             const float base = 2.0f - 2.0f * dot(q, c);
             for (uint64_t t = 0; t < d; ++t) resid[t] = q[t] - c[t];
             build(resid.data(), false, table.data());
             float l2_scale = 1.0f, l2_offset = 0.0f;
-            round_table(table.data(), m, list_lut.data(), l2_scale, l2_offset);
+            if (fs) round_table(table.data(), m, list_lut.data(), l2_scale, l2_offset);
             for (uint32_t i : by_list[l]) {
                 const uint8_t* code = codes.data() + size_t{i} * m;
-                uint32_t a = 0, b = 0;
-                for (uint64_t s = 0; s < m; ++s) {
-                    a += lut[s * 16 + code[s]];
-                    b += list_lut[s * 16 + code[s]];
+                float s_ip = 0.0f, s_l2 = 0.0f;
+                if (fs) {
+                    uint32_t a = 0, b = 0;
+                    for (uint64_t s = 0; s < m; ++s) {
+                        a += lut[s * 16 + code[s]];
+                        b += list_lut[s * 16 + code[s]];
+                    }
+                    s_ip = static_cast<float>(a) / ip_scale + ip_offset;
+                    s_l2 = static_cast<float>(b) / l2_scale + l2_offset;
+                } else {
+                    for (uint64_t s = 0; s < m; ++s) {
+                        s_ip += ip_table[s * K + code[s]];
+                        s_l2 += table[s * K + code[s]];
+                    }
                 }
                 exact.emplace_back(-dot(q, x + size_t{i} * d), i);
-                ip.emplace_back(base + (static_cast<float>(a) / ip_scale + ip_offset), i);
-                l2.emplace_back(static_cast<float>(b) / l2_scale + l2_offset, i);
+                ip.emplace_back(base + s_ip, i);
+                l2.emplace_back(s_l2, i);
             }
         }
         const size_t top = std::min<size_t>(kTop, exact.size()), kept = std::min<size_t>(kKept, exact.size());
@@ -574,10 +604,25 @@ bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint
     return total_ip >= total_l2;
 }
 
+// detail::SQ::encode (src/sq.cpp): per dimension round((x - vmin) / step)
+// (0 where step is 0), clamped to the code range; 4-bit codes two to a byte.
+static void sq_encode(const IvfData& x, const float* v, uint8_t* out) {
+    const uint64_t d = x.dim;
+    const float top = static_cast<float>((1 << x.sq_bits) - 1);
+    if (x.sq_bits == 4) std::fill_n(out, (d + 1) / 2, uint8_t{0});
+    for (uint64_t t = 0; t < d; ++t) {
+        const float c = x.sq_step[t] > 0 ? std::round((v[t] - x.sq_vmin[t]) / x.sq_step[t]) : 0.0f;
+        const auto code = static_cast<uint8_t>(std::clamp(c, 0.0f, top));
+        if (x.sq_bits == 8) out[t] = code;
+        else out[t / 2] |= static_cast<uint8_t>(code << ((t & 1) * 4));
+    }
+}
+
+// IVFIndex::add_batch_impl (src/ivf_index.cpp:111-195): rotation, the nearest
+// list of each vector (on the GPU), its PQ code of the residual, SQ code or the
+// raw row, and each list filled in input order.
 void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
     if (!d_->trained) throw std::logic_error("train() has not be run yet");
-    if (!d_->has_pq || !d_->fast_scan)
-        throw std::runtime_error("qann_amd adds to IVF fast-scan indexes only");
     d_->ids.check(n, idv);
     const uint64_t d = d_->dim;
     std::vector<float> rot;
@@ -588,16 +633,20 @@ void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
     std::vector<uint32_t> assign(n);
     assign_gpu(vecs, n, d, d_->centroids.data(), d_->nlist, assign.data());
     const PQ& pq = d_->pq;
-    const std::vector<float> cent_t = pq_transpose(pq);
-    const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
-    std::vector<uint8_t> codes(n * pq.m);
-    parallel_for(n, [&](size_t i) {
-        std::vector<float> r(d);
-        const float* c = rc + size_t{assign[i]} * d;
-        for (uint64_t t = 0; t < d; ++t) r[t] = vecs[i * d + t] - c[t];
-        pq_encode(cent_t, pq.m, pq.sub, pq.K, r.data(), codes.data() + i * pq.m);
-    });
-    // each list filled in input order (ivf_index.cpp:160-194), codes packed with fs_set
+    const uint64_t w = d_->sq_bits ? (d_->sq_bits == 8 ? d : (d + 1) / 2) : d_->has_pq ? pq.m : 0;
+    std::vector<uint8_t> codes(n * w);
+    if (d_->sq_bits) {
+        parallel_for(n, [&](size_t i) { sq_encode(*d_, vecs + i * d, codes.data() + i * w); });
+    } else if (d_->has_pq) {
+        const std::vector<float> cent_t = pq_transpose(pq);
+        const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
+        parallel_for(n, [&](size_t i) {
+            std::vector<float> r(d);
+            const float* c = rc + size_t{assign[i]} * d;
+            for (uint64_t t = 0; t < d; ++t) r[t] = vecs[i * d + t] - c[t];
+            pq_encode(cent_t, pq.m, pq.sub, pq.K, r.data(), codes.data() + i * w);
+        });
+    }
     const uint32_t first = static_cast<uint32_t>(d_->ids.slots);
     std::vector<std::vector<uint32_t>> members(d_->nlist);
     for (uint64_t i = 0; i < n; ++i) members[assign[i]].push_back(static_cast<uint32_t>(i));
@@ -607,13 +656,21 @@ void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
         for (uint32_t i : members[l]) {
             const uint64_t pos = list.ids.size();
             list.ids.push_back(first + i);
-            if (pos % 32 == 0) list.codes.resize(list.codes.size() + bb, 0);
-            uint8_t* block = list.codes.data() + (pos / 32) * bb;
-            const uint64_t v = pos % 32;
-            for (uint64_t s = 0; s < pq.m; ++s) {
-                uint8_t& b = block[s * 16 + (v % 16)];
-                const uint8_t code = codes[i * pq.m + s];
-                b = v < 16 ? static_cast<uint8_t>((b & 0xF0) | code) : static_cast<uint8_t>((b & 0x0F) | (code << 4));
+            if (d_->has_pq && d_->fast_scan) {
+                // fs_set (src/fast_scan.hpp:25-28): vector v of a block in the low
+                // nibble of byte v (v < 16) or the high nibble of byte v - 16
+                if (pos % 32 == 0) list.codes.resize(list.codes.size() + bb, 0);
+                uint8_t* block = list.codes.data() + (pos / 32) * bb;
+                const uint64_t v = pos % 32;
+                for (uint64_t s = 0; s < pq.m; ++s) {
+                    uint8_t& b = block[s * 16 + (v % 16)];
+                    const uint8_t code = codes[i * w + s];
+                    b = v < 16 ? static_cast<uint8_t>((b & 0xF0) | code) : static_cast<uint8_t>((b & 0x0F) | (code << 4));
+                }
+            } else if (w) {
+                list.codes.insert(list.codes.end(), codes.begin() + i * w, codes.begin() + (i + 1) * w);
+            } else {
+                list.vecs.insert(list.vecs.end(), vecs + i * d, vecs + (i + 1) * d);
             }
         }
     });

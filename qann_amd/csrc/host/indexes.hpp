@@ -127,6 +127,8 @@ public:
 
     // IVFIndex::enable_pq (src/ivf_index.cpp:83-94)
     void enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate);
+    // IVFIndex::enable_sq (src/ivf_index.cpp:103-109)
+    void enable_sq(int bits);
     // IVFIndex::train (src/ivf_index.cpp:725-806), with this library's own
     // k-means (the clustering is not restated; the index it makes is in the
     // reference's format and searches identically in both)

@@ -203,7 +203,9 @@ def test_gpu_config_100k_x_128_matches_oracle(tmp_path, oracle):
     assert recall > 0.74, recall
 
 
-@pytest.mark.parametrize("name", ["l2_fs", "ip_fs_custom", "cos_fs_refine", "cos_l2_refine", "rot_refine"])
+@pytest.mark.parametrize("name", ["l2_fs", "ip_fs_custom", "cos_fs_refine", "cos_l2_refine", "rot_refine", "l2_pq8",
+                                  "ip_pq6_custom", "cos_pq8_refine", "l2_sq8", "ip_sq4_custom", "cos_sq8_refine",
+                                  "l2_float", "ip_float_refine"])
 def test_gpu_add_writes_the_reference_bytes(tmp_path, reference, name):
     # IVFIndex::add_batch_impl (src/ivf_index.cpp:111-195): the same vectors added
     # to the same reference-built index by qann_amd (nearest list on the GPU, PQ
@@ -224,3 +226,44 @@ def test_gpu_add_writes_the_reference_bytes(tmp_path, reference, name):
     a = (tmp_path / "ours.qann").read_bytes()
     b = (tmp_path / "ref.qann").read_bytes()
     assert len(a) == len(b) and a == b, f"{name}: files differ ({len(a)} vs {len(b)} bytes)"
+
+
+STORAGES = [
+    # metric, dim, how the lists store vectors, k_factor
+    (qann_amd.Metric.L2, 32, "pq8", 0),
+    (qann_amd.Metric.Cosine, 40, "pq8", 4),
+    (qann_amd.Metric.InnerProduct, 24, "sq4", 0),
+    (qann_amd.Metric.L2, 21, "sq8", 3),
+    (qann_amd.Metric.Cosine, 19, "float", 0),
+    (qann_amd.Metric.InnerProduct, 16, "float", 2),
+]
+
+
+@pytest.mark.parametrize("metric,dim,storage,kf", STORAGES)
+def test_gpu_built_storages_match_oracle(tmp_path, oracle, metric, dim, storage, kf):
+    # indexes of PQ (one byte per subspace), SQ or float lists trained and
+    # filled by qann_amd, searched on the GPU and by the oracle from the file
+    rng = np.random.default_rng(dim * 13 + kf)
+    x = mixture(rng, 8000, dim)
+    q = mixture(rng, 120, dim)
+    ivf = qann_amd.IVFIndex(dim, 100, 8, metric=metric)
+    if storage == "pq8":
+        ivf.enable_pq(dim // 4)
+    elif storage.startswith("sq"):
+        ivf.enable_sq(int(storage[2:]))
+    ivf.train(x, seed=5)
+    idx = qann_amd.RefineIndex(ivf, kf) if kf else ivf
+    idx.add(x)
+    idx.remove(rng.choice(8000, 150, replace=False))
+    path = tmp_path / "built.qann"
+    idx.save(str(path))
+    orc = oracle.load(path)
+    for npb in (1, 8, 30):
+        ivf.nprobe = npb
+        orc.set_nprobe(npb)
+        for k in (1, 10, 64):
+            assert_same(idx.batch_query(q, k), orc.batch_query(q, k), f"{storage} nprobe {npb} k {k}")
+    for size in (200, 4000):  # a direct scan (no PQ, at most 3 nlist) and a probed filter
+        allowed = rng.choice(8000, size, replace=False)
+        assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed),
+                    f"{storage} filtered {size}")
