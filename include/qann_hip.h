@@ -191,6 +191,22 @@ const char* qann_hip_stage_name(int stage);
 /* Device bytes the index holds. */
 int qann_hip_index_bytes(qann_hip_index* index, uint64_t* bytes);
 
+/*
+ * Free and total bytes of the current GPU's memory (hipMemGetInfo). No
+ * counterpart in the reference: for callers that admit work by free memory,
+ * as a serving loop does.
+ */
+int qann_hip_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/*
+ * An upper estimate of the device scratch one qann_hip_search of nq queries
+ * with these parameters takes (it is held only during the call). No
+ * counterpart in the reference: a serving loop sums it over the launches it
+ * runs at once to stay within free memory.
+ */
+int qann_hip_search_scratch_bytes(qann_hip_index* index, uint64_t nq, const qann_hip_search_params* params,
+                                  uint64_t* bytes);
+
 void qann_hip_index_destroy(qann_hip_index* index);
 
 #ifdef __cplusplus

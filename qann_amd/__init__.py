@@ -12,7 +12,15 @@ from __future__ import annotations
 
 import pathlib
 
+import importlib.util
+
 _here = pathlib.Path(__file__).resolve().parent
+# torch ships its own HIP runtime under the same soname as ROCm's. Whichever
+# loads first serves the process, and torch only works with its own; so when
+# torch is installed it is imported first, and both share its runtime.
+if importlib.util.find_spec("torch") is not None:
+    import torch  # noqa: F401
+
 try:
     from . import _qann  # noqa: F401  (loads libqann_hip.so through its rpath)
 except ImportError as e:  # pragma: no cover - exercised only on a broken build
@@ -26,15 +34,18 @@ from ._qann import (  # noqa: E402
     IVFIndex,
     Metric,
     RefineIndex,
+    Server,
+    ServeFuture,
     get_num_threads,
     load,
+    serve_load,
     set_num_threads,
 )
 
 Index = DenseIndex
 __version__ = "0.1.0"
 __all__ = ["DenseIndex", "Index", "FlatIndex", "IVFIndex", "RefineIndex", "Metric", "load", "set_num_threads",
-           "get_num_threads", "native_library"]
+           "get_num_threads", "native_library", "Server", "ServeFuture", "serve_load"]
 
 
 def native_library() -> pathlib.Path:

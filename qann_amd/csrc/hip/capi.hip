@@ -1054,6 +1054,39 @@ const char* qann_hip_stage_name(int stage) {
     return stage >= 0 && stage < QANN_STAGE_COUNT ? names[stage] : "";
 }
 
+int qann_hip_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guarded([&] {
+        size_t f = 0, t = 0;
+        check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+    });
+}
+
+int qann_hip_search_scratch_bytes(qann_hip_index* h, uint64_t nq, const qann_hip_search_params* p, uint64_t* bytes) {
+    return guarded([&] {
+        if (!h || !p || !bytes) invalid("null argument");
+        const uint64_t k = p->k;
+        uint64_t b = 0;
+        if (h->kind == 1) {
+            b = std::min<uint64_t>(nq * h->slots * 4, uint64_t{1} << 32) + nq * k * 16;
+        } else {
+            const qann_hip_index* ivfh = h->kind == 3 ? h->base : h;
+            const qd::DevIvf& ix = ivfh->ivf;
+            const uint64_t nlist = static_cast<uint64_t>(ix.nlist);
+            const uint64_t M = p->allowed_words ? std::min<uint64_t>(nlist, qd::kSelectMax) : std::max<uint32_t>(p->nprobe, 1);
+            const uint64_t kp = h->kind == 3 ? k * std::max<uint32_t>(p->k_factor, 1) : k;
+            const uint64_t vb = ix.storage == QANN_IVF_FASTSCAN ? 2 : 4;
+            const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];
+            const uint64_t stride = std::max<uint64_t>(wblocks, 1) * qd::kFsBlock;
+            const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * vb)));
+            b = nq * nlist * 4 + nq * M * 34 + nq * static_cast<uint64_t>(ix.m) * std::max(ix.K, 16) * 4 +
+                chunk * stride * vb * 33 / 32 + nq * kp * 24 + nq * ix.dim * 4 + (uint64_t{1} << 20);
+        }
+        *bytes = b;
+    });
+}
+
 int qann_hip_index_bytes(qann_hip_index* h, uint64_t* bytes) {
     return guarded([&] { *bytes = h ? h->bytes : 0; });
 }

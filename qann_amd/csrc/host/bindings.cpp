@@ -4,6 +4,7 @@
 // index (and a RefineIndex's base) shared; writes hold them exclusively
 // (read_locked / write_locked, python_bindings.cpp:97-131).
 #include "indexes.hpp"
+#include "serve.hpp"
 
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -132,6 +133,51 @@ py::array_t<int64_t> add(DenseIndex& self, FloatArray data, const py::object& id
     py::array_t<int64_t> res(static_cast<py::ssize_t>(out.size()));
     std::copy(out.begin(), out.end(), res.mutable_data());
     return res;
+}
+
+// A pending serving result: result() waits (without the GIL) and returns
+// (ids, distances, latency_ms) with the request's shape.
+struct ServeFuture {
+    std::shared_ptr<std::future<ServeResult>> f;
+    uint64_t nq;
+    uint32_t k;
+    py::tuple result() {
+        ServeResult r;
+        {
+            py::gil_scoped_release release;
+            r = f->get();
+        }
+        py::array_t<int64_t> ids({static_cast<py::ssize_t>(nq), static_cast<py::ssize_t>(k)});
+        py::array_t<float> dists({static_cast<py::ssize_t>(nq), static_cast<py::ssize_t>(k)});
+        std::copy(r.ids.begin(), r.ids.end(), ids.mutable_data());
+        std::copy(r.dists.begin(), r.dists.end(), dists.mutable_data());
+        return py::make_tuple(ids, dists, r.latency_ms);
+    }
+};
+
+std::shared_ptr<const SlotFilter> slot_filter(const DenseIndex& idx, py::handle allowed) {
+    py::array_t<int64_t> a = as_ids(allowed);
+    auto f = std::make_shared<SlotFilter>();
+    read_locked(idx, [&] {
+        *f = idx.ids().filter(a.data(), static_cast<size_t>(a.size()));
+        return 0;
+    });
+    return f;
+}
+
+py::dict stats_dict(const ServeStats& s) {
+    py::dict d;
+    d["requests"] = s.requests;
+    d["queries"] = s.queries;
+    d["launches"] = s.launches;
+    d["p50_ms"] = s.p50_ms;
+    d["p90_ms"] = s.p90_ms;
+    d["p99_ms"] = s.p99_ms;
+    d["mean_ms"] = s.mean_ms;
+    d["max_ms"] = s.max_ms;
+    d["seconds"] = s.seconds;
+    d["qps"] = s.seconds > 0 ? static_cast<double>(s.queries) / s.seconds : 0.0;
+    return d;
 }
 
 } // namespace
@@ -301,4 +347,66 @@ PYBIND11_MODULE(_qann, m) {
               return load_index(path);
           },
           py::arg("path"), py::arg("mmap") = false, "Load an index saved by qann or qann_amd");
+
+    py::class_<ServeFuture>(m, "ServeFuture", "A pending serving result")
+        .def("result", &ServeFuture::result, "Waits; returns (ids, distances, latency_ms)");
+
+    py::class_<Server, std::shared_ptr<Server>>(m, "Server",
+                                                "A serving loop: requests of any size, k, nprobe and filter, coalesced "
+                                                "into launches and run concurrently within a device-memory budget")
+        .def(py::init([](std::shared_ptr<DenseIndex> index, int workers, uint64_t max_batch, double memory_fraction) {
+                 py::gil_scoped_release release;
+                 return std::make_shared<Server>(std::move(index), workers, max_batch, memory_fraction);
+             }),
+             py::arg("index"), py::kw_only(), py::arg("workers") = 4, py::arg("max_batch") = 4096,
+             py::arg("memory_fraction") = 0.5)
+        .def("submit",
+             [](Server& self, py::handle queries, uint32_t k, uint32_t nprobe, py::handle allowed) {
+                 FloatArray q = FloatArray::ensure(queries);
+                 if (!q || q.ndim() != 2 || static_cast<uint64_t>(q.shape(1)) != self.index().dim())
+                     throw py::value_error("queries must be a 2-D array with the index's dimension");
+                 ServeRequest r;
+                 r.nq = static_cast<uint64_t>(q.shape(0));
+                 r.k = k;
+                 r.nprobe = nprobe;
+                 r.queries.assign(q.data(), q.data() + q.size());
+                 if (!allowed.is_none()) r.filter = slot_filter(self.index(), allowed);
+                 const uint64_t nq = r.nq;
+                 auto f = std::make_shared<std::future<ServeResult>>(self.submit(std::move(r)));
+                 return ServeFuture{f, nq, k};
+             },
+             py::arg("queries"), py::arg("k"), py::kw_only(), py::arg("nprobe") = 0, py::arg("allowed") = py::none())
+        .def("stats", [](Server& self, bool reset) { return stats_dict(self.stats(reset)); }, py::arg("reset") = false)
+        .def_property_readonly("memory_budget", &Server::memory_budget)
+        .def("close", [](Server& self) {
+            py::gil_scoped_release release;
+            self.close();
+        });
+
+    m.def("serve_load",
+          [](Server& server, py::handle pool, double rate, double seconds, uint64_t seed, std::vector<uint32_t> ks,
+             std::vector<uint32_t> nprobes, double filtered_fraction, py::list filters, uint32_t max_nq) {
+              FloatArray p = FloatArray::ensure(pool);
+              if (!p || p.ndim() != 2 || static_cast<uint64_t>(p.shape(1)) != server.index().dim())
+                  throw py::value_error("pool must be a 2-D array with the index's dimension");
+              LoadSpec spec;
+              spec.rate = rate;
+              spec.seconds = seconds;
+              spec.seed = seed;
+              spec.ks = std::move(ks);
+              spec.nprobes = std::move(nprobes);
+              spec.filtered_fraction = filtered_fraction;
+              spec.max_nq = max_nq;
+              for (py::handle f : filters) spec.filters.push_back(slot_filter(server.index(), f));
+              ServeStats st;
+              {
+                  py::gil_scoped_release release;
+                  st = run_load(server, p.data(), static_cast<uint64_t>(p.shape(0)), spec);
+              }
+              return stats_dict(st);
+          },
+          py::arg("server"), py::arg("pool"), py::arg("rate"), py::arg("seconds"), py::kw_only(), py::arg("seed") = 1,
+          py::arg("ks") = std::vector<uint32_t>{10}, py::arg("nprobes") = std::vector<uint32_t>{0},
+          py::arg("filtered_fraction") = 0.0, py::arg("filters") = py::list(), py::arg("max_nq") = 256,
+          "An open-loop Poisson load of requests on a server; returns its latency percentiles and throughput");
 }

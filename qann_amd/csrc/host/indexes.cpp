@@ -254,6 +254,37 @@ void DenseIndex::batch_query(const float* queries, uint64_t nq, uint32_t k, cons
         for (size_t i = 0; i < total; ++i) out_dists[i] *= 0.5f;
 }
 
+void DenseIndex::search_prepared(const float* prepared, uint64_t nq, uint32_t k, uint32_t nprobe,
+                                 const SlotFilter* filter, int64_t* out_ids, float* out_dists) const {
+    const size_t total = static_cast<size_t>(nq) * k;
+    std::fill_n(out_ids, total, int64_t{-1});
+    std::fill_n(out_dists, total, INFINITY);
+    if (nq == 0 || k == 0 || (filter && filter->count == 0)) return;
+    qann_hip_search_params p{};
+    fill_params(&p);
+    p.k = k;
+    if (nprobe) p.nprobe = nprobe;
+    if (filter) {
+        p.allowed_words = filter->words.data();
+        p.allowed_count = filter->count;
+    }
+    check_status(qann_hip_search(device(), prepared, nq, &p, out_ids, out_dists));
+    if (metric() == Cosine)
+        for (size_t i = 0; i < total; ++i) out_dists[i] *= 0.5f;
+}
+
+uint64_t DenseIndex::scratch_bytes(uint64_t nq, uint32_t k, uint32_t nprobe, bool filtered) const {
+    qann_hip_search_params p{};
+    fill_params(&p);
+    p.k = k;
+    if (nprobe) p.nprobe = nprobe;
+    static const uint64_t word = 1;
+    if (filtered) p.allowed_words = &word;
+    uint64_t b = 0;
+    check_status(qann_hip_search_scratch_bytes(device(), nq, &p, &b));
+    return b;
+}
+
 void DenseIndex::search_device(const float* d_queries, uint64_t nq, uint32_t k, int64_t* d_ids, float* d_dists,
                                void* stream) {
     qann_hip_search_params p{};

@@ -42,6 +42,16 @@ public:
     // enqueued on stream (qann_hip_search_async). No filter.
     void search_device(const float* d_queries, uint64_t nq, uint32_t k, int64_t* d_ids, float* d_dists, void* stream);
 
+    // batch_query's search on queries already prepared, with a slot filter (or
+    // null) and an nprobe for IVF (0: the index's own); distances converted and
+    // rows padded as batch_query leaves them. The caller holds the locks.
+    void search_prepared(const float* prepared, uint64_t nq, uint32_t k, uint32_t nprobe, const SlotFilter* filter,
+                         int64_t* out_ids, float* out_dists) const;
+    // An upper estimate of the device scratch such a search takes.
+    uint64_t scratch_bytes(uint64_t nq, uint32_t k, uint32_t nprobe, bool filtered) const;
+    // the queries prepared for the metric (unit length for Cosine)
+    std::vector<float> prepare(const float* q, uint64_t n) const;
+
     // add_batch (src/index.cpp:33-49): returns the added ids.
     std::vector<int64_t> add(const float* vecs, uint64_t n, const int64_t* ids);
     virtual size_t remove(const int64_t* ids, size_t n) = 0;
@@ -68,7 +78,6 @@ protected:
     // builds or refreshes the device copy; called with mu held shared
     virtual qann_hip_index* device() const = 0;
     virtual void fill_params(void* params) const = 0;
-    std::vector<float> prepare(const float* q, uint64_t n) const;
     mutable std::mutex dev_mu;
 };
 

@@ -1,0 +1,134 @@
+"""Serving benchmark: the serving configuration on one MI355X.
+
+Index: 10,000,000 x 96 float32 L2 vectors from a 4,096-centre Gaussian mixture
+(centres U[0,1)^96, sigma 0.05, seed 4); IVF nlist 16384 with 4-bit PQ M=48
+fast-scan; a RefineIndex (k_factor 10) whose raw vectors stay resident in HBM.
+Load: a Poisson stream of requests, each a batch of 1-256 queries (log-uniform),
+k from {10, 100}, nprobe from {16, 64}; 30% of requests carry an allowed-id
+set of 10% of the ids (one of eight seeded sets). The saturated throughput is
+measured first (requests offered far faster than served), then p50 / p99
+latency at 25%, 50% and 90% of it. recall@10 is measured on 1,000 queries
+against exact search, and a sample of served results is checked against the
+same queries run alone.
+
+Run from the repository root:  python3 -m qann_amd.serving_bench [--scale 0.1]
+Prints one JSON object (last line of stdout); progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_data(n, nq, dim, centres, sigma, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = rng.random((centres, dim), dtype=np.float32)
+    x = np.empty((n, dim), np.float32)
+    for s in range(0, n, 1_000_000):
+        e = min(n, s + 1_000_000)
+        x[s:e] = c[rng.integers(0, centres, e - s)] + sigma * rng.standard_normal((e - s, dim), dtype=np.float32)
+    q = c[rng.integers(0, centres, nq)] + sigma * rng.standard_normal((nq, dim), dtype=np.float32)
+    return x, q.astype(np.float32)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 10M vectors (and of nlist)")
+    ap.add_argument("--seconds", type=float, default=6.0, help="length of each load level")
+    ap.add_argument("--workers", type=int, default=4)
+    args = ap.parse_args()
+
+    import torch  # noqa: F401  (first: torch and the library share the HIP runtime)
+    import qann_amd
+
+    n = int(10_000_000 * args.scale)
+    nlist = max(256, int(16384 * args.scale))
+    dim, m, kf = 96, 48, 10
+    t0 = time.perf_counter()
+    x, pool = make_data(n, 20_000, dim, 4096, 0.05, 4)
+    log(f"data {x.shape} in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    ivf = qann_amd.IVFIndex(dim, nlist, 16)
+    ivf.enable_pq(m, fast_scan=True)
+    rng = np.random.default_rng(4)
+    ivf.train(x[rng.choice(n, min(n, 64 * nlist), replace=False)], max_iters=10, seed=4)
+    index = qann_amd.RefineIndex(ivf, kf)
+    index.add(x)
+    index.batch_query(pool[:8], 10)  # uploads the index
+    log(f"index built in {time.perf_counter() - t0:.1f}s")
+
+    # recall@10 of 1,000 queries at each nprobe against exact search (torch, FP32)
+    import torch as th
+
+    xs = th.from_numpy(x).cuda()
+    xsq = (xs * xs).sum(1)[None, :]
+    truth = np.empty((1000, 10), dtype=np.int64)
+    for s in range(0, 1000, 100):
+        qq = th.from_numpy(pool[s:s + 100]).cuda()
+        d2 = (qq * qq).sum(1, keepdim=True) - 2 * qq @ xs.T + xsq
+        truth[s:s + 100] = d2.topk(10, dim=1, largest=False).indices.cpu().numpy()
+    del xs, xsq, qq, d2
+    th.cuda.empty_cache()
+    recall = {}
+    for npb in (16, 64):
+        ivf.nprobe = npb
+        ids, _ = index.batch_query(pool[:1000], 10)
+        recall[npb] = float(np.mean([len(set(a) & set(b)) / 10 for a, b in zip(ids, truth)]))
+    log(f"recall@10: {recall}")
+
+    ids_all = np.arange(n, dtype=np.int64)
+    frng = np.random.default_rng(5)
+    filters = [np.sort(frng.choice(ids_all, n // 10, replace=False)) for _ in range(8)]
+    spec = dict(ks=[10, 100], nprobes=[16, 64], filtered_fraction=0.3, filters=filters, max_nq=256)
+
+    server = qann_amd.Server(index, workers=args.workers, max_batch=4096)
+    # served results are each query's own: a sample checked against single calls
+    checks = []
+    for r in range(40):
+        lo = int(frng.integers(0, 19_000))
+        nq = int(frng.integers(1, 257))
+        k = int(frng.choice([10, 100]))
+        npb = int(frng.choice([16, 64]))
+        allowed = filters[r % 8] if r % 3 == 0 else None
+        checks.append((lo, nq, k, npb, allowed, server.submit(pool[lo:lo + nq], k, nprobe=npb, allowed=allowed)))
+    same = 0
+    for lo, nq, k, npb, allowed, f in checks:
+        got_ids, got_d, _ = f.result()
+        ivf.nprobe = npb
+        want_ids, want_d = index.batch_query(pool[lo:lo + nq], k, allowed=allowed)
+        same += int(np.array_equal(got_ids, want_ids) and np.array_equal(got_d.view(np.uint32), want_d.view(np.uint32)))
+    log(f"served results identical to single calls: {same}/{len(checks)}")
+
+    mean_nq = float(np.mean(np.clip(np.rint(np.exp(np.random.default_rng(0).random(200_000) * np.log(256))), 1, 256)))
+    sat = qann_amd.serve_load(server, pool, rate=20_000.0, seconds=args.seconds / 2, seed=11, **spec)
+    sat_qps = sat["qps"]
+    log(f"saturated: {sat_qps:,.0f} queries/s over {sat['requests']} requests in {sat['launches']} launches")
+    levels = {}
+    for frac in (0.25, 0.5, 0.9):
+        rate = frac * sat_qps / mean_nq
+        st = qann_amd.serve_load(server, pool, rate=rate, seconds=args.seconds, seed=int(frac * 100), **spec)
+        levels[f"{int(frac * 100)}%"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in st.items()}
+        log(f"{int(frac * 100)}% load ({rate:,.0f} requests/s): {st['qps']:,.0f} queries/s, p50 {st['p50_ms']:.2f} ms, "
+            f"p99 {st['p99_ms']:.2f} ms, {st['launches']} launches for {st['requests']} requests")
+    server.close()
+    print(json.dumps({
+        "workload": f"serving: {n:,} x {dim} L2, IVF{nlist} + PQ{m}x4 fast-scan + RefineIndex(k_factor={kf}) in HBM; "
+                    "Poisson requests of 1-256 queries, k in {10, 100}, nprobe in {16, 64}, 30% filtered at 10%",
+        "saturated_qps": round(sat_qps, 1),
+        "levels": levels,
+        "recall@10": recall,
+        "served_equal_to_single_calls": f"{same}/{len(checks)}",
+        "workers": args.workers,
+    }), flush=True)
+
+
+if __name__ == "__main__":
+    main()
