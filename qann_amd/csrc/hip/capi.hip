@@ -267,9 +267,20 @@ void fill_empty(int64_t* d_ids, float* d_dists, size_t n, hipStream_t s) {
     check_launch("k_fill_results");
 }
 
+// *flag = 1 when any of n statuses is set.
+__global__ void k_any_set(const int* status, int n, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && status[i]) atomicOr(flag, 1);
+}
+
 // The device search of an index for nq prepared queries already in HBM.
+// A filtered IVF search selects the nearest M lists of each query and probes
+// them in order until IVFIndex::scan's stopping rule holds. With d_more set,
+// M is a first guess (twice the probes the rule asks for at least) and a query
+// that would need more sets *d_more; the caller then searches again with every
+// list it may need (at most 4096). Without it, M is that limit at once.
 void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip_search_params& p,
-                const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc) {
+                const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc, int* d_more = nullptr) {
     hipStream_t st = sc.stream;
     const uint32_t k = p.k;
     if (nq == 0 || k == 0) return;
@@ -408,8 +419,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const uint32_t nlist = static_cast<uint32_t>(ix.nlist);
     const uint32_t nprobe = p.nprobe;
     if (nprobe == 0 || nprobe > nlist) invalid("nprobe must be in [1, nlist]");
-    const uint32_t M = filtered ? std::min<uint32_t>(nlist, qd::kSelectMax) : nprobe;
-    if (M > static_cast<uint32_t>(qd::kSelectMax)) invalid("nprobe above 4096 is not supported");
+    if (nprobe > static_cast<uint32_t>(qd::kSelectMax)) invalid("nprobe above 4096 is not supported");
     const bool fs = ix.storage == QANN_IVF_FASTSCAN;
     const bool has_pq = fs || ix.storage == QANN_IVF_PQ;
     const bool cos_pq = has_pq && ix.metric == QANN_METRIC_COSINE && ix.cosine_ip;
@@ -432,6 +442,10 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             min_probes = static_cast<int>(std::clamp<size_t>(static_cast<size_t>(std::min(want, double(nlist))), nprobe, nlist));
         }
     }
+    const uint32_t M_all = std::min<uint32_t>(nlist, qd::kSelectMax);
+    const uint32_t M = !filtered ? nprobe
+                       : d_more  ? std::min<uint32_t>(M_all, std::max<uint32_t>(64, 2 * static_cast<uint32_t>(min_probes)))
+                                 : M_all;
     // sums of one query take at most the blocks of its M largest probed lists;
     // queries go through the two scan phases in chunks that bound that buffer
     const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];
@@ -495,11 +509,17 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     check_launch("k_probe_plan");
     t_sel.done();
     if (filtered && M < nlist) {
-        std::vector<int> hs(nq);
-        check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        for (int v : hs)
-            if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
+        if (d_more && M < M_all) {
+            hipLaunchKernelGGL(k_any_set, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0, st, status,
+                               static_cast<int>(nq), d_more);
+            check_launch("k_any_set");
+        } else {
+            std::vector<int> hs(nq);
+            check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            for (int v : hs)
+                if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
+        }
     }
     float* C = nullptr;
     uint8_t* lut = nullptr;
@@ -942,7 +962,17 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
                 check(hipMemcpyAsync(df, params->allowed_words, (h->slots + 63) / 64 * 8, hipMemcpyHostToDevice,
                                      io->stream),
                       "hipMemcpyAsync");
-            run_search(h, dq, nq, *params, df, dids, dd, *io);
+            int* dmore = df ? reinterpret_cast<int*>(buf + qb + ob + db + fb) : nullptr;
+            if (dmore) {
+                hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, io->stream, reinterpret_cast<uint32_t*>(dmore), 1);
+                check_launch("k_zero_u32");
+            }
+            run_search(h, dq, nq, *params, df, dids, dd, *io, dmore);
+            int more = 0;
+            if (dmore) check(hipMemcpyAsync(&more, dmore, 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
+            check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
+            // some query needs more lists than the first guess: every list it may need
+            if (more) run_search(h, dq, nq, *params, df, dids, dd, *io);
             check(hipMemcpyAsync(out_ids, dids, nq * k * 8, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipMemcpyAsync(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
