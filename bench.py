@@ -102,6 +102,20 @@ def ground_truth(x, q, k):
     return ids, q.shape[0] / best
 
 
+PROFILE = ROOT / "profiles" / "ivf_fastscan_1.2Mx100_nprobe128.md"
+
+
+def measured_traffic(nprobe):
+    """HBM bytes per k_fs_sums launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs of this workload (scaled by a copy of known size), or None."""
+    if not PROFILE.exists():
+        return None
+    for line in PROFILE.read_text().splitlines():
+        if line.startswith(f"traffic_bytes_per_launch k_fs_sums nprobe {nprobe}:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def recall(ids, gt):
     k = gt.shape[1]
     hits = [len(set(a[:k]) & set(b)) for a, b in zip(ids, gt)]
