@@ -131,10 +131,11 @@ def hip_lib():
 
 
 def stage_times(lib, reset=True):
-    ms = (ctypes.c_double * 8)()
-    n = (ctypes.c_uint64 * 8)()
+    ms = (ctypes.c_double * 32)()
+    n = (ctypes.c_uint64 * 32)()
     lib.qann_hip_profile_read(ms, n, int(reset))
-    return {lib.qann_hip_stage_name(i).decode(): (ms[i], n[i]) for i in range(8)}
+    names = [lib.qann_hip_stage_name(i).decode() for i in range(32)]
+    return {name: (ms[i], n[i]) for i, name in enumerate(names) if name}
 
 
 def cpu_baseline(path, q, nprobe, gpu_ids, gpu_d, budget_s):

@@ -166,7 +166,10 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
  * Per-stage kernel timing, the device counterpart of the reference's
  * compile-time section timers (src/profile.hpp: coarse, table, scan, drain,
  * rerank). QANN_STAGE_SCAN is the fast-scan sums kernel, QANN_STAGE_REPLAY the
- * kernel replaying the TopK pushes over them.
+ * kernel replaying the TopK pushes over them. QANN_STAGE_FLAT is a FlatIndex
+ * search by the BF16 GEMM (or, for fewer than 8 queries, by direct exact
+ * scoring), QANN_STAGE_FLAT_FP32 one by the FP32 GEMM, queries the BF16 path
+ * hands over included.
  * While enabled, every search synchronizes its stream after each stage and
  * adds the stage's HIP-event time to a process-wide total; read returns the
  * totals in milliseconds and launch counts for QANN_STAGE_COUNT stages, in the
@@ -182,7 +185,8 @@ enum {
     QANN_STAGE_RERANK = 5,
     QANN_STAGE_FLAT = 6,
     QANN_STAGE_REPLAY = 7,
-    QANN_STAGE_COUNT = 8,
+    QANN_STAGE_FLAT_FP32 = 8,
+    QANN_STAGE_COUNT = 9,
 };
 int qann_hip_profile_enable(int on);
 int qann_hip_profile_read(double* ms, uint64_t* launches, int reset);

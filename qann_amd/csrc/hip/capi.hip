@@ -2,11 +2,13 @@
 // pipelines that run on them.
 #include "../../../include/qann_hip.h"
 #include "device_common.hpp"
+#include "gemm_bf16.hpp"
 #include "ivf_kernels.hpp"
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -154,6 +156,13 @@ struct qann_hip_index {
     float* rows = nullptr;
     float* row_sq = nullptr; // flat: squared norms (exact order), for the GEMM path
     float xnorm_max = 0.0f;  // flat: an upper bound on every row's norm
+    // flat: BF16 copy of the rows (zero-padded to kp) and, on the device, the
+    // largest rounding residual and BF16 norm; null below kFlatBf16MinRows
+    uint16_t* rows_bf16 = nullptr;
+    float* bf16_maxes = nullptr;
+    int kp = 0;
+    // cleared when most queries of a search needed the FP32 path
+    std::atomic<bool> bf16_ok{true};
     qann_hip_index* base = nullptr;
     std::vector<void*> allocs;
     uint64_t bytes = 0;
@@ -267,6 +276,151 @@ void fill_empty(int64_t* d_ids, float* d_dists, size_t n, hipStream_t s) {
     check_launch("k_fill_results");
 }
 
+__global__ void k_gather_rows(const float* src, const uint32_t* idx, uint32_t cnt, uint32_t dim, float* dst) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<uint64_t>(cnt) * dim) return;
+    const uint64_t r = t / dim, e = t % dim;
+    dst[t] = src[static_cast<uint64_t>(idx[r]) * dim + e];
+}
+
+__global__ void k_scatter_results(const int64_t* sids, const float* sd, const uint32_t* idx, uint32_t cnt, uint32_t k,
+                                  int64_t* ids, float* dists) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<uint64_t>(cnt) * k) return;
+    const uint64_t r = t / k, j = t % k, o = static_cast<uint64_t>(idx[r]) * k + j;
+    ids[o] = sids[t];
+    dists[o] = sd[t];
+}
+
+// FlatIndex rows below this count keep to the FP32 GEMM path.
+constexpr uint64_t kFlatBf16MinRows = 65536;
+
+// FlatIndex::query_impl by the FP32 MFMA GEMM: scores of a chunk of queries
+// against every row into D (chunk * n floats), then per query the exact k
+// nearest from a provably sufficient shortlist.
+void flat_fp32(qann_hip_index* h, const float* d_q, uint64_t nq, uint32_t k, const uint64_t* excl,
+               const uint64_t* incl, int64_t* d_ids, float* d_dists, float* D, uint64_t chunk, hipStream_t st) {
+    const uint64_t n = h->slots;
+    const int ip = h->metric == QANN_METRIC_IP;
+    for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
+        const uint64_t nc = std::min(chunk, nq - q0);
+        const float* qc = d_q + q0 * h->dim;
+        qd::launch_coarse_gemm(qc, static_cast<int>(nc), h->rows, h->row_sq, static_cast<int>(n),
+                               static_cast<int>(h->dim), ip, D, st, excl, incl);
+        check_launch("k_coarse_gemm");
+        qd::launch_flat_select(D, qc, static_cast<int>(nc), h->rows, static_cast<int>(n), static_cast<int>(h->dim), ip,
+                               h->xnorm_max, static_cast<int>(k), excl, incl, h->labels, d_ids + q0 * k,
+                               d_dists + q0 * k, st);
+        check_launch("k_flat_select");
+    }
+}
+
+uint64_t flat_fp32_chunk(uint64_t nq, uint64_t n) {
+    return std::max<uint64_t>(1, std::min<uint64_t>(nq, (uint64_t{1} << 32) / (n * 4)));
+}
+
+// The BF16 plan of a flat search: candidates per query come from rows under
+// a limit that a strided sample of the rows gives (about want * stride rows
+// each); the stride keeps them well inside the candidate buffer.
+struct FlatBf16Plan {
+    uint32_t stride = 0, ns = 0;
+    uint64_t chunk = 0, per_query = 0;
+};
+
+FlatBf16Plan flat_bf16_plan(const qann_hip_index* h, uint64_t nq, uint32_t want) {
+    FlatBf16Plan p;
+    p.stride = std::clamp<uint32_t>(static_cast<uint32_t>(qd::kFlatCandCap) / (4 * std::max<uint32_t>(want, 1)), 4, 64);
+    p.ns = static_cast<uint32_t>((h->slots + p.stride - 1) / p.stride);
+    p.per_query = pad256(static_cast<uint64_t>(h->kp) * 2) + 12 + static_cast<uint64_t>(p.ns) * 4 + 8 +
+                  static_cast<uint64_t>(qd::kFlatCandCap) * 8;
+    p.chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, (uint64_t{1} << 30) / p.per_query));
+    return p;
+}
+
+// FlatIndex::query_impl by the BF16 GEMM with exact rescoring (gemm_bf16.hpp).
+// Returns the queries it could not finish, for flat_fp32_subset.
+std::vector<uint32_t> flat_bf16(qann_hip_index* h, const float* d_q, uint64_t nq, uint32_t k, uint32_t want,
+                                const uint64_t* excl, const uint64_t* incl, int64_t* d_ids, float* d_dists,
+                                Scratch& sc) {
+    hipStream_t st = sc.stream;
+    const FlatBf16Plan pl = flat_bf16_plan(h, nq, want);
+    const uint64_t ch = pl.chunk;
+    sc.used = 0;
+    sc.reserve(pad256(ch * h->kp * 2) + pad256(ch * 12) + pad256(ch * pl.ns * 4) + 2 * pad256(ch * 4) +
+               2 * pad256(ch * qd::kFlatCandCap * 4) + pad256(nq * 4) + 256);
+    uint16_t* qb = sc.take<uint16_t>(ch * h->kp);
+    float* qinfo = sc.take<float>(ch * 3);
+    float* Ds = sc.take<float>(ch * pl.ns);
+    float* lim = sc.take<float>(ch);
+    uint32_t* cnt = sc.take<uint32_t>(ch);
+    float* cd = sc.take<float>(ch * qd::kFlatCandCap);
+    uint32_t* ci = sc.take<uint32_t>(ch * qd::kFlatCandCap);
+    uint32_t* ovf = sc.take<uint32_t>(nq);
+    uint32_t* ovf_cnt = sc.take<uint32_t>(1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ovf_cnt, 1u);
+    qd::FlatBf16 f{};
+    f.xb = h->rows_bf16;
+    f.rows = h->rows;
+    f.xsq = h->row_sq;
+    f.maxes = h->bf16_maxes;
+    f.xnorm_max = h->xnorm_max;
+    f.n = static_cast<int>(h->slots);
+    f.dim = static_cast<int>(h->dim);
+    f.kp = h->kp;
+    f.ip = h->metric == QANN_METRIC_IP;
+    f.excl = excl;
+    f.incl = incl;
+    f.labels = h->labels;
+    for (uint64_t q0 = 0; q0 < nq; q0 += ch) {
+        const int nc = static_cast<int>(std::min(ch, nq - q0));
+        const float* qc = d_q + q0 * h->dim;
+        qd::launch_queries_bf16(qc, nc, f.dim, f.kp, qb, qinfo, st);
+        check_launch("k_queries_bf16");
+        qd::launch_flat_bf16(f, qc, qb, qinfo, nc, static_cast<int>(k), static_cast<int>(want),
+                             static_cast<int>(pl.stride), static_cast<int>(pl.ns), Ds, lim, cnt, cd, ci,
+                             d_ids + q0 * k, d_dists + q0 * k, ovf, ovf_cnt, static_cast<int>(q0), st);
+        check_launch("k_flat_bf16");
+    }
+    uint32_t novf = 0;
+    check(hipMemcpyAsync(&novf, ovf_cnt, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    std::vector<uint32_t> list(novf);
+    if (novf == 0) return list;
+    if (2ull * novf > nq) h->bf16_ok = false;
+    check(hipMemcpyAsync(list.data(), ovf, novf * 4ull, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return list;
+}
+
+// flat_fp32 for the listed queries of a batch, gathered and scattered back.
+void flat_fp32_subset(qann_hip_index* h, const float* d_q, const std::vector<uint32_t>& list, uint32_t k,
+                      const uint64_t* excl, const uint64_t* incl, int64_t* d_ids, float* d_dists, Scratch& sc) {
+    hipStream_t st = sc.stream;
+    const uint32_t novf = static_cast<uint32_t>(list.size());
+    const uint64_t fch = flat_fp32_chunk(novf, h->slots);
+    sc.used = 0;
+    sc.reserve(pad256(novf * 4ull) + pad256(novf * static_cast<uint64_t>(h->dim) * 4) + pad256(novf * k * 8ull) +
+               pad256(novf * k * 4ull) +
+               pad256(fch * h->slots * 4));
+    uint32_t* d_list = sc.take<uint32_t>(novf);
+    float* qg = sc.take<float>(novf * static_cast<uint64_t>(h->dim));
+    int64_t* gids = sc.take<int64_t>(novf * static_cast<uint64_t>(k));
+    float* gd = sc.take<float>(novf * static_cast<uint64_t>(k));
+    float* D = sc.take<float>(fch * h->slots);
+    check(hipMemcpyAsync(d_list, list.data(), novf * 4ull, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    const uint64_t ge = static_cast<uint64_t>(novf) * h->dim;
+    hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>((ge + 255) / 256)), dim3(256), 0, st, d_q, d_list,
+                       novf, h->dim, qg);
+    check_launch("k_gather_rows");
+    flat_fp32(h, qg, novf, k, excl, incl, gids, gd, D, fch, st);
+    const uint64_t re = static_cast<uint64_t>(novf) * k;
+    hipLaunchKernelGGL(k_scatter_results, dim3(static_cast<unsigned>((re + 255) / 256)), dim3(256), 0, st, gids, gd,
+                       d_list, novf, k, d_ids, d_dists);
+    check_launch("k_scatter_results");
+    // list and the staging above must outlive the copies
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
 // *flag = 1 when any of n statuses is set.
 __global__ void k_any_set(const int* status, int n, int* flag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -301,27 +455,32 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         const bool gemm = nq >= 8 && n > 0 && n <= 0x7FFFFFFFull && k <= 4096 && qlds + fcap * 8 <= qd::kLdsMax &&
                           (!filtered || p.allowed_count * 8 >= n);
         if (gemm) {
-            // MFMA scores of a chunk of queries against every row, then per query the
-            // exact k nearest from a provably sufficient shortlist
-            const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, (uint64_t{1} << 32) / (n * 4)));
-            sc.used = 0;
-            sc.reserve(pad256(chunk * n * 4));
-            float* D = sc.take<float>(chunk * n);
-            const uint64_t* excl = filtered ? nullptr : h->deleted;
+            const uint64_t* excl = filtered ? nullptr : (h->live < h->slots ? h->deleted : nullptr);
             const uint64_t* incl = filtered ? d_filter : nullptr;
+            const uint64_t eligible = filtered ? p.allowed_count : h->live;
+            const uint32_t want = static_cast<uint32_t>(std::min<uint64_t>(k, eligible));
             StageTimer t(QANN_STAGE_FLAT, st);
-            for (uint64_t q0 = 0; q0 < nq; q0 += chunk) {
-                const uint64_t nc = std::min(chunk, nq - q0);
-                const float* qc = d_q + q0 * h->dim;
-                qd::launch_coarse_gemm(qc, static_cast<int>(nc), h->rows, h->row_sq, static_cast<int>(n),
-                                       static_cast<int>(h->dim), ip, D, st, excl, incl);
-                check_launch("k_coarse_gemm");
-                qd::launch_flat_select(D, qc, static_cast<int>(nc), h->rows, static_cast<int>(n),
-                                       static_cast<int>(h->dim), ip, h->xnorm_max, static_cast<int>(k), excl, incl,
-                                       h->labels, d_ids + q0 * k, d_dists + q0 * k, st);
-                check_launch("k_flat_select");
+            if (want == 0) {
+                fill_empty(d_ids, d_dists, nq * k, st);
+                t.done();
+            } else if (h->rows_bf16 && h->bf16_ok && want <= 256) {
+                const std::vector<uint32_t> rest = flat_bf16(h, d_q, nq, k, want, excl, incl, d_ids, d_dists, sc);
+                t.done();
+                if (!rest.empty()) {
+                    StageTimer t2(QANN_STAGE_FLAT_FP32, st);
+                    flat_fp32_subset(h, d_q, rest, k, excl, incl, d_ids, d_dists, sc);
+                    t2.done();
+                }
+            } else {
+                t.done();
+                StageTimer t2(QANN_STAGE_FLAT_FP32, st);
+                const uint64_t chunk = flat_fp32_chunk(nq, n);
+                sc.used = 0;
+                sc.reserve(pad256(chunk * n * 4));
+                float* D = sc.take<float>(chunk * n);
+                flat_fp32(h, d_q, nq, k, excl, incl, d_ids, d_dists, D, chunk, st);
+                t2.done();
             }
-            t.done();
             return;
         }
         sc.used = 0;
@@ -885,6 +1044,19 @@ int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint6
                                h->row_sq, nullptr);
             check_launch("k_sqnorms");
             h->xnorm_max = norm_bound(rows, slots, dim);
+            // QANN_AMD_FLAT_FP32=1 keeps the index on the FP32 GEMM path
+            const char* fp32_only = std::getenv("QANN_AMD_FLAT_FP32");
+            const bool bf16 = !(fp32_only && *fp32_only && std::strcmp(fp32_only, "0") != 0);
+            if (bf16 && slots >= kFlatBf16MinRows &&
+                qd::bf16_kpad(static_cast<int>(dim)) * 4 + qd::kFlatCandCap * 8 <= qd::kLdsMax) {
+                h->kp = qd::bf16_kpad(static_cast<int>(dim));
+                h->rows_bf16 = h->alloc<uint16_t>(slots * static_cast<uint64_t>(h->kp));
+                h->bf16_maxes = h->alloc<float>(2);
+                check(hipMemset(h->bf16_maxes, 0, 8), "hipMemset");
+                qd::launch_rows_bf16(h->rows, slots, static_cast<int>(dim), h->kp, h->rows_bf16, h->bf16_maxes, nullptr);
+                check_launch("k_rows_bf16");
+                check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            }
         }
         uint64_t set = 0;
         std::vector<uint64_t> bm = to_bitmap(nullptr, slots, set);
@@ -1080,7 +1252,7 @@ int qann_hip_profile_read(double* ms, uint64_t* launches, int reset) {
 
 const char* qann_hip_stage_name(int stage) {
     static const char* names[QANN_STAGE_COUNT] = {"rotate", "coarse", "select", "tables", "scan", "rerank", "flat",
-                                                  "replay"};
+                                                  "replay", "flat_fp32"};
     return stage >= 0 && stage < QANN_STAGE_COUNT ? names[stage] : "";
 }
 
@@ -1100,6 +1272,10 @@ int qann_hip_search_scratch_bytes(qann_hip_index* h, uint64_t nq, const qann_hip
         uint64_t b = 0;
         if (h->kind == 1) {
             b = std::min<uint64_t>(nq * h->slots * 4, uint64_t{1} << 32) + nq * k * 16;
+            if (h->rows_bf16 && k <= 256) {
+                const FlatBf16Plan pl = flat_bf16_plan(h, nq, static_cast<uint32_t>(k));
+                b = pl.chunk * pl.per_query + nq * (k * 16 + 4) + (uint64_t{1} << 20);
+            }
         } else {
             const qann_hip_index* ivfh = h->kind == 3 ? h->base : h;
             const qd::DevIvf& ix = ivfh->ivf;

@@ -1,0 +1,408 @@
+// BF16 MFMA scoring with a proven error bound, for exact FlatIndex search
+// (FlatIndex::query_impl, src/flat_index.cpp:33-64) on gfx950.
+//
+//   k_rows_bf16 / k_queries_bf16   FP32 -> BF16 (round to nearest even) and the
+//                                  norms the error bound needs
+//   k_gemm_bf16<..., Epi>          C = A * B^T by v_mfma_f32_32x32x16_bf16,
+//                                  LDS double-buffered, XCD-aware tile order;
+//                                  the epilogue writes scores or keeps candidates
+//   k_flat_limit                   per query, an upper bound on its k-th smallest
+//                                  approximate score from a strided row sample
+//   k_flat_finish                  per query, the exact k nearest among the
+//                                  candidates, or a hand-over to the FP32 path
+#include <algorithm>
+
+#include "device_common.hpp"
+#include "gemm_bf16.hpp"
+#include "select.hpp"
+
+namespace qd {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// Round to nearest even; infinities and NaN keep their class.
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// sqrt of a float sum of n squares, rounded up past the sum's own rounding
+// (relative error at most n u, u = 2^-24; n <= 2^13 keeps that under 2^-11).
+__device__ __forceinline__ float norm_up(float sumsq) { return sqrtf(sumsq) * (1.0f + 0x1p-10f); }
+
+// One wave per row: BF16 copy, zero-padded to kp, and the row's norms.
+__global__ void __launch_bounds__(256) k_rows_bf16(const float* rows, uint64_t n, int dim, int kp, uint16_t* out,
+                                                   float* maxes) {
+    const uint64_t r = blockIdx.x * 4ull + threadIdx.x / kWave;
+    const int lane = lane_id();
+    if (r >= n) return;
+    const float* x = rows + r * dim;
+    uint16_t* o = out + r * kp;
+    float res = 0.0f, nt = 0.0f;
+    for (int e = lane; e < kp; e += kWave) {
+        uint16_t b = 0;
+        if (e < dim) {
+            b = to_bf16(x[e]);
+            const float t = from_bf16(b), d = x[e] - t;
+            res += d * d;
+            nt += t * t;
+        }
+        o[e] = b;
+    }
+    res = wave_sum(res);
+    nt = wave_sum(nt);
+    if (lane == 0) {
+        // non-negative floats order as their bit patterns
+        atomicMax(reinterpret_cast<unsigned int*>(maxes), __float_as_uint(norm_up(res)));
+        atomicMax(reinterpret_cast<unsigned int*>(maxes + 1), __float_as_uint(norm_up(nt)));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_queries_bf16(const float* q, int nq, int dim, int kp, uint16_t* out,
+                                                      float* qinfo) {
+    const int i = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = lane_id();
+    if (i >= nq) return;
+    const float* x = q + static_cast<size_t>(i) * dim;
+    uint16_t* o = out + static_cast<size_t>(i) * kp;
+    float s = 0.0f, res = 0.0f, nt = 0.0f;
+    for (int e = lane; e < kp; e += kWave) {
+        uint16_t b = 0;
+        if (e < dim) {
+            b = to_bf16(x[e]);
+            const float t = from_bf16(b), d = x[e] - t;
+            s += x[e] * x[e];
+            res += d * d;
+            nt += t * t;
+        }
+        o[e] = b;
+    }
+    s = wave_sum(s);
+    res = wave_sum(res);
+    nt = wave_sum(nt);
+    if (lane == 0) {
+        qinfo[3 * i] = norm_up(s);
+        qinfo[3 * i + 1] = norm_up(nt);
+        qinfo[3 * i + 2] = norm_up(res);
+    }
+}
+
+// A bound on |approximate - exact + const| for one query, const = ||q||^2 for
+// L2 (the approximate score leaves it out) and 0 for inner product. With
+// q~, x~ the BF16 values and s~ the FP32-accumulated q~ . x~:
+//   |s~ - q.x| <= |q~||x - x~| + |q - q~||x| + (n + 2) u |q~||x~|
+// inner product: approximate = -s~; the exact score's own rounding adds
+//   (n + 3) u |q||x|;
+// L2: approximate = ||x||^2 (rounded, (n + 3) u |x|^2) - 2 s~, the subtraction
+// rounds by u times its size, and the exact sum of squared differences is
+// within (n + 4) u (|q| + |x|)^2 of ||q - x||^2. Each norm is the largest
+// bound over the rows; the total is raised by 1/16, and by n 2^-100 for
+// products flushed to zero.
+__device__ __forceinline__ float flat_err(int ip, int dim, const float* qinfo, const float* maxes, float xmax) {
+    const float u = 5.9604645e-8f, n = static_cast<float>(dim);
+    const float qn = qinfo[0], qtn = qinfo[1], rq = qinfo[2];
+    const float rx = maxes[0], xt = maxes[1];
+    const float dot = qtn * rx + rq * xmax + (n + 2.0f) * u * qtn * xt;
+    float e;
+    if (ip) {
+        e = dot + (n + 3.0f) * u * qn * xmax;
+    } else {
+        const float s = qn + xmax;
+        e = (n + 3.0f) * u * xmax * xmax + 2.0f * dot + u * (xmax * xmax + 2.0f * qtn * xt) + (n + 4.0f) * u * s * s;
+    }
+    return e * 1.0625f + n * 7.9e-31f;
+}
+
+// Every approximate score within 2 e of tau: a limit that float rounding
+// cannot pull below tau + 2 e.
+__device__ __forceinline__ float within(float tau, float e) {
+    return tau + (2.0f * e + fabsf(tau) * 0x1p-22f);
+}
+
+constexpr int kLdsRow = kBf16K + 8; // BF16 per LDS row: 64 bytes of data, 16 of padding
+
+// C = A * B^T over K = kp (both row-major, BF16, rows zero-padded), 4 waves
+// of (32 TM) x (32 TN), a workgroup tile of (64 TM) x (64 TN). Lane t gives
+// A[t % 32][8 (t / 32) .. +7] and B[t % 32][...] of each 32 x 16 step and
+// holds C[8 (v / 4) + 4 (t / 32) + v % 4][t % 32] in accumulator v. Tiles are
+// ordered along the big operand, and a workgroup's tile is picked so that the
+// workgroups of one XCD (every 8th) take consecutive tiles: the big operand's
+// tile is read into that XCD's L2 once and reused across the small operand.
+template <int TM, int TN, bool A_BIG, class Epi>
+__global__ void __launch_bounds__(256) k_gemm_bf16(const uint16_t* __restrict__ A, int M, size_t lda,
+                                                   const uint16_t* __restrict__ B, int N, size_t ldb, int kp,
+                                                   int per_xcd, Epi epi) {
+    constexpr int BM = 64 * TM, BN = 64 * TN;
+    __shared__ __align__(16) uint16_t As[2][BM * kLdsRow];
+    __shared__ __align__(16) uint16_t Bs[2][BN * kLdsRow];
+    const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    const int slot = blockIdx.x / 8, xcd = blockIdx.x % 8;
+    if (slot >= per_xcd) return;
+    const int t = xcd * per_xcd + slot;
+    if (t >= tiles_m * tiles_n) return;
+    const int tm = A_BIG ? t / tiles_n : t % tiles_m;
+    const int tn = A_BIG ? t % tiles_n : t / tiles_m;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const int wm = wave >> 1, wn = wave & 1;
+
+    constexpr int CA = BM * 4 / 256, CB = BN * 4 / 256; // 16-byte chunks per thread per stage
+    u32x4 ra[CA], rb[CB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int c = 0; c < CA; ++c) {
+            const int e = tid + 256 * c, r = e >> 2, ch = e & 3;
+            const int g = m0 + r;
+            ra[c] = g < M ? *reinterpret_cast<const u32x4*>(A + static_cast<size_t>(g) * lda + k0 + 8 * ch)
+                          : u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            const int e = tid + 256 * c, r = e >> 2, ch = e & 3;
+            const int g = n0 + r;
+            rb[c] = g < N ? *reinterpret_cast<const u32x4*>(B + static_cast<size_t>(g) * ldb + k0 + 8 * ch)
+                          : u32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int c = 0; c < CA; ++c) {
+            const int e = tid + 256 * c, r = e >> 2, ch = e & 3;
+            *reinterpret_cast<u32x4*>(&As[buf][r * kLdsRow + 8 * ch]) = ra[c];
+        }
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            const int e = tid + 256 * c, r = e >> 2, ch = e & 3;
+            *reinterpret_cast<u32x4*>(&Bs[buf][r * kLdsRow + 8 * ch]) = rb[c];
+        }
+    };
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+    const int nk = kp / kBf16K;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int arow = wm * 32 * TM + (lane & 31), brow = wn * 32 * TN + (lane & 31), kofs = 8 * (lane >> 5);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * kBf16K);
+#pragma unroll
+        for (int ks = 0; ks < kBf16K / 16; ++ks) {
+            bf16x8 a[TM], b[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+                a[i] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const u32x4*>(&As[buf][(arow + 32 * i) * kLdsRow + 16 * ks + kofs]));
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                b[j] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const u32x4*>(&Bs[buf][(brow + 32 * j) * kLdsRow + 16 * ks + kofs]));
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = n0 + wn * 32 * TN + 32 * j + (lane & 31);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = m0 + wm * 32 * TM + 32 * i + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
+                if (row < M && col < N) epi(row, col, acc[i][j][v]);
+            }
+        }
+}
+
+// Scores of queries (A rows) against every stride-th row (B rows, B's row j
+// is row j * stride): ||x||^2 - 2 q.x, or -q.x, and +inf for rows the query
+// may not return.
+struct EpiScores {
+    float* D;
+    size_t ldd;
+    int ip, stride;
+    const float* xsq;
+    const uint64_t* excl;
+    const uint64_t* incl;
+    __device__ __forceinline__ void operator()(int i, int j, float s) const {
+        const uint32_t r = static_cast<uint32_t>(j) * static_cast<uint32_t>(stride);
+        const bool out = (excl && bit_test(excl, r)) || (incl && !bit_test(incl, r));
+        D[static_cast<size_t>(i) * ldd + j] = out ? INFINITY : ip ? -s : xsq[r] - 2.0f * s;
+    }
+};
+
+// Rows (A rows) against queries (B rows): a row whose score is at most the
+// query's limit, and which the query may return, joins its candidates.
+struct EpiFilter {
+    const float* lim;
+    int ip, cap;
+    const float* xsq;
+    const uint64_t* excl;
+    const uint64_t* incl;
+    uint32_t* cnt;
+    float* cd;
+    uint32_t* ci;
+    __device__ __forceinline__ void operator()(int i, int j, float s) const {
+        const float a = ip ? -s : xsq[i] - 2.0f * s;
+        if (!(a <= lim[j])) return;
+        const uint32_t r = static_cast<uint32_t>(i);
+        if ((excl && bit_test(excl, r)) || (incl && !bit_test(incl, r))) return;
+        const uint32_t pos = atomicAdd(&cnt[j], 1u);
+        if (pos < static_cast<uint32_t>(cap)) {
+            cd[static_cast<size_t>(j) * cap + pos] = a;
+            ci[static_cast<size_t>(j) * cap + pos] = r;
+        }
+    }
+};
+
+// Per query: tau, an upper bound on the want-th smallest approximate score of
+// the rows it may return (the want-th smallest of a sample is one), and the
+// candidate limit tau + 2 e.
+__global__ void __launch_bounds__(256) k_flat_limit(const float* Ds, int ns, int want, const float* qinfo,
+                                                    const float* maxes, float xmax, int ip, int dim, float* lim,
+                                                    uint32_t* cnt) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_wsum[4], s_digit, s_below;
+    __shared__ float s_red8[8];
+    const int i = blockIdx.x;
+    const float* row = Ds + static_cast<size_t>(i) * ns;
+    const int target = want + max(64, want / 4);
+    const float tau = ns <= 16 * 256
+                          ? select_threshold<true, true>(row, ns, want, target, hist, s_wsum, &s_digit, &s_below, s_red8)
+                          : select_threshold<false, true>(row, ns, want, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+    if (threadIdx.x == 0) {
+        const float e = flat_err(ip, dim, qinfo + 3 * i, maxes, xmax);
+        lim[i] = tau <= 3.40282347e38f ? within(tau, e) : INFINITY;
+        cnt[i] = 0;
+    }
+}
+
+// Per query: the want-th smallest approximate score tau among the candidates
+// (they hold every row under the limit, so it is the query's own), the rows
+// within tau + 2 e rescored exactly in the reference's order, sorted by
+// (distance, slot), the first k written. A query whose candidates overflowed,
+// or fell short of want, goes to ovf instead.
+__global__ void __launch_bounds__(256) k_flat_finish(const float* q, int dim, int ip, const float* rows,
+                                                     const float* qinfo, const float* maxes, float xmax, int k,
+                                                     int want, int cap, const uint32_t* cnt, const float* cd,
+                                                     const uint32_t* ci, const int64_t* labels, int64_t* out_ids,
+                                                     float* out_dists, uint32_t* ovf, uint32_t* ovf_cnt, int ovf_base) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
+    __shared__ float s_red8[8];
+    const int i = blockIdx.x, tid = threadIdx.x;
+    const uint32_t c = cnt[i];
+    if (c > static_cast<uint32_t>(cap) || c < static_cast<uint32_t>(want)) {
+        if (tid == 0) ovf[atomicAdd(ovf_cnt, 1u)] = static_cast<uint32_t>(ovf_base + i);
+        return;
+    }
+    float* qsh = reinterpret_cast<float*>(lds);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
+    for (int t = tid; t < dim; t += blockDim.x) qsh[t] = q[static_cast<size_t>(i) * dim + t];
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+    const float* crow = cd + static_cast<size_t>(i) * cap;
+    const uint32_t* irow = ci + static_cast<size_t>(i) * cap;
+    const int n = static_cast<int>(c);
+    float limit = INFINITY;
+    if (want > 0) {
+        const int target = want + max(64, want / 4);
+        const float tau =
+            select_threshold<true, true>(crow, n, want, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+        limit = within(tau, flat_err(ip, dim, qinfo + 3 * i, maxes, xmax));
+    }
+    for (int t = tid; t < n; t += blockDim.x)
+        if (crow[t] <= limit) keys[atomicAdd(&s_count, 1u)] = irow[t];
+    __syncthreads();
+    const int m = static_cast<int>(s_count);
+    int np = 1;
+    while (np < m) np <<= 1;
+    for (int t = tid; t < np; t += blockDim.x) {
+        if (t < m) {
+            const uint32_t l = static_cast<uint32_t>(keys[t]);
+            keys[t] = dist_key(exact_score(ip != 0, qsh, rows + static_cast<size_t>(l) * dim, dim), l);
+        } else {
+            keys[t] = ~0ull;
+        }
+    }
+    block_bitonic_sort(keys, np);
+    for (int t = tid; t < k; t += blockDim.x) {
+        int64_t id = -1;
+        float d = INFINITY;
+        if (t < m) {
+            const uint32_t l = key_index(keys[t]);
+            id = labels ? labels[l] : static_cast<int64_t>(l);
+            d = key_dist(keys[t]);
+        }
+        out_ids[static_cast<size_t>(i) * k + t] = id;
+        out_dists[static_cast<size_t>(i) * k + t] = d;
+    }
+}
+
+inline unsigned cdiv(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+template <int TM, int TN, bool A_BIG, class Epi>
+void gemm(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp, const Epi& epi,
+          hipStream_t s) {
+    if (M <= 0 || N <= 0) return;
+    const size_t tiles = static_cast<size_t>(cdiv(M, 64 * TM)) * cdiv(N, 64 * TN);
+    const int per_xcd = static_cast<int>((tiles + 7) / 8);
+    hipLaunchKernelGGL((k_gemm_bf16<TM, TN, A_BIG, Epi>), dim3(static_cast<unsigned>(8 * per_xcd)), dim3(256), 0, s,
+                       A, M, lda, B, N, ldb, kp, per_xcd, epi);
+}
+
+} // namespace
+
+void launch_rows_bf16(const float* rows, uint64_t n, int dim, int kp, uint16_t* out, float* maxes, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_rows_bf16, dim3(cdiv(n, 4)), dim3(256), 0, s, rows, n, dim, kp, out, maxes);
+}
+
+void launch_queries_bf16(const float* q, int nq, int dim, int kp, uint16_t* out, float* qinfo, hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_queries_bf16, dim3(cdiv(nq, 4)), dim3(256), 0, s, q, nq, dim, kp, out, qinfo);
+}
+
+void launch_flat_bf16(const FlatBf16& f, const float* q, const uint16_t* qb, const float* qinfo, int nq, int k,
+                      int want, int stride, int ns, float* Ds, float* lim, uint32_t* cnt, float* cd, uint32_t* ci,
+                      int64_t* out_ids, float* out_dists, uint32_t* ovf, uint32_t* ovf_cnt, int ovf_base,
+                      hipStream_t s) {
+    if (nq <= 0) return;
+    EpiScores es{Ds, static_cast<size_t>(ns), f.ip, stride, f.xsq, f.excl, f.incl};
+    gemm<2, 2, false>(qb, nq, static_cast<size_t>(f.kp), f.xb, ns, static_cast<size_t>(stride) * f.kp, f.kp, es, s);
+    hipLaunchKernelGGL(k_flat_limit, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s, Ds, ns, want, qinfo, f.maxes,
+                       f.xnorm_max, f.ip, f.dim, lim, cnt);
+    EpiFilter ef{lim, f.ip, kFlatCandCap, f.xsq, f.excl, f.incl, cnt, cd, ci};
+    gemm<2, 2, true>(f.xb, f.n, static_cast<size_t>(f.kp), qb, nq, static_cast<size_t>(f.kp), f.kp, ef, s);
+    const size_t lds = ((static_cast<size_t>(f.dim) * 4 + 15) & ~size_t{15}) + static_cast<size_t>(kFlatCandCap) * 8;
+    hipLaunchKernelGGL(k_flat_finish, dim3(static_cast<unsigned>(nq)), dim3(256), lds, s, q, f.dim, f.ip, f.rows,
+                       qinfo, f.maxes, f.xnorm_max, k, want, kFlatCandCap, cnt, cd, ci, f.labels, out_ids, out_dists,
+                       ovf, ovf_cnt, ovf_base);
+}
+
+} // namespace qd

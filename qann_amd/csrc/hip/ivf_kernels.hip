@@ -20,6 +20,7 @@
 
 #include "device_common.hpp"
 #include "ivf_kernels.hpp"
+#include "select.hpp"
 
 namespace qd {
 
@@ -139,189 +140,6 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
     put(acc01, 0, 1);
     put(acc10, 1, 0);
     put(acc11, 1, 1);
-}
-
-// The digit d of a 256-bin histogram (one bin per thread of a 256-thread
-// block) whose running count first reaches `remain`, and the count below it:
-// an inclusive scan of the bins within each wave, then across the four waves.
-__device__ __forceinline__ void block_find_digit(const uint32_t* hist, uint32_t remain, uint32_t* s_wsum, uint32_t* s_digit,
-                                 uint32_t* s_below) {
-    const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
-    const uint32_t h = hist[t];
-    uint32_t x = h;
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), off, kWave));
-        if (lane >= off) x += y;
-    }
-    if (lane == kWave - 1) s_wsum[w] = x;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int v = 0; v < w; ++v) before += s_wsum[v];
-    const uint32_t incl = before + x, excl = incl - h;
-    if (excl < remain && incl >= remain) {
-        *s_digit = static_cast<uint32_t>(t);
-        *s_below = excl;
-    }
-    __syncthreads();
-}
-
-// Radix select over a row of n values by a 32- or 64-bit key (8 bits a pass):
-// the key of rank `want` (1-based) among those key(l) gives. 256 threads;
-// with CACHE, the row's keys (n <= 16 * 256) stay in registers across passes.
-template <typename Key, bool CACHE, typename KeyOf>
-__device__ __forceinline__ Key block_radix_select(int n, uint32_t want, KeyOf key, uint32_t* hist, uint32_t* s_wsum, uint32_t* s_digit,
-                                  uint32_t* s_below) {
-    Key cached[CACHE ? 16 : 1];
-    if (CACHE) {
-#pragma unroll
-        for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
-            const int l = threadIdx.x + j * 256;
-            cached[j] = l < n ? key(l) : ~Key(0);
-        }
-    }
-    Key prefix = 0;
-    uint32_t remain = want;
-    constexpr int kBits = 8 * static_cast<int>(sizeof(Key));
-    for (int shift = kBits - 8; shift >= 0; shift -= 8) {
-        hist[threadIdx.x] = 0;
-        __syncthreads();
-        const Key hi_mask = shift == kBits - 8 ? Key(0) : static_cast<Key>(~Key(0) << (shift + 8));
-        if (CACHE) {
-#pragma unroll
-            for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
-                const int l = threadIdx.x + j * 256;
-                if (l < n && (cached[j] & hi_mask) == prefix) atomicAdd(&hist[(cached[j] >> shift) & 255], 1u);
-            }
-        } else {
-            for (int l = threadIdx.x; l < n; l += 256) {
-                const Key k = key(l);
-                if ((k & hi_mask) == prefix) atomicAdd(&hist[(k >> shift) & 255], 1u);
-            }
-        }
-        __syncthreads();
-        block_find_digit(hist, remain, s_wsum, s_digit, s_below);
-        prefix |= static_cast<Key>(*s_digit) << shift;
-        remain -= *s_below;
-        __syncthreads();
-    }
-    return prefix;
-}
-
-// Selection keys of (distance, index): ckey(d) (float order, -0 tied with +0)
-// above the index, with the index shifted up one bit so the lowest bit can
-// record a -0 distance -- indexes are unique, so that bit never decides the
-// order, and the key gives back the distance's exact bits.
-__device__ __forceinline__ uint64_t dist_key(float d, uint32_t idx) {
-    const uint32_t negzero = (d == 0.0f && (__float_as_uint(d) >> 31)) ? 1u : 0u;
-    return (static_cast<uint64_t>(ckey(d)) << 32) | (static_cast<uint64_t>(idx) << 1) | negzero;
-}
-__device__ __forceinline__ uint32_t key_index(uint64_t key) { return static_cast<uint32_t>(key & 0xFFFFFFFFu) >> 1; }
-__device__ __forceinline__ float key_dist(uint64_t key) {
-    return (key & 1u) ? -0.0f : from_fkey(static_cast<uint32_t>(key >> 32));
-}
-
-// Minimum / maximum over a 256-thread block (every thread gets the result).
-__device__ __forceinline__ float block_min256(float v, float* s_red) {
-    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
-    __syncthreads();
-    if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
-    __syncthreads();
-    return fminf(fminf(s_red[0], s_red[1]), fminf(s_red[2], s_red[3]));
-}
-
-__device__ __forceinline__ float block_max256(float v, float* s_red) {
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
-    __syncthreads();
-    if (lane_id() == 0) s_red[threadIdx.x / kWave] = v;
-    __syncthreads();
-    return fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
-}
-
-// A value tau of the row with at least M values <= tau and, as a rule, few
-// more than M: values are bucketed linearly over [min, max] (256 buckets, the
-// bucket a monotone function of the value), the bucket where the running count
-// reaches M is split the same way once more when it holds many, and tau is the
-// largest value in the buckets up to the one found. With CACHE the row
-// (n <= 16 * 256) stays in registers.
-template <bool CACHE>
-__device__ __forceinline__ float select_threshold(const float* row, int n, int M, int target, uint32_t* hist, uint32_t* s_wsum,
-                                  uint32_t* s_digit, uint32_t* s_below, float* s_red) {
-    const int tid = threadIdx.x;
-    float cached[CACHE ? 16 : 1];
-    if (CACHE) {
-#pragma unroll
-        for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
-            const int l = tid + 256 * j;
-            cached[j] = l < n ? row[l] : 0.0f;
-        }
-    }
-    auto each = [&](auto&& fn) {
-        if (CACHE) {
-#pragma unroll
-            for (int j = 0; j < (CACHE ? 16 : 1); ++j)
-                if (tid + 256 * j < n) fn(cached[j]);
-        } else {
-            for (int l = tid; l < n; l += 256) fn(row[l]);
-        }
-    };
-    float lo = INFINITY, hi = -INFINITY;
-    each([&](float v) {
-        lo = fminf(lo, v);
-        hi = fmaxf(hi, v);
-    });
-    lo = block_min256(lo, s_red);
-    hi = block_max256(hi, s_red + 4);
-    if (!(hi > lo) || !isfinite(hi - lo)) return hi;
-    const float s1 = 256.0f / (hi - lo);
-    auto b1 = [&](float v) { return min(255, static_cast<int>((v - lo) * s1)); };
-    hist[tid] = 0;
-    __syncthreads();
-    each([&](float v) { atomicAdd(&hist[b1(v)], 1u); });
-    __syncthreads();
-    block_find_digit(hist, static_cast<uint32_t>(M), s_wsum, s_digit, s_below);
-    const int d1 = static_cast<int>(*s_digit);
-    const uint32_t below1 = *s_below, c1 = hist[d1];
-    __syncthreads();
-    if (below1 + c1 <= static_cast<uint32_t>(target)) {
-        float m = -INFINITY;
-        each([&](float v) {
-            if (b1(v) <= d1) m = fmaxf(m, v);
-        });
-        return block_max256(m, s_red);
-    }
-    float lo2 = INFINITY, hi2 = -INFINITY;
-    each([&](float v) {
-        if (b1(v) == d1) {
-            lo2 = fminf(lo2, v);
-            hi2 = fmaxf(hi2, v);
-        }
-    });
-    lo2 = block_min256(lo2, s_red);
-    hi2 = block_max256(hi2, s_red + 4);
-    if (!(hi2 > lo2)) {
-        float m = -INFINITY;
-        each([&](float v) {
-            if (b1(v) <= d1) m = fmaxf(m, v);
-        });
-        return block_max256(m, s_red);
-    }
-    const float s2 = 256.0f / (hi2 - lo2);
-    auto b2 = [&](float v) { return min(255, max(0, static_cast<int>((v - lo2) * s2))); };
-    hist[tid] = 0;
-    __syncthreads();
-    each([&](float v) {
-        if (b1(v) == d1) atomicAdd(&hist[b2(v)], 1u);
-    });
-    __syncthreads();
-    block_find_digit(hist, static_cast<uint32_t>(M) - below1, s_wsum, s_digit, s_below);
-    const int d2 = static_cast<int>(*s_digit);
-    __syncthreads();
-    float m = -INFINITY;
-    each([&](float v) {
-        const int b = b1(v);
-        if (b < d1 || (b == d1 && b2(v) <= d2)) m = fmaxf(m, v);
-    });
-    return block_max256(m, s_red);
 }
 
 // Per query (one workgroup): the M smallest (A, list), A exact, sorted. The
@@ -459,8 +277,10 @@ __global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, c
     ss = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
     const int M = min(k, n);
     const int target = min(cap, M + max(64, M / 4));
-    const float tau = n <= 16 * 256 ? select_threshold<true>(row, n, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
-                                    : select_threshold<false>(row, n, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+    // rows the query may not return score +inf and take no part in tau
+    const float tau = n <= 16 * 256
+                          ? select_threshold<true, true>(row, n, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
+                          : select_threshold<false, true>(row, n, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
     const float u = 5.9604645e-8f, nd = static_cast<float>(dim);
     const float qn = sqrtf(ss * (1.0f + 2.0f * (nd + 2.0f) * u)) * 1.0001f;
     const float delta = 2.0f * (ip ? 2.0f * (nd + 1.0f) * u * qn * xnorm_max

@@ -175,6 +175,107 @@ def test_gpu_flat_gemm_many_ties(tmp_path, oracle):
         assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"k {k}")
 
 
+def _stage_launches():
+    """Launches per search stage since the last read (the library's profiler,
+    which times every stage while it is on)."""
+    import ctypes
+    import pathlib
+
+    lib = ctypes.CDLL(str(pathlib.Path(qann_amd.__file__).parent / "libqann_hip.so"))
+    lib.qann_hip_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.c_int]
+    lib.qann_hip_stage_name.restype = ctypes.c_char_p
+    ms = (ctypes.c_double * 32)()
+    n = (ctypes.c_uint64 * 32)()
+    lib.qann_hip_profile_read(ms, n, 1)
+    names = [lib.qann_hip_stage_name(i).decode() for i in range(32)]
+    return {name: int(n[i]) for i, name in enumerate(names) if name}
+
+
+def _profiling(on):
+    import ctypes
+    import pathlib
+
+    lib = ctypes.CDLL(str(pathlib.Path(qann_amd.__file__).parent / "libqann_hip.so"))
+    lib.qann_hip_profile_enable.argtypes = [ctypes.c_int]
+    lib.qann_hip_profile_enable(int(on))
+
+
+def spectrum_rows(rng, n, dim=768):
+    lam = np.arange(1, dim + 1, dtype=np.float64) ** -0.8
+    x = (rng.standard_normal((n, dim)) * np.sqrt(lam)).astype(np.float32)
+    return x / np.linalg.norm(x, axis=1, keepdims=True)
+
+
+def test_gpu_flat_bf16_matches_oracle(tmp_path, oracle):
+    # 70,000 rows (the BF16 GEMM path starts at 65,536) of the precision
+    # configuration's kind: every query finishes on the BF16 path, and the
+    # results are the reference's, bit for bit
+    rng = np.random.default_rng(769)
+    x = spectrum_rows(rng, 70_000)
+    q = spectrum_rows(rng, 24)
+    _profiling(True)
+    try:
+        for metric in (qann_amd.Metric.InnerProduct, qann_amd.Metric.L2):
+            flat = qann_amd.FlatIndex(768, metric=metric)
+            flat.add(x)
+            flat.remove(rng.choice(70_000, 500, replace=False))
+            path = tmp_path / "flat_bf16.qann"
+            flat.save(str(path))
+            orc = oracle.load(path)
+            _stage_launches()
+            for k in (1, 10, 100):
+                assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"{metric} k {k}")
+            allowed = rng.choice(70_000, 40_000, replace=False)
+            assert_same(flat.batch_query(q, 100, allowed=allowed), orc.batch_query(q, 100, allowed=allowed),
+                        f"{metric} filtered")
+            launches = _stage_launches()
+            assert launches["flat"] == 4 and launches["flat_fp32"] == 0, launches
+    finally:
+        _profiling(False)
+
+
+def test_gpu_flat_bf16_hands_queries_to_fp32(tmp_path, oracle):
+    # 4,500 copies of one row: the queries next to it have more rows within the
+    # BF16 bound than the candidate buffer holds and go to the FP32 path, the
+    # others finish on the BF16 path; all match the reference
+    rng = np.random.default_rng(770)
+    x = spectrum_rows(rng, 70_000)
+    x[rng.choice(70_000, 4500, replace=False)] = x[7]
+    near = x[7][None] + 0.01 * spectrum_rows(rng, 4)
+    q = np.concatenate([spectrum_rows(rng, 12), near / np.linalg.norm(near, axis=1, keepdims=True)]).astype(np.float32)
+    flat = qann_amd.FlatIndex(768, metric=qann_amd.Metric.InnerProduct, custom_ids=True)
+    flat.add(x, ids=rng.choice(10**9, 70_000, replace=False))
+    path = tmp_path / "flat_dup.qann"
+    flat.save(str(path))
+    orc = oracle.load(path)
+    _profiling(True)
+    try:
+        _stage_launches()
+        for k in (10, 100):
+            assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"k {k}")
+        launches = _stage_launches()
+        assert launches["flat"] == 2 and launches["flat_fp32"] == 2, launches
+    finally:
+        _profiling(False)
+
+
+def test_gpu_flat_bf16_loose_bound_uses_fp32(tmp_path, oracle):
+    # rows far from the origin and close together: the BF16 bound admits whole
+    # clusters, so the queries go to the FP32 path (and, after a search where
+    # most did, the index keeps to it); the results match the reference
+    rng = np.random.default_rng(771)
+    x = mixture(rng, 70_000, 64, centers=8, sigma=0.02)
+    q = mixture(rng, 16, 64, centers=8, sigma=0.02)
+    flat = qann_amd.FlatIndex(64, metric=qann_amd.Metric.L2)
+    flat.add(x)
+    path = tmp_path / "flat_far.qann"
+    flat.save(str(path))
+    orc = oracle.load(path)
+    for k in (10, 100):
+        assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"k {k}")
+
+
 def test_gpu_config_100k_x_128_matches_oracle(tmp_path, oracle):
     # the CPU-runnable configuration at its full size: 100,000 x 128 L2 from a
     # 64-centre mixture (centres U[0,1)^128, sigma 0.05), seed 1, IVF nlist 256
