@@ -449,29 +449,79 @@ struct WaveTopK {
     int size;       // wave-uniform
     int k;
     float thr;      // threshold(): +inf until the first trim
+    uint32_t* hist; // LDS, 256 counters for the trim's radix select
 };
-
-// count of buffered keys <= t, by the whole wave
-__device__ __forceinline__ int count_le(const WaveTopK& tk, int n, uint32_t t) {
-    int c = 0;
-    for (int base = 0; base < n; base += kWave) {
-        const int i = base + lane_id();
-        const bool p = i < n && fkey(tk.dist[i]) <= t;
-        c += __popcll(ballot(p));
-    }
-    return c;
-}
 
 // TopK::trim (src/topk.hpp:94-131): keep the k smallest keys, ties at the
 // k-th in push order; the threshold becomes the k-th smallest distance.
+// The k-th smallest key t is found by radix select: the bits every key shares
+// (those above the highest bit where the smallest and largest keys differ)
+// are t's, then 8 bits a pass from a histogram of the keys still matching
+// the prefix found so far.
 __device__ void wave_trim(WaveTopK& tk) {
-    const int n = tk.size;
-    uint32_t t = 0;
-    for (int b = 31; b >= 0; --b) {
-        const uint32_t below = b == 0 ? 0u : ((1u << b) - 1u);
-        if (count_le(tk, n, t | below) < tk.k) t |= 1u << b;
+    const int n = tk.size, lane = lane_id();
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (int i = lane; i < n; i += kWave) {
+        const uint32_t key = fkey(tk.dist[i]);
+        lo = min(lo, key);
+        hi = max(hi, key);
     }
-    int ties = tk.k - (t == 0 ? 0 : count_le(tk, n, t - 1));
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), off, kWave)));
+        hi = max(hi, static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), off, kWave)));
+    }
+    uint32_t t = lo, below = 0;
+    if (lo != hi) {
+        int top = 31 - __clz(static_cast<int>(lo ^ hi));
+        uint32_t prefix = top == 31 ? 0u : lo & ~((2u << top) - 1u);
+        uint32_t remain = static_cast<uint32_t>(tk.k);
+        while (top >= 0) {
+            const int low = max(top - 7, 0);
+            const uint32_t above = top == 31 ? 0u : ~((2u << top) - 1u);
+            const uint32_t dmask = (2u << (top - low)) - 1u;
+            for (int b = lane; b < 256; b += kWave) tk.hist[b] = 0u;
+            wave_sync();
+            for (int i = lane; i < n; i += kWave) {
+                const uint32_t key = fkey(tk.dist[i]);
+                if ((key & above) == prefix) atomicAdd(&tk.hist[(key >> low) & dmask], 1u);
+            }
+            wave_sync();
+            const uint32_t c0 = tk.hist[4 * lane], c1 = tk.hist[4 * lane + 1], c2 = tk.hist[4 * lane + 2],
+                           c3 = tk.hist[4 * lane + 3];
+            const uint32_t mine = c0 + c1 + c2 + c3;
+            uint32_t incl = mine;
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, kWave));
+                if (lane >= off) incl += y;
+            }
+            const uint32_t excl = incl - mine;
+            // the lane whose four bins hold the remain-th key, then the bin
+            const uint64_t at = ballot(excl < remain && incl >= remain);
+            const int src = __builtin_ctzll(at);
+            uint32_t acc = excl, d = 0;
+            if (acc + c0 < remain) {
+                acc += c0;
+                d = 1;
+                if (acc + c1 < remain) {
+                    acc += c1;
+                    d = 2;
+                    if (acc + c2 < remain) {
+                        acc += c2;
+                        d = 3;
+                    }
+                }
+            }
+            const uint32_t digit = static_cast<uint32_t>(__shfl(static_cast<int>(4 * lane + d), src, kWave));
+            const uint32_t nb = static_cast<uint32_t>(__shfl(static_cast<int>(acc), src, kWave));
+            prefix |= digit << low;
+            remain -= nb;
+            below += nb;
+            top = low - 1;
+            wave_sync();
+        }
+        t = prefix;
+    }
+    int ties = tk.k - static_cast<int>(below);
     int w = 0;
     for (int base = 0; base < n; base += kWave) {
         const int i = base + lane_id();
@@ -1000,8 +1050,11 @@ __global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t
 // smallest sum (distance) cannot pass the current limit (threshold) is
 // skipped whole, and within a list only blocks whose smallest can are read
 // (the threshold only falls). The TopK holds slots; they become ids (labels)
-// for the final sort.
-template <bool FL>
+// for the final sort. With STAGE, a list's candidate blocks have their sums
+// and slots loaded 16 blocks at a time into LDS before they are replayed, so
+// one query waits for one round of loads per 16 blocks rather than per block:
+// the form for small batches, whose few waves cannot hide the latency.
+template <bool FL, bool STAGE>
 __device__ void fs_replay(const ReplayArgs& a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevIvf& ix = a.ix;
@@ -1012,7 +1065,8 @@ __device__ void fs_replay(const ReplayArgs& a) {
     float* tk_dist = tk_lds ? reinterpret_cast<float*>(tk_label + 2 * kp) : a.g_dist + static_cast<size_t>(qi) * 2 * kp;
     const int lane = lane_id();
     const bool filtered = a.filter != nullptr;
-    WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY};
+    __shared__ uint32_t trim_hist[256];
+    WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY, trim_hist};
     const int P = a.nprobe_q[qi];
     const size_t qoff = static_cast<size_t>(qi) * a.stride_slots;
     const size_t qboff = static_cast<size_t>(qi) * (a.stride_slots / kFsBlock);
@@ -1049,86 +1103,204 @@ __device__ void fs_replay(const ReplayArgs& a) {
             const uint32_t* ids = ix.list_ids + ix.list_off[list];
             const size_t voff = qoff + static_cast<size_t>(blk0) * kFsBlock;
             const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
-            for (uint32_t bc = 0; bc < nb; bc += kWave) {
-                const uint32_t bl = bc + lane;
-                uint32_t bm = 0xFFFFFFFFu;
-                if (bl < nb) bm = FL ? a.fbmin[qboff + blk0 + bl] : a.bmin[qboff + blk0 + bl];
-                uint64_t cand = ballot(bm < limit_now());
-                while (cand) {
-                    // the next two candidate blocks: lanes 0-31 and 32-63
-                    const int b1 = __builtin_ctzll(cand);
-                    cand &= cand - 1;
-                    int b2 = -1;
-                    if (cand) {
-                        b2 = __builtin_ctzll(cand);
-                        cand &= cand - 1;
-                    }
-                    const int mb = (lane >> 5) == 0 ? b1 : b2;
+            if constexpr (STAGE) {
+                constexpr int kStage = 16; // candidate blocks staged at a time
+                __shared__ int st_blk[kWave];
+                __shared__ uint32_t st_val[kStage * kFsBlock];
+                __shared__ uint32_t st_id[kStage * kFsBlock];
+                for (uint32_t bc = 0; bc < nb; bc += kWave) {
+                    const uint32_t bl = bc + lane;
+                    uint32_t bm = 0xFFFFFFFFu;
+                    if (bl < nb) bm = FL ? a.fbmin[qboff + blk0 + bl] : a.bmin[qboff + blk0 + bl];
+                    const uint64_t cand = ballot(bm < limit_now());
+                    if (cand == 0) continue;
+                    // the candidate blocks in order; their sums (distances) and slots
+                    // are staged kStage blocks at a time, all loads of a stage in flight
+                    const int nc = __popcll(cand);
+                    if ((cand >> lane) & 1ull) st_blk[lanes_below(cand)] = lane;
+                    wave_sync();
                     const int v = lane & 31;
-                    const uint32_t e = (bc + static_cast<uint32_t>(mb)) * kFsBlock + v;
-                    const bool in = mb >= 0 && e < n;
-                    uint32_t sum = 0xFFFFu;
-                    float fd = INFINITY;
-                    if (in) {
-                        if (FL) fd = a.fdists[voff + e];
-                        else sum = a.sums[voff + e];
+                    for (int g0 = 0; g0 < nc; g0 += kStage) {
+                        const int gn = min(kStage, nc - g0);
+                        uint32_t sv[kStage / 2], iv[kStage / 2];
+    #pragma unroll
+                        for (int u = 0; u < kStage / 2; ++u) {
+                            const int r = 2 * u + (lane >> 5);
+                            sv[u] = FL ? __float_as_uint(INFINITY) : 0xFFFFu;
+                            iv[u] = 0u;
+                            if (r < gn) {
+                                const uint32_t e = (bc + static_cast<uint32_t>(st_blk[g0 + r])) * kFsBlock + v;
+                                if (e < n) {
+                                    sv[u] = FL ? __float_as_uint(a.fdists[voff + e]) : static_cast<uint32_t>(a.sums[voff + e]);
+                                    iv[u] = ids[e];
+                                }
+                            }
+                        }
+    #pragma unroll
+                        for (int u = 0; u < kStage / 2; ++u) {
+                            const int r = 2 * u + (lane >> 5);
+                            st_val[r * kFsBlock + v] = sv[u];
+                            st_id[r * kFsBlock + v] = iv[u];
+                        }
+                        wave_sync();
+                        // two candidate blocks at a time: lanes 0-31 and 32-63
+                        for (int r = 0; r < gn; r += 2) {
+                            const int rr = r + (lane >> 5);
+                            const bool has2 = r + 1 < gn;
+                            const int mb = rr < gn ? st_blk[g0 + rr] : -1;
+                            const uint32_t e = mb >= 0 ? (bc + static_cast<uint32_t>(mb)) * kFsBlock + v : 0u;
+                            const bool in = mb >= 0 && e < n;
+                            uint32_t sum = 0xFFFFu;
+                            float fd = INFINITY;
+                            uint32_t sid = 0u;
+                            if (in) {
+                                const uint32_t x = st_val[rr * kFsBlock + v];
+                                if (FL) fd = __uint_as_float(x);
+                                else sum = x;
+                                sid = st_id[rr * kFsBlock + v];
+                            }
+                            auto passes_now = [&]() -> bool { return FL ? fd < tk.thr : sum < limit_now(); };
+                            if (ballot(in && passes_now()) == 0) continue;
+                            for (int h = 0; h < 2; ++h) {
+                                if (h == 1 && !has2) break;
+                                const bool mine = (lane >> 5) == h;
+                                const bool valid = mine && in;
+                                const uint32_t slot = sid;
+                                bool allowed = valid;
+                                if (!FL && a.block_filter) {
+                                    allowed = valid && bit_test(a.filter, slot);
+                                    if (ballot(allowed) == 0) continue;
+                                }
+                                const bool surv = allowed && passes_now();
+                                uint64_t rem = ballot(surv);
+                                if (rem == 0) continue;
+                                float dist;
+                                bool ok;
+                                if (FL) {
+                                    dist = fd; // +inf already marks what may not be returned
+                                    ok = surv;
+                                } else {
+                                    dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                                    ok = surv && (filtered ? bit_test(a.filter, slot)
+                                                           : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                                }
+                                while (rem) {
+                                    const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
+                                    const uint64_t cm = ballot(cond);
+                                    if (cm == 0) break;
+                                    const int space = 2 * kp - tk.size;
+                                    const int cnt = __popcll(cm);
+                                    if (cnt < space) {
+                                        if (cond) {
+                                            const int pos = tk.size + static_cast<int>(lanes_below(cm));
+                                            tk.dist[pos] = dist;
+                                            tk.label[pos] = slot;
+                                        }
+                                        tk.size += cnt;
+                                        wave_sync();
+                                        break;
+                                    }
+                                    // the space-th candidate fills the buffer and triggers a trim
+                                    const uint32_t rank = lanes_below(cm);
+                                    const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
+                                    const int bpos = __builtin_ctzll(at);
+                                    if (cond && lane <= bpos) {
+                                        const int pos = tk.size + static_cast<int>(rank);
+                                        tk.dist[pos] = dist;
+                                        tk.label[pos] = slot;
+                                    }
+                                    tk.size = 2 * kp;
+                                    wave_sync();
+                                    wave_trim(tk);
+                                    rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
+                                }
+                            }
+                        }
+                        wave_sync(); // the next stage overwrites st_val / st_id
                     }
-                    auto passes_now = [&]() -> bool { return FL ? fd < tk.thr : sum < limit_now(); };
-                    if (ballot(in && passes_now()) == 0) continue;
-                    for (int h = 0; h < 2; ++h) {
-                        if (h == 1 && b2 < 0) break;
-                        const bool mine = (lane >> 5) == h;
-                        const bool valid = mine && in;
-                        uint32_t slot = 0;
-                        bool allowed = valid;
-                        if (!FL && a.block_filter) {
-                            slot = valid ? ids[e] : 0u;
-                            allowed = valid && bit_test(a.filter, slot);
-                            if (ballot(allowed) == 0) continue;
+                }
+            } else {
+                for (uint32_t bc = 0; bc < nb; bc += kWave) {
+                    const uint32_t bl = bc + lane;
+                    uint32_t bm = 0xFFFFFFFFu;
+                    if (bl < nb) bm = FL ? a.fbmin[qboff + blk0 + bl] : a.bmin[qboff + blk0 + bl];
+                    uint64_t cand = ballot(bm < limit_now());
+                    while (cand) {
+                        // the next two candidate blocks: lanes 0-31 and 32-63
+                        const int b1 = __builtin_ctzll(cand);
+                        cand &= cand - 1;
+                        int b2 = -1;
+                        if (cand) {
+                            b2 = __builtin_ctzll(cand);
+                            cand &= cand - 1;
                         }
-                        const bool surv = allowed && passes_now();
-                        uint64_t rem = ballot(surv);
-                        if (rem == 0) continue;
-                        if ((FL || !a.block_filter) && surv) slot = ids[e];
-                        float dist;
-                        bool ok;
-                        if (FL) {
-                            dist = fd; // +inf already marks what may not be returned
-                            ok = surv;
-                        } else {
-                            dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
-                            ok = surv && (filtered ? bit_test(a.filter, slot)
-                                                   : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                        const int mb = (lane >> 5) == 0 ? b1 : b2;
+                        const int v = lane & 31;
+                        const uint32_t e = (bc + static_cast<uint32_t>(mb)) * kFsBlock + v;
+                        const bool in = mb >= 0 && e < n;
+                        uint32_t sum = 0xFFFFu;
+                        float fd = INFINITY;
+                        if (in) {
+                            if (FL) fd = a.fdists[voff + e];
+                            else sum = a.sums[voff + e];
                         }
-                        while (rem) {
-                            const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
-                            const uint64_t cm = ballot(cond);
-                            if (cm == 0) break;
-                            const int space = 2 * kp - tk.size;
-                            const int cnt = __popcll(cm);
-                            if (cnt < space) {
-                                if (cond) {
-                                    const int pos = tk.size + static_cast<int>(lanes_below(cm));
+                        auto passes_now = [&]() -> bool { return FL ? fd < tk.thr : sum < limit_now(); };
+                        if (ballot(in && passes_now()) == 0) continue;
+                        for (int h = 0; h < 2; ++h) {
+                            if (h == 1 && b2 < 0) break;
+                            const bool mine = (lane >> 5) == h;
+                            const bool valid = mine && in;
+                            uint32_t slot = 0;
+                            bool allowed = valid;
+                            if (!FL && a.block_filter) {
+                                slot = valid ? ids[e] : 0u;
+                                allowed = valid && bit_test(a.filter, slot);
+                                if (ballot(allowed) == 0) continue;
+                            }
+                            const bool surv = allowed && passes_now();
+                            uint64_t rem = ballot(surv);
+                            if (rem == 0) continue;
+                            if ((FL || !a.block_filter) && surv) slot = ids[e];
+                            float dist;
+                            bool ok;
+                            if (FL) {
+                                dist = fd; // +inf already marks what may not be returned
+                                ok = surv;
+                            } else {
+                                dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                                ok = surv && (filtered ? bit_test(a.filter, slot)
+                                                       : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                            }
+                            while (rem) {
+                                const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
+                                const uint64_t cm = ballot(cond);
+                                if (cm == 0) break;
+                                const int space = 2 * kp - tk.size;
+                                const int cnt = __popcll(cm);
+                                if (cnt < space) {
+                                    if (cond) {
+                                        const int pos = tk.size + static_cast<int>(lanes_below(cm));
+                                        tk.dist[pos] = dist;
+                                        tk.label[pos] = slot;
+                                    }
+                                    tk.size += cnt;
+                                    wave_sync();
+                                    break;
+                                }
+                                // the space-th candidate fills the buffer and triggers a trim
+                                const uint32_t rank = lanes_below(cm);
+                                const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
+                                const int bpos = __builtin_ctzll(at);
+                                if (cond && lane <= bpos) {
+                                    const int pos = tk.size + static_cast<int>(rank);
                                     tk.dist[pos] = dist;
                                     tk.label[pos] = slot;
                                 }
-                                tk.size += cnt;
+                                tk.size = 2 * kp;
                                 wave_sync();
-                                break;
+                                wave_trim(tk);
+                                rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
                             }
-                            // the space-th candidate fills the buffer and triggers a trim
-                            const uint32_t rank = lanes_below(cm);
-                            const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
-                            const int bpos = __builtin_ctzll(at);
-                            if (cond && lane <= bpos) {
-                                const int pos = tk.size + static_cast<int>(rank);
-                                tk.dist[pos] = dist;
-                                tk.label[pos] = slot;
-                            }
-                            tk.size = 2 * kp;
-                            wave_sync();
-                            wave_trim(tk);
-                            rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
                         }
                     }
                 }
@@ -1196,8 +1368,10 @@ __device__ void fs_replay(const ReplayArgs& a) {
     if (lane == 0) a.out_count[qi] = n;
 }
 
-__global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) { fs_replay<false>(a); }
-__global__ void __launch_bounds__(64) k_fl_replay(const ReplayArgs a) { fs_replay<true>(a); }
+template <bool STAGE>
+__global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) { fs_replay<false, STAGE>(a); }
+template <bool STAGE>
+__global__ void __launch_bounds__(64) k_fl_replay(const ReplayArgs a) { fs_replay<true, STAGE>(a); }
 
 // ---------------------------------------------------------------------------
 // Exact re-rank (RefineIndex::rerank): the candidates' exact distances, the k
@@ -1520,8 +1694,15 @@ void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out
 
 void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
     const size_t lds = a.g_dist ? 0 : static_cast<size_t>(2 * a.kp) * 12;
-    if (a.fdists) hipLaunchKernelGGL(k_fl_replay, dim3(nq), dim3(kWave), lds, s, a);
-    else hipLaunchKernelGGL(k_fs_replay, dim3(nq), dim3(kWave), lds, s, a);
+    // staged loads pay off while the batch leaves most of the chip's wave slots idle
+    const bool stage = nq < kReplayStageBelow;
+    if (a.fdists) {
+        if (stage) hipLaunchKernelGGL(k_fl_replay<true>, dim3(nq), dim3(kWave), lds, s, a);
+        else hipLaunchKernelGGL(k_fl_replay<false>, dim3(nq), dim3(kWave), lds, s, a);
+    } else {
+        if (stage) hipLaunchKernelGGL(k_fs_replay<true>, dim3(nq), dim3(kWave), lds, s, a);
+        else hipLaunchKernelGGL(k_fs_replay<false>, dim3(nq), dim3(kWave), lds, s, a);
+    }
 }
 
 size_t rerank_keys(int kc) {

@@ -115,6 +115,9 @@ struct ReplayArgs {
 // Largest LDS a kernel here asks for (the default dynamic limit).
 constexpr size_t kLdsMax = 64 * 1024;
 
+// Replays of batches below this many queries stage their loads through LDS.
+constexpr int kReplayStageBelow = 2048;
+
 // LDS bytes of k_fs_replay's TopK buffer.
 inline size_t replay_lds_bytes(int kp) { return static_cast<size_t>(2 * kp) * 12; }
 

@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 10M vectors (and of nlist)")
     ap.add_argument("--seconds", type=float, default=6.0, help="length of each load level")
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--stage-seconds", type=float, default=2.0, help="length of the stage-profiled load (0: none)")
     args = ap.parse_args()
 
     import torch  # noqa: F401  (first: torch and the library share the HIP runtime)
@@ -118,6 +119,32 @@ def main():
         levels[f"{int(frac * 100)}%"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in st.items()}
         log(f"{int(frac * 100)}% load ({rate:,.0f} requests/s): {st['qps']:,.0f} queries/s, p50 {st['p50_ms']:.2f} ms, "
             f"p99 {st['p99_ms']:.2f} ms, {st['launches']} launches for {st['requests']} requests")
+    # per-stage kernel time of the searches (the library's stage profiler: each
+    # stage is synchronized and timed with HIP events, so this pass is slower)
+    stages = {}
+    if args.stage_seconds > 0:
+        import ctypes
+        import pathlib
+
+        lib = ctypes.CDLL(str(pathlib.Path(qann_amd.__file__).parent / "libqann_hip.so"))
+        lib.qann_hip_profile_enable.argtypes = [ctypes.c_int]
+        lib.qann_hip_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.c_int]
+        lib.qann_hip_stage_name.restype = ctypes.c_char_p
+        ms = (ctypes.c_double * 32)()
+        cnt = (ctypes.c_uint64 * 32)()
+        lib.qann_hip_profile_read(ms, cnt, 1)
+        lib.qann_hip_profile_enable(1)
+        qann_amd.serve_load(server, pool, rate=0.25 * sat_qps / mean_nq, seconds=args.stage_seconds, seed=7, **spec)
+        lib.qann_hip_profile_enable(0)
+        lib.qann_hip_profile_read(ms, cnt, 1)
+        total = sum(ms[i] for i in range(32))
+        for i in range(32):
+            name = lib.qann_hip_stage_name(i).decode()
+            if name and cnt[i]:
+                stages[name] = {"ms_per_launch": round(ms[i] / cnt[i], 4), "share": round(ms[i] / total, 4)}
+        log("stages (ms per launch, share of stage time): " +
+            ", ".join(f"{k} {v['ms_per_launch']:.3f} ({100 * v['share']:.0f}%)" for k, v in stages.items()))
     server.close()
     print(json.dumps({
         "workload": f"serving: {n:,} x {dim} L2, IVF{nlist} + PQ{m}x4 fast-scan + RefineIndex(k_factor={kf}) in HBM; "
@@ -127,6 +154,7 @@ def main():
         "recall@10": recall,
         "served_equal_to_single_calls": f"{same}/{len(checks)}",
         "workers": args.workers,
+        "stage_ms_per_launch_at_25pct": stages,
     }), flush=True)
 
 

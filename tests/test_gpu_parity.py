@@ -45,6 +45,23 @@ def test_gpu_reproduces_reference_fixtures(oracle, name, queries):
     assert_same((got_ids, got_d), oracle.load(path).batch_query(q, ids.shape[1]), name)
 
 
+@pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
+def test_gpu_large_batch_matches_goldens(name):
+    # the goldens' queries repeated past 2,048 rows: batches that large replay
+    # the TopK without staging its loads (smaller ones stage them)
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    reps = 2048 // q.shape[0] + 1
+    qq = np.tile(q, (reps, 1))
+    for npb, k, fname, allowed, stem in synth_queries(exp, params):
+        if npb is not None:
+            ivf.nprobe = npb
+        got = idx.batch_query(qq, k, allowed=allowed)
+        want = (np.tile(exp[stem + "_ids"], (reps, 1)), np.tile(exp[stem + "_dists"], (reps, 1)))
+        assert_same(got, want, f"{name} {stem}")
+
+
 def test_gpu_single_query_matches_batch_row():
     path, q, _, _ = synth_case("l2_fs_refine")
     idx = qann_amd.load(str(path))
