@@ -292,6 +292,10 @@ __global__ void k_scatter_results(const int64_t* sids, const float* sd, const ui
     dists[o] = sd[t];
 }
 
+// Fast-scan chunks of at least this many queries are scanned in the order of
+// their nearest list (query_order), XCD by XCD.
+constexpr uint64_t kOrderMinQueries = 1024;
+
 // FlatIndex rows below this count keep to the FP32 GEMM path.
 constexpr uint64_t kFlatBf16MinRows = 65536;
 
@@ -628,6 +632,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + pad256(nq * 4);
     need += gtopk ? pad256(nq * 2 * kp * 4) + pad256(nq * 2 * kp * 8) : 0;
     need += grerank ? pad256(nq * rkeys * 8) : 0;
+    const bool order_queries = fs && chunk >= kOrderMinQueries;
+    need += order_queries ? pad256(nlist * 4) + pad256(chunk * 4) : 0;
     sc.used = 0;
     sc.reserve(need);
     const float* qr = d_q;
@@ -711,6 +717,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         fdists = sc.take<float>(chunk * stride);
         fbmin = sc.take<uint32_t>(chunk * (stride / qd::kFsBlock));
     }
+    uint32_t* ord_hist = order_queries ? sc.take<uint32_t>(nlist) : nullptr;
+    uint32_t* order = order_queries ? sc.take<uint32_t>(chunk) : nullptr;
     float* cd = sc.take<float>(nq * kp);
     int64_t* cl = sc.take<int64_t>(nq * kp);
     int* ccount = sc.take<int>(nq);
@@ -792,6 +800,12 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.item_offset = item_offset + q0 * M;
         a.item_min = item_min + q0 * M;
         if (qd::fs_sums_lds(a) == 0) throw Error(QANN_EINVAL, "the fast-scan tables of this index do not fit in LDS");
+        if (order && nc >= kOrderMinQueries) {
+            qd::launch_query_order(a.probe_idx, static_cast<int>(M), static_cast<int>(nc), static_cast<int>(nlist),
+                                   ord_hist, order, st);
+            check_launch("k_order");
+            a.order = order;
+        }
         StageTimer t_scan(QANN_STAGE_SCAN, st);
         qd::launch_fs_sums(a, static_cast<int>(nc), st);
         check_launch("k_fs_sums");

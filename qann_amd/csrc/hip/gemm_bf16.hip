@@ -11,6 +11,10 @@
 //   k_flat_finish                  per query, the exact k nearest among the
 //                                  candidates, or a hand-over to the FP32 path
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "device_common.hpp"
 #include "gemm_bf16.hpp"
@@ -376,7 +380,96 @@ void gemm(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t
                        A, M, lda, B, N, ldb, kp, per_xcd, epi);
 }
 
+__global__ void k_zero_cnt(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// The tile shapes the launches choose from: wave tiles of (32 TM) x (32 TN)
+// with (TM, TN) = (2, 2), (2, 4), (4, 2).
+constexpr int kVariants = 3;
+
+template <bool A_BIG, class Epi>
+void gemm_variant(int v, const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp,
+                  const Epi& epi, hipStream_t s) {
+    switch (v) {
+    case 1: gemm<2, 4, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
+    case 2: gemm<4, 2, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
+    default: gemm<2, 2, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
+    }
+}
+
+// The autotuner: per problem class (which epilogue, K, the size classes of M
+// and N), the first launch times every tile shape on the problem itself (HIP
+// events on the caller's stream; reset() undoes an epilogue's effect between
+// trials) and keeps the fastest for the life of the process.
+struct TuneKey {
+    int kind, kp, mclass, nclass;
+    bool operator<(const TuneKey& o) const {
+        return kind != o.kind ? kind < o.kind : kp != o.kp ? kp < o.kp : mclass != o.mclass ? mclass < o.mclass : nclass < o.nclass;
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tuned;
+
+int size_class(int x) {
+    int c = 0;
+    while ((1 << c) < x) ++c;
+    return c;
+}
+
+template <bool A_BIG, class Epi, class Reset>
+void gemm_tuned(int kind, const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp,
+                const Epi& epi, Reset reset, hipStream_t s) {
+    const TuneKey key{kind, kp, size_class(M), size_class(N)};
+    int best = -1;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) best = it->second;
+    }
+    if (best < 0) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            best = 0;
+        } else {
+            float best_ms = 0.0f, times[kVariants] = {};
+            for (int v = 0; v < kVariants; ++v) {
+                reset();
+                (void)hipEventRecord(a, s);
+                gemm_variant<A_BIG>(v, A, M, lda, B, N, ldb, kp, epi, s);
+                (void)hipEventRecord(b, s);
+                float ms = 0.0f;
+                if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) continue;
+                times[v] = ms;
+                if (best < 0 || ms < best_ms) {
+                    best = v;
+                    best_ms = ms;
+                }
+            }
+            // QANN_AMD_TUNE_LOG=1 prints each choice and its trial times
+            const char* log = std::getenv("QANN_AMD_TUNE_LOG");
+            if (log && *log && *log != '0')
+                std::fprintf(stderr, "qann_amd tune: gemm kind %d M %d N %d K %d -> shape %d (ms: %.3f %.3f %.3f)\n", kind,
+                             M, N, kp, best, times[0], times[1], times[2]);
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (best < 0) best = 0;
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            g_tuned[key] = best;
+        }
+        reset();
+    }
+    gemm_variant<A_BIG>(best, A, M, lda, B, N, ldb, kp, epi, s);
+}
+
 } // namespace
+
+int gemm_tuned_variant(int kind, int kp, int M, int N) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find(TuneKey{kind, kp, size_class(M), size_class(N)});
+    return it == g_tuned.end() ? -1 : it->second;
+}
 
 void launch_rows_bf16(const float* rows, uint64_t n, int dim, int kp, uint16_t* out, float* maxes, hipStream_t s) {
     if (n == 0) return;
@@ -394,11 +487,15 @@ void launch_flat_bf16(const FlatBf16& f, const float* q, const uint16_t* qb, con
                       hipStream_t s) {
     if (nq <= 0) return;
     EpiScores es{Ds, static_cast<size_t>(ns), f.ip, stride, f.xsq, f.excl, f.incl};
-    gemm<2, 2, false>(qb, nq, static_cast<size_t>(f.kp), f.xb, ns, static_cast<size_t>(stride) * f.kp, f.kp, es, s);
+    gemm_tuned<false>(kGemmScores, qb, nq, static_cast<size_t>(f.kp), f.xb, ns, static_cast<size_t>(stride) * f.kp,
+                      f.kp, es, [] {}, s);
     hipLaunchKernelGGL(k_flat_limit, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s, Ds, ns, want, qinfo, f.maxes,
                        f.xnorm_max, f.ip, f.dim, lim, cnt);
     EpiFilter ef{lim, f.ip, kFlatCandCap, f.xsq, f.excl, f.incl, cnt, cd, ci};
-    gemm<2, 2, true>(f.xb, f.n, static_cast<size_t>(f.kp), qb, nq, static_cast<size_t>(f.kp), f.kp, ef, s);
+    // k_flat_limit zeroed the counters; a tuning trial zeroes them again
+    auto reset = [&] { hipLaunchKernelGGL(k_zero_cnt, dim3(cdiv(nq, 256)), dim3(256), 0, s, cnt, nq); };
+    gemm_tuned<true>(kGemmFilter, f.xb, f.n, static_cast<size_t>(f.kp), qb, nq, static_cast<size_t>(f.kp), f.kp, ef,
+                     reset, s);
     const size_t lds = ((static_cast<size_t>(f.dim) * 4 + 15) & ~size_t{15}) + static_cast<size_t>(kFlatCandCap) * 8;
     hipLaunchKernelGGL(k_flat_finish, dim3(static_cast<unsigned>(nq)), dim3(256), lds, s, q, f.dim, f.ip, f.rows,
                        qinfo, f.maxes, f.xnorm_max, k, want, kFlatCandCap, cnt, cd, ci, f.labels, out_ids, out_dists,

@@ -61,4 +61,9 @@ void launch_flat_bf16(const FlatBf16& f, const float* q, const uint16_t* qb, con
                       int64_t* out_ids, float* out_dists, uint32_t* ovf, uint32_t* ovf_cnt, int ovf_base,
                       hipStream_t s);
 
+// Autotuned GEMM classes, and the tile shape chosen for one (index 0: 64x64
+// wave tiles, 1: 64x128, 2: 128x64; -1: not tuned yet).
+enum { kGemmScores = 0, kGemmFilter = 1 };
+int gemm_tuned_variant(int kind, int kp, int M, int N);
+
 } // namespace qd

@@ -624,12 +624,24 @@ __device__ __forceinline__ void l2_row(const DevIvf& ix, const float* B, const f
     }
 }
 
+// The LUT as k_fs_sums keeps it in LDS: every byte complemented except those
+// of entries 4 and 12 (a table row's four words XORed as lut_code does).
+// A nibble x then reads as perm(entries 0..7, x) ^ perm(entries 8..15, x ^ 8):
+// the half the nibble does not index answers 0x00 (v_perm selector 12, or
+// selectors 8..11, the sign of an entry that was not complemented) or 0xFF
+// (selectors 13..15, or the sign of a complemented entry), and the half it
+// indexes holds the entry complemented exactly where the other answers 0xFF.
+__device__ __forceinline__ uint4 lut_code(uint4 w) {
+    return make_uint4(w.x ^ 0xFFFFFFFFu, w.y ^ 0xFFFFFF00u, w.z ^ 0xFFFFFFFFu, w.w ^ 0xFFFFFF00u);
+}
+__device__ __forceinline__ uint32_t lut_code_word(uint32_t w, int i) { return w ^ ((i & 1) ? 0xFFFFFF00u : 0xFFFFFFFFu); }
+
 // The L2 table of one (query, probed list), rounded to bytes (quantize_lut,
 // src/fast_scan.cpp:22-46), by one wave, one table row per lane: the row's
 // minimum and maximum taken in entry order as std::min / std::max do, one
 // scale 63 / widest over all rows, offset (p0 + p1) + (p2 + p3) with p[i % 4]
 // summing row minimums in row order, bytes rounded half to even. Rows m ..
-// m4 - 1 of the LUT are zero.
+// m4 - 1 of the LUT are zero. The rows are written coded (lut_code).
 __device__ void wave_lut_l2(const DevIvf& ix, const float* q, int list, float A, const float* Cq, float* rowmin,
                             float* resid, uint8_t* lut, float& scale_out, float& offset_out) {
     const int m = ix.m, lane = lane_id();
@@ -648,8 +660,21 @@ __device__ void wave_lut_l2(const DevIvf& ix, const float* q, int list, float A,
             float lo = t[0], hi = t[0];
 #pragma unroll
             for (int j = 1; j < 16; ++j) {
-                lo = t[j] < lo ? t[j] : lo;
-                hi = hi < t[j] ? t[j] : hi;
+                lo = fminf(lo, t[j]);
+                hi = fmaxf(hi, t[j]);
+            }
+            // std::min / std::max in entry order give the same values except
+            // for a NaN first entry (kept) and the sign of a zero minimum (the
+            // first zero's); the maximum's zero sign never reaches a result
+            if (t[0] != t[0]) {
+                lo = t[0];
+                hi = t[0];
+            }
+            if (ballot(lo == 0.0f)) { // rare: taken by the wave only when some row needs it
+                if (lo == 0.0f) {
+                    for (int j = 15; j >= 0; --j)
+                        if (t[j] == 0.0f) lo = t[j];
+                }
             }
             rowmin[r] = lo;
             lo_keep = lo;
@@ -677,44 +702,29 @@ __device__ void wave_lut_l2(const DevIvf& ix, const float* q, int list, float A,
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 uint32_t v = 0;
+                // each value is a whole number in [0, 63]: the byte conversion is exact
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    v |= static_cast<uint32_t>(__builtin_rintf((t[4 * i + j] - lo) * scale)) << (8 * j);
+                    v = __builtin_amdgcn_cvt_pk_u8_f32(__builtin_rintf((t[4 * i + j] - lo) * scale), j, v);
                 w[i] = v;
             }
-            *reinterpret_cast<uint4*>(lut + r * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+            *reinterpret_cast<uint4*>(lut + r * 16) = lut_code(make_uint4(w[0], w[1], w[2], w[3]));
         } else if (r < ix.m4) {
-            *reinterpret_cast<uint4*>(lut + r * 16) = make_uint4(0u, 0u, 0u, 0u);
+            *reinterpret_cast<uint4*>(lut + r * 16) = lut_code(make_uint4(0u, 0u, 0u, 0u));
         }
     }
     scale_out = scale;
     offset_out = (p0 + p1) + (p2 + p3);
 }
 
-// Byte masks of bit 3 of each nibble of a code word, 0xFF where set: v_perm's
-// selectors 8..11 give the sign of bytes 1, 3 of its second source and bytes
-// 1, 3 of its first; with the word (shifted so the wanted bit is each byte's
-// bit 7) as the second source and the same shifted by 8 as the first, selector
-// 0x090B080A puts the sign of byte b of the word in byte b.
-__device__ __forceinline__ uint32_t bit3_mask_lo(uint32_t w) {
-    return __builtin_amdgcn_perm(w << 12, w << 4, 0x090B080Au);
-}
-__device__ __forceinline__ uint32_t bit3_mask_hi(uint32_t w) {
-    return __builtin_amdgcn_perm(w << 8, w, 0x090B080Au);
-}
-
 // Eight LUT lookups of one code word (4 bytes = vectors 4g..4g+3 in the low
-// nibbles, 16+4g.. in the high): each 16-byte LUT row is held in four
-// registers; v_perm picks from entries 0..7 and 8..15 with the nibble's low 3
-// bits and bit 3 chooses between them. The results are added bytewise.
+// nibbles, 16+4g.. in the high) against one coded 16-byte LUT row held in
+// four registers; the results are added bytewise.
 __device__ __forceinline__ void lookup8(uint32_t w, const uint4 r, uint32_t& lo8, uint32_t& hi8) {
-    const uint32_t sl = w & 0x07070707u;
-    const uint32_t sh = (w >> 4) & 0x07070707u;
-    const uint32_t ml = bit3_mask_lo(w), mh = bit3_mask_hi(w);
-    const uint32_t a0 = __builtin_amdgcn_perm(r.y, r.x, sl), a1 = __builtin_amdgcn_perm(r.w, r.z, sl);
-    const uint32_t b0 = __builtin_amdgcn_perm(r.y, r.x, sh), b1 = __builtin_amdgcn_perm(r.w, r.z, sh);
-    lo8 += (a1 & ml) | (a0 & ~ml);
-    hi8 += (b1 & mh) | (b0 & ~mh);
+    const uint32_t xl = w & 0x0F0F0F0Fu;
+    const uint32_t xh = (w >> 4) & 0x0F0F0F0Fu;
+    lo8 += __builtin_amdgcn_perm(r.y, r.x, xl) ^ __builtin_amdgcn_perm(r.w, r.z, xl ^ 0x08080808u);
+    hi8 += __builtin_amdgcn_perm(r.y, r.x, xh) ^ __builtin_amdgcn_perm(r.w, r.z, xh ^ 0x08080808u);
 }
 
 __device__ __forceinline__ uint4 load_code16(const uint8_t* p) { return *reinterpret_cast<const uint4*>(p); }
@@ -734,7 +744,18 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
     __shared__ uint32_t w_min[kSumsWin];
     const DevIvf& ix = a.ix;
     const int m = ix.m, lb = ix.m4 * 16, win = a.win;
-    const int qi = blockIdx.x / a.nwin, p0 = (blockIdx.x % a.nwin) * win;
+    int qi, p0;
+    if (a.order) {
+        // workgroup b runs on XCD b % 8: XCD x takes ranks x * per_xcd .. of the order
+        const int xcd = static_cast<int>(blockIdx.x % 8), slot = static_cast<int>(blockIdx.x / 8);
+        const int rank = xcd * a.per_xcd + slot / a.nwin;
+        if (rank >= a.nq) return;
+        qi = static_cast<int>(a.order[rank]);
+        p0 = (slot % a.nwin) * win;
+    } else {
+        qi = static_cast<int>(blockIdx.x) / a.nwin;
+        p0 = (static_cast<int>(blockIdx.x) % a.nwin) * win;
+    }
     const int P = a.nprobe_q[qi];
     if (p0 >= P) return;
     const int np = min(win, P - p0);
@@ -756,7 +777,8 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
     }
     if (ip) {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ip_lut + static_cast<size_t>(qi) * m * 16);
-        for (int e = tid; e < lb / 4; e += blockDim.x) reinterpret_cast<uint32_t*>(luts)[e] = e < m * 4 ? src[e] : 0u;
+        for (int e = tid; e < lb / 4; e += blockDim.x)
+            reinterpret_cast<uint32_t*>(luts)[e] = lut_code_word(e < m * 4 ? src[e] : 0u, e);
         if (tid < np) {
             a.item_scale[item0 + tid] = a.ip_scale[qi];
             a.item_offset[item0 + tid] = a.ip_offset[qi] + a.probe_base[item0 + tid];
@@ -991,6 +1013,45 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
     }
     __syncthreads();
     if (tid < np) a.item_min[item0 + tid] = w_min[tid];
+}
+
+// query_order: a counting sort of the queries by their nearest probed list.
+__global__ void k_order_zero(uint32_t* hist, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hist[i] = 0u;
+}
+
+__global__ void k_order_count(const uint16_t* probe_idx, int M, int nq, uint32_t* hist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) atomicAdd(&hist[probe_idx[static_cast<size_t>(i) * M]], 1u);
+}
+
+// Exclusive prefix sum of the nlist counters, in place, by one 1024-thread block.
+__global__ void __launch_bounds__(1024) k_order_scan(uint32_t* hist, int n) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x, per = (n + 1023) / 1024;
+    const int b = min(n, t * per), e = min(n, b + per);
+    uint32_t sum = 0;
+    for (int i = b; i < e; ++i) sum += hist[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int i = b; i < e; ++i) {
+        const uint32_t c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+__global__ void k_order_scatter(const uint16_t* probe_idx, int M, int nq, uint32_t* offs, uint32_t* order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) order[atomicAdd(&offs[probe_idx[static_cast<size_t>(i) * M]], 1u)] = static_cast<uint32_t>(i);
 }
 
 // Probes each query scans and where each probe's sums start: nprobe lists
@@ -1649,8 +1710,19 @@ size_t fs_sums_lds(SumsArgs& a) {
 
 void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
     const size_t lds = fs_sums_lds(a);
-    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(static_cast<size_t>(nq) * a.nwin)),
-                       dim3(kSumsWaves * kWave), lds, s, a);
+    a.nq = nq;
+    a.per_xcd = (nq + 7) / 8;
+    const size_t grid = a.order ? static_cast<size_t>(8) * a.per_xcd * a.nwin : static_cast<size_t>(nq) * a.nwin;
+    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(grid)), dim3(kSumsWaves * kWave), lds, s, a);
+}
+
+void launch_query_order(const uint16_t* probe_idx, int M, int nq, int nlist, uint32_t* hist, uint32_t* order,
+                        hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(k_order_zero, dim3(cdiv(nlist, 256)), dim3(256), 0, s, hist, nlist);
+    hipLaunchKernelGGL(k_order_count, dim3(cdiv(nq, 256)), dim3(256), 0, s, probe_idx, M, nq, hist);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, s, hist, nlist);
+    hipLaunchKernelGGL(k_order_scatter, dim3(cdiv(nq, 256)), dim3(256), 0, s, probe_idx, M, nq, hist, order);
 }
 
 size_t ivf_dists_lds(DistArgs& a) {

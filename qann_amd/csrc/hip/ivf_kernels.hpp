@@ -65,6 +65,11 @@ struct SumsArgs {
     uint32_t* item_min;        // smallest sum of the probe's list
     int win, nwin;             // probes per workgroup and workgroups per query (set by the launcher)
     int resid_in_lds;
+    // Optional order of the queries (query_order): each XCD then takes a
+    // contiguous range of it, so queries that probe the same lists share
+    // that XCD's L2. nq and per_xcd are set by the launcher.
+    const uint32_t* order;
+    int nq, per_xcd;
 };
 
 // Phase 1 for PQ (ADC), SQ and float lists: the float distance of every
@@ -151,6 +156,11 @@ void launch_query_tables(const float* q, const DevIvf& ix, int nq, int mode, flo
 // Dynamic LDS of k_fs_sums (0 when a table does not fit); sets win, nwin and the LDS flags.
 size_t fs_sums_lds(SumsArgs& a);
 void launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
+// The queries ordered by their nearest probed list (a counting sort over the
+// lists; hist takes nlist counters): consecutive queries in it probe many of
+// the same lists.
+void launch_query_order(const uint16_t* probe_idx, int M, int nq, int nlist, uint32_t* hist, uint32_t* order,
+                        hipStream_t s);
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
                        const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
                        int* status, hipStream_t s);
