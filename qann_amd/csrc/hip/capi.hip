@@ -325,7 +325,9 @@ uint64_t flat_fp32_chunk(uint64_t nq, uint64_t n) {
 
 // The BF16 plan of a flat search: candidates per query come from rows under
 // a limit that a strided sample of the rows gives (about want * stride rows
-// each); the stride keeps them well inside the candidate buffer.
+// each). The stride keeps them within a quarter of the candidate buffer, and
+// leaves the sample at least 4 * want tiles of 128 rows, whose minima bound
+// the want-th score tightly.
 struct FlatBf16Plan {
     uint32_t stride = 0, ns = 0;
     uint64_t chunk = 0, per_query = 0;
@@ -333,10 +335,12 @@ struct FlatBf16Plan {
 
 FlatBf16Plan flat_bf16_plan(const qann_hip_index* h, uint64_t nq, uint32_t want) {
     FlatBf16Plan p;
-    p.stride = std::clamp<uint32_t>(static_cast<uint32_t>(qd::kFlatCandCap) / (4 * std::max<uint32_t>(want, 1)), 4, 64);
+    const uint64_t w = std::max<uint32_t>(want, 1);
+    const uint64_t by_cap = static_cast<uint64_t>(qd::kFlatCandCap) / (4 * w), by_tiles = h->slots / (512 * w);
+    p.stride = static_cast<uint32_t>(std::clamp<uint64_t>(std::min(by_cap, by_tiles), 1, 64));
     p.ns = static_cast<uint32_t>((h->slots + p.stride - 1) / p.stride);
-    p.per_query = pad256(static_cast<uint64_t>(h->kp) * 2) + 12 + static_cast<uint64_t>(p.ns) * 4 + 8 +
-                  static_cast<uint64_t>(qd::kFlatCandCap) * 8;
+    p.per_query = pad256(static_cast<uint64_t>(h->kp) * 2) + 12 + static_cast<uint64_t>(qd::flat_sample_tiles(p.ns)) * 4 +
+                  8 + static_cast<uint64_t>(qd::kFlatCandCap) * 8;
     p.chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, (uint64_t{1} << 30) / p.per_query));
     return p;
 }
@@ -350,11 +354,12 @@ std::vector<uint32_t> flat_bf16(qann_hip_index* h, const float* d_q, uint64_t nq
     const FlatBf16Plan pl = flat_bf16_plan(h, nq, want);
     const uint64_t ch = pl.chunk;
     sc.used = 0;
-    sc.reserve(pad256(ch * h->kp * 2) + pad256(ch * 12) + pad256(ch * pl.ns * 4) + 2 * pad256(ch * 4) +
+    const uint64_t ntile = static_cast<uint64_t>(qd::flat_sample_tiles(static_cast<int>(pl.ns)));
+    sc.reserve(pad256(ch * h->kp * 2) + pad256(ch * 12) + pad256(ch * ntile * 4) + 2 * pad256(ch * 4) +
                2 * pad256(ch * qd::kFlatCandCap * 4) + pad256(nq * 4) + 256);
     uint16_t* qb = sc.take<uint16_t>(ch * h->kp);
     float* qinfo = sc.take<float>(ch * 3);
-    float* Ds = sc.take<float>(ch * pl.ns);
+    float* Ds = sc.take<float>(ch * ntile);
     float* lim = sc.take<float>(ch);
     uint32_t* cnt = sc.take<uint32_t>(ch);
     float* cd = sc.take<float>(ch * qd::kFlatCandCap);

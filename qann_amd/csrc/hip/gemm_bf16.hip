@@ -229,23 +229,29 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(const uint16_t* __restrict__ 
         if (kt + 1 < nk) lstore(buf ^ 1);
         __syncthreads();
     }
+    if constexpr (Epi::kTile) {
+        // a whole-tile epilogue, with the (now idle) LDS for its reductions
+        epi.template tile<TM, TN>(acc, m0, n0, wm, wn, lane, M, N, reinterpret_cast<float*>(&As[0][0]));
+    } else {
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const int col = n0 + wn * 32 * TN + 32 * j + (lane & 31);
+            for (int j = 0; j < TN; ++j) {
+                const int col = n0 + wn * 32 * TN + 32 * j + (lane & 31);
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int row = m0 + wm * 32 * TM + 32 * i + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
-                if (row < M && col < N) epi(row, col, acc[i][j][v]);
+                for (int v = 0; v < 16; ++v) {
+                    const int row = m0 + wm * 32 * TM + 32 * i + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
+                    if (row < M && col < N) epi(row, col, acc[i][j][v]);
+                }
             }
-        }
+    }
 }
 
 // Scores of queries (A rows) against every stride-th row (B rows, B's row j
 // is row j * stride): ||x||^2 - 2 q.x, or -q.x, and +inf for rows the query
 // may not return.
 struct EpiScores {
+    static constexpr bool kTile = false;
     float* D;
     size_t ldd;
     int ip, stride;
@@ -262,6 +268,7 @@ struct EpiScores {
 // Rows (A rows) against queries (B rows): a row whose score is at most the
 // query's limit, and which the query may return, joins its candidates.
 struct EpiFilter {
+    static constexpr bool kTile = false;
     const float* lim;
     int ip, cap;
     const float* xsq;
@@ -283,21 +290,73 @@ struct EpiFilter {
     }
 };
 
+// Every stride-th row (A rows; A's row i is row i * stride) against queries
+// (B rows): per query, the smallest approximate score over each tile of 128
+// sampled rows that it may return (+inf if none) into out[query * ntile + tile].
+// The want-th smallest of those minima is an upper bound on the want-th
+// smallest score of the sample, and so of every row: the minima belong to
+// distinct rows.
+struct EpiTileMin {
+    static constexpr bool kTile = true;
+    float* out;
+    int ntile, ip, stride;
+    const float* xsq;
+    const uint64_t* excl;
+    const uint64_t* incl;
+    template <int TM, int TN>
+    __device__ __forceinline__ void tile(const f32x16 (&acc)[TM][TN], int m0, int n0, int wm, int wn, int lane, int M,
+                                         int N, float* red) const {
+        float mn[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mn[j] = INFINITY;
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = m0 + wm * 32 * TM + 32 * i + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
+                if (row >= M) continue;
+                const uint32_t r = static_cast<uint32_t>(row) * static_cast<uint32_t>(stride);
+                if ((excl && bit_test(excl, r)) || (incl && !bit_test(incl, r))) continue;
+                const float x2 = ip ? 0.0f : xsq[r];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const float s = acc[i][j][v];
+                    mn[j] = fminf(mn[j], ip ? -s : x2 - 2.0f * s);
+                }
+            }
+        // lanes t and t + 32 hold the two row halves of one query column
+#pragma unroll
+        for (int j = 0; j < TN; ++j) mn[j] = fminf(mn[j], __shfl_xor(mn[j], 32, kWave));
+        if (wm == 1 && lane < 32)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) red[(wn * TN + j) * 32 + lane] = mn[j];
+        __syncthreads();
+        if (wm == 0 && lane < 32) {
+            const int t = m0 / (64 * TM);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int col = n0 + wn * 32 * TN + 32 * j + lane;
+                if (col < N) out[static_cast<size_t>(col) * ntile + t] = fminf(mn[j], red[(wn * TN + j) * 32 + lane]);
+            }
+        }
+    }
+};
+
 // Per query: tau, an upper bound on the want-th smallest approximate score of
-// the rows it may return (the want-th smallest of a sample is one), and the
+// the rows it may return (the want-th smallest of its tile minima), and the
 // candidate limit tau + 2 e.
-__global__ void __launch_bounds__(256) k_flat_limit(const float* Ds, int ns, int want, const float* qinfo,
+__global__ void __launch_bounds__(256) k_flat_limit(const float* tmin, int ntile, int want, const float* qinfo,
                                                     const float* maxes, float xmax, int ip, int dim, float* lim,
                                                     uint32_t* cnt) {
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_wsum[4], s_digit, s_below;
     __shared__ float s_red8[8];
     const int i = blockIdx.x;
-    const float* row = Ds + static_cast<size_t>(i) * ns;
-    const int target = want + max(64, want / 4);
-    const float tau = ns <= 16 * 256
-                          ? select_threshold<true, true>(row, ns, want, target, hist, s_wsum, &s_digit, &s_below, s_red8)
-                          : select_threshold<false, true>(row, ns, want, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+    const float* row = tmin + static_cast<size_t>(i) * ntile;
+    const int target = want + max(16, want / 8);
+    const float tau = ntile <= 16 * 256
+                          ? select_threshold<true, true>(row, ntile, want, target, hist, s_wsum, &s_digit, &s_below, s_red8)
+                          : select_threshold<false, true>(row, ntile, want, target, hist, s_wsum, &s_digit, &s_below, s_red8);
     if (threadIdx.x == 0) {
         const float e = flat_err(ip, dim, qinfo + 3 * i, maxes, xmax);
         lim[i] = tau <= 3.40282347e38f ? within(tau, e) : INFINITY;
@@ -486,11 +545,12 @@ void launch_flat_bf16(const FlatBf16& f, const float* q, const uint16_t* qb, con
                       int64_t* out_ids, float* out_dists, uint32_t* ovf, uint32_t* ovf_cnt, int ovf_base,
                       hipStream_t s) {
     if (nq <= 0) return;
-    EpiScores es{Ds, static_cast<size_t>(ns), f.ip, stride, f.xsq, f.excl, f.incl};
-    gemm_tuned<false>(kGemmScores, qb, nq, static_cast<size_t>(f.kp), f.xb, ns, static_cast<size_t>(stride) * f.kp,
-                      f.kp, es, [] {}, s);
-    hipLaunchKernelGGL(k_flat_limit, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s, Ds, ns, want, qinfo, f.maxes,
-                       f.xnorm_max, f.ip, f.dim, lim, cnt);
+    // the sample: every stride-th row, one minimum per query per 128 of them
+    const int ntile = flat_sample_tiles(ns);
+    EpiTileMin et{Ds, ntile, f.ip, stride, f.xsq, f.excl, f.incl};
+    gemm<2, 2, true>(f.xb, ns, static_cast<size_t>(stride) * f.kp, qb, nq, static_cast<size_t>(f.kp), f.kp, et, s);
+    hipLaunchKernelGGL(k_flat_limit, dim3(static_cast<unsigned>(nq)), dim3(256), 0, s, Ds, ntile, want, qinfo,
+                       f.maxes, f.xnorm_max, f.ip, f.dim, lim, cnt);
     EpiFilter ef{lim, f.ip, kFlatCandCap, f.xsq, f.excl, f.incl, cnt, cd, ci};
     // k_flat_limit zeroed the counters; a tuning trial zeroes them again
     auto reset = [&] { hipLaunchKernelGGL(k_zero_cnt, dim3(cdiv(nq, 256)), dim3(256), 0, s, cnt, nq); };

@@ -51,8 +51,12 @@ struct FlatBf16 {
     const int64_t* labels; // slot -> id, or null
 };
 
+// Tile minima per query of a sample of ns rows (the sample GEMM's row tiles).
+inline int flat_sample_tiles(int ns) { return (ns + 127) / 128; }
+
 // One chunk of queries (qb, qinfo from launch_queries_bf16; q the FP32 rows):
-// sample GEMM into Ds (nq * ns floats, rows stride, stride * ns >= n), limits,
+// sample GEMM (every stride-th row, stride * ns >= n) into Ds, its tile minima
+// (nq * flat_sample_tiles(ns) floats), limits,
 // filtered GEMM into the candidate buffers (cnt nq, cd / ci nq * kFlatCandCap),
 // then the exact selection into out_ids / out_dists (nq * k). Queries it
 // cannot finish go to ovf (ovf_cnt counts them; the caller zeroes it).
