@@ -102,7 +102,7 @@ std::atomic<int> g_prof_on{0};
 std::mutex g_prof_mu;
 double g_prof_ms[QANN_STAGE_COUNT] = {};
 // bytes of u16 sums one search keeps at once (queries are scanned in chunks within it)
-uint64_t g_sums_budget = uint64_t{2} << 30;
+uint64_t g_sums_budget = uint64_t{8} << 30;
 uint64_t g_prof_n[QANN_STAGE_COUNT] = {};
 
 // Times one stage when profiling is on: events around the launches, then a sync.
