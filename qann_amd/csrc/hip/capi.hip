@@ -639,6 +639,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += grerank ? pad256(nq * rkeys * 8) : 0;
     const bool order_queries = fs && chunk >= kOrderMinQueries;
     need += order_queries ? pad256(nlist * 4) + pad256(chunk * 4) : 0;
+    need += fs ? pad256(8 * 4) : 0; // work-queue counters of the persistent sums kernel
     sc.used = 0;
     sc.reserve(need);
     const float* qr = d_q;
@@ -722,6 +723,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         fdists = sc.take<float>(chunk * stride);
         fbmin = sc.take<uint32_t>(chunk * (stride / qd::kFsBlock));
     }
+    uint32_t* sums_queue = fs ? sc.take<uint32_t>(8) : nullptr;
     uint32_t* ord_hist = order_queries ? sc.take<uint32_t>(nlist) : nullptr;
     uint32_t* order = order_queries ? sc.take<uint32_t>(chunk) : nullptr;
     float* cd = sc.take<float>(nq * kp);
@@ -805,6 +807,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.item_offset = item_offset + q0 * M;
         a.item_min = item_min + q0 * M;
         if (qd::fs_sums_lds(a) == 0) throw Error(QANN_EINVAL, "the fast-scan tables of this index do not fit in LDS");
+        a.queue = sums_queue;
         if (order && nc >= kOrderMinQueries) {
             qd::launch_query_order(a.probe_idx, static_cast<int>(M), static_cast<int>(nc), static_cast<int>(nlist),
                                    ord_hist, order, st);

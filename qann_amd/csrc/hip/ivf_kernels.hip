@@ -17,6 +17,12 @@
 //   k_rerank          exact FP32 re-rank of the candidates (RefineIndex::rerank, src/refine_index.cpp:112-140)
 //   k_block_select    exact top-k over a slot set (FlatIndex::query_impl, RefineIndex::direct_scan)
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "device_common.hpp"
 #include "ivf_kernels.hpp"
@@ -737,25 +743,11 @@ __device__ __forceinline__ uint4 load_code16(const uint8_t* p) { return *reinter
 // column g's words of subspaces 4i..4i+3 together) and writes the u16 sums of
 // vectors 4g..4g+3 and 16+4g..16+4g+3 (fast_scan_block, src/fast_scan.cpp:256-262).
 // Each probe's smallest sum goes to item_min for the replay's list skip.
-__global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t w_start[kSumsWin + 1];
-    __shared__ int w_list[kSumsWin];
-    __shared__ uint32_t w_min[kSumsWin];
+// One (query, window) of phase 1; the workgroup's shared state is passed in.
+__device__ __forceinline__ void sums_item(const SumsArgs& a, int qi, int p0, unsigned char* lds, uint32_t* w_start,
+                                          int* w_list, uint32_t* w_min) {
     const DevIvf& ix = a.ix;
     const int m = ix.m, lb = ix.m4 * 16, win = a.win;
-    int qi, p0;
-    if (a.order) {
-        // workgroup b runs on XCD b % 8: XCD x takes ranks x * per_xcd .. of the order
-        const int xcd = static_cast<int>(blockIdx.x % 8), slot = static_cast<int>(blockIdx.x / 8);
-        const int rank = xcd * a.per_xcd + slot / a.nwin;
-        if (rank >= a.nq) return;
-        qi = static_cast<int>(a.order[rank]);
-        p0 = (slot % a.nwin) * win;
-    } else {
-        qi = static_cast<int>(blockIdx.x) / a.nwin;
-        p0 = (static_cast<int>(blockIdx.x) % a.nwin) * win;
-    }
     const int P = a.nprobe_q[qi];
     if (p0 >= P) return;
     const int np = min(win, P - p0);
@@ -870,6 +862,77 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a
     }
     __syncthreads();
     if (tid < np) a.item_min[item0 + tid] = w_min[tid];
+}
+
+// The (query, window) of work item `item` of the XCD-ordered form: XCD x owns
+// ranks x * per_xcd .. of the order, window by window.
+__device__ __forceinline__ void ordered_item(const SumsArgs& a, int x, int i, int& qi, int& p0, bool& ok) {
+    const int rank = x * a.per_xcd + i / a.nwin;
+    ok = rank < a.nq;
+    qi = ok ? static_cast<int>(a.order ? a.order[rank] : static_cast<uint32_t>(rank)) : 0;
+    p0 = (i % a.nwin) * a.win;
+}
+
+__global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t w_start[kSumsWin + 1];
+    __shared__ int w_list[kSumsWin];
+    __shared__ uint32_t w_min[kSumsWin];
+    int qi, p0;
+    if (a.order) {
+        // workgroup b runs on XCD b % 8: XCD x takes ranks x * per_xcd .. of the order
+        bool ok;
+        ordered_item(a, static_cast<int>(blockIdx.x % 8), static_cast<int>(blockIdx.x / 8), qi, p0, ok);
+        if (!ok) return;
+    } else {
+        qi = static_cast<int>(blockIdx.x) / a.nwin;
+        p0 = (static_cast<int>(blockIdx.x) % a.nwin) * a.win;
+    }
+    sums_item(a, qi, p0, lds, w_start, w_list, w_min);
+}
+
+// Phase 1 as a persistent kernel: a grid that fills the chip once, each
+// workgroup pulling (query, window) items from a device queue per XCD (the
+// XCD-ordered items of that XCD; a drained queue sends it to the others') until
+// every queue is drained.
+// One pull from the queues, starting at the home XCD's: (queue, index) or x < 0.
+__device__ __forceinline__ void pull_item(const SumsArgs& a, int home, int per_queue, int& x_out, int& i_out) {
+    x_out = -1;
+    i_out = 0;
+    for (int y = 0; y < 8; ++y) {
+        const int x = (home + y) % 8;
+        if (__atomic_load_n(&a.queue[x], __ATOMIC_RELAXED) >= static_cast<uint32_t>(per_queue)) continue;
+        const uint32_t i = atomicAdd(&a.queue[x], 1u);
+        if (i < static_cast<uint32_t>(per_queue)) {
+            x_out = x;
+            i_out = static_cast<int>(i);
+            return;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums_queue(const SumsArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t w_start[kSumsWin + 1];
+    __shared__ int w_list[kSumsWin];
+    __shared__ uint32_t w_min[kSumsWin];
+    __shared__ int s_x[2], s_i[2];
+    // the XCD this workgroup runs on (hardware register XCC_ID), whose queue it drains first
+    const int home = static_cast<int>(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7);
+    const int per_queue = a.per_xcd * a.nwin;
+    if (threadIdx.x == 0) pull_item(a, home, per_queue, s_x[0], s_i[0]);
+    __syncthreads();
+    for (int cur = 0;; cur ^= 1) {
+        const int x = s_x[cur], i = s_i[cur];
+        if (x < 0) return;
+        // the next item's pull is in flight while this one runs
+        if (threadIdx.x == 0) pull_item(a, home, per_queue, s_x[cur ^ 1], s_i[cur ^ 1]);
+        int qi, p0;
+        bool ok;
+        ordered_item(a, x, i, qi, p0, ok);
+        if (ok) sums_item(a, qi, p0, lds, w_start, w_list, w_min);
+        __syncthreads();
+    }
 }
 
 // Phase 1 for lists stored as PQ codes (ADC), SQ codes or raw vectors: one
@@ -1708,12 +1771,91 @@ size_t fs_sums_lds(SumsArgs& a) {
     return (ip ? lb : static_cast<size_t>(a.win) * lb) + fixed;
 }
 
+namespace {
+
+struct SumsKey {
+    int m4, win, nwin, nclass, ordered;
+    bool operator<(const SumsKey& o) const {
+        return std::tie(m4, win, nwin, nclass, ordered) < std::tie(o.m4, o.win, o.nwin, o.nclass, o.ordered);
+    }
+};
+std::mutex g_sums_mu;
+std::map<SumsKey, int> g_sums_form;
+
+// workgroups of k_fs_sums_queue that fill the chip once (a multiple of 8, one per XCD slot)
+unsigned persistent_grid(size_t lds) {
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_fs_sums_queue, kSumsWaves * kWave, lds) != hipSuccess ||
+        per <= 0)
+        per = 1;
+    return static_cast<unsigned>((cus * per + 7) / 8 * 8);
+}
+
+void sums_form(int form, const SumsArgs& a, int nq, size_t lds, hipStream_t s) {
+    if (form == 1) {
+        hipLaunchKernelGGL(k_order_zero, dim3(1), dim3(64), 0, s, a.queue, 8);
+        hipLaunchKernelGGL(k_fs_sums_queue, dim3(persistent_grid(lds)), dim3(kSumsWaves * kWave), lds, s, a);
+        return;
+    }
+    const size_t grid = a.order ? static_cast<size_t>(8) * a.per_xcd * a.nwin : static_cast<size_t>(nq) * a.nwin;
+    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(grid)), dim3(kSumsWaves * kWave), lds, s, a);
+}
+
+} // namespace
+
 void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
     const size_t lds = fs_sums_lds(a);
     a.nq = nq;
     a.per_xcd = (nq + 7) / 8;
-    const size_t grid = a.order ? static_cast<size_t>(8) * a.per_xcd * a.nwin : static_cast<size_t>(nq) * a.nwin;
-    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(grid)), dim3(kSumsWaves * kWave), lds, s, a);
+    if (!a.queue) {
+        sums_form(0, a, nq, lds, s);
+        return;
+    }
+    // QANN_AMD_SUMS_FORM=grid or =queue fixes the form (tests run both)
+    const char* force = std::getenv("QANN_AMD_SUMS_FORM");
+    if (force && (std::strcmp(force, "grid") == 0 || std::strcmp(force, "queue") == 0)) {
+        sums_form(std::strcmp(force, "queue") == 0 ? 1 : 0, a, nq, lds, s);
+        return;
+    }
+    int nclass = 0;
+    while ((1 << nclass) < nq) ++nclass;
+    const SumsKey key{a.ix.m4, a.win, a.nwin, nclass, a.order != nullptr};
+    int form = -1;
+    {
+        std::lock_guard<std::mutex> g(g_sums_mu);
+        auto it = g_sums_form.find(key);
+        if (it != g_sums_form.end()) form = it->second;
+    }
+    if (form < 0) {
+        // both forms write the same outputs, so the trials can run on the real problem
+        float t[2] = {0.0f, 0.0f};
+        hipEvent_t e0, e1;
+        form = 0;
+        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+            for (int f = 0; f < 2; ++f) {
+                (void)hipEventRecord(e0, s);
+                sums_form(f, a, nq, lds, s);
+                (void)hipEventRecord(e1, s);
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t[f], e0, e1) != hipSuccess)
+                    t[f] = 1e30f;
+            }
+            form = t[1] < t[0] ? 1 : 0;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        const char* log = std::getenv("QANN_AMD_TUNE_LOG");
+        if (log && *log && *log != '0')
+            std::fprintf(stderr,
+                         "qann_amd tune: fast-scan sums nq %d m4 %d windows %d -> %s (ms: %.3f grid, %.3f queue of %u "
+                         "workgroups)\n",
+                         nq, a.ix.m4, a.nwin, form ? "persistent queue" : "one workgroup per window", t[0], t[1],
+                         persistent_grid(lds));
+        std::lock_guard<std::mutex> g(g_sums_mu);
+        g_sums_form[key] = form;
+    }
+    sums_form(form, a, nq, lds, s);
 }
 
 void launch_query_order(const uint16_t* probe_idx, int M, int nq, int nlist, uint32_t* hist, uint32_t* order,

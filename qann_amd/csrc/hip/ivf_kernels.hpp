@@ -70,6 +70,8 @@ struct SumsArgs {
     // that XCD's L2. nq and per_xcd are set by the launcher.
     const uint32_t* order;
     int nq, per_xcd;
+    // 8 counters (one per XCD) for the persistent form, or null for the plain launch only
+    uint32_t* queue;
 };
 
 // Phase 1 for PQ (ADC), SQ and float lists: the float distance of every
@@ -155,6 +157,9 @@ void launch_query_tables(const float* q, const DevIvf& ix, int nq, int mode, flo
                          float* offset, hipStream_t s);
 // Dynamic LDS of k_fs_sums (0 when a table does not fit); sets win, nwin and the LDS flags.
 size_t fs_sums_lds(SumsArgs& a);
+// Phase 1, as one workgroup per (query, window) or as the persistent
+// work-queue kernel; which of the two a problem class runs is chosen by timing
+// both on its first launch (QANN_AMD_TUNE_LOG=1 prints the choice).
 void launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
 // The queries ordered by their nearest probed list (a counting sort over the
 // lists; hist takes nlist counters): consecutive queries in it probe many of

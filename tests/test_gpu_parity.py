@@ -62,6 +62,25 @@ def test_gpu_large_batch_matches_goldens(name):
         assert_same(got, want, f"{name} {stem}")
 
 
+@pytest.mark.parametrize("form", ["grid", "queue"])
+@pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp"])
+def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
+    # phase 1 as one workgroup per (query, window) and as the persistent
+    # work-queue kernel, each forced, small and large (XCD-ordered) batches
+    monkeypatch.setenv("QANN_AMD_SUMS_FORM", form)
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    reps = 2048 // q.shape[0] + 1
+    for qq, r in ((q, 1), (np.tile(q, (reps, 1)), reps)):
+        for npb, k, fname, allowed, stem in synth_queries(exp, params):
+            if npb is not None:
+                ivf.nprobe = npb
+            got = idx.batch_query(qq, k, allowed=allowed)
+            want = (np.tile(exp[stem + "_ids"], (r, 1)), np.tile(exp[stem + "_dists"], (r, 1)))
+            assert_same(got, want, f"{name} {form} {stem}")
+
+
 def test_gpu_single_query_matches_batch_row():
     path, q, _, _ = synth_case("l2_fs_refine")
     idx = qann_amd.load(str(path))
