@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -61,6 +62,21 @@ struct Scratch {
     size_t cap = 0, used = 0;
     char* io = nullptr; // staging for the synchronous entry point's inputs and outputs
     size_t io_cap = 0;
+    // searches captured as HIP graphs, by shape (queries padded to a power of
+    // two, k, nprobe, k_factor), with the buffers they were captured over
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        char* base = nullptr;
+        char* io = nullptr;
+    };
+    std::map<std::tuple<uint64_t, uint32_t, uint32_t, uint32_t>, Graph> graphs;
+    std::map<std::tuple<uint64_t, uint32_t, uint32_t, uint32_t>, int> graph_seen;
+
+    void drop_graphs() {
+        for (auto& kv : graphs)
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        graphs.clear();
+    }
 
     char* reserve_io(size_t bytes) {
         if (bytes > io_cap) {
@@ -196,6 +212,8 @@ struct qann_hip_index {
     }
     ~qann_hip_index() {
         for (Scratch* s : free_scratch) {
+            if (s->own_stream) (void)hipStreamSynchronize(s->stream);
+            s->drop_graphs();
             if (s->base) (void)hipFree(s->base);
             if (s->io) (void)hipFree(s->io);
             if (s->own_stream) (void)hipStreamDestroy(s->stream);
@@ -875,6 +893,71 @@ void checkin(qann_hip_index* h, Scratch* s) {
     h->free_scratch.push_back(s);
 }
 
+// Searches of at most this many queries are captured as HIP graphs.
+constexpr uint64_t kGraphMaxQueries = 4096;
+
+// QANN_AMD_GRAPHS=0 turns the capture off.
+bool graphs_enabled() {
+    const char* e = std::getenv("QANN_AMD_GRAPHS");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+constexpr size_t kGraphsPerScratch = 64;
+constexpr int kGraphAfter = 3;
+
+// run_search of an unfiltered IVF / refine batch through a HIP graph: a shape
+// runs plainly a few times, is then captured, and later searches of it replay
+// the capture -- as long as the scratch and staging buffers it was captured
+// over are the ones the Scratch still holds.
+void run_graphed(qann_hip_index* h, const float* dq, uint64_t nr, const qann_hip_search_params& p, int64_t* dids,
+                 float* dd, Scratch& io) {
+    const auto key = std::make_tuple(nr, p.k, p.nprobe, p.k_factor);
+    auto it = io.graphs.find(key);
+    if (it != io.graphs.end()) {
+        if (it->second.base == io.base && it->second.io == io.io) {
+            check(hipGraphLaunch(it->second.exec, io.stream), "hipGraphLaunch");
+            return;
+        }
+        // captured over buffers since replaced: capture again
+        (void)hipGraphExecDestroy(it->second.exec);
+        io.graphs.erase(it);
+        io.graph_seen[key] = kGraphAfter;
+    }
+    // a shape is captured once it has run plainly kGraphAfter times (the first
+    // sizes the scratch and lets the launch autotuners settle; the wait keeps
+    // one-off shapes from paying for captures); -1: never (cache full or failed)
+    int& seen = io.graph_seen[key];
+    if (seen == kGraphAfter && io.graphs.size() >= kGraphsPerScratch) seen = -1;
+    if (seen != kGraphAfter) {
+        if (seen >= 0) ++seen;
+        run_search(h, dq, nr, p, nullptr, dids, dd, io);
+        return;
+    }
+    seen = -1;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    bool ok = hipStreamBeginCapture(io.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        try {
+            run_search(h, dq, nr, p, nullptr, dids, dd, io);
+        } catch (...) {
+            ok = false;
+        }
+        ok = hipStreamEndCapture(io.stream, &g) == hipSuccess && ok && g;
+        ok = ok && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();
+    if (!ok) {
+        // not capturable: this shape runs plainly from now on
+        if (ex) (void)hipGraphExecDestroy(ex);
+        run_search(h, dq, nr, p, nullptr, dids, dd, io);
+        return;
+    }
+    io.graphs[key] = Scratch::Graph{ex, io.base, io.io};
+    check(hipGraphLaunch(ex, io.stream), "hipGraphLaunch");
+}
+
 } // namespace
 
 extern "C" {
@@ -1143,8 +1226,13 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
         check(hipSetDevice(h->device), "hipSetDevice");
         Scratch* io = checkout(h);
         try {
+            // unfiltered IVF / refine searches of up to kGraphMaxQueries queries whose shape
+            // repeats run as HIP graphs (at most kGraphsPerScratch shapes per stream)
+            const bool graph = graphs_enabled() && !params->allowed_words && h->kind != 1 && !g_prof_on.load() &&
+                               nq <= kGraphMaxQueries;
+            const uint64_t nr = nq;
             // inputs and outputs in a second buffer so run_search owns the scratch
-            const size_t qb = pad256(nq * h->dim * 4), ob = pad256(nq * k * 8), db = pad256(nq * k * 4);
+            const size_t qb = pad256(nr * h->dim * 4), ob = pad256(nr * k * 8), db = pad256(nr * k * 4);
             const size_t fb = params->allowed_words ? pad256((h->slots + 63) / 64 * 8) : 0;
             char* buf = io->reserve_io(qb + ob + db + fb + 256);
             float* dq = reinterpret_cast<float*>(buf);
@@ -1152,6 +1240,14 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
             float* dd = reinterpret_cast<float*>(buf + qb + ob);
             uint64_t* df = fb ? reinterpret_cast<uint64_t*>(buf + qb + ob + db) : nullptr;
             check(hipMemcpyAsync(dq, queries, nq * h->dim * 4, hipMemcpyHostToDevice, io->stream), "hipMemcpyAsync");
+            if (graph) {
+                run_graphed(h, dq, nr, *params, dids, dd, *io);
+                check(hipMemcpyAsync(out_ids, dids, nq * k * 8, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
+                check(hipMemcpyAsync(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
+                check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
+                checkin(h, io);
+                return;
+            }
             if (df)
                 check(hipMemcpyAsync(df, params->allowed_words, (h->slots + 63) / 64 * 8, hipMemcpyHostToDevice,
                                      io->stream),

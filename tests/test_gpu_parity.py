@@ -81,6 +81,32 @@ def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
             assert_same(got, want, f"{name} {form} {stem}")
 
 
+@pytest.mark.parametrize("name", ["l2_fs_refine", "cos_fs_refine", "ip_pq6_custom", "l2_sq8"])
+def test_gpu_graph_replays_match_goldens(name):
+    # unfiltered searches run plainly three times, are captured as a HIP graph
+    # the fourth and replayed after that: every call, with other searches,
+    # filtered ones and host synchronizations between them, gives the goldens
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    runs = [(npb, k, stem) for npb, k, fname, allowed, stem in synth_queries(exp, params) if allowed is None]
+    filt = [(npb, k, allowed, stem) for npb, k, fname, allowed, stem in synth_queries(exp, params) if allowed is not None]
+    for rep in range(8):
+        for npb, k, stem in runs:
+            if npb is not None:
+                ivf.nprobe = npb
+            # 40 queries and 13 of them: two shapes, each captured once
+            assert_same(idx.batch_query(q, k), (exp[stem + "_ids"], exp[stem + "_dists"]), f"{name} {stem} rep {rep}")
+            assert_same(idx.batch_query(q[:13], k), (exp[stem + "_ids"][:13], exp[stem + "_dists"][:13]),
+                        f"{name} {stem} 13 rows rep {rep}")
+        if filt:
+            npb, k, allowed, stem = filt[rep % len(filt)]
+            if npb is not None:
+                ivf.nprobe = npb
+            assert_same(idx.batch_query(q, k, allowed=allowed), (exp[stem + "_ids"], exp[stem + "_dists"]),
+                        f"{name} {stem} filtered rep {rep}")
+
+
 def test_gpu_single_query_matches_batch_row():
     path, q, _, _ = synth_case("l2_fs_refine")
     idx = qann_amd.load(str(path))
