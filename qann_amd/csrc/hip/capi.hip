@@ -796,6 +796,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.out_dist = cd + q0 * kp;
         r.out_label = cl + q0 * kp;
         r.out_count = ccount + q0;
+        r.unsorted = refine ? 1 : 0;
         r.g_dist = gd ? gd + q0 * 2 * kp : nullptr;
         r.g_label = gl ? gl + q0 * 2 * kp : nullptr;
         StageTimer t_rep(QANN_STAGE_REPLAY, st);
@@ -854,6 +855,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.out_dist = cd + q0 * kp;
         r.out_label = cl + q0 * kp;
         r.out_count = ccount + q0;
+        r.unsorted = refine ? 1 : 0;
         r.g_dist = gd ? gd + q0 * 2 * kp : nullptr;
         r.g_label = gl ? gl + q0 * 2 * kp : nullptr;
         StageTimer t_rep(QANN_STAGE_REPLAY, st);

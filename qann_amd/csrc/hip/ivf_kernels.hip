@@ -1432,9 +1432,20 @@ __device__ void fs_replay(const ReplayArgs& a) {
         }
     }
 
-    // take_sorted (src/topk.hpp:53-76)
+    // take_sorted (src/topk.hpp:53-76); RefineIndex::rerank (src/refine_index.cpp:112-140)
+    // orders the candidates itself by (exact distance, slot), so for it the set suffices
     if (tk.size > kp) wave_trim(tk);
     const int n = tk.size;
+    if (a.unsorted) {
+        float* od = a.out_dist + static_cast<size_t>(qi) * kp;
+        int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
+        for (int i = lane; i < kp; i += kWave) {
+            od[i] = i < n ? tk.dist[i] : INFINITY;
+            ol[i] = i < n ? (ix.labels ? ix.labels[static_cast<uint32_t>(tk.label[i])] : tk.label[i]) : -1;
+        }
+        if (lane == 0) a.out_count[qi] = n;
+        return;
+    }
     // slots become ids; (key, id) order when every id fits 32 bits, else (float, id) order
     bool big = false;
     for (int base = 0; base < n; base += kWave) {

@@ -100,6 +100,7 @@ enum { kDistIp = 0, kDistL2Pre = 1, kDistL2Resid = 2, kDistSq = 3, kDistFloat = 
 struct ReplayArgs {
     DevIvf ix;
     int kp, M, block_filter;
+    int unsorted;              // 1: the candidates go to a re-rank, which needs only their set
     const uint16_t* probe_idx;
     const int* nprobe_q;
     const uint32_t* item_blk;
