@@ -145,7 +145,7 @@ constexpr int kLdsRow = kBf16K + 8; // BF16 per LDS row: 64 bytes of data, 16 of
 // workgroups of one XCD (every 8th) take consecutive tiles: the big operand's
 // tile is read into that XCD's L2 once and reused across the small operand.
 template <int TM, int TN, bool A_BIG, class Epi>
-__global__ void __launch_bounds__(256) k_gemm_bf16(const uint16_t* __restrict__ A, int M, size_t lda,
+__global__ void __launch_bounds__(256, 2) k_gemm_bf16(const uint16_t* __restrict__ A, int M, size_t lda,
                                                    const uint16_t* __restrict__ B, int N, size_t ldb, int kp,
                                                    int per_xcd, Epi epi) {
     constexpr int BM = 64 * TM, BN = 64 * TN;
@@ -266,9 +266,10 @@ struct EpiScores {
 };
 
 // Rows (A rows) against queries (B rows): a row whose score is at most the
-// query's limit, and which the query may return, joins its candidates.
+// query's limit, and which the query may return, joins its candidates. Each
+// lane holds TN query columns: their limits are read once per tile.
 struct EpiFilter {
-    static constexpr bool kTile = false;
+    static constexpr bool kTile = true;
     const float* lim;
     int ip, cap;
     const float* xsq;
@@ -277,16 +278,37 @@ struct EpiFilter {
     uint32_t* cnt;
     float* cd;
     uint32_t* ci;
-    __device__ __forceinline__ void operator()(int i, int j, float s) const {
-        const float a = ip ? -s : xsq[i] - 2.0f * s;
-        if (!(a <= lim[j])) return;
-        const uint32_t r = static_cast<uint32_t>(i);
-        if ((excl && bit_test(excl, r)) || (incl && !bit_test(incl, r))) return;
-        const uint32_t pos = atomicAdd(&cnt[j], 1u);
-        if (pos < static_cast<uint32_t>(cap)) {
-            cd[static_cast<size_t>(j) * cap + pos] = a;
-            ci[static_cast<size_t>(j) * cap + pos] = r;
+    template <int TM, int TN>
+    __device__ __forceinline__ void tile(const f32x16 (&acc)[TM][TN], int m0, int n0, int wm, int wn, int lane, int M,
+                                         int N, float*) const {
+        float lj[TN];
+        int cj[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            cj[j] = n0 + wn * 32 * TN + 32 * j + (lane & 31);
+            lj[j] = cj[j] < N ? lim[cj[j]] : -INFINITY;
         }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = m0 + wm * 32 * TM + 32 * i + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
+                if (row >= M) continue;
+                const float x2 = ip ? 0.0f : xsq[row];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const float s = acc[i][j][v];
+                    const float a = ip ? -s : x2 - 2.0f * s;
+                    if (!(a <= lj[j])) continue;
+                    const uint32_t r = static_cast<uint32_t>(row);
+                    if ((excl && bit_test(excl, r)) || (incl && !bit_test(incl, r))) continue;
+                    const uint32_t pos = atomicAdd(&cnt[cj[j]], 1u);
+                    if (pos < static_cast<uint32_t>(cap)) {
+                        cd[static_cast<size_t>(cj[j]) * cap + pos] = a;
+                        ci[static_cast<size_t>(cj[j]) * cap + pos] = r;
+                    }
+                }
+            }
     }
 };
 
