@@ -157,13 +157,14 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
 // plus n 2^-125 for flushed denormals). If more than `cap` lists qualify (or
 // fewer than M, with NaN scores), every list is scored exactly and selected by
 // the full 64-bit key.
-__global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
                                                        int dim, int ip, float cnorm_max, int M, int cap,
                                                        uint16_t* out_idx, float* out_val) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
-    __shared__ float s_red[4], s_red8[8];
+    __shared__ float s_red[NT / kWave], s_red8[2 * NT / kWave];
     float* qsh = reinterpret_cast<float*>(lds);
     uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
     const int i = blockIdx.x, tid = threadIdx.x;
@@ -178,30 +179,32 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
     if (lane_id() == 0) s_red[tid / kWave] = ss;
     if (tid == 0) s_count = 0;
     __syncthreads();
-    ss = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    ss = 0.0f;
+    for (int w = 0; w < NT / kWave; ++w) ss += s_red[w];
 
     const int target = std::min(cap, M + std::max(64, M / 4));
-    const float tau = nlist <= 16 * 256
-                          ? select_threshold<true>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
-                          : select_threshold<false>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+    const float tau =
+        nlist <= 16 * NT
+            ? select_threshold<true, false, NT>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
+            : select_threshold<false, false, NT>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
     const float u = 5.9604645e-8f, n = static_cast<float>(dim);
     const float qn = sqrtf(ss * (1.0f + 2.0f * (n + 2.0f) * u)) * 1.0001f;
     const float delta = 2.0f * (ip ? 2.0f * (n + 1.0f) * u * qn * cnorm_max
                                    : (2.0f * n + 5.0f) * u * (qn + cnorm_max) * (qn + cnorm_max)) +
                         n * 2.3509887e-38f;
     const float limit = tau + 3.0f * delta;
-    for (int l0 = 0; l0 < nlist; l0 += 16 * 256) {
+    for (int l0 = 0; l0 < nlist; l0 += 16 * NT) {
         float v[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int l = l0 + tid + 256 * j;
+            const int l = l0 + tid + NT * j;
             v[j] = l < nlist ? row[l] : INFINITY;
         }
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             if (v[j] <= limit) {
                 const uint32_t pos = atomicAdd(&s_count, 1u);
-                if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l0 + tid + 256 * j);
+                if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l0 + tid + NT * j);
             }
         }
     }
@@ -231,7 +234,7 @@ __global__ void __launch_bounds__(256) k_coarse_select(float* D, const float* q,
     __syncthreads();
     auto key64 = [&](int l) { return dist_key(row[l], static_cast<uint32_t>(l)); };
     const uint64_t kth =
-        block_radix_select<uint64_t, false>(nlist, static_cast<uint32_t>(M), key64, hist, s_wsum, &s_digit, &s_below);
+        block_radix_select<uint64_t, false, decltype(key64), NT>(nlist, static_cast<uint32_t>(M), key64, hist, s_wsum, &s_digit, &s_below);
     int Mp = 1;
     while (Mp < M) Mp <<= 1;
     if (tid == 0) s_count = 0;
@@ -1737,8 +1740,13 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     if (nq <= 0) return;
     const size_t cap = coarse_select_cap(M);
     const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
-    hipLaunchKernelGGL(k_coarse_select, dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip, cnorm_max, M,
-                       static_cast<int>(cap), out_idx, out_val);
+    // rows of up to 16,384 lists stay in registers with 1,024 threads a query
+    if (nlist > 16 * 256 && nlist <= 16 * 1024)
+        hipLaunchKernelGGL(k_coarse_select<1024>, dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim, ip, cnorm_max,
+                           M, static_cast<int>(cap), out_idx, out_val);
+    else
+        hipLaunchKernelGGL(k_coarse_select<256>, dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip, cnorm_max,
+                           M, static_cast<int>(cap), out_idx, out_val);
 }
 
 void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,

@@ -183,6 +183,31 @@ def test_gpu_built_index_matches_oracle(tmp_path, oracle, metric, dim, m, nlist,
     assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "filtered")
 
 
+def test_gpu_many_lists_matches_oracle(tmp_path, oracle):
+    # 5,000 lists: the coarse select keeps each query's row of 5,000 scores in
+    # registers with 1,024 threads (up to 16,384 lists), where smaller indexes
+    # use 256 threads
+    rng = np.random.default_rng(5000)
+    x = mixture(rng, 100_000, 32, centers=400, sigma=0.05)
+    q = mixture(rng, 300, 32, centers=400, sigma=0.05)
+    ivf = qann_amd.IVFIndex(32, 5000, 16)
+    ivf.enable_pq(16, fast_scan=True)
+    ivf.train(x, max_iters=4, seed=2)
+    idx = qann_amd.RefineIndex(ivf, 4)
+    idx.add(x)
+    path = tmp_path / "many_lists.qann"
+    idx.save(str(path))
+    orc = oracle.load(path)
+    for npb in (1, 16, 300):
+        ivf.nprobe = npb
+        orc.set_nprobe(npb)
+        assert_same(idx.batch_query(q, 10), orc.batch_query(q, 10), f"nprobe {npb}")
+    allowed = rng.choice(100_000, 20_000, replace=False)
+    ivf.nprobe = 16
+    orc.set_nprobe(16)
+    assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "filtered")
+
+
 def test_gpu_flat_matches_oracle(tmp_path, oracle):
     rng = np.random.default_rng(11)
     for metric in (qann_amd.Metric.L2, qann_amd.Metric.InnerProduct, qann_amd.Metric.Cosine):
