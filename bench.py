@@ -272,7 +272,8 @@ def main():
                     help="cpu100k: the CPU-runnable 100k x 128 configuration instead of the headline one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py runs on one GPU only: the search path has no multi-GPU mode (the IVF lists would shard "
+              "across GPUs by list, with a per-query merge of the top k)",
               file=sys.stderr)
         sys.exit(2)
 

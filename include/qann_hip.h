@@ -170,11 +170,13 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
  * search by the BF16 GEMM (or, for fewer than 8 queries, by direct exact
  * scoring), QANN_STAGE_FLAT_FP32 one by the FP32 GEMM, queries the BF16 path
  * hands over included.
- * While enabled, every search synchronizes its stream after each stage and
- * adds the stage's HIP-event time to a process-wide total; read returns the
- * totals in milliseconds and launch counts for QANN_STAGE_COUNT stages, in the
- * order of qann_hip_stage_name. Timing serializes the stages: use it for a
- * measurement pass, not the throughput pass.
+ * While enabled, every search records HIP events around each stage on the
+ * stream the stage runs on (no synchronization: stages of concurrent streams
+ * stay concurrent, and a stage's time includes what it waited for while others
+ * ran); read waits for the recorded stages and returns the process-wide totals
+ * in milliseconds and launch counts for QANN_STAGE_COUNT stages, in the order
+ * of qann_hip_stage_name. Searches run while enabled are not captured as
+ * graphs.
  */
 enum {
     QANN_STAGE_ROTATE = 0,

@@ -27,7 +27,8 @@ ARCH = os.environ.get("QANN_AMD_ARCH", "gfx950")
 
 # -ffp-contract=off: every fused multiply-add in the kernels is an explicit fmaf,
 # so results keep the bits of the reference's CPU arithmetic.
-# -packed-fp32-ops: no v_pk_*_f32 (build flags).
+# -packed-fp32-ops: the compiler emits no packed FP32 instructions (v_pk_add_f32,
+# v_pk_mul_f32, v_pk_fma_f32); every FP32 operation is a scalar VALU one.
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -71,11 +71,41 @@ struct Scratch {
     };
     std::map<std::tuple<uint64_t, uint32_t, uint32_t, uint32_t>, Graph> graphs;
     std::map<std::tuple<uint64_t, uint32_t, uint32_t, uint32_t>, int> graph_seen;
+    // a pipelined search's parts: one Scratch (own stream) per part, and the
+    // events that order them (inputs ready; per part: scan done, part done)
+    std::vector<Scratch*> parts;
+    hipEvent_t ev_in = nullptr;
+    std::vector<hipEvent_t> ev_scan, ev_done;
+    bool pipe_warm = false; // a first pipelined search runs its parts one after another
 
     void drop_graphs() {
         for (auto& kv : graphs)
             if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         graphs.clear();
+    }
+
+    // Frees the buffers, graphs, parts and events (and the stream when owned),
+    // after the stream's work has finished.
+    void release() {
+        (void)hipStreamSynchronize(stream);
+        for (Scratch* s : parts) {
+            s->release();
+            delete s;
+        }
+        parts.clear();
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        for (hipEvent_t e : ev_scan) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_done) (void)hipEventDestroy(e);
+        ev_in = nullptr;
+        ev_scan.clear();
+        ev_done.clear();
+        drop_graphs();
+        if (base) (void)hipFree(base);
+        if (io) (void)hipFree(io);
+        base = io = nullptr;
+        cap = io_cap = 0;
+        if (own_stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
     }
 
     char* reserve_io(size_t bytes) {
@@ -121,33 +151,53 @@ double g_prof_ms[QANN_STAGE_COUNT] = {};
 uint64_t g_sums_budget = uint64_t{8} << 30;
 uint64_t g_prof_n[QANN_STAGE_COUNT] = {};
 
-// Times one stage when profiling is on: events around the launches, then a sync.
+// Stage timings recorded but not yet read: events around a stage's launches on
+// its stream, resolved when the timings are read.
+struct PendingStage {
+    int stage;
+    hipEvent_t a, b;
+};
+std::vector<PendingStage> g_prof_pending; // guarded by g_prof_mu
+
+// Adds the pending timings to the totals (waits for their end events); the
+// caller holds g_prof_mu.
+void resolve_stages_locked() {
+    for (const PendingStage& p : g_prof_pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            g_prof_ms[p.stage] += ms;
+            g_prof_n[p.stage] += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    g_prof_pending.clear();
+}
+
+// Times one stage when profiling is on: events around its launches on its
+// stream, with no synchronization, so that stages of concurrent streams keep
+// running concurrently while they are timed.
 struct StageTimer {
     int stage;
     hipStream_t st;
-    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t a = nullptr;
     StageTimer(int s, hipStream_t stream) : stage(s), st(stream) {
         if (!g_prof_on.load()) return;
         check(hipEventCreate(&a), "hipEventCreate");
-        check(hipEventCreate(&b), "hipEventCreate");
         check(hipEventRecord(a, st), "hipEventRecord");
     }
     void done() {
         if (!a) return;
+        hipEvent_t b = nullptr;
+        check(hipEventCreate(&b), "hipEventCreate");
         check(hipEventRecord(b, st), "hipEventRecord");
-        check(hipEventSynchronize(b), "hipEventSynchronize");
-        float ms = 0.0f;
-        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        a = b = nullptr;
         std::lock_guard<std::mutex> g(g_prof_mu);
-        g_prof_ms[stage] += ms;
-        g_prof_n[stage] += 1;
+        g_prof_pending.push_back(PendingStage{stage, a, b});
+        a = nullptr;
+        if (g_prof_pending.size() >= 8192) resolve_stages_locked();
     }
     ~StageTimer() {
         if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
     }
 };
 
@@ -212,16 +262,11 @@ struct qann_hip_index {
     }
     ~qann_hip_index() {
         for (Scratch* s : free_scratch) {
-            if (s->own_stream) (void)hipStreamSynchronize(s->stream);
-            s->drop_graphs();
-            if (s->base) (void)hipFree(s->base);
-            if (s->io) (void)hipFree(s->io);
-            if (s->own_stream) (void)hipStreamDestroy(s->stream);
+            s->release();
             delete s;
         }
         for (auto& kv : stream_scratch) {
-            (void)hipStreamSynchronize(kv.first);
-            if (kv.second->base) (void)hipFree(kv.second->base);
+            kv.second->release();
             delete kv.second;
         }
         for (void* p : allocs) (void)hipFree(p);
@@ -460,8 +505,12 @@ __global__ void k_any_set(const int* status, int n, int* flag) {
 // M is a first guess (twice the probes the rule asks for at least) and a query
 // that would need more sets *d_more; the caller then searches again with every
 // list it may need (at most 4096). Without it, M is that limit at once.
+// One search on sc's stream. scan_wait / scan_done (fast-scan lists only, may
+// be null): the scan waits for scan_wait and records scan_done after its last
+// launch, so that a pipelined search's parts scan one after another.
 void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip_search_params& p,
-                const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc, int* d_more = nullptr) {
+                const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc, int* d_more = nullptr,
+                hipEvent_t scan_wait = nullptr, hipEvent_t scan_done = nullptr) {
     hipStream_t st = sc.stream;
     const uint32_t k = p.k;
     if (nq == 0 || k == 0) return;
@@ -827,6 +876,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.item_min = item_min + q0 * M;
         if (qd::fs_sums_lds(a) == 0) throw Error(QANN_EINVAL, "the fast-scan tables of this index do not fit in LDS");
         a.queue = sums_queue;
+        if (q0 == 0 && scan_wait) check(hipStreamWaitEvent(st, scan_wait, 0), "hipStreamWaitEvent");
         if (order && nc >= kOrderMinQueries) {
             qd::launch_query_order(a.probe_idx, static_cast<int>(M), static_cast<int>(nc), static_cast<int>(nlist),
                                    ord_hist, order, st);
@@ -837,6 +887,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         qd::launch_fs_sums(a, static_cast<int>(nc), st);
         check_launch("k_fs_sums");
         t_scan.done();
+        if (q0 + nc >= nq && scan_done) check(hipEventRecord(scan_done, st), "hipEventRecord");
         qd::ReplayArgs r{};
         r.ix = ixs;
         r.kp = static_cast<int>(kp);
@@ -875,6 +926,81 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
                            static_cast<int>(kp), static_cast<int>(k), static_cast<int>(nq), d_ids, d_dists);
         check_launch("k_copy_results");
     }
+}
+
+// Unfiltered fast-scan searches of at least kPipeMinQueries queries run as a
+// pipeline of parts (at most kPipeParts, each at least kPipeMinPart queries),
+// each part a whole search on a stream of its own. The parts' scans run one
+// after another; a part's coarse quantizer, probe selection and tables overlap
+// the previous part's scan, and its replay and re-rank overlap the next one's.
+// Above kGraphMaxQueries, so graph capture never meets a pipelined search.
+constexpr uint64_t kPipeMinQueries = 8192;
+constexpr uint64_t kPipeMinPart = 2048;
+constexpr int kPipeParts = 4;
+
+// Parts for a search (1: not pipelined). QANN_AMD_PIPE=n forces at most n
+// parts for searches the pipeline applies to (1 turns it off).
+int pipe_parts(const qann_hip_index* h, uint64_t nq, const qann_hip_search_params& p) {
+    if (p.allowed_words || h->kind == 1) return 1;
+    const qann_hip_index* ivfh = h->kind == 3 ? h->base : h;
+    if (!ivfh || ivfh->ivf.storage != QANN_IVF_FASTSCAN) return 1;
+    int parts = kPipeParts;
+    if (const char* e = std::getenv("QANN_AMD_PIPE")) {
+        if (*e) parts = std::clamp(std::atoi(e), 1, 64);
+    } else {
+        return 1;
+    }
+    if (nq < kPipeMinQueries) return 1;
+    return static_cast<int>(std::clamp<uint64_t>(nq / kPipeMinPart, 1, static_cast<uint64_t>(parts)));
+}
+
+// A search as a pipeline of parts on the parts' streams; the results are in
+// d_ids / d_dists once sc's stream reaches its end.
+void run_pipelined(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip_search_params& p, int parts,
+                   int64_t* d_ids, float* d_dists, Scratch& sc) {
+    while (static_cast<int>(sc.parts.size()) < parts) {
+        auto* s = new Scratch();
+        if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete s;
+            throw Error(QANN_EDEVICE, "hipStreamCreate failed");
+        }
+        s->own_stream = true;
+        sc.parts.push_back(s);
+        hipEvent_t a = nullptr, b = nullptr;
+        check(hipEventCreateWithFlags(&a, hipEventDisableTiming), "hipEventCreate");
+        sc.ev_scan.push_back(a);
+        check(hipEventCreateWithFlags(&b, hipEventDisableTiming), "hipEventCreate");
+        sc.ev_done.push_back(b);
+    }
+    if (!sc.ev_in) check(hipEventCreateWithFlags(&sc.ev_in, hipEventDisableTiming), "hipEventCreate");
+    // the first pipelined search on this Scratch runs its parts in turn (the
+    // launch autotuners time their candidates with the chip to themselves)
+    const bool serial = !sc.pipe_warm;
+    check(hipEventRecord(sc.ev_in, sc.stream), "hipEventRecord");
+    const uint64_t per = (nq + parts - 1) / parts;
+    const uint64_t d = h->dim, k = p.k;
+    for (int i = 0; i < parts; ++i) {
+        const uint64_t q0 = static_cast<uint64_t>(i) * per;
+        if (q0 >= nq) break;
+        const uint64_t nc = std::min(per, nq - q0);
+        Scratch& ps = *sc.parts[i];
+        check(hipStreamWaitEvent(ps.stream, i > 0 && serial ? sc.ev_done[i - 1] : sc.ev_in, 0), "hipStreamWaitEvent");
+        run_search(h, d_q + q0 * d, nc, p, nullptr, d_ids + q0 * k, d_dists + q0 * k, ps, nullptr,
+                   i > 0 ? sc.ev_scan[i - 1] : nullptr, sc.ev_scan[i]);
+        check(hipEventRecord(sc.ev_done[i], ps.stream), "hipEventRecord");
+        check(hipStreamWaitEvent(sc.stream, sc.ev_done[i], 0), "hipStreamWaitEvent");
+    }
+    sc.pipe_warm = true;
+}
+
+// run_search, or run_pipelined for the searches the pipeline applies to.
+void search_any(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip_search_params& p,
+                const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc, int* d_more = nullptr) {
+    const int parts = pipe_parts(h, nq, p);
+    if (parts > 1)
+        run_pipelined(h, d_q, nq, p, parts, d_ids, d_dists, sc);
+    else
+        run_search(h, d_q, nq, p, d_filter, d_ids, d_dists, sc, d_more);
 }
 
 Scratch* checkout(qann_hip_index* h) {
@@ -1259,7 +1385,7 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, io->stream, reinterpret_cast<uint32_t*>(dmore), 1);
                 check_launch("k_zero_u32");
             }
-            run_search(h, dq, nq, *params, df, dids, dd, *io, dmore);
+            search_any(h, dq, nq, *params, df, dids, dd, *io, dmore);
             int more = 0;
             if (dmore) check(hipMemcpyAsync(&more, dmore, 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
@@ -1294,7 +1420,7 @@ int qann_hip_search_async(qann_hip_index* h, const float* d_queries, uint64_t nq
                 sc = it->second;
             }
         }
-        run_search(h, d_queries, nq, *params, params->allowed_words, d_out_ids, d_out_dists, *sc);
+        search_any(h, d_queries, nq, *params, params->allowed_words, d_out_ids, d_out_dists, *sc);
     });
 }
 
@@ -1359,6 +1485,7 @@ int qann_hip_profile_enable(int on) {
 
 int qann_hip_profile_read(double* ms, uint64_t* launches, int reset) {
     std::lock_guard<std::mutex> g(g_prof_mu);
+    resolve_stages_locked();
     for (int i = 0; i < QANN_STAGE_COUNT; ++i) {
         if (ms) ms[i] = g_prof_ms[i];
         if (launches) launches[i] = g_prof_n[i];
@@ -1405,7 +1532,9 @@ int qann_hip_search_scratch_bytes(qann_hip_index* h, uint64_t nq, const qann_hip
             const uint64_t vb = ix.storage == QANN_IVF_FASTSCAN ? 2 : 4;
             const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];
             const uint64_t stride = std::max<uint64_t>(wblocks, 1) * qd::kFsBlock;
-            const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * vb)));
+            // a pipelined search's parts hold a sums buffer each
+            const uint64_t parts = static_cast<uint64_t>(pipe_parts(h, nq, *p));
+            const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, parts * (g_sums_budget / (stride * vb))));
             b = nq * nlist * 4 + nq * M * 34 + nq * static_cast<uint64_t>(ix.m) * std::max(ix.K, 16) * 4 +
                 chunk * stride * vb * 33 / 32 + nq * kp * 24 + nq * ix.dim * 4 + (uint64_t{1} << 20);
         }

@@ -2,18 +2,25 @@
 //
 // One batch of prepared queries runs as:
 //   k_rotate          q * R for rotated indexes           (src/rotation.cpp:165-185)
-//   k_coarse_dist     exact A for every (query, list)     (IVFIndex::coarse / coarse_all, src/ivf_index.cpp:305-383)
-//   k_select_sorted   the M nearest lists, sorted by (A, list)
+//   k_coarse_gemm     approximate coarse scores for every (query, list), MFMA
+//   k_coarse_select   the M nearest lists by exact A, sorted by (A, list)
+//                     (IVFIndex::coarse / coarse_batch, src/ivf_index.cpp:305-419)
 //   k_probe_base      per-probe A (residual centres) and the IP/cosine base term
 //   k_query_tables    C = -2<q_s, y_sj> (PQCodebook::compute_inner_table, src/pq.cpp:186-204),
 //                     or the inner-product LUT quantized once per query (scan, :448-456)
 //   k_probe_plan      probes each query scans (filters: min_probes / wanted) and sum offsets
-//   k_fs_sums         one wave per (query, probe): the u8 LUT in LDS (build_quantize_lut,
-//                     src/fast_scan.cpp:243-254) and the u16 sums of every 4-bit code of
-//                     the list (fast_scan_block, :256-262) with v_perm lookups
-//   k_fs_replay       one wave per query: the reference's scan replayed push by push over
+//   k_order_*         the queries ordered by nearest list, for L2 sharing in k_fs_sums
+//   k_fs_sums         per (query, window of 16 probes): the u8 LUTs in LDS
+//                     (build_quantize_lut, src/fast_scan.cpp:243-254) and the u16 sums of
+//                     every 4-bit code of those lists (fast_scan_block, :256-262) with
+//                     v_perm lookups, plus block minima (k_fs_sums_queue: the same work
+//                     pulled from per-XCD queues)
+//   k_fs_replay_win   one wave per query: the reference's scan replayed push by push over
 //                     those sums -- limit16, survivors in bit order, TopK trims
-//                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543)
+//                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543);
+//                     k_fs_replay the same without LDS windows (large TopK buffers)
+//   k_ivf_dists,      PQ (ADC) / SQ / float lists: float distances, then the same replay
+//   k_fl_replay_win     over them (:552-611)
 //   k_rerank          exact FP32 re-rank of the candidates (RefineIndex::rerank, src/refine_index.cpp:112-140)
 //   k_block_select    exact top-k over a slot set (FlatIndex::query_impl, RefineIndex::direct_scan)
 #include <algorithm>
@@ -23,6 +30,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "ivf_kernels.hpp"
@@ -1167,6 +1175,83 @@ __global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t
     }
 }
 
+// The end of a replay: take_sorted (src/topk.hpp:53-76) of the TopK (which
+// holds slots) into the query's output row, or, for a re-rank
+// (RefineIndex::rerank, src/refine_index.cpp:112-140, orders the candidates
+// itself by (exact distance, slot)), just the set.
+__device__ void replay_finish(WaveTopK& tk, const ReplayArgs& a, int qi) {
+    const DevIvf& ix = a.ix;
+    const int kp = a.kp;
+    const int lane = lane_id();
+
+    const int n = tk.size;
+    if (a.unsorted) {
+        float* od = a.out_dist + static_cast<size_t>(qi) * kp;
+        int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
+        for (int i = lane; i < kp; i += kWave) {
+            od[i] = i < n ? tk.dist[i] : INFINITY;
+            ol[i] = i < n ? (ix.labels ? ix.labels[static_cast<uint32_t>(tk.label[i])] : tk.label[i]) : -1;
+        }
+        if (lane == 0) a.out_count[qi] = n;
+        return;
+    }
+    // slots become ids; (key, id) order when every id fits 32 bits, else (float, id) order
+    bool big = false;
+    for (int base = 0; base < n; base += kWave) {
+        const int i = base + lane;
+        if (i < n && ix.labels) tk.label[i] = ix.labels[static_cast<uint32_t>(tk.label[i])];
+        big |= __popcll(ballot(i < n && static_cast<uint64_t>(tk.label[i]) > 0xFFFFFFFFull)) != 0;
+    }
+    int np = 1;
+    while (np < n) np <<= 1;
+    for (int i = n + lane; i < np; i += kWave) {
+        tk.dist[i] = __uint_as_float(0x7FC00000u);
+        tk.label[i] = -1;
+    }
+    wave_sync();
+    // padding entries (label -1) sort last wherever the network moves them
+    auto sort_key = [&](int i, uint32_t& k32, uint64_t& id) {
+        if (tk.label[i] < 0) {
+            k32 = 0xFFFFFFFFu;
+            id = ~0ull;
+            return;
+        }
+        const float d = tk.dist[i];
+        k32 = big ? ckey(d) : fkey(d);
+        id = static_cast<uint64_t>(tk.label[i]);
+    };
+    for (int size = 2; size <= np; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = lane; t < (np >> 1); t += kWave) {
+                const int x = 2 * t - (t & (stride - 1));
+                const int y = x + stride;
+                const bool up = (x & size) == 0;
+                uint32_t kx, ky;
+                uint64_t ix_, iy;
+                sort_key(x, kx, ix_);
+                sort_key(y, ky, iy);
+                const bool gt = kx != ky ? kx > ky : ix_ > iy;
+                if (gt == up) {
+                    const float dx = tk.dist[x], dy = tk.dist[y];
+                    const int64_t lx = tk.label[x], ly = tk.label[y];
+                    tk.dist[x] = dy;
+                    tk.dist[y] = dx;
+                    tk.label[x] = ly;
+                    tk.label[y] = lx;
+                }
+            }
+            wave_sync();
+        }
+    }
+    float* od = a.out_dist + static_cast<size_t>(qi) * kp;
+    int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
+    for (int i = lane; i < kp; i += kWave) {
+        od[i] = i < n ? tk.dist[i] : INFINITY;
+        ol[i] = i < n ? tk.label[i] : -1;
+    }
+    if (lane == 0) a.out_count[qi] = n;
+}
+
 // Phase 2: one wave per query replays the reference's scan in probe order.
 // Fast-scan (FL false): over the u16 sums -- limit16 from the threshold at
 // each block (scan_fast, ivf_index.cpp:482-485), survivors in bit order
@@ -1435,79 +1520,303 @@ __device__ void fs_replay(const ReplayArgs& a) {
         }
     }
 
-    // take_sorted (src/topk.hpp:53-76); RefineIndex::rerank (src/refine_index.cpp:112-140)
-    // orders the candidates itself by (exact distance, slot), so for it the set suffices
     if (tk.size > kp) wave_trim(tk);
-    const int n = tk.size;
-    if (a.unsorted) {
-        float* od = a.out_dist + static_cast<size_t>(qi) * kp;
-        int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
-        for (int i = lane; i < kp; i += kWave) {
-            od[i] = i < n ? tk.dist[i] : INFINITY;
-            ol[i] = i < n ? (ix.labels ? ix.labels[static_cast<uint32_t>(tk.label[i])] : tk.label[i]) : -1;
-        }
-        if (lane == 0) a.out_count[qi] = n;
-        return;
-    }
-    // slots become ids; (key, id) order when every id fits 32 bits, else (float, id) order
-    bool big = false;
-    for (int base = 0; base < n; base += kWave) {
-        const int i = base + lane;
-        if (i < n && ix.labels) tk.label[i] = ix.labels[static_cast<uint32_t>(tk.label[i])];
-        big |= __popcll(ballot(i < n && static_cast<uint64_t>(tk.label[i]) > 0xFFFFFFFFull)) != 0;
-    }
-    int np = 1;
-    while (np < n) np <<= 1;
-    for (int i = n + lane; i < np; i += kWave) {
-        tk.dist[i] = __uint_as_float(0x7FC00000u);
-        tk.label[i] = -1;
-    }
-    wave_sync();
-    // padding entries (label -1) sort last wherever the network moves them
-    auto sort_key = [&](int i, uint32_t& k32, uint64_t& id) {
-        if (tk.label[i] < 0) {
-            k32 = 0xFFFFFFFFu;
-            id = ~0ull;
+    replay_finish(tk, a, qi);
+}
+
+// TopK::push of one block's survivors in bit order (src/topk.hpp:44-50): the
+// lanes of rem whose entry may be pushed (ok) and is below the threshold go
+// into the buffer; the one that fills it triggers a trim, and the lanes after
+// it are tested again against the new threshold.
+__device__ __forceinline__ void push_survivors(WaveTopK& tk, uint64_t rem, bool ok, float dist, int64_t label) {
+    const int lane = lane_id();
+    while (rem) {
+        const bool cond = ((rem >> lane) & 1ull) && ok && dist < tk.thr;
+        const uint64_t cm = ballot(cond);
+        if (cm == 0) break;
+        const int space = 2 * tk.k - tk.size;
+        const int cnt = __popcll(cm);
+        if (cnt < space) {
+            if (cond) {
+                const int pos = tk.size + static_cast<int>(lanes_below(cm));
+                tk.dist[pos] = dist;
+                tk.label[pos] = label;
+            }
+            tk.size += cnt;
+            wave_sync();
             return;
         }
-        const float d = tk.dist[i];
-        k32 = big ? ckey(d) : fkey(d);
-        id = static_cast<uint64_t>(tk.label[i]);
-    };
-    for (int size = 2; size <= np; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = lane; t < (np >> 1); t += kWave) {
-                const int x = 2 * t - (t & (stride - 1));
-                const int y = x + stride;
-                const bool up = (x & size) == 0;
-                uint32_t kx, ky;
-                uint64_t ix_, iy;
-                sort_key(x, kx, ix_);
-                sort_key(y, ky, iy);
-                const bool gt = kx != ky ? kx > ky : ix_ > iy;
-                if (gt == up) {
-                    const float dx = tk.dist[x], dy = tk.dist[y];
-                    const int64_t lx = tk.label[x], ly = tk.label[y];
-                    tk.dist[x] = dy;
-                    tk.dist[y] = dx;
-                    tk.label[x] = ly;
-                    tk.label[y] = lx;
+        const uint32_t rank = lanes_below(cm);
+        const uint64_t at = ballot(cond && static_cast<int>(rank) == space - 1);
+        const int bpos = __builtin_ctzll(at);
+        if (cond && lane <= bpos) {
+            const int pos = tk.size + static_cast<int>(rank);
+            tk.dist[pos] = dist;
+            tk.label[pos] = label;
+        }
+        tk.size = 2 * tk.k;
+        wave_sync();
+        wave_trim(tk);
+        rem &= (bpos >= 63) ? 0ull : (~0ull << (bpos + 1));
+    }
+}
+
+// Candidate blocks a windowed replay loads at once: kPfRows rows of two.
+constexpr int kPfRows = 8;
+
+// Block minima a windowed replay keeps in LDS at once.
+constexpr uint32_t kReplayWindow = 2048;
+
+// The replay of fs_replay with the TopK always in LDS and its per-probe costs
+// cut (the same pushes in the same order, so the same TopK):
+//  * the query's block minima are copied into LDS a window of kReplayWindow
+//    blocks at a time (a query's probes own consecutive blocks), so that a
+//    probe's candidate blocks are found without a trip to HBM;
+//  * of each 64 probes, the ones whose list minimum passes its limit are
+//    found at once, one probe per lane, and only those are visited (a probe
+//    that fails now fails at its turn: the limit only falls, and nothing is
+//    pushed between);
+//  * a probe's candidate blocks go 16 at a time: their sums (distances) are
+//    loaded together and staged in LDS; the TopK holds list entries, whose
+//    slots are read when the query ends, so without a filter or deleted rows
+//    no slot is read on the way; with them, the slots and filter / deleted
+//    bits of every entry passing the limit at the group's start (a superset
+//    of what a later, lower limit passes) are read for the group at once.
+template <bool FL>
+__device__ void fs_replay_win(const ReplayArgs& a) {
+    using BT = typename std::conditional<FL, uint32_t, uint16_t>::type;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const DevIvf& ix = a.ix;
+    const int kp = a.kp;
+    const int qi = blockIdx.x;
+    int64_t* tk_label = reinterpret_cast<int64_t*>(lds);
+    float* tk_dist = reinterpret_cast<float*>(tk_label + 2 * kp);
+    BT* sb = reinterpret_cast<BT*>(tk_dist + 2 * kp);
+    const int lane = lane_id();
+    const int v = lane & 31;
+    const bool filtered = a.filter != nullptr;
+    const bool need_slot = !FL && (filtered || ix.deleted != nullptr);
+    __shared__ uint32_t trim_hist[256];
+    __shared__ int st_blk[kWave];
+    __shared__ uint32_t st_val[2 * kPfRows * kFsBlock];
+    WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY, trim_hist};
+    const int P = a.nprobe_q[qi];
+    const size_t qoff = static_cast<size_t>(qi) * a.stride_slots;
+    const size_t qboff = static_cast<size_t>(qi) * (a.stride_slots / kFsBlock);
+    const BT* gbmin;
+    if constexpr (FL) gbmin = a.fbmin + qboff;
+    else gbmin = a.bmin + qboff;
+    uint32_t win0 = 0, win1 = 0; // staged blocks [win0, win1) of the query
+    for (int c0 = 0; c0 < P; c0 += kWave) {
+        int m_list = 0;
+        uint32_t m_n = 0, m_blk = 0, m_min = 0xFFFFFFFFu, m_first = 0;
+        float m_scale = 1.0f, m_off = 0.0f;
+        if (c0 + lane < P) {
+            const size_t item = static_cast<size_t>(qi) * a.M + c0 + lane;
+            m_list = a.probe_idx[item];
+            m_first = ix.list_off[m_list];
+            m_n = ix.list_off[m_list + 1] - m_first;
+            m_blk = a.item_blk[item];
+            m_min = a.item_min[item];
+            if (!FL) {
+                m_scale = a.item_scale[item];
+                m_off = a.item_offset[item];
+            }
+        }
+        const int cn = min(kWave, P - c0);
+        // the end of this chunk's blocks (a window never reaches past it)
+        const uint32_t cend = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_blk), cn - 1)) +
+                              (static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_n), cn - 1)) +
+                               kFsBlock - 1) / kFsBlock;
+        // the probes whose list minimum passes their limit now, one per lane
+        auto live_now = [&]() -> uint64_t {
+            bool live;
+            if (FL) {
+                live = m_min < fkey(tk.thr);
+            } else {
+                const float limit = (tk.thr - m_off) * m_scale;
+                const uint32_t l16 = limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
+                live = m_min < l16;
+            }
+            return ballot(c0 + lane < P && m_n != 0 && live);
+        };
+        uint64_t todo = live_now();
+        while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t n = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_n), j));
+            const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
+            const float scale = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_scale), j));
+            const float offset = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_off), j));
+            const uint32_t first = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_first), j));
+            const uint32_t blk0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_blk), j));
+            auto limit_now = [&]() -> uint32_t {
+                if (FL) return fkey(tk.thr);
+                const float limit = (tk.thr - offset) * scale;
+                return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
+            };
+            // the window: restaged from this probe on when it does not hold the probe's blocks
+            if ((blk0 < win0 || blk0 + nb > win1) && nb <= kReplayWindow) {
+                win0 = blk0;
+                win1 = min(blk0 + kReplayWindow, cend);
+                wave_sync();
+                for (uint32_t i = static_cast<uint32_t>(lane); i < win1 - win0; i += kWave) sb[i] = gbmin[win0 + i];
+                wave_sync();
+            }
+            const bool staged = blk0 >= win0 && blk0 + nb <= win1;
+            const float thr_in = tk.thr;
+            const size_t voff = qoff + static_cast<size_t>(blk0) * kFsBlock;
+            for (uint32_t bc = 0; bc < nb; bc += kWave) {
+                const uint32_t bl = bc + static_cast<uint32_t>(lane);
+                uint32_t bm = 0xFFFFFFFFu;
+                if (bl < nb) bm = staged ? static_cast<uint32_t>(sb[blk0 - win0 + bl]) : static_cast<uint32_t>(gbmin[blk0 + bl]);
+                uint64_t cand = ballot(bm < limit_now());
+                while (cand) {
+                    // the next (up to) 2 * kPfRows candidate blocks, in order; row r
+                    // holds candidates 2r (lanes 0-31) and 2r + 1 (lanes 32-63)
+                    const int ncand = __popcll(cand);
+                    const int nc = min(ncand, 2 * kPfRows);
+                    if ((cand >> lane) & 1ull) st_blk[lanes_below(cand)] = lane;
+                    wave_sync();
+                    const int rows = (nc + 1) / 2;
+                    const uint32_t lim0 = limit_now();
+                    uint32_t sv[kPfRows];
+#pragma unroll
+                    for (int r = 0; r < kPfRows; ++r) {
+                        sv[r] = FL ? __float_as_uint(INFINITY) : 0xFFFFu;
+                        if (r < rows) {
+                            const int rr = 2 * r + (lane >> 5);
+                            const int mb = rr < nc ? st_blk[rr] : -1;
+                            const uint32_t e = (bc + static_cast<uint32_t>(mb)) * kFsBlock + v;
+                            if (mb >= 0 && e < n)
+                                sv[r] = FL ? __float_as_uint(a.fdists[voff + e]) : static_cast<uint32_t>(a.sums[voff + e]);
+                        }
+                    }
+                    if (need_slot) {
+                        // bit 16 set: the entry may not be returned (filtered out or deleted)
+                        uint32_t sl[kPfRows];
+#pragma unroll
+                        for (int r = 0; r < kPfRows; ++r) {
+                            sl[r] = 0u;
+                            if (r < rows && sv[r] < lim0) {
+                                const int rr = 2 * r + (lane >> 5);
+                                const uint32_t e = (bc + static_cast<uint32_t>(st_blk[rr])) * kFsBlock + v;
+                                sl[r] = ix.list_ids[first + e];
+                            }
+                        }
+#pragma unroll
+                        for (int r = 0; r < kPfRows; ++r) {
+                            if (r < rows && sv[r] < lim0) {
+                                const bool okb = filtered ? bit_test(a.filter, sl[r]) : !bit_test(ix.deleted, sl[r]);
+                                if (!okb) sv[r] |= 0x10000u;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < kPfRows; ++r)
+                        if (r < rows) st_val[r * kWave + lane] = sv[r];
+                    wave_sync();
+                    // row r's entry in this lane: its value, whether it is in the list and
+                    // may be returned, and its distance
+                    auto row_entry = [&](int r, uint32_t& e, bool& in, bool& okb, uint32_t& sum, float& fd) {
+                        const int rr = 2 * r + (lane >> 5);
+                        const int mb = rr < nc ? st_blk[rr] : -1;
+                        e = (bc + static_cast<uint32_t>(mb)) * kFsBlock + v;
+                        in = mb >= 0 && e < n;
+                        const uint32_t x = st_val[r * kWave + lane];
+                        okb = FL || !(x & 0x10000u);
+                        sum = 0xFFFFu;
+                        fd = INFINITY;
+                        if (in) {
+                            if (FL) fd = __uint_as_float(x);
+                            else sum = x & 0xFFFFu;
+                        }
+                    };
+                    // Rows whose pushes cannot fill the buffer leave the threshold
+                    // (so every block's limit) as it is: their survivors are counted
+                    // and written at once, in push order (row by row; in a row, the
+                    // first block's entries, lanes 0-31, then the second's). The row
+                    // that would fill it replays push by push, trims included.
+                    int r = 0;
+                    while (r < rows) {
+                        const uint32_t lim = limit_now();
+                        const float thr = tk.thr;
+                        for (; r < rows; ++r) {
+                            uint32_t e, sum;
+                            bool in, okb;
+                            float fd;
+                            row_entry(r, e, in, okb, sum, fd);
+                            const float dist = FL ? fd : __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                            const bool surv = in && okb && (FL ? fd < thr : sum < lim) && dist < thr;
+                            const uint64_t m = ballot(surv);
+                            const int c = __popcll(m);
+                            if (tk.size + c >= 2 * kp) break;
+                            if (surv) {
+                                const int pos = tk.size + static_cast<int>(lanes_below(m));
+                                tk.dist[pos] = dist;
+                                tk.label[pos] = static_cast<int64_t>(first + e);
+                            }
+                            tk.size += c;
+                        }
+                        wave_sync();
+                        if (r >= rows) break;
+                        // row r fills the buffer: its two blocks push by push
+                        {
+                            uint32_t e, sum;
+                            bool in, okb;
+                            float fd;
+                            row_entry(r, e, in, okb, sum, fd);
+                            auto passes_now = [&]() -> bool { return FL ? fd < tk.thr : sum < limit_now(); };
+                            for (int h = 0; h < 2; ++h) {
+                                if (h == 1 && 2 * r + 1 >= nc) break;
+                                const bool valid = (lane >> 5) == h && in;
+                                bool allowed = valid;
+                                if (!FL && a.block_filter) {
+                                    allowed = valid && okb;
+                                    if (ballot(allowed) == 0) continue;
+                                }
+                                const bool surv = allowed && passes_now();
+                                const uint64_t rem = ballot(surv);
+                                if (rem == 0) continue;
+                                float dist;
+                                bool ok;
+                                if (FL) {
+                                    dist = fd; // +inf already marks what may not be returned
+                                    ok = surv;
+                                } else {
+                                    dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                                    ok = surv && okb;
+                                }
+                                push_survivors(tk, rem, ok, dist, static_cast<int64_t>(first + e));
+                            }
+                        }
+                        ++r;
+                    }
+                    // drop the candidates just replayed
+                    if (nc == ncand) {
+                        cand = 0;
+                    } else {
+                        const int last = __builtin_amdgcn_readfirstlane(st_blk[nc - 1]);
+                        cand &= ~((2ull << last) - 1ull);
+                    }
+                    wave_sync(); // the next group overwrites st_blk / st_val
                 }
             }
-            wave_sync();
+            // a push lowered the threshold: drop the probes that now fail
+            if (tk.thr != thr_in) todo &= live_now();
         }
     }
-    float* od = a.out_dist + static_cast<size_t>(qi) * kp;
-    int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
-    for (int i = lane; i < kp; i += kWave) {
-        od[i] = i < n ? tk.dist[i] : INFINITY;
-        ol[i] = i < n ? tk.label[i] : -1;
-    }
-    if (lane == 0) a.out_count[qi] = n;
+    // entries become slots
+    if (tk.size > kp) wave_trim(tk);
+    for (int i = lane; i < tk.size; i += kWave) tk.label[i] = ix.list_ids[static_cast<uint32_t>(tk.label[i])];
+    wave_sync();
+    replay_finish(tk, a, qi);
 }
 
 template <bool STAGE>
 __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) { fs_replay<false, STAGE>(a); }
+__global__ void __launch_bounds__(64) k_fs_replay_win(const ReplayArgs a) { fs_replay_win<false>(a); }
+__global__ void __launch_bounds__(64) k_fl_replay_win(const ReplayArgs a) { fs_replay_win<true>(a); }
 template <bool STAGE>
 __global__ void __launch_bounds__(64) k_fl_replay(const ReplayArgs a) { fs_replay<true, STAGE>(a); }
 
@@ -1927,8 +2236,23 @@ void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out
 
 void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
     const size_t lds = a.g_dist ? 0 : static_cast<size_t>(2 * a.kp) * 12;
+    // QANN_AMD_REPLAY=plain / stage / win forces a form (where it applies)
+    int forced = -1;
+    if (const char* e = std::getenv("QANN_AMD_REPLAY")) {
+        if (std::strcmp(e, "plain") == 0) forced = 0;
+        else if (std::strcmp(e, "stage") == 0) forced = 1;
+        else if (std::strcmp(e, "win") == 0) forced = 2;
+    }
+    // the window form, whenever its TopK and window fit LDS (its fixed arrays
+    // take about 3.3 KB more)
+    const size_t wl = lds + static_cast<size_t>(kReplayWindow) * (a.fdists ? 4 : 2);
+    if ((forced < 0 || forced == 2) && !a.g_dist && wl + 4096 <= kLdsMax) {
+        if (a.fdists) hipLaunchKernelGGL(k_fl_replay_win, dim3(nq), dim3(kWave), wl, s, a);
+        else hipLaunchKernelGGL(k_fs_replay_win, dim3(nq), dim3(kWave), wl, s, a);
+        return;
+    }
     // staged loads pay off while the batch leaves most of the chip's wave slots idle
-    const bool stage = nq < kReplayStageBelow;
+    const bool stage = forced == 1 || (forced < 0 && nq < kReplayStageBelow);
     if (a.fdists) {
         if (stage) hipLaunchKernelGGL(k_fl_replay<true>, dim3(nq), dim3(kWave), lds, s, a);
         else hipLaunchKernelGGL(k_fl_replay<false>, dim3(nq), dim3(kWave), lds, s, a);

@@ -62,6 +62,68 @@ def test_gpu_large_batch_matches_goldens(name):
         assert_same(got, want, f"{name} {stem}")
 
 
+@pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp"])
+def test_gpu_pipelined_batch_matches_goldens(monkeypatch, name):
+    # unfiltered fast-scan batches of at least 8,192 queries run as a pipeline
+    # of parts, each on a stream of its own (the first such search runs its
+    # parts in turn, later ones overlap them): parts of uneven size, 4 and 3
+    # parts, three searches each, through the synchronous entry point and, for
+    # an L2 index, the stream-ordered one on device buffers
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    reps = 8200 // q.shape[0] + 1
+    qq = np.ascontiguousarray(np.tile(q, (reps, 1)))
+    runs = [(npb, k, stem) for npb, k, fname, allowed, stem in synth_queries(exp, params) if allowed is None]
+    assert runs
+    for parts in ("4", "3"):
+        monkeypatch.setenv("QANN_AMD_PIPE", parts)
+        for rep in range(3):
+            for npb, k, stem in runs:
+                if npb is not None:
+                    ivf.nprobe = npb
+                want = (np.tile(exp[stem + "_ids"], (reps, 1)), np.tile(exp[stem + "_dists"], (reps, 1)))
+                assert_same(idx.batch_query(qq, k), want, f"{name} {stem} parts {parts} rep {rep}")
+    if name == "l2_fs_refine":
+        import torch
+
+        npb, k, stem = runs[0]
+        if npb is not None:
+            ivf.nprobe = npb
+        dq = torch.from_numpy(qq).cuda()
+        ids = torch.empty((qq.shape[0], k), dtype=torch.int64, device="cuda")
+        d = torch.empty((qq.shape[0], k), dtype=torch.float32, device="cuda")
+        sptr = torch.cuda.current_stream().cuda_stream
+        want = (np.tile(exp[stem + "_ids"], (reps, 1)), np.tile(exp[stem + "_dists"], (reps, 1)))
+        for rep in range(3):
+            ids.fill_(-7)
+            idx.search_device(dq.data_ptr(), qq.shape[0], k, ids.data_ptr(), d.data_ptr(), sptr)
+            torch.cuda.synchronize()
+            assert_same((ids.cpu().numpy(), d.cpu().numpy()), want, f"{name} {stem} device rep {rep}")
+
+
+@pytest.mark.parametrize("form", ["plain", "stage", "win"])
+@pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
+def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
+    # the TopK replay with its loads one block pair at a time, staged through
+    # LDS, and with windows of block minima in LDS, grouped candidate loads and
+    # bulk pushes (entries in the TopK, slots looked up at the end), each
+    # forced, on every golden index: filtered, deleted, custom ids, PQ / SQ /
+    # float lists; small and large batches
+    monkeypatch.setenv("QANN_AMD_REPLAY", form)
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    reps = 2048 // q.shape[0] + 1
+    for qq, r in ((q, 1), (np.tile(q, (reps, 1)), reps)):
+        for npb, k, fname, allowed, stem in synth_queries(exp, params):
+            if npb is not None:
+                ivf.nprobe = npb
+            got = idx.batch_query(qq, k, allowed=allowed)
+            want = (np.tile(exp[stem + "_ids"], (r, 1)), np.tile(exp[stem + "_dists"], (r, 1)))
+            assert_same(got, want, f"{name} {form} {stem}")
+
+
 @pytest.mark.parametrize("form", ["grid", "queue"])
 @pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp"])
 def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
