@@ -165,7 +165,11 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
 // plus n 2^-125 for flushed denormals). If more than `cap` lists qualify (or
 // fewer than M, with NaN scores), every list is scored exactly and selected by
 // the full 64-bit key.
-template <int NT>
+// CD: values of the row each thread keeps in registers (the row, nlist <=
+// CD * NT, is then read once; CD 0: read from memory at every pass). SAMPLE:
+// tau comes first from a sample of the row (below).
+constexpr int kSelectSample = 4096;
+template <int NT, int CD, bool SAMPLE>
 __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
                                                        int dim, int ip, float cnorm_max, int M, int cap,
                                                        uint16_t* out_idx, float* out_val) {
@@ -191,28 +195,104 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
     for (int w = 0; w < NT / kWave; ++w) ss += s_red[w];
 
     const int target = std::min(cap, M + std::max(64, M / 4));
-    const float tau =
-        nlist <= 16 * NT
-            ? select_threshold<true, false, NT>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8)
-            : select_threshold<false, false, NT>(row, nlist, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
     const float u = 5.9604645e-8f, n = static_cast<float>(dim);
     const float qn = sqrtf(ss * (1.0f + 2.0f * (n + 2.0f) * u)) * 1.0001f;
     const float delta = 2.0f * (ip ? 2.0f * (n + 1.0f) * u * qn * cnorm_max
                                    : (2.0f * n + 5.0f) * u * (qn + cnorm_max) * (qn + cnorm_max)) +
                         n * 2.3509887e-38f;
-    const float limit = tau + 3.0f * delta;
-    for (int l0 = 0; l0 < nlist; l0 += 16 * NT) {
-        float v[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int l = l0 + tid + NT * j;
-            v[j] = l < nlist ? row[l] : INFINITY;
+    // every list scoring at most `limit` into keys (up to cap of them)
+    auto collect = [&](float v, int l, float limit) {
+        if (v <= limit) {
+            const uint32_t pos = atomicAdd(&s_count, 1u);
+            if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l);
         }
+    };
+    bool found = false;
+    if (SAMPLE) {
+        // tau from a sample of kSelectSample scores, one every nlist /
+        // kSelectSample lists: the rank r in the sample that puts about tc
+        // scores of the row at or below it. One pass over the row counts them
+        // and collects the candidates; when fewer than M are at or below (or
+        // more than cap are candidates) the row's own threshold is found below.
+        constexpr int kPer = kSelectSample / NT;
+        const int tc = M + std::max(96, M / 2);
+        if (tc <= cap * 3 / 4) {
+            const int stride = nlist / kSelectSample;
+            float smp[kPer];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (v[j] <= limit) {
-                const uint32_t pos = atomicAdd(&s_count, 1u);
-                if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l0 + tid + NT * j);
+            for (int j = 0; j < kPer; ++j) smp[j] = row[static_cast<size_t>(tid + NT * j) * stride];
+            const int r = std::min(kSelectSample, (tc * kSelectSample + nlist - 1) / nlist);
+            auto each_smp = [&](auto&& fn) {
+#pragma unroll
+                for (int j = 0; j < kPer; ++j) fn(smp[j]);
+            };
+            const float ts = select_threshold_each<false, NT>(each_smp, r, r + std::max(16, r / 4), hist, s_wsum,
+                                                              &s_digit, &s_below, s_red8);
+            const float limit = ts + 3.0f * delta;
+            uint32_t le = 0;
+            for (int l0 = 0; l0 < nlist; l0 += 16 * NT) {
+                float v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int l = l0 + tid + NT * j;
+                    v[j] = l < nlist ? row[l] : INFINITY;
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int l = l0 + tid + NT * j;
+                    if (l < nlist) {
+                        le += v[j] <= ts ? 1u : 0u;
+                        collect(v[j], l, limit);
+                    }
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) le += static_cast<uint32_t>(__shfl_xor(static_cast<int>(le), off, kWave));
+            if (lane_id() == 0) s_red[tid / kWave] = __uint_as_float(le);
+            __syncthreads();
+            uint32_t total = 0;
+            for (int w = 0; w < NT / kWave; ++w) total += __float_as_uint(s_red[w]);
+            found = total >= static_cast<uint32_t>(M) && s_count <= static_cast<uint32_t>(cap);
+            __syncthreads();
+            if (!found && tid == 0) s_count = 0;
+            __syncthreads();
+        }
+    }
+    if (!found) {
+        constexpr int kC = CD > 0 ? CD : 1;
+        float cached[kC];
+        if (CD > 0) {
+#pragma unroll
+            for (int j = 0; j < kC; ++j) {
+                const int l = tid + NT * j;
+                cached[j] = l < nlist ? row[l] : INFINITY;
+            }
+        }
+        auto each_all = [&](auto&& fn) {
+            if (CD > 0) {
+#pragma unroll
+                for (int j = 0; j < kC; ++j)
+                    if (tid + NT * j < nlist) fn(cached[j]);
+            } else {
+                for (int l = tid; l < nlist; l += NT) fn(row[l]);
+            }
+        };
+        const float tau = select_threshold_each<false, NT>(each_all, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
+        const float limit = tau + 3.0f * delta;
+        if (CD > 0) {
+#pragma unroll
+            for (int j = 0; j < kC; ++j)
+                if (tid + NT * j < nlist) collect(cached[j], tid + NT * j, limit);
+        } else {
+            for (int l0 = 0; l0 < nlist; l0 += 16 * NT) {
+                float v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int l = l0 + tid + NT * j;
+                    v[j] = l < nlist ? row[l] : INFINITY;
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (l0 + tid + NT * j < nlist) collect(v[j], l0 + tid + NT * j, limit);
             }
         }
     }
@@ -2049,13 +2129,18 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     if (nq <= 0) return;
     const size_t cap = coarse_select_cap(M);
     const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
-    // rows of up to 16,384 lists stay in registers with 1,024 threads a query
-    if (nlist > 16 * 256 && nlist <= 16 * 1024)
-        hipLaunchKernelGGL(k_coarse_select<1024>, dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim, ip, cnorm_max,
-                           M, static_cast<int>(cap), out_idx, out_val);
+    // each query's row of scores stays in registers, 16 values a thread: with
+    // 256 threads up to 4,096 lists, with 1,024 up to 16,384; larger rows are
+    // read from memory, with a threshold from a sample of the row first
+    if (nlist <= 16 * 256)
+        hipLaunchKernelGGL((k_coarse_select<256, 16, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
+                           cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+    else if (nlist <= 16 * 1024)
+        hipLaunchKernelGGL((k_coarse_select<1024, 16, false>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim,
+                           ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
     else
-        hipLaunchKernelGGL(k_coarse_select<256>, dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip, cnorm_max,
-                           M, static_cast<int>(cap), out_idx, out_val);
+        hipLaunchKernelGGL((k_coarse_select<256, 0, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
+                           cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
 }
 
 void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,

@@ -117,35 +117,21 @@ __device__ __forceinline__ float block_max256(float v, float* s_red) {
     return r;
 }
 
-// A value tau of the row with at least M values <= tau and, as a rule, few
-// more than M: values are bucketed linearly over [min, max] (256 buckets, the
-// bucket a monotone function of the value), the bucket where the running count
-// reaches M is split the same way once more when it holds many, and tau is the
-// largest value in the buckets up to the one found. With CACHE the row
-// (n <= 16 * NT) stays in registers; NT threads, s_red holds 2 * NT / 64 floats. With FINITE, +inf and NaN values (rows
-// that may not be returned) take no part, and the result is +inf when fewer
-// than M values remain.
-template <bool CACHE, bool FINITE = false, int NT = 256>
-__device__ __forceinline__ float select_threshold(const float* row, int n, int M, int target, uint32_t* hist, uint32_t* s_wsum,
-                                  uint32_t* s_digit, uint32_t* s_below, float* s_red) {
+// A value tau of a row of scores with at least M values <= tau and, as a
+// rule, few more than M: values are bucketed linearly over [min, max] (256
+// buckets, the bucket a monotone function of the value), the bucket where the
+// running count reaches M is split the same way once more when it holds many,
+// and tau is the largest value in the buckets up to the one found. The row is
+// given by each_all(fn), which calls fn on every value this thread holds (the
+// block's threads together holding the row once). NT threads; s_red holds
+// 2 * NT / 64 floats. With FINITE, +inf and NaN values (rows that may not be
+// returned) take no part, and the result is +inf when fewer than M values
+// remain.
+template <bool FINITE, int NT, typename EachAll>
+__device__ __forceinline__ float select_threshold_each(EachAll&& each_all, int M, int target, uint32_t* hist,
+                                                       uint32_t* s_wsum, uint32_t* s_digit, uint32_t* s_below,
+                                                       float* s_red) {
     const int tid = threadIdx.x;
-    float cached[CACHE ? 16 : 1];
-    if (CACHE) {
-#pragma unroll
-        for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
-            const int l = tid + NT * j;
-            cached[j] = l < n ? row[l] : 0.0f;
-        }
-    }
-    auto each_all = [&](auto&& fn) {
-        if (CACHE) {
-#pragma unroll
-            for (int j = 0; j < (CACHE ? 16 : 1); ++j)
-                if (tid + NT * j < n) fn(cached[j]);
-        } else {
-            for (int l = tid; l < n; l += NT) fn(row[l]);
-        }
-    };
     auto each = [&](auto&& fn) {
         each_all([&](float v) {
             if (!FINITE || v <= 3.40282347e38f) fn(v);
@@ -220,6 +206,32 @@ __device__ __forceinline__ float select_threshold(const float* row, int n, int M
         if (b < d1 || (b == d1 && b2(v) <= d2)) m = fmaxf(m, v);
     });
     return block_max256<NT>(m, s_red);
+}
+
+// select_threshold_each over row[0 .. n): with CACHE the row (n <= 16 * NT)
+// is read once into registers, else it is read from memory at every pass.
+template <bool CACHE, bool FINITE = false, int NT = 256>
+__device__ __forceinline__ float select_threshold(const float* row, int n, int M, int target, uint32_t* hist, uint32_t* s_wsum,
+                                  uint32_t* s_digit, uint32_t* s_below, float* s_red) {
+    const int tid = threadIdx.x;
+    float cached[CACHE ? 16 : 1];
+    if (CACHE) {
+#pragma unroll
+        for (int j = 0; j < (CACHE ? 16 : 1); ++j) {
+            const int l = tid + NT * j;
+            cached[j] = l < n ? row[l] : 0.0f;
+        }
+    }
+    auto each_all = [&](auto&& fn) {
+        if (CACHE) {
+#pragma unroll
+            for (int j = 0; j < (CACHE ? 16 : 1); ++j)
+                if (tid + NT * j < n) fn(cached[j]);
+        } else {
+            for (int l = tid; l < n; l += NT) fn(row[l]);
+        }
+    };
+    return select_threshold_each<FINITE, NT>(each_all, M, target, hist, s_wsum, s_digit, s_below, s_red);
 }
 
 } // namespace qd

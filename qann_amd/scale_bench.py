@@ -102,6 +102,20 @@ def main():
     torch.cuda.empty_cache()
     log(f"exact top-{k} of {nt} queries in {time.perf_counter() - t0:.1f}s")
 
+    import ctypes
+    import pathlib
+
+    lib = ctypes.CDLL(str(pathlib.Path(qann_amd.__file__).parent / "libqann_hip.so"))
+    lib.qann_hip_profile_enable.argtypes = [ctypes.c_int]
+    lib.qann_hip_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.c_int]
+    lib.qann_hip_stage_name.restype = ctypes.c_char_p
+    ms = (ctypes.c_double * 32)()
+    cnt = (ctypes.c_uint64 * 32)()
+    cents = torch.from_numpy(ivf.centroids()).cuda()
+    csq = (cents * cents).sum(1)
+    lblocks = torch.from_numpy(((sizes + 31) // 32).astype(np.int64)).cuda()
+
     dq = torch.from_numpy(q).cuda()
     ids = torch.empty((args.nq, k), dtype=torch.int64, device="cuda")
     dists = torch.empty((args.nq, k), dtype=torch.float32, device="cuda")
@@ -120,10 +134,31 @@ def main():
         got = ids[:nt].cpu().numpy()
         rec = float(np.mean([len(set(a) & set(b)) / k for a, b in zip(got, truth)]))
         best = min(times)
+        # one more batch with the library's stage timers (HIP events on each
+        # stage's stream), and the code-block bytes its probes cover
+        lib.qann_hip_profile_read(ms, cnt, 1)
+        lib.qann_hip_profile_enable(1)
+        index.search_device(dq.data_ptr(), args.nq, k, ids.data_ptr(), dists.data_ptr(), sptr)
+        torch.cuda.synchronize()
+        lib.qann_hip_profile_enable(0)
+        lib.qann_hip_profile_read(ms, cnt, 1)
+        stages = {lib.qann_hip_stage_name(i).decode(): {"ms": round(ms[i], 3), "launches": int(cnt[i])}
+                  for i in range(32) if cnt[i]}
+        blocks = 0
+        for s in range(0, args.nq, 1000):
+            d2 = csq[None, :] - 2.0 * (dq[s:s + 1000] @ cents.T)
+            blocks += int(lblocks[d2.topk(npb, dim=1, largest=False).indices].sum().item())
+        scan_bytes = blocks * args.m * 16
+        scan = stages.get("scan", {"ms": 0.0, "launches": 0})
+        gbs = scan_bytes / (scan["ms"] * 1e-3) / 1e9 if scan["ms"] else 0.0
         results[str(npb)] = {"qps": round(args.nq / best, 1), "ms_per_batch": round(1e3 * best, 2),
-                             "recall@10": round(rec, 4)}
+                             "recall@10": round(rec, 4), "stages": stages,
+                             "scan_code_bytes": scan_bytes, "scan_gb_per_s": round(gbs, 1)}
         log(f"nprobe {npb}: {args.nq / best:,.0f} QPS, {1e3 * best:.1f} ms per {args.nq}-query batch, "
             f"recall@10 {rec:.4f}")
+        log("  stages (ms per batch, launches): " +
+            ", ".join(f"{n} {v['ms']:.2f} ({v['launches']})" for n, v in stages.items()))
+        log(f"  scan: {scan_bytes / 1e9:.1f} GB of code blocks, {gbs:,.0f} GB/s over the scan launches' time")
     print(json.dumps({
         "workload": f"{args.n:,} x {dim} L2, Zipf(1.1) {args.centres}-centre mixture; IVF{args.nlist} + "
                     f"PQ{args.m}x4 fast-scan + RefineIndex(k_factor={kf}) in HBM; {args.nq:,}-query batch, k={k}",

@@ -245,14 +245,15 @@ def test_gpu_built_index_matches_oracle(tmp_path, oracle, metric, dim, m, nlist,
     assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "filtered")
 
 
-def test_gpu_many_lists_matches_oracle(tmp_path, oracle):
-    # 5,000 lists: the coarse select keeps each query's row of 5,000 scores in
-    # registers with 1,024 threads (up to 16,384 lists), where smaller indexes
-    # use 256 threads
-    rng = np.random.default_rng(5000)
+@pytest.mark.parametrize("nlist", [5000, 30000])
+def test_gpu_many_lists_matches_oracle(tmp_path, oracle, nlist):
+    # the coarse select keeps each query's row of scores in registers: 5,000
+    # lists with 1,024 threads of 16 scores (up to 16,384 lists), 30,000 with
+    # 1,024 threads of 64 (up to 65,536), where smaller indexes use 256 threads
+    rng = np.random.default_rng(nlist)
     x = mixture(rng, 100_000, 32, centers=400, sigma=0.05)
     q = mixture(rng, 300, 32, centers=400, sigma=0.05)
-    ivf = qann_amd.IVFIndex(32, 5000, 16)
+    ivf = qann_amd.IVFIndex(32, nlist, 16)
     ivf.enable_pq(16, fast_scan=True)
     ivf.train(x, max_iters=4, seed=2)
     idx = qann_amd.RefineIndex(ivf, 4)
