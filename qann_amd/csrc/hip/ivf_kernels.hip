@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
 // the full 64-bit key.
 // CD: values of the row each thread keeps in registers (the row, nlist <=
 // CD * NT, is then read once; CD 0: read from memory at every pass). SAMPLE:
-// tau comes first from a sample of the row (below).
+// tau comes first from a sample of at most kSelectSample scores of the row.
 constexpr int kSelectSample = 4096;
 template <int NT, int CD, bool SAMPLE>
 __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
@@ -207,29 +207,22 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
             if (pos < static_cast<uint32_t>(cap)) keys[pos] = static_cast<uint64_t>(l);
         }
     };
-    bool found = false;
-    if (SAMPLE) {
-        // tau from a sample of kSelectSample scores, one every nlist /
-        // kSelectSample lists: the rank r in the sample that puts about tc
-        // scores of the row at or below it. One pass over the row counts them
-        // and collects the candidates; when fewer than M are at or below (or
-        // more than cap are candidates) the row's own threshold is found below.
-        constexpr int kPer = kSelectSample / NT;
-        const int tc = M + std::max(96, M / 2);
-        if (tc <= cap * 3 / 4) {
-            const int stride = nlist / kSelectSample;
-            float smp[kPer];
+    // the row: kept in registers (CD values a thread) or read from memory
+    constexpr int kC = CD > 0 ? CD : 1;
+    float cached[kC];
+    if (CD > 0) {
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) smp[j] = row[static_cast<size_t>(tid + NT * j) * stride];
-            const int r = std::min(kSelectSample, (tc * kSelectSample + nlist - 1) / nlist);
-            auto each_smp = [&](auto&& fn) {
+        for (int j = 0; j < kC; ++j) {
+            const int l = tid + NT * j;
+            cached[j] = l < nlist ? row[l] : INFINITY;
+        }
+    }
+    auto each_vl = [&](auto&& fn) {
+        if (CD > 0) {
 #pragma unroll
-                for (int j = 0; j < kPer; ++j) fn(smp[j]);
-            };
-            const float ts = select_threshold_each<false, NT>(each_smp, r, r + std::max(16, r / 4), hist, s_wsum,
-                                                              &s_digit, &s_below, s_red8);
-            const float limit = ts + 3.0f * delta;
-            uint32_t le = 0;
+            for (int j = 0; j < kC; ++j)
+                if (tid + NT * j < nlist) fn(cached[j], tid + NT * j);
+        } else {
             for (int l0 = 0; l0 < nlist; l0 += 16 * NT) {
                 float v[16];
 #pragma unroll
@@ -238,14 +231,43 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
                     v[j] = l < nlist ? row[l] : INFINITY;
                 }
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int l = l0 + tid + NT * j;
-                    if (l < nlist) {
-                        le += v[j] <= ts ? 1u : 0u;
-                        collect(v[j], l, limit);
-                    }
-                }
+                for (int j = 0; j < 16; ++j)
+                    if (l0 + tid + NT * j < nlist) fn(v[j], l0 + tid + NT * j);
             }
+        }
+    };
+    bool found = false;
+    if (SAMPLE) {
+        // tau from a sample of ns scores, one every nlist / ns lists: the rank r
+        // in the sample that puts about tc scores of the row at or below it. One
+        // pass over the row counts them and collects the candidates; when fewer
+        // than M are at or below (or more than cap are candidates) the row's own
+        // threshold is found below.
+        constexpr int kPer = kSelectSample / NT;
+        const int ns = std::min(kSelectSample, nlist / 16);
+        const int tc = 2 * M + 96;
+        if (ns >= 64 && tc <= cap * 3 / 4) {
+            const int stride = nlist / ns;
+            float smp[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int si = tid + NT * j;
+                smp[j] = si < ns ? row[static_cast<size_t>(si) * stride] : INFINITY;
+            }
+            const int r = std::min(ns, (tc * ns + nlist - 1) / nlist);
+            auto each_smp = [&](auto&& fn) {
+#pragma unroll
+                for (int j = 0; j < kPer; ++j)
+                    if (tid + NT * j < ns) fn(smp[j]);
+            };
+            const float ts = select_threshold_each<false, NT>(each_smp, r, r + std::max(16, r / 4), hist, s_wsum,
+                                                              &s_digit, &s_below, s_red8);
+            const float limit = ts + 3.0f * delta;
+            uint32_t le = 0;
+            each_vl([&](float v, int l) {
+                le += v <= ts ? 1u : 0u;
+                collect(v, l, limit);
+            });
             for (int off = 32; off > 0; off >>= 1) le += static_cast<uint32_t>(__shfl_xor(static_cast<int>(le), off, kWave));
             if (lane_id() == 0) s_red[tid / kWave] = __uint_as_float(le);
             __syncthreads();
@@ -258,43 +280,10 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
         }
     }
     if (!found) {
-        constexpr int kC = CD > 0 ? CD : 1;
-        float cached[kC];
-        if (CD > 0) {
-#pragma unroll
-            for (int j = 0; j < kC; ++j) {
-                const int l = tid + NT * j;
-                cached[j] = l < nlist ? row[l] : INFINITY;
-            }
-        }
-        auto each_all = [&](auto&& fn) {
-            if (CD > 0) {
-#pragma unroll
-                for (int j = 0; j < kC; ++j)
-                    if (tid + NT * j < nlist) fn(cached[j]);
-            } else {
-                for (int l = tid; l < nlist; l += NT) fn(row[l]);
-            }
-        };
+        auto each_all = [&](auto&& fn) { each_vl([&](float v, int) { fn(v); }); };
         const float tau = select_threshold_each<false, NT>(each_all, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
         const float limit = tau + 3.0f * delta;
-        if (CD > 0) {
-#pragma unroll
-            for (int j = 0; j < kC; ++j)
-                if (tid + NT * j < nlist) collect(cached[j], tid + NT * j, limit);
-        } else {
-            for (int l0 = 0; l0 < nlist; l0 += 16 * NT) {
-                float v[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int l = l0 + tid + NT * j;
-                    v[j] = l < nlist ? row[l] : INFINITY;
-                }
-#pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (l0 + tid + NT * j < nlist) collect(v[j], l0 + tid + NT * j, limit);
-            }
-        }
+        each_vl([&](float v, int l) { collect(v, l, limit); });
     }
     __syncthreads();
     const int cnt = static_cast<int>(s_count);
@@ -2131,16 +2120,35 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
     // each query's row of scores stays in registers, 16 values a thread: with
     // 256 threads up to 4,096 lists, with 1,024 up to 16,384; larger rows are
-    // read from memory, with a threshold from a sample of the row first
-    if (nlist <= 16 * 256)
-        hipLaunchKernelGGL((k_coarse_select<256, 16, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
-                           cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
-    else if (nlist <= 16 * 1024)
-        hipLaunchKernelGGL((k_coarse_select<1024, 16, false>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim,
-                           ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
-    else
-        hipLaunchKernelGGL((k_coarse_select<256, 0, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
-                           cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+    // read from memory, and take a threshold from a sample first (a row in
+    // registers costs less to search than the sample's larger candidate set).
+    // QANN_AMD_SELECT_SAMPLE=1 samples every row of more than 2,048 lists, 0 none.
+    const char* se = std::getenv("QANN_AMD_SELECT_SAMPLE");
+    const bool smp = se && std::strcmp(se, "1") == 0   ? nlist > 2048
+                     : se && std::strcmp(se, "0") == 0 ? false
+                                                        : nlist > 16 * 1024;
+    if (nlist <= 16 * 256) {
+        if (smp)
+            hipLaunchKernelGGL((k_coarse_select<256, 16, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+        else
+            hipLaunchKernelGGL((k_coarse_select<256, 16, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+    } else if (nlist <= 16 * 1024) {
+        if (smp)
+            hipLaunchKernelGGL((k_coarse_select<1024, 16, true>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim,
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+        else
+            hipLaunchKernelGGL((k_coarse_select<1024, 16, false>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist,
+                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+    } else {
+        if (smp)
+            hipLaunchKernelGGL((k_coarse_select<256, 0, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
+                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+        else
+            hipLaunchKernelGGL((k_coarse_select<256, 0, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+    }
 }
 
 void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,

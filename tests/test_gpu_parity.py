@@ -245,11 +245,14 @@ def test_gpu_built_index_matches_oracle(tmp_path, oracle, metric, dim, m, nlist,
     assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "filtered")
 
 
-@pytest.mark.parametrize("nlist", [5000, 30000])
-def test_gpu_many_lists_matches_oracle(tmp_path, oracle, nlist):
-    # the coarse select keeps each query's row of scores in registers: 5,000
-    # lists with 1,024 threads of 16 scores (up to 16,384 lists), 30,000 with
-    # 1,024 threads of 64 (up to 65,536), where smaller indexes use 256 threads
+@pytest.mark.parametrize("sample", ["1", "0"])
+@pytest.mark.parametrize("nlist", [3000, 5000, 30000])
+def test_gpu_many_lists_matches_oracle(tmp_path, monkeypatch, oracle, nlist, sample):
+    # the coarse select's shapes: a row of 3,000 scores in registers of 256
+    # threads, 5,000 in registers of 1,024 threads (up to 16,384 lists), 30,000
+    # read from memory; each with its threshold from a sample of the row first
+    # (the default above 16,384 lists) and without
+    monkeypatch.setenv("QANN_AMD_SELECT_SAMPLE", sample)
     rng = np.random.default_rng(nlist)
     x = mixture(rng, 100_000, 32, centers=400, sigma=0.05)
     q = mixture(rng, 300, 32, centers=400, sigma=0.05)
