@@ -156,6 +156,91 @@ __global__ void __launch_bounds__(256) k_coarse_gemm(const float* q, int nq, con
     put(acc11, 1, 1);
 }
 
+// The same scores by v_mfma_f32_32x32x2f32 over 128 x 128 tiles: four waves
+// of 64 x 64 (2 x 2 tiles of 32 x 32), K staged through LDS 16 at a time in
+// K-major order (a wave's 32 lanes read 32 consecutive rows of one k), the
+// next step's tiles loaded into registers while the current one's MFMAs run.
+// Lane t gives A[t % 32][t / 32] and B[t / 32][t % 32] of each 32 x 2 x 32
+// step and holds D[8 (v / 4) + 4 (t / 32) + v % 4][t % 32] in accumulator v.
+// The scores differ from k_coarse_gemm's only by summation order, which the
+// select's bound on |A - approx - const| covers.
+constexpr int kG2Tile = 128, kG2K = 16, kG2Pad = 4;
+__global__ void __launch_bounds__(256) k_coarse_gemm32(const float* q, int nq, const float* cents, const float* csq,
+                                                       int nlist, int dim, int ip, float* D, const uint64_t* excl,
+                                                       const uint64_t* incl) {
+    __shared__ float qs[2][kG2K][kG2Tile + kG2Pad];
+    __shared__ float cs[2][kG2K][kG2Tile + kG2Pad];
+    const int l0 = blockIdx.x * kG2Tile, q0 = blockIdx.y * kG2Tile;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const int wq = (wave >> 1) * 64, wl = (wave & 1) * 64;
+    // each thread stages 8 values of each operand per K step: element e =
+    // tid + 256 j is row e / 16, k e % 16 (16 threads read 64 contiguous bytes)
+    constexpr int kPer = kG2Tile * kG2K / 256;
+    float pa[kPer], pb[kPer];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int e = tid + 256 * j, r = e / kG2K, k = k0 + e % kG2K;
+            const int gq = q0 + r, gl = l0 + r;
+            pa[j] = (gq < nq && k < dim) ? q[static_cast<size_t>(gq) * dim + k] : 0.0f;
+            pb[j] = (gl < nlist && k < dim) ? cents[static_cast<size_t>(gl) * dim + k] : 0.0f;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int e = tid + 256 * j, r = e / kG2K, k = e % kG2K;
+            qs[buf][k][r] = pa[j];
+            cs[buf][k][r] = pb[j];
+        }
+    };
+    typedef float v16f __attribute__((ext_vector_type(16)));
+    v16f acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+    const int nk = (dim + kG2K - 1) / kG2K;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int rr = lane % 32, kh = lane / 32;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * kG2K);
+#pragma unroll
+        for (int kk = 0; kk < kG2K; kk += 2) {
+            const float a0 = qs[buf][kk + kh][wq + rr], a1 = qs[buf][kk + kh][wq + 32 + rr];
+            const float b0 = cs[buf][kk + kh][wl + rr], b1 = cs[buf][kk + kh][wl + 32 + rr];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (kt + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+        const int l = l0 + wl + 32 * tj + rr;
+        if (l >= nlist) continue;
+        const float c2 = ip ? 0.0f : csq[l];
+        // a row the query may not return scores +inf
+        const bool out = (excl && bit_test(excl, static_cast<uint32_t>(l))) ||
+                         (incl && !bit_test(incl, static_cast<uint32_t>(l)));
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = q0 + wq + 32 * ti + 8 * (v / 4) + 4 * kh + (v % 4);
+                if (i < nq)
+                    D[static_cast<size_t>(i) * nlist + l] = out ? INFINITY : ip ? -acc[ti][tj][v] : c2 - 2.0f * acc[ti][tj][v];
+            }
+    }
+}
+
 // Per query (one workgroup): the M smallest (A, list), A exact, sorted. The
 // candidates are the lists whose approximate score is at most tau + 3 delta,
 // tau an approximate score with at least M at or below it (so every list of
@@ -2104,12 +2189,21 @@ size_t coarse_select_cap(int M) {
 
 void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
                         float* D, hipStream_t s, const uint64_t* excl, const uint64_t* incl) {
-    const int rows = 65535 * kGemmTile; // grid y limit
+    // QANN_AMD_COARSE_GEMM=16 keeps the 64 x 64 tiles of 16 x 16 x 4 MFMAs
+    const char* e = std::getenv("QANN_AMD_COARSE_GEMM");
+    const bool small = e && std::strcmp(e, "16") == 0;
+    const int tile = small ? kGemmTile : kG2Tile;
+    const int rows = 65535 * tile; // grid y limit
     for (int q0 = 0; q0 < nq; q0 += rows) {
         const int n = std::min(rows, nq - q0);
-        hipLaunchKernelGGL(k_coarse_gemm, dim3(cdiv(nlist, kGemmTile), cdiv(n, kGemmTile)), dim3(256), 0, s,
-                           q + static_cast<size_t>(q0) * dim, n, cents, csq, nlist, dim, ip,
-                           D + static_cast<size_t>(q0) * nlist, excl, incl);
+        if (small)
+            hipLaunchKernelGGL(k_coarse_gemm, dim3(cdiv(nlist, tile), cdiv(n, tile)), dim3(256), 0, s,
+                               q + static_cast<size_t>(q0) * dim, n, cents, csq, nlist, dim, ip,
+                               D + static_cast<size_t>(q0) * nlist, excl, incl);
+        else
+            hipLaunchKernelGGL(k_coarse_gemm32, dim3(cdiv(nlist, tile), cdiv(n, tile)), dim3(256), 0, s,
+                               q + static_cast<size_t>(q0) * dim, n, cents, csq, nlist, dim, ip,
+                               D + static_cast<size_t>(q0) * nlist, excl, incl);
     }
 }
 
