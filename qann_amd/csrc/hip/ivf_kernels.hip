@@ -873,10 +873,11 @@ __device__ void wave_lut_l2(const DevIvf& ix, const float* q, int list, float A,
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 uint32_t v = 0;
-                // each value is a whole number in [0, 63]: the byte conversion is exact
+                // the byte conversion rounds half to even, as nearbyint does; the
+                // values lie in [0, 63 + rounding]
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    v = __builtin_amdgcn_cvt_pk_u8_f32(__builtin_rintf((t[4 * i + j] - lo) * scale), j, v);
+                    v = __builtin_amdgcn_cvt_pk_u8_f32((t[4 * i + j] - lo) * scale, j, v);
                 w[i] = v;
             }
             *reinterpret_cast<uint4*>(lut + r * 16) = lut_code(make_uint4(w[0], w[1], w[2], w[3]));
