@@ -1313,6 +1313,60 @@ __global__ void k_probe_plan(const DevIvf ix, int nq, int M, int nprobe, int min
     status[i] = st;
 }
 
+// k_probe_plan with one wave per query: 64 probes at a time, the block
+// offsets and allowed counts by wave prefix sums, and the loop's stop rule
+// (checked before probe p is taken: p >= nprobe, p >= min_probes and the
+// allowed entries of probes before p at least `wanted`) found by a ballot.
+__global__ void __launch_bounds__(256) k_probe_plan_wave(const DevIvf ix, int nq, int M, int nprobe, int min_probes,
+                                                          int wanted, const uint16_t* probe_idx,
+                                                          const uint32_t* list_allowed, int* nprobe_q,
+                                                          uint32_t* item_blk, int* status) {
+    const int i = blockIdx.x * 4 + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = lane_id();
+    if (i >= nq) return;
+    const uint16_t* pl = probe_idx + static_cast<size_t>(i) * M;
+    uint32_t blk = 0;  // blocks of the probes taken so far
+    uint64_t seen = 0; // their allowed entries
+    auto incl_scan = [&](uint32_t v) {
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), off, kWave));
+            if (lane >= off) v += y;
+        }
+        return v;
+    };
+    for (int c0 = 0; c0 < M; c0 += kWave) {
+        const int p = c0 + lane;
+        uint32_t nb = 0, al = 0;
+        if (p < M) {
+            const int list = pl[p];
+            nb = (ix.list_off[list + 1] - ix.list_off[list] + kFsBlock - 1) / kFsBlock;
+            if (list_allowed) al = list_allowed[list];
+        }
+        const uint32_t inb = incl_scan(nb), ial = incl_scan(al);
+        const uint64_t seen_before = seen + (ial - al);
+        const bool stop = p < M && p >= nprobe && p >= min_probes && seen_before >= static_cast<uint64_t>(wanted);
+        const uint64_t sm = ballot(stop);
+        const int first = sm ? __builtin_ctzll(sm) : kWave;
+        if (p < M && lane < first) item_blk[static_cast<size_t>(i) * M + p] = blk + (inb - nb);
+        if (sm) {
+            if (lane == 0) {
+                nprobe_q[i] = c0 + first;
+                status[i] = 0;
+            }
+            return;
+        }
+        blk += static_cast<uint32_t>(__shfl(static_cast<int>(inb), kWave - 1, kWave));
+        seen += static_cast<uint32_t>(__shfl(static_cast<int>(ial), kWave - 1, kWave));
+    }
+    // every selected probe taken: the loop stops at M, needing more lists
+    // when its rule does not hold there and lists remain
+    if (lane == 0) {
+        const bool stop = M >= nprobe && M >= min_probes && seen >= static_cast<uint64_t>(wanted);
+        nprobe_q[i] = M;
+        status[i] = (!stop && M < ix.nlist) ? 1 : 0;
+    }
+}
+
 // Allowed entries of each list under a filter.
 __global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t* out) {
     const int l = blockIdx.x;
@@ -2414,8 +2468,14 @@ void launch_ivf_dists(DistArgs a, int nq, hipStream_t s) {
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
                        const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
                        int* status, hipStream_t s) {
-    hipLaunchKernelGGL(k_probe_plan, dim3(cdiv(nq, 128)), dim3(128), 0, s, ix, nq, M, nprobe, min_probes, wanted,
-                       probe_idx, list_allowed, nprobe_q, item_blk, status);
+    // QANN_AMD_PROBE_PLAN=thread keeps one thread per query
+    const char* e = std::getenv("QANN_AMD_PROBE_PLAN");
+    if (e && std::strcmp(e, "thread") == 0)
+        hipLaunchKernelGGL(k_probe_plan, dim3(cdiv(nq, 128)), dim3(128), 0, s, ix, nq, M, nprobe, min_probes, wanted,
+                           probe_idx, list_allowed, nprobe_q, item_blk, status);
+    else
+        hipLaunchKernelGGL(k_probe_plan_wave, dim3(cdiv(nq, 4)), dim3(256), 0, s, ix, nq, M, nprobe, min_probes,
+                           wanted, probe_idx, list_allowed, nprobe_q, item_blk, status);
 }
 
 void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s) {

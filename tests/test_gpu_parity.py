@@ -124,6 +124,22 @@ def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
             assert_same(got, want, f"{name} {form} {stem}")
 
 
+@pytest.mark.parametrize("form", ["thread", "wave"])
+@pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
+def test_gpu_probe_plan_forms_match_goldens(monkeypatch, name, form):
+    # the probe plan (probes taken, their block offsets, the filtered stop
+    # rule) by one thread per query and by one wave per query
+    monkeypatch.setenv("QANN_AMD_PROBE_PLAN", form)
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    for npb, k, fname, allowed, stem in synth_queries(exp, params):
+        if npb is not None:
+            ivf.nprobe = npb
+        assert_same(idx.batch_query(q, k, allowed=allowed), (exp[stem + "_ids"], exp[stem + "_dists"]),
+                    f"{name} {form} {stem}")
+
+
 @pytest.mark.parametrize("form", ["grid", "queue"])
 @pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp"])
 def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
