@@ -537,3 +537,38 @@ def test_gpu_built_storages_match_oracle(tmp_path, oracle, metric, dim, storage,
         allowed = rng.choice(8000, size, replace=False)
         assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed),
                     f"{storage} filtered {size}")
+
+
+def test_gpu_stage_profiler_times_without_changing_results():
+    # the stage timers record events around each stage without synchronizing:
+    # results under them are the goldens, every stage of the IVF + refine path
+    # is counted once per search, times are positive, and a read resets
+    import ctypes
+    import pathlib
+
+    lib = ctypes.CDLL(str(pathlib.Path(qann_amd.__file__).parent / "libqann_hip.so"))
+    lib.qann_hip_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.c_int]
+    lib.qann_hip_stage_name.restype = ctypes.c_char_p
+    path, q, exp, params = synth_case("l2_fs_refine")
+    idx = qann_amd.load(str(path))
+    ivf = idx.base
+    runs = [(npb, k, stem) for npb, k, fname, allowed, stem in synth_queries(exp, params) if allowed is None]
+    _stage_launches()
+    _profiling(True)
+    try:
+        for npb, k, stem in runs:
+            if npb is not None:
+                ivf.nprobe = npb
+            assert_same(idx.batch_query(q, k), (exp[stem + "_ids"], exp[stem + "_dists"]), f"profiled {stem}")
+    finally:
+        _profiling(False)
+    ms = (ctypes.c_double * 32)()
+    n = (ctypes.c_uint64 * 32)()
+    lib.qann_hip_profile_read(ms, n, 1)
+    stages = {lib.qann_hip_stage_name(i).decode(): (ms[i], int(n[i])) for i in range(32)
+              if lib.qann_hip_stage_name(i)}
+    for name in ("coarse", "select", "scan", "replay", "rerank"):
+        assert stages[name][1] == len(runs), (name, stages[name])
+        assert stages[name][0] > 0.0, (name, stages[name])
+    assert all(v == 0 for v in _stage_launches().values())
