@@ -229,6 +229,10 @@ struct qann_hip_index {
     int kp = 0;
     // cleared when most queries of a search needed the FP32 path
     std::atomic<bool> bf16_ok{true};
+    // IVF: the centroids as split BF16 rows [hi | lo | hi] of split_kp each,
+    // for the coarse scores by one BF16 GEMM
+    uint16_t* cent_split = nullptr;
+    int split_kp = 0;
     qann_hip_index* base = nullptr;
     std::vector<void*> allocs;
     uint64_t bytes = 0;
@@ -505,6 +509,18 @@ __global__ void k_any_set(const int* status, int n, int* flag) {
 // M is a first guess (twice the probes the rule asks for at least) and a query
 // that would need more sets *d_more; the caller then searches again with every
 // list it may need (at most 4096). Without it, M is that limit at once.
+// Coarse scores by the split BF16 product (hi.hi + hi.lo + lo.hi, one BF16
+// MFMA GEMM over three times K) instead of the FP32 MFMA GEMM, when
+// QANN_AMD_COARSE=bf16. Its proven error is about three times the FP32
+// GEMM's, so the exact shortlist is wider: where many lists crowd around a
+// query (Zipf-sized clusters split over many lists) queries overflow into
+// scoring every list exactly, so the FP32 GEMM stays the default.
+bool coarse_split(const qann_hip_index* ivfh) {
+    if (!ivfh->cent_split) return false;
+    const char* e = std::getenv("QANN_AMD_COARSE");
+    return e && std::strcmp(e, "bf16") == 0;
+}
+
 // One search on sc's stream. scan_wait / scan_done (fast-scan lists only, may
 // be null): the scan waits for scan_wait and records scan_done after its last
 // launch, so that a pipelined search's parts scan one after another.
@@ -693,6 +709,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     size_t need = 0;
     need += ix.rotation ? pad256(nq * d * 4) : 0;
     need += pad256(nq * nlist * 4); // coarse distances
+    const bool split = coarse_split(ivfh);
+    need += split ? pad256(nq * 3 * static_cast<size_t>(ivfh->split_kp) * 2) : 0;
     need += pad256(nq * M * 2) + 3 * pad256(nq * M * 4);
     need += use_pre || (ip_pq && !fs) ? pad256(nq * tsz * 4) : 0;
     need += ip_pq && fs ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
@@ -719,15 +737,24 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         qr = r;
     }
     float* D = sc.take<float>(nq * nlist);
+    uint16_t* q3 = split ? sc.take<uint16_t>(nq * 3 * static_cast<size_t>(ivfh->split_kp)) : nullptr;
     StageTimer t_coarse(QANN_STAGE_COARSE, st);
-    qd::launch_coarse_gemm(qr, static_cast<int>(nq), ix.centroids, ix.centroid_sqnorms, ix.nlist, ix.dim, ip, D, st);
-    check_launch("k_coarse_gemm");
+    if (split) {
+        qd::launch_split_bf16(qr, nq, ix.dim, ivfh->split_kp, 0, q3, st);
+        qd::launch_coarse_split(q3, static_cast<int>(nq), ivfh->cent_split, ix.nlist, ivfh->split_kp,
+                                ix.centroid_sqnorms, ip, D, st);
+        check_launch("k_gemm_bf16 (coarse)");
+    } else {
+        qd::launch_coarse_gemm(qr, static_cast<int>(nq), ix.centroids, ix.centroid_sqnorms, ix.nlist, ix.dim, ip, D,
+                               st);
+        check_launch("k_coarse_gemm");
+    }
     t_coarse.done();
     uint16_t* pidx = sc.take<uint16_t>(nq * M);
     float* pval = sc.take<float>(nq * M);
     StageTimer t_sel(QANN_STAGE_SELECT, st);
     qd::launch_coarse_select(D, qr, static_cast<int>(nq), ix.centroids, ix.cnorm_max, ix.nlist, ix.dim, ip,
-                             static_cast<int>(M), pidx, pval, st);
+                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0);
     check_launch("k_coarse_select");
     float* pA = sc.take<float>(nq * M);
     float* pbase = sc.take<float>(nq * M);
@@ -1181,6 +1208,9 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
             ix.sq_step = h->upload(dsc.sq_step, d);
         }
         if (dsc.rotation) ix.rotation = h->upload(dsc.rotation, d * d);
+        h->split_kp = qd::bf16_kpad(static_cast<int>(d));
+        h->cent_split = h->alloc<uint16_t>(nl * 3 * static_cast<size_t>(h->split_kp));
+        qd::launch_split_bf16(cent, nl, static_cast<int>(d), h->split_kp, 1, h->cent_split, st);
         check_launch("index setup kernels");
         check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         *out = h.release();
@@ -1535,7 +1565,8 @@ int qann_hip_search_scratch_bytes(qann_hip_index* h, uint64_t nq, const qann_hip
             // a pipelined search's parts hold a sums buffer each
             const uint64_t parts = static_cast<uint64_t>(pipe_parts(h, nq, *p));
             const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, parts * (g_sums_budget / (stride * vb))));
-            b = nq * nlist * 4 + nq * M * 34 + nq * static_cast<uint64_t>(ix.m) * std::max(ix.K, 16) * 4 +
+            b = nq * nlist * 4 + nq * 3 * static_cast<uint64_t>(ivfh->split_kp) * 2 + nq * M * 34 +
+                nq * static_cast<uint64_t>(ix.m) * std::max(ix.K, 16) * 4 +
                 chunk * stride * vb * 33 / 32 + nq * kp * 24 + nq * ix.dim * 4 + (uint64_t{1} << 20);
         }
         *bytes = b;

@@ -103,6 +103,28 @@ __global__ void __launch_bounds__(256) k_queries_bf16(const float* q, int nq, in
     }
 }
 
+// Each row as three BF16 segments of kp (zero-padded) for a split product:
+// x = hi + lo + r with hi = bf16(x), lo = bf16(x - hi), |r| <= 2^-16 |x|.
+// order 0 writes [hi | hi | lo], order 1 [hi | lo | hi], so that one GEMM over
+// K = 3 kp of an order-0 row against an order-1 row sums hi.hi + hi.lo + lo.hi.
+__global__ void __launch_bounds__(256) k_split_bf16(const float* x, uint64_t n, int dim, int kp, int order,
+                                                    uint16_t* out) {
+    const uint64_t r = blockIdx.x * 4ull + threadIdx.x / kWave;
+    if (r >= n) return;
+    const float* xr = x + r * dim;
+    uint16_t* o = out + r * 3 * static_cast<uint64_t>(kp);
+    for (int e = lane_id(); e < kp; e += kWave) {
+        uint16_t hi = 0, lo = 0;
+        if (e < dim) {
+            hi = to_bf16(xr[e]);
+            lo = to_bf16(xr[e] - from_bf16(hi));
+        }
+        o[e] = hi;
+        o[kp + e] = order == 0 ? hi : lo;
+        o[2 * kp + e] = order == 0 ? lo : hi;
+    }
+}
+
 // A bound on |approximate - exact + const| for one query, const = ||q||^2 for
 // L2 (the approximate score leaves it out) and 0 for inner product. With
 // q~, x~ the BF16 values and s~ the FP32-accumulated q~ . x~:
@@ -550,6 +572,19 @@ int gemm_tuned_variant(int kind, int kp, int M, int N) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tuned.find(TuneKey{kind, kp, size_class(M), size_class(N)});
     return it == g_tuned.end() ? -1 : it->second;
+}
+
+void launch_split_bf16(const float* x, uint64_t n, int dim, int kp, int order, uint16_t* out, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_split_bf16, dim3(cdiv(n, 4)), dim3(256), 0, s, x, n, dim, kp, order, out);
+}
+
+void launch_coarse_split(const uint16_t* q3, int nq, const uint16_t* c3, int nlist, int kp, const float* csq, int ip,
+                         float* D, hipStream_t s) {
+    if (nq <= 0 || nlist <= 0) return;
+    EpiScores e{D, static_cast<size_t>(nlist), ip, 1, csq, nullptr, nullptr};
+    const size_t k3 = 3 * static_cast<size_t>(kp);
+    gemm_tuned<true>(kGemmScores, q3, nq, k3, c3, nlist, k3, static_cast<int>(k3), e, [] {}, s);
 }
 
 void launch_rows_bf16(const float* rows, uint64_t n, int dim, int kp, uint16_t* out, float* maxes, hipStream_t s) {

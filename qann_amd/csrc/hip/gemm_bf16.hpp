@@ -38,6 +38,17 @@ void launch_rows_bf16(const float* rows, uint64_t n, int dim, int kp, uint16_t* 
 // ||q||, ||bf16(q)|| and ||q - bf16(q)||.
 void launch_queries_bf16(const float* q, int nq, int dim, int kp, uint16_t* out, float* qinfo, hipStream_t s);
 
+// Rows as three BF16 segments of kp for the split coarse product (order 0:
+// [hi | hi | lo] for queries, order 1: [hi | lo | hi] for centroids), out
+// n * 3 kp.
+void launch_split_bf16(const float* x, uint64_t n, int dim, int kp, int order, uint16_t* out, hipStream_t s);
+
+// Coarse scores by the split product: D[i][l] = csq[l] - 2 s (or -s for inner
+// product) with s = hi.hi + hi.lo + lo.hi of query i and centroid l, one BF16
+// GEMM over K = 3 kp with FP32 accumulation.
+void launch_coarse_split(const uint16_t* q3, int nq, const uint16_t* c3, int nlist, int kp, const float* csq, int ip,
+                         float* D, hipStream_t s);
+
 // The arguments shared by the flat-search launches.
 struct FlatBf16 {
     const uint16_t* xb;    // rows, n * kp BF16

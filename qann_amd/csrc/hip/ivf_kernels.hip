@@ -257,7 +257,7 @@ constexpr int kSelectSample = 4096;
 template <int NT, int CD, bool SAMPLE>
 __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
                                                        int dim, int ip, float cnorm_max, int M, int cap,
-                                                       uint16_t* out_idx, float* out_val) {
+                                                       uint16_t* out_idx, float* out_val, int split_kp) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
@@ -282,9 +282,21 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
     const int target = std::min(cap, M + std::max(64, M / 4));
     const float u = 5.9604645e-8f, n = static_cast<float>(dim);
     const float qn = sqrtf(ss * (1.0f + 2.0f * (n + 2.0f) * u)) * 1.0001f;
-    const float delta = 2.0f * (ip ? 2.0f * (n + 1.0f) * u * qn * cnorm_max
-                                   : (2.0f * n + 5.0f) * u * (qn + cnorm_max) * (qn + cnorm_max)) +
-                        n * 2.3509887e-38f;
+    float delta = 2.0f * (ip ? 2.0f * (n + 1.0f) * u * qn * cnorm_max
+                             : (2.0f * n + 5.0f) * u * (qn + cnorm_max) * (qn + cnorm_max)) +
+                  n * 2.3509887e-38f;
+    if (split_kp > 0) {
+        // scores from the split BF16 product over K = 3 kp: the terms it leaves
+        // out (hi.r + lo.lo + lo.r + r.c) are at most 3.02 2^-16 |q||c|, its
+        // FP32 accumulation at most (3 kp + 2) u times 1.02 |q||c|; for L2 the
+        // rounding of |c|^2, of the subtraction and of the exact side add
+        // (2n + 6) u (|q| + |c|)^2, for inner product the exact side (n + 1) u |q||c|
+        const float k3 = 3.0f * static_cast<float>(split_kp);
+        const float es = (3.05f * 1.52587890625e-5f + (k3 + 2.0f) * u * 1.02f) * qn * cnorm_max;
+        delta = 2.0f * (ip ? es + (n + 1.0f) * u * qn * cnorm_max
+                           : 2.0f * es + (2.0f * n + 6.0f) * u * (qn + cnorm_max) * (qn + cnorm_max)) +
+                k3 * 2.3509887e-38f;
+    }
     // every list scoring at most `limit` into keys (up to cap of them)
     auto collect = [&](float v, int l, float limit) {
         if (v <= limit) {
@@ -2263,7 +2275,7 @@ void launch_coarse_gemm(const float* q, int nq, const float* cents, const float*
 }
 
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
-                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s) {
+                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp) {
     if (nq <= 0) return;
     const size_t cap = coarse_select_cap(M);
     const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
@@ -2279,24 +2291,24 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     if (nlist <= 16 * 256) {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<256, 16, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
         else
             hipLaunchKernelGGL((k_coarse_select<256, 16, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
     } else if (nlist <= 16 * 1024) {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<1024, 16, true>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
         else
             hipLaunchKernelGGL((k_coarse_select<1024, 16, false>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist,
-                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
     } else {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<256, 0, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
-                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
         else
             hipLaunchKernelGGL((k_coarse_select<256, 0, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
     }
 }
 

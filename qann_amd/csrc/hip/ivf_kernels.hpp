@@ -141,8 +141,10 @@ void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, 
 size_t coarse_select_cap(int M);
 void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
                         float* D, hipStream_t s, const uint64_t* excl = nullptr, const uint64_t* incl = nullptr);
+// split_kp > 0: D holds the split BF16 product's scores over K = 3 split_kp
+// (launch_coarse_split), and the shortlist bound is that product's.
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
-                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s);
+                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp = 0);
 // FlatIndex over D from launch_coarse_gemm (rows as the "centroids", masked by
 // excl / incl): the k nearest rows per query, exact, in (distance, slot) order.
 void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,

@@ -124,6 +124,25 @@ def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
             assert_same(got, want, f"{name} {form} {stem}")
 
 
+@pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
+def test_gpu_split_bf16_coarse_matches_goldens(monkeypatch, name):
+    # coarse scores by the split BF16 product (one BF16 MFMA GEMM over three
+    # times K) with its own shortlist bound: the same lists, so the goldens;
+    # small and large batches
+    monkeypatch.setenv("QANN_AMD_COARSE", "bf16")
+    path, q, exp, params = synth_case(name)
+    idx = qann_amd.load(str(path))
+    ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
+    reps = 2048 // q.shape[0] + 1
+    for qq, r in ((q, 1), (np.tile(q, (reps, 1)), reps)):
+        for npb, k, fname, allowed, stem in synth_queries(exp, params):
+            if npb is not None:
+                ivf.nprobe = npb
+            got = idx.batch_query(qq, k, allowed=allowed)
+            want = (np.tile(exp[stem + "_ids"], (r, 1)), np.tile(exp[stem + "_dists"], (r, 1)))
+            assert_same(got, want, f"{name} split coarse {stem}")
+
+
 @pytest.mark.parametrize("form", ["thread", "wave"])
 @pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
 def test_gpu_probe_plan_forms_match_goldens(monkeypatch, name, form):
