@@ -1786,7 +1786,7 @@ __device__ __forceinline__ void push_survivors(WaveTopK& tk, uint64_t rem, bool 
 constexpr int kPfRows = 8;
 
 // Block minima a windowed replay keeps in LDS at once.
-constexpr uint32_t kReplayWindow = 2048;
+constexpr uint32_t kReplayWindow = 512; // 2,048 measured 10% slower at 10k queries (fewer waves per CU)
 
 // The replay of fs_replay with the TopK always in LDS and its per-probe costs
 // cut (the same pushes in the same order, so the same TopK):
@@ -1877,9 +1877,9 @@ __device__ void fs_replay_win(const ReplayArgs& a) {
                 return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
             };
             // the window: restaged from this probe on when it does not hold the probe's blocks
-            if ((blk0 < win0 || blk0 + nb > win1) && nb <= kReplayWindow) {
+            if ((blk0 < win0 || blk0 + nb > win1) && nb <= static_cast<uint32_t>(a.window)) {
                 win0 = blk0;
-                win1 = min(blk0 + kReplayWindow, cend);
+                win1 = min(blk0 + static_cast<uint32_t>(a.window), cend);
                 wave_sync();
                 for (uint32_t i = static_cast<uint32_t>(lane); i < win1 - win0; i += kWave) sb[i] = gbmin[win0 + i];
                 wave_sync();
@@ -2505,10 +2505,17 @@ void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
     }
     // the window form, whenever its TopK and window fit LDS (its fixed arrays
     // take about 3.3 KB more)
-    const size_t wl = lds + static_cast<size_t>(kReplayWindow) * (a.fdists ? 4 : 2);
+    // the window's blocks (QANN_AMD_REPLAY_WINDOW overrides kReplayWindow)
+    ReplayArgs aw = a;
+    aw.window = static_cast<int>(kReplayWindow);
+    if (const char* w = std::getenv("QANN_AMD_REPLAY_WINDOW")) {
+        const int v = std::atoi(w);
+        if (v >= 64 && v <= 8192) aw.window = v;
+    }
+    const size_t wl = lds + static_cast<size_t>(aw.window) * (a.fdists ? 4 : 2);
     if ((forced < 0 || forced == 2) && !a.g_dist && wl + 4096 <= kLdsMax) {
-        if (a.fdists) hipLaunchKernelGGL(k_fl_replay_win, dim3(nq), dim3(kWave), wl, s, a);
-        else hipLaunchKernelGGL(k_fs_replay_win, dim3(nq), dim3(kWave), wl, s, a);
+        if (a.fdists) hipLaunchKernelGGL(k_fl_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
+        else hipLaunchKernelGGL(k_fs_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
         return;
     }
     // staged loads pay off while the batch leaves most of the chip's wave slots idle

@@ -118,6 +118,7 @@ struct ReplayArgs {
     int* out_count;            // nq
     float* g_dist;             // nq * 2kp TopK buffers in HBM when they do not fit LDS, else null
     int64_t* g_label;
+    int window;                // block minima the window replay keeps in LDS (set by the launcher)
 };
 
 // Largest LDS a kernel here asks for (the default dynamic limit).

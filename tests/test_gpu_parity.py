@@ -102,7 +102,7 @@ def test_gpu_pipelined_batch_matches_goldens(monkeypatch, name):
             assert_same((ids.cpu().numpy(), d.cpu().numpy()), want, f"{name} {stem} device rep {rep}")
 
 
-@pytest.mark.parametrize("form", ["plain", "stage", "win"])
+@pytest.mark.parametrize("form", ["plain", "stage", "win", "win64"])
 @pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
 def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
     # the TopK replay with its loads one block pair at a time, staged through
@@ -110,7 +110,10 @@ def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
     # bulk pushes (entries in the TopK, slots looked up at the end), each
     # forced, on every golden index: filtered, deleted, custom ids, PQ / SQ /
     # float lists; small and large batches
-    monkeypatch.setenv("QANN_AMD_REPLAY", form)
+    # (win64: windows of 64 block minima, restaged many times a query)
+    monkeypatch.setenv("QANN_AMD_REPLAY", "win" if form == "win64" else form)
+    if form == "win64":
+        monkeypatch.setenv("QANN_AMD_REPLAY_WINDOW", "64")
     path, q, exp, params = synth_case(name)
     idx = qann_amd.load(str(path))
     ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
