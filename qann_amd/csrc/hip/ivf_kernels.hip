@@ -1051,7 +1051,10 @@ __device__ __forceinline__ void ordered_item(const SumsArgs& a, int x, int i, in
     p0 = (i % a.nwin) * a.win;
 }
 
-__global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums(const SumsArgs a) {
+// OCC: waves per SIMD the register allocation leaves room for.
+constexpr int kSumsHighOccQueries = 1024;
+template <int OCC>
+__global__ void __launch_bounds__(kSumsWaves * kWave, OCC) k_fs_sums(const SumsArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t w_start[kSumsWin + 1];
     __shared__ int w_list[kSumsWin];
@@ -2382,7 +2385,16 @@ void sums_form(int form, const SumsArgs& a, int nq, size_t lds, hipStream_t s) {
         return;
     }
     const size_t grid = a.order ? static_cast<size_t>(8) * a.per_xcd * a.nwin : static_cast<size_t>(nq) * a.nwin;
-    hipLaunchKernelGGL(k_fs_sums, dim3(static_cast<unsigned>(grid)), dim3(kSumsWaves * kWave), lds, s, a);
+    // large batches: registers for six waves per SIMD, whose extra waves hide
+    // the table build's and the code loads' latency (12-15% faster at 10k
+    // queries); small ones keep four, whose longer code load runs suit them
+    // (QANN_AMD_SUMS_OCC=4|6 forces either)
+    const char* e = std::getenv("QANN_AMD_SUMS_OCC");
+    const int occ = e ? std::atoi(e) : (nq >= kSumsHighOccQueries ? 6 : 4);
+    if (occ == 6)
+        hipLaunchKernelGGL(k_fs_sums<6>, dim3(static_cast<unsigned>(grid)), dim3(kSumsWaves * kWave), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_fs_sums<4>, dim3(static_cast<unsigned>(grid)), dim3(kSumsWaves * kWave), lds, s, a);
 }
 
 } // namespace

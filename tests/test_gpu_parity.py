@@ -162,12 +162,15 @@ def test_gpu_probe_plan_forms_match_goldens(monkeypatch, name, form):
                     f"{name} {form} {stem}")
 
 
-@pytest.mark.parametrize("form", ["grid", "queue"])
+@pytest.mark.parametrize("form", ["grid", "grid4", "grid6", "queue"])
 @pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp"])
 def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
-    # phase 1 as one workgroup per (query, window) and as the persistent
-    # work-queue kernel, each forced, small and large (XCD-ordered) batches
-    monkeypatch.setenv("QANN_AMD_SUMS_FORM", form)
+    # phase 1 as one workgroup per (query, window) (registers for four or six
+    # waves per SIMD, or by batch size) and as the persistent work-queue
+    # kernel, each forced, small and large (XCD-ordered) batches
+    monkeypatch.setenv("QANN_AMD_SUMS_FORM", "queue" if form == "queue" else "grid")
+    if form in ("grid4", "grid6"):
+        monkeypatch.setenv("QANN_AMD_SUMS_OCC", form[-1])
     path, q, exp, params = synth_case(name)
     idx = qann_amd.load(str(path))
     ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
