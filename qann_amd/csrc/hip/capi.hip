@@ -656,6 +656,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             scorer.sq_step = ix.sq_step;
             scorer.sq_bits = ix.sq_bits;
             scorer.vbytes = ix.vbytes;
+            scorer.sq_chunked = ix.sq_chunked;
         } else {
             scorer.rows = reinterpret_cast<const float*>(ix.codes);
         }
@@ -1170,6 +1171,7 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
         ix.sub = pq ? static_cast<int>(dsc.dim / dsc.m) : 0;
         ix.cosine_ip = dsc.cosine_ip ? 1 : 0;
         ix.sq_bits = dsc.storage == QANN_IVF_SQ ? static_cast<int>(dsc.sq_bits) : 0;
+        ix.sq_chunked = ix.sq_bits == 8 && dsc.dim % 16 == 0 ? 1 : 0;
         ix.vbytes = dsc.storage == QANN_IVF_PQ     ? static_cast<int>(dsc.m)
                     : dsc.storage == QANN_IVF_SQ    ? static_cast<int>(dsc.sq_bits == 8 ? dsc.dim : (dsc.dim + 1) / 2)
                     : dsc.storage == QANN_IVF_FLOAT ? static_cast<int>(dsc.dim * 4)
@@ -1252,6 +1254,16 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
                             std::memcpy(db + (sp / 4) * 64 + g * 16 + (sp % 4) * 4, sb + sp * 16 + 4 * g, 4);
                 }
             }
+        } else if (ix.sq_chunked) {
+            // 8-bit SQ rows of a multiple of 16 dimensions: each group of 32
+            // entries chunk-major, chunk c of entry e at
+            // (e / 32) * 32 * dim + c * 512 + (e % 32) * 16 (sq_row)
+            const size_t vb = static_cast<size_t>(ix.vbytes);
+            packed.assign(std::max<uint64_t>((total + 31) / 32 * 32 * vb, 4), 0);
+            for (uint64_t e = 0; e < total; ++e)
+                for (size_t c = 0; c < vb / 16; ++c)
+                    std::memcpy(packed.data() + (e / 32) * 32 * vb + c * 512 + (e % 32) * 16, codes + e * vb + c * 16, 16);
+            std::fill(coff.begin(), coff.end(), 0);
         } else {
             // PQ / SQ codes or float rows: vbytes per entry, in entry order
             const size_t vb = static_cast<size_t>(ix.vbytes);

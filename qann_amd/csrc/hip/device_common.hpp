@@ -131,9 +131,11 @@ __device__ __forceinline__ float exact_score(bool ip, const float* a, const floa
 // dimensions per step of the vector loop.
 template <bool IP>
 __device__ __forceinline__ float sq_score(const float* q, const uint8_t* code, const float* vmin, const float* step,
-                                          int n, int bits) {
+                                          int n, int bits, int cstride = 16) {
+    // 8-bit code of dimension d at code[(d / 16) * cstride + d % 16] (sq_row)
     auto code_at = [&](int d) -> float {
-        return bits == 8 ? static_cast<float>(code[d]) : static_cast<float>((code[d >> 1] >> ((d & 1) * 4)) & 0x0F);
+        return bits == 8 ? static_cast<float>(code[(d >> 4) * cstride + (d & 15)])
+                         : static_cast<float>((code[d >> 1] >> ((d & 1) * 4)) & 0x0F);
     };
     float acc[8];
 #pragma unroll
@@ -173,6 +175,19 @@ __device__ __forceinline__ float sq_score(const float* q, const uint8_t* code, c
         }
     }
     return sum;
+}
+
+// The SQ codes of list entry e and the stride between its 16-byte chunks.
+// Rows of vbytes in entry order (stride 16: the row is contiguous), or, for
+// 8-bit codes of a multiple of 16 dimensions (chunked), each group of 32
+// entries stored chunk-major: chunk c of entry e at
+// (e / 32) * 32 * dim + c * 512 + (e % 32) * 16, so one 16-byte load per
+// lane of chunk c over consecutive entries reads contiguous bytes.
+__device__ __forceinline__ const uint8_t* sq_row(const uint8_t* codes, uint32_t e, int vbytes, int chunked,
+                                                 int& cstride) {
+    cstride = chunked ? 512 : 16;
+    return chunked ? codes + static_cast<size_t>(e >> 5) * 32 * vbytes + (e & 31) * 16
+                   : codes + static_cast<size_t>(e) * vbytes;
 }
 
 // Filter bit of a slot (detail::SlotFilter::test, id_map.hpp:27).

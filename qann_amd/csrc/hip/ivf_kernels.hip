@@ -1236,9 +1236,10 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
                         for (int sidx = 0; sidx < m; ++sidx) d += tb[sidx * K + code[sidx]];
                         dist = mode == kDistIp ? d + w_base[p] : d;
                     } else if (mode == kDistSq) {
-                        const uint8_t* code = ix.codes + static_cast<size_t>(entry) * ix.vbytes;
-                        dist = ix.metric == 2 ? -sq_score<true>(qs, code, sqv, sqv + dim, dim, ix.sq_bits)
-                                              : sq_score<false>(qs, code, sqv, sqv + dim, dim, ix.sq_bits);
+                        int cs;
+                        const uint8_t* code = sq_row(ix.codes, entry, ix.vbytes, ix.sq_chunked, cs);
+                        dist = ix.metric == 2 ? -sq_score<true>(qs, code, sqv, sqv + dim, dim, ix.sq_bits, cs)
+                                              : sq_score<false>(qs, code, sqv, sqv + dim, dim, ix.sq_bits, cs);
                     } else {
                         const float* row = reinterpret_cast<const float*>(ix.codes) + static_cast<size_t>(entry) * dim;
                         dist = exact_score(ix.metric == 2, qs, row, dim);
@@ -1253,6 +1254,168 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
         if (t < T && v == 0) {
             qbmin[t] = key;
             atomicMin(&w_min[p], key);
+        }
+    }
+    __syncthreads();
+    if (tid < np) a.item_min[item0 + tid] = w_min[tid];
+}
+
+// Phase 1 for 8-bit SQ lists stored chunk-major (sq_row): the same work item
+// and outputs as k_ivf_dists. Each lane scores kSq8Rows vectors, vector
+// v = lane % 32 of blocks g + 2r + lane / 32, one 16-byte chunk at a time: a
+// wave's load of chunk c covers 64 consecutive entries' chunks, contiguous
+// 512-byte runs, and every LDS read of q / vmin / step serves kSq8Rows
+// vectors. Each distance keeps SQ::distance's order (src/sq.cpp:126-211), as
+// sq_score does: 8 accumulators over dimensions d = l (mod 8),
+// v = fma(code, step, vmin), then (q - v)^2 or q * v fused, folded as
+// ((a0+a4)+(a2+a6)) + ((a1+a5)+(a3+a7)).
+constexpr int kSq8Rows = 2;
+template <bool IP>
+__global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t w_start[kSumsWin + 1];
+    __shared__ int w_list[kSumsWin];
+    __shared__ uint32_t w_min[kSumsWin];
+    __shared__ uint32_t w_off[kSumsWin], w_len[kSumsWin];
+    const DevIvf& ix = a.ix;
+    const int dim = ix.dim, win = a.win;
+    const int qi = blockIdx.x / a.nwin, p0 = (blockIdx.x % a.nwin) * win;
+    const int P = a.nprobe_q[qi];
+    if (p0 >= P) return;
+    const int np = min(win, P - p0);
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
+    float* qs = reinterpret_cast<float*>(lds); // q, vmin, step: dim each
+    float* vmin = qs + dim;
+    float* step = vmin + dim;
+    const float* q = a.q + static_cast<size_t>(qi) * dim;
+    for (int t = tid; t < dim; t += blockDim.x) {
+        qs[t] = q[t];
+        vmin[t] = ix.sq_vmin[t];
+        step[t] = ix.sq_step[t];
+    }
+    const uint32_t blk0 = a.item_blk[item0];
+    if (tid < np) {
+        const int list = a.probe_idx[item0 + tid];
+        const uint32_t off = ix.list_off[list], n = ix.list_off[list + 1] - off;
+        w_list[tid] = list;
+        w_off[tid] = off;
+        w_len[tid] = n;
+        w_min[tid] = 0xFFFFFFFFu;
+        const uint32_t rel = a.item_blk[item0 + tid] - blk0;
+        w_start[tid] = rel;
+        if (tid == np - 1) w_start[np] = rel + (n + kFsBlock - 1) / kFsBlock;
+    }
+    __syncthreads();
+
+    const uint32_t T = w_start[np];
+    float* qd = a.dists + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
+    uint32_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock) + blk0;
+    const int v = lane & 31;
+    constexpr uint32_t kGroup = 2 * kSq8Rows; // blocks of one wave's pass
+    for (uint32_t g = wave * kGroup; g < T; g += kSumsWaves * kGroup) {
+        const uint4* row[kSq8Rows];
+        uint32_t t[kSq8Rows], entry[kSq8Rows];
+        int pr[kSq8Rows];
+        bool ok[kSq8Rows];
+#pragma unroll
+        for (int r = 0; r < kSq8Rows; ++r) {
+            t[r] = g + 2 * r + (lane >> 5);
+            // a block past the window's end scores a valid row of the last block, unused
+            const uint32_t tt = min(t[r], T - 1);
+            int p = 0;
+#pragma unroll
+            for (int s2 = kSumsWin / 2; s2 > 0; s2 >>= 1) {
+                const int c = min(p + s2, np - 1);
+                p = w_start[c] <= tt ? c : p;
+            }
+            pr[r] = p;
+            const uint32_t n = w_len[p]; // >= 1: block tt is probe p's
+            const uint32_t j = (tt - w_start[p]) * kFsBlock + v;
+            entry[r] = w_off[p] + min(j, n - 1);
+            ok[r] = t[r] < T && j < n;
+            int cs;
+            row[r] = reinterpret_cast<const uint4*>(sq_row(ix.codes, entry[r], dim, 1, cs));
+        }
+        // every row's slot, then its filter or deleted bit, loaded together
+        uint32_t slot[kSq8Rows];
+#pragma unroll
+        for (int r = 0; r < kSq8Rows; ++r) slot[r] = ix.list_ids[entry[r]];
+        const uint64_t* bits = a.filter ? a.filter : ix.deleted;
+        if (bits) {
+            uint64_t word[kSq8Rows];
+#pragma unroll
+            for (int r = 0; r < kSq8Rows; ++r) word[r] = bits[(slot[r] == 0xFFFFFFFFu ? 0u : slot[r]) >> 6];
+#pragma unroll
+            for (int r = 0; r < kSq8Rows; ++r) {
+                const bool bit = (word[r] >> (slot[r] & 63)) & 1ull;
+                ok[r] = ok[r] && (a.filter ? bit : !bit);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kSq8Rows; ++r) ok[r] = ok[r] && slot[r] != 0xFFFFFFFFu;
+        float acc[kSq8Rows][8];
+#pragma unroll
+        for (int r = 0; r < kSq8Rows; ++r)
+#pragma unroll
+            for (int l = 0; l < 8; ++l) acc[r][l] = 0.0f;
+        // chunk c + 1 of every row is in flight while chunk c is scored (512
+        // bytes between chunks; the last pass reloads the last chunk, unused)
+        const int nc = dim / 16;
+        uint4 cw[kSq8Rows];
+#pragma unroll
+        for (int r = 0; r < kSq8Rows; ++r) cw[r] = row[r][0];
+        for (int c = 0; c < nc; ++c) {
+            uint4 nx[kSq8Rows];
+            const int cn = min(c + 1, nc - 1);
+#pragma unroll
+            for (int r = 0; r < kSq8Rows; ++r) nx[r] = row[r][cn * 32];
+            // keeps those loads here: without it they are sunk to the next pass's start
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int w4 = 0; w4 < 4; ++w4) {
+                const int d0 = c * 16 + w4 * 4;
+                const float4 q4 = *reinterpret_cast<const float4*>(qs + d0);
+                const float4 m4 = *reinterpret_cast<const float4*>(vmin + d0);
+                const float4 s4 = *reinterpret_cast<const float4*>(step + d0);
+                const float qv[4] = {q4.x, q4.y, q4.z, q4.w};
+                const float mv[4] = {m4.x, m4.y, m4.z, m4.w};
+                const float sv[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+                for (int r = 0; r < kSq8Rows; ++r) {
+                    const uint32_t word = w4 == 0 ? cw[r].x : w4 == 1 ? cw[r].y : w4 == 2 ? cw[r].z : cw[r].w;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int l = (w4 * 4 + b) & 7;
+                        const float val = fmaf(static_cast<float>((word >> (8 * b)) & 0xFFu), sv[b], mv[b]);
+                        if (IP) {
+                            acc[r][l] = fmaf(qv[b], val, acc[r][l]);
+                        } else {
+                            const float diff = qv[b] - val;
+                            acc[r][l] = fmaf(diff, diff, acc[r][l]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kSq8Rows; ++r) cw[r] = nx[r];
+        }
+#pragma unroll
+        for (int r = 0; r < kSq8Rows; ++r) {
+            float h[4];
+#pragma unroll
+            for (int l = 0; l < 4; ++l) h[l] = acc[r][l] + acc[r][l + 4];
+            const float sum = (h[0] + h[2]) + (h[1] + h[3]);
+            const float dist = ok[r] ? (IP ? -sum : sum) : INFINITY;
+            uint32_t key = ok[r] ? fkey(dist) : 0xFFFFFFFFu;
+            if (t[r] < T) qd[static_cast<size_t>(t[r]) * kFsBlock + v] = dist;
+            // the block's 32 lanes: their smallest key
+            for (int off = 16; off > 0; off >>= 1)
+                key = min(key, static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), off, 32)));
+            if (t[r] < T && v == 0) {
+                qbmin[t[r]] = key;
+                atomicMin(&w_min[pr[r]], key);
+            }
         }
     }
     __syncthreads();
@@ -2096,9 +2259,10 @@ __device__ __forceinline__ float slot_score(const SlotScorer& sc, bool ip, const
     const uint32_t e = sc.entry_of_slot ? sc.entry_of_slot[slot] : slot;
     if (e == 0xFFFFFFFFu) return INFINITY; // a slot in no list
     if (sc.sq_codes) {
-        const uint8_t* code = sc.sq_codes + static_cast<size_t>(e) * sc.vbytes;
-        return ip ? -sq_score<true>(q, code, sc.sq_vmin, sc.sq_step, dim, sc.sq_bits)
-                  : sq_score<false>(q, code, sc.sq_vmin, sc.sq_step, dim, sc.sq_bits);
+        int cs;
+        const uint8_t* code = sq_row(sc.sq_codes, e, sc.vbytes, sc.sq_chunked, cs);
+        return ip ? -sq_score<true>(q, code, sc.sq_vmin, sc.sq_step, dim, sc.sq_bits, cs)
+                  : sq_score<false>(q, code, sc.sq_vmin, sc.sq_step, dim, sc.sq_bits, cs);
     }
     return exact_score(ip, q, sc.rows + static_cast<size_t>(e) * dim, dim);
 }
@@ -2483,10 +2647,25 @@ size_t ivf_dists_lds(DistArgs& a) {
     return fixed;
 }
 
+// Chunk-major 8-bit SQ lists go to k_sq8_dists unless QANN_AMD_SQ8=thread
+// keeps them on k_ivf_dists (one thread per vector, one byte load per dimension).
+static bool sq8_rows_form(const DistArgs& a) {
+    if (a.mode != kDistSq || !a.ix.sq_chunked) return false;
+    const char* e = std::getenv("QANN_AMD_SQ8");
+    return !(e && std::strcmp(e, "thread") == 0);
+}
+
 void launch_ivf_dists(DistArgs a, int nq, hipStream_t s) {
     const size_t lds = ivf_dists_lds(a);
-    hipLaunchKernelGGL(k_ivf_dists, dim3(static_cast<unsigned>(static_cast<size_t>(nq) * a.nwin)),
-                       dim3(kSumsWaves * kWave), lds, s, a);
+    const dim3 grid(static_cast<unsigned>(static_cast<size_t>(nq) * a.nwin)), block(kSumsWaves * kWave);
+    if (sq8_rows_form(a)) {
+        if (a.ix.metric == 2)
+            hipLaunchKernelGGL(k_sq8_dists<true>, grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL(k_sq8_dists<false>, grid, block, lds, s, a);
+        return;
+    }
+    hipLaunchKernelGGL(k_ivf_dists, grid, block, lds, s, a);
 }
 
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,

@@ -24,6 +24,7 @@ struct DevIvf {
     int K;                         // PQ centroids per subspace
     int sq_bits;                   // SQ: 4 or 8
     int vbytes;                    // bytes per entry in codes for PQ / SQ / FLOAT storage
+    int sq_chunked;                // 1: 8-bit SQ rows stored chunk-major (sq_row, device_common.hpp)
     const float* sq_vmin;          // dim, SQ only
     const float* sq_step;          // dim, SQ only
     const uint32_t* entry_of_slot; // slot -> list entry (list_off[l] + j), PQ-less storages only
@@ -192,7 +193,7 @@ struct SlotScorer {
     const uint8_t* sq_codes;       // non-null: SQ scoring of sq_codes + entry * vbytes
     const float* sq_vmin;
     const float* sq_step;
-    int sq_bits, vbytes;
+    int sq_bits, vbytes, sq_chunked;
 };
 // Bytes of one query's selection buffers; in HBM (gbuf, nq of them) when above kLdsMax.
 size_t block_select_bytes(int k, bool by_label);

@@ -4,8 +4,8 @@ Rows: 100,000,000 x 128 float32, L2, from a 16,384-centre Gaussian mixture
 whose cluster sizes follow Zipf(1.1) (centres U[0,1)^128, sigma 0.05), drawn on
 the GPU with torch's seeded generator (seed 5) in chunks of 10M rows, rows then
 queries; 100,000 queries the same way. IVF nlist 65535 (the reference's
-largest), 4-bit PQ M=64 fast-scan (32 bytes per vector) trained on 4.19M of
-the rows, a RefineIndex (k_factor 10) whose 51.2 GB of rows stay in HBM, k 10,
+largest), 4-bit PQ M=64 fast-scan (32 bytes per vector; --storage sq8: 8-bit
+SQ, 128 bytes per vector) trained on 4.19M of the rows, a RefineIndex (k_factor 10) whose 51.2 GB of rows stay in HBM, k 10,
 nprobe 32 and 128. The 100,000 queries run as one batch with queries and
 results in HBM; recall@10 is measured on the first 1,000 against exact FP32
 search over all rows.
@@ -34,6 +34,8 @@ def main():
     ap.add_argument("--nlist", type=int, default=65535)
     ap.add_argument("--centres", type=int, default=16384)
     ap.add_argument("--m", type=int, default=64)
+    ap.add_argument("--storage", choices=("pq4fs", "sq8"), default="pq4fs",
+                    help="list storage: 4-bit PQ fast-scan blocks, or 8-bit SQ rows (dim bytes per vector)")
     ap.add_argument("--nprobes", default="32,128")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--chunk", type=int, default=10_000_000)
@@ -66,7 +68,10 @@ def main():
 
     t0 = time.perf_counter()
     ivf = qann_amd.IVFIndex(dim, args.nlist, 32)
-    ivf.enable_pq(args.m, fast_scan=True)
+    if args.storage == "sq8":
+        ivf.enable_sq(8)
+    else:
+        ivf.enable_pq(args.m, fast_scan=True)
     rng = np.random.default_rng(5)
     train_rows = min(args.n, 64 * args.nlist)
     ivf.train(x[np.sort(rng.choice(args.n, train_rows, replace=False))], max_iters=10, seed=5)
@@ -114,7 +119,12 @@ def main():
     cnt = (ctypes.c_uint64 * 32)()
     cents = torch.from_numpy(ivf.centroids()).cuda()
     csq = (cents * cents).sum(1)
-    lblocks = torch.from_numpy(((sizes + 31) // 32).astype(np.int64)).cuda()
+    # bytes the scan must read per probed list: its code blocks (fast-scan),
+    # or one dim-byte row per vector (SQ8)
+    if args.storage == "sq8":
+        lbytes = torch.from_numpy(sizes.astype(np.int64) * dim).cuda()
+    else:
+        lbytes = torch.from_numpy(((sizes + 31) // 32).astype(np.int64) * args.m * 16).cuda()
 
     dq = torch.from_numpy(q).cuda()
     ids = torch.empty((args.nq, k), dtype=torch.int64, device="cuda")
@@ -144,11 +154,10 @@ def main():
         lib.qann_hip_profile_read(ms, cnt, 1)
         stages = {lib.qann_hip_stage_name(i).decode(): {"ms": round(ms[i], 3), "launches": int(cnt[i])}
                   for i in range(32) if cnt[i]}
-        blocks = 0
+        scan_bytes = 0
         for s in range(0, args.nq, 1000):
             d2 = csq[None, :] - 2.0 * (dq[s:s + 1000] @ cents.T)
-            blocks += int(lblocks[d2.topk(npb, dim=1, largest=False).indices].sum().item())
-        scan_bytes = blocks * args.m * 16
+            scan_bytes += int(lbytes[d2.topk(npb, dim=1, largest=False).indices].sum().item())
         scan = stages.get("scan", {"ms": 0.0, "launches": 0})
         gbs = scan_bytes / (scan["ms"] * 1e-3) / 1e9 if scan["ms"] else 0.0
         results[str(npb)] = {"qps": round(args.nq / best, 1), "ms_per_batch": round(1e3 * best, 2),
@@ -158,10 +167,11 @@ def main():
             f"recall@10 {rec:.4f}")
         log("  stages (ms per batch, launches): " +
             ", ".join(f"{n} {v['ms']:.2f} ({v['launches']})" for n, v in stages.items()))
-        log(f"  scan: {scan_bytes / 1e9:.1f} GB of code blocks, {gbs:,.0f} GB/s over the scan launches' time")
+        log(f"  scan: {scan_bytes / 1e9:.1f} GB of codes, {gbs:,.0f} GB/s over the scan launches' time")
     print(json.dumps({
         "workload": f"{args.n:,} x {dim} L2, Zipf(1.1) {args.centres}-centre mixture; IVF{args.nlist} + "
-                    f"PQ{args.m}x4 fast-scan + RefineIndex(k_factor={kf}) in HBM; {args.nq:,}-query batch, k={k}",
+                    + ("SQ8" if args.storage == "sq8" else f"PQ{args.m}x4 fast-scan")
+                    + f" + RefineIndex(k_factor={kf}) in HBM; {args.nq:,}-query batch, k={k}",
         "results": results,
         "lists": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
     }), flush=True)

@@ -531,6 +531,10 @@ STORAGES = [
     (qann_amd.Metric.L2, 21, "sq8", 3),
     (qann_amd.Metric.Cosine, 19, "float", 0),
     (qann_amd.Metric.InnerProduct, 16, "float", 2),
+    # 8-bit SQ rows of a multiple of 16 bytes take the multi-row scan kernel
+    (qann_amd.Metric.L2, 64, "sq8", 0),
+    (qann_amd.Metric.InnerProduct, 96, "sq8", 3),
+    (qann_amd.Metric.Cosine, 128, "sq8", 0),
 ]
 
 
@@ -562,6 +566,34 @@ def test_gpu_built_storages_match_oracle(tmp_path, oracle, metric, dim, storage,
         allowed = rng.choice(8000, size, replace=False)
         assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed),
                     f"{storage} filtered {size}")
+
+
+@pytest.mark.parametrize("form", ["rows", "thread"])
+@pytest.mark.parametrize("metric", [qann_amd.Metric.L2, qann_amd.Metric.InnerProduct])
+def test_gpu_sq8_scan_forms_match_oracle(tmp_path, monkeypatch, oracle, metric, form):
+    # 8-bit SQ lists of 128-byte rows, scored four rows per lane (the default)
+    # or one byte load per dimension (QANN_AMD_SQ8=thread); lists of about 600
+    # rows, so a window spans many block groups; removed rows and filters
+    if form == "thread":
+        monkeypatch.setenv("QANN_AMD_SQ8", "thread")
+    rng = np.random.default_rng(128 + int(metric == qann_amd.Metric.InnerProduct))
+    x = mixture(rng, 60000, 128)
+    q = mixture(rng, 300, 128)
+    ivf = qann_amd.IVFIndex(128, 100, 8, metric=metric)
+    ivf.enable_sq(8)
+    ivf.train(x, seed=3)
+    ivf.add(x)
+    ivf.remove(rng.choice(60000, 500, replace=False))
+    path = tmp_path / "sq8.qann"
+    ivf.save(str(path))
+    orc = oracle.load(path)
+    for npb in (3, 20):
+        ivf.nprobe = npb
+        orc.set_nprobe(npb)
+        for k in (10, 100):
+            assert_same(ivf.batch_query(q, k), orc.batch_query(q, k), f"{form} nprobe {npb} k {k}")
+    allowed = rng.choice(60000, 24000, replace=False)
+    assert_same(ivf.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), f"{form} filtered")
 
 
 def test_gpu_stage_profiler_times_without_changing_results():
