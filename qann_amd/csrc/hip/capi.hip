@@ -510,15 +510,101 @@ __global__ void k_any_set(const int* status, int n, int* flag) {
 // that would need more sets *d_more; the caller then searches again with every
 // list it may need (at most 4096). Without it, M is that limit at once.
 // Coarse scores by the split BF16 product (hi.hi + hi.lo + lo.hi, one BF16
-// MFMA GEMM over three times K) instead of the FP32 MFMA GEMM, when
-// QANN_AMD_COARSE=bf16. Its proven error is about three times the FP32
-// GEMM's, so the exact shortlist is wider: where many lists crowd around a
-// query (Zipf-sized clusters split over many lists) queries overflow into
-// scoring every list exactly, so the FP32 GEMM stays the default.
-bool coarse_split(const qann_hip_index* ivfh) {
-    if (!ivfh->cent_split) return false;
+// MFMA GEMM over three times K) or by the FP32 MFMA GEMM. Both lead
+// k_coarse_select to the same probes. The split product's proven error is
+// about three times the FP32 GEMM's, so its exact shortlist is wider: where
+// many lists crowd around a query (Zipf-sized clusters split over many lists)
+// queries overflow into scoring every list exactly, and the FP32 GEMM is
+// faster; elsewhere the BF16 GEMM is. QANN_AMD_COARSE=bf16 or =fp32 forces a
+// form; otherwise the first search of a problem class times both (GEMM and
+// select) and the class keeps the faster (QANN_AMD_TUNE_LOG=1 prints it).
+// -1: not forced.
+int coarse_forced(const qann_hip_index* ivfh) {
+    if (!ivfh->cent_split) return 0;
     const char* e = std::getenv("QANN_AMD_COARSE");
-    return e && std::strcmp(e, "bf16") == 0;
+    if (e && std::strcmp(e, "bf16") == 0) return 1;
+    if (e && std::strcmp(e, "fp32") == 0) return 0;
+    return -1;
+}
+
+struct CoarseKey {
+    int nlist, dim, ip, M, nclass;
+    bool operator<(const CoarseKey& o) const {
+        return std::tie(nlist, dim, ip, M, nclass) < std::tie(o.nlist, o.dim, o.ip, o.M, o.nclass);
+    }
+};
+std::mutex g_coarse_mu;
+std::map<CoarseKey, int> g_coarse_form;
+
+// The coarse scores, by the FP32 GEMM (split false) or the split BF16 GEMM
+// (q3: the queries' split rows, scratch).
+void coarse_scores(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int ip, bool split, uint16_t* q3,
+                   float* D, hipStream_t st) {
+    const qd::DevIvf& ix = ivfh->ivf;
+    if (split) {
+        qd::launch_split_bf16(qr, nq, ix.dim, ivfh->split_kp, 0, q3, st);
+        qd::launch_coarse_split(q3, static_cast<int>(nq), ivfh->cent_split, ix.nlist, ivfh->split_kp,
+                                ix.centroid_sqnorms, ip, D, st);
+        check_launch("k_gemm_bf16 (coarse)");
+    } else {
+        qd::launch_coarse_gemm(qr, static_cast<int>(nq), ix.centroids, ix.centroid_sqnorms, ix.nlist, ix.dim, ip, D,
+                               st);
+        check_launch("k_coarse_gemm");
+    }
+}
+
+// The M nearest lists of each query from those scores (the bound of the form
+// that wrote them).
+void coarse_probes(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int ip, uint64_t M, bool split,
+                   float* D, uint16_t* pidx, float* pval, hipStream_t st) {
+    const qd::DevIvf& ix = ivfh->ivf;
+    qd::launch_coarse_select(D, qr, static_cast<int>(nq), ix.centroids, ix.cnorm_max, ix.nlist, ix.dim, ip,
+                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0);
+    check_launch("k_coarse_select");
+}
+
+// The coarse form of this search: forced, known for its class, or timed now
+// (both forms run on the real problem; their outputs are the same).
+bool coarse_split(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int ip, uint64_t M, uint16_t* q3,
+                  float* D, uint16_t* pidx, float* pval, hipStream_t st) {
+    const int forced = coarse_forced(ivfh);
+    if (forced >= 0) return forced == 1;
+    const qd::DevIvf& ix = ivfh->ivf;
+    int nclass = 0;
+    while ((uint64_t{1} << nclass) < nq) ++nclass;
+    const CoarseKey key{ix.nlist, ix.dim, ip, static_cast<int>(M), nclass};
+    {
+        std::lock_guard<std::mutex> g(g_coarse_mu);
+        auto it = g_coarse_form.find(key);
+        if (it != g_coarse_form.end()) return it->second == 1;
+    }
+    // the better of two runs of each form, alternated
+    float t[2] = {1e30f, 1e30f};
+    int form = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        for (int run = 0; run < 4; ++run) {
+            const int f = run & 1;
+            float ms = 0.0f;
+            (void)hipEventRecord(e0, st);
+            coarse_scores(ivfh, qr, nq, ip, f == 1, q3, D, st);
+            coarse_probes(ivfh, qr, nq, ip, M, f == 1, D, pidx, pval, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+                t[f] = std::min(t[f], ms);
+        }
+        form = t[1] < t[0] ? 1 : 0;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const char* log = std::getenv("QANN_AMD_TUNE_LOG");
+    if (log && *log && *log != '0')
+        std::fprintf(stderr, "qann_amd tune: coarse nq %llu nlist %d dim %d probes %llu -> %s (ms: %.3f fp32, %.3f bf16)\n",
+                     static_cast<unsigned long long>(nq), ix.nlist, ix.dim, static_cast<unsigned long long>(M),
+                     form ? "split bf16" : "fp32", t[0], t[1]);
+    std::lock_guard<std::mutex> g(g_coarse_mu);
+    g_coarse_form[key] = form;
+    return form == 1;
 }
 
 // One search on sc's stream. scan_wait / scan_done (fast-scan lists only, may
@@ -710,8 +796,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     size_t need = 0;
     need += ix.rotation ? pad256(nq * d * 4) : 0;
     need += pad256(nq * nlist * 4); // coarse distances
-    const bool split = coarse_split(ivfh);
-    need += split ? pad256(nq * 3 * static_cast<size_t>(ivfh->split_kp) * 2) : 0;
+    const bool may_split = coarse_forced(ivfh) != 0;
+    need += may_split ? pad256(nq * 3 * static_cast<size_t>(ivfh->split_kp) * 2) : 0;
     need += pad256(nq * M * 2) + 3 * pad256(nq * M * 4);
     need += use_pre || (ip_pq && !fs) ? pad256(nq * tsz * 4) : 0;
     need += ip_pq && fs ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
@@ -738,25 +824,15 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         qr = r;
     }
     float* D = sc.take<float>(nq * nlist);
-    uint16_t* q3 = split ? sc.take<uint16_t>(nq * 3 * static_cast<size_t>(ivfh->split_kp)) : nullptr;
-    StageTimer t_coarse(QANN_STAGE_COARSE, st);
-    if (split) {
-        qd::launch_split_bf16(qr, nq, ix.dim, ivfh->split_kp, 0, q3, st);
-        qd::launch_coarse_split(q3, static_cast<int>(nq), ivfh->cent_split, ix.nlist, ivfh->split_kp,
-                                ix.centroid_sqnorms, ip, D, st);
-        check_launch("k_gemm_bf16 (coarse)");
-    } else {
-        qd::launch_coarse_gemm(qr, static_cast<int>(nq), ix.centroids, ix.centroid_sqnorms, ix.nlist, ix.dim, ip, D,
-                               st);
-        check_launch("k_coarse_gemm");
-    }
-    t_coarse.done();
+    uint16_t* q3 = may_split ? sc.take<uint16_t>(nq * 3 * static_cast<size_t>(ivfh->split_kp)) : nullptr;
     uint16_t* pidx = sc.take<uint16_t>(nq * M);
     float* pval = sc.take<float>(nq * M);
+    const bool split = coarse_split(ivfh, qr, nq, ip, M, q3, D, pidx, pval, st);
+    StageTimer t_coarse(QANN_STAGE_COARSE, st);
+    coarse_scores(ivfh, qr, nq, ip, split, q3, D, st);
+    t_coarse.done();
     StageTimer t_sel(QANN_STAGE_SELECT, st);
-    qd::launch_coarse_select(D, qr, static_cast<int>(nq), ix.centroids, ix.cnorm_max, ix.nlist, ix.dim, ip,
-                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0);
-    check_launch("k_coarse_select");
+    coarse_probes(ivfh, qr, nq, ip, M, split, D, pidx, pval, st);
     float* pA = sc.take<float>(nq * M);
     float* pbase = sc.take<float>(nq * M);
     qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);

@@ -127,12 +127,14 @@ def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
             assert_same(got, want, f"{name} {form} {stem}")
 
 
+@pytest.mark.parametrize("form", ["bf16", "fp32"])
 @pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
-def test_gpu_split_bf16_coarse_matches_goldens(monkeypatch, name):
+def test_gpu_coarse_forms_match_goldens(monkeypatch, name, form):
     # coarse scores by the split BF16 product (one BF16 MFMA GEMM over three
-    # times K) with its own shortlist bound: the same lists, so the goldens;
-    # small and large batches
-    monkeypatch.setenv("QANN_AMD_COARSE", "bf16")
+    # times K) with its own shortlist bound, or by the FP32 GEMM, each forced
+    # (unforced, the first search of a shape times both): the same lists, so
+    # the goldens; small and large batches
+    monkeypatch.setenv("QANN_AMD_COARSE", form)
     path, q, exp, params = synth_case(name)
     idx = qann_amd.load(str(path))
     ivf = idx.base if isinstance(idx, qann_amd.RefineIndex) else idx
@@ -143,7 +145,7 @@ def test_gpu_split_bf16_coarse_matches_goldens(monkeypatch, name):
                 ivf.nprobe = npb
             got = idx.batch_query(qq, k, allowed=allowed)
             want = (np.tile(exp[stem + "_ids"], (r, 1)), np.tile(exp[stem + "_dists"], (r, 1)))
-            assert_same(got, want, f"{name} split coarse {stem}")
+            assert_same(got, want, f"{name} {form} coarse {stem}")
 
 
 @pytest.mark.parametrize("form", ["thread", "wave"])
