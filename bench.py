@@ -13,6 +13,11 @@ HBM (prepared as the reference prepares them) and the results left in HBM.
 The reported value is queries per second at the smallest swept nprobe whose
 recall@10 against exact search reaches 0.95.
 
+--workload cpu100k runs the CPU-runnable 100k x 128 configuration instead, and
+--workload sq8 the scale configuration's SQ8 variant at 20M rows (IVF16384,
+8-bit SQ lists, 20,000 queries, nprobe 32), each with the reference library
+timed on the host over the same saved index.
+
 Prints one JSON line (last line of stdout). Progress goes to stderr.
 """
 from __future__ import annotations
@@ -257,6 +262,148 @@ def run_cpu_config(args):
     }), flush=True)
 
 
+def run_sq8_config(args):
+    """The scale configuration's SQ8 variant at 20M rows (--workload sq8):
+    20,000,000 x 128 L2 rows from a 4,096-centre Gaussian mixture with Zipf(1.1)
+    cluster sizes (centres U[0,1)^128, sigma 0.05), drawn on the GPU with torch's
+    generator (seed 5) in chunks, rows then 20,000 queries; IVF nlist 16384
+    trained on 1,048,576 of the rows, lists stored as 8-bit SQ codes (128 bytes
+    per vector), a RefineIndex (k_factor 10) with the rows in HBM; nprobe 32,
+    k 10. GPU queries/s of the 20,000-query batch (queries and results in
+    HBM), the SQ8 scan kernel's algorithmic bytes (the codes of every probed
+    list) over its launch time, recall@10 of the first 1,000 queries against
+    exact search, and the reference library on the host over the same saved
+    index (ids and distance bits compared)."""
+    import torch
+    import qann_amd
+
+    qann_amd.set_num_threads(min(16, os.cpu_count() or 1))
+    n, nq, dim, nlist, centres_n, nprobe, chunk = 20_000_000, 20_000, 128, 16384, 4096, 32, 5_000_000
+    t0 = time.perf_counter()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    centres = torch.rand((centres_n, dim), generator=g, device="cuda")
+    w = torch.arange(1, centres_n + 1, device="cuda", dtype=torch.float64) ** -1.1
+    w = (w / w.sum()).float()
+
+    def draw(count):
+        out = np.empty((count, dim), np.float32)
+        for s in range(0, count, chunk):
+            e = min(count, s + chunk)
+            lab = torch.multinomial(w, e - s, replacement=True, generator=g)
+            out[s:e] = (centres[lab] + 0.05 * torch.randn((e - s, dim), generator=g, device="cuda")).cpu().numpy()
+        return out
+
+    x = draw(n)
+    q = draw(nq)
+    log(f"sq8: data {x.shape} + {q.shape} in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    ivf = qann_amd.IVFIndex(dim, nlist, nprobe)
+    ivf.enable_sq(8)
+    rng = np.random.default_rng(5)
+    ivf.train(x[np.sort(rng.choice(n, 64 * nlist, replace=False))], max_iters=10, seed=5)
+    index = qann_amd.RefineIndex(ivf, K_FACTOR)
+    for s in range(0, n, chunk):
+        index.add(x[s:s + chunk])
+    sizes = ivf.list_sizes()
+    log(f"sq8: index built in {time.perf_counter() - t0:.1f}s; lists median {int(np.median(sizes))}, "
+        f"largest {sizes.max()}")
+
+    dq = torch.from_numpy(q).cuda()
+    d_ids = torch.empty((nq, K), dtype=torch.int64, device="cuda")
+    d_dists = torch.empty((nq, K), dtype=torch.float32, device="cuda")
+    sptr = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        index.search_device(dq.data_ptr(), nq, K, d_ids.data_ptr(), d_dists.data_ptr(), sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - a)
+    total = time.perf_counter() - start
+    ms = np.array(times) * 1e3
+    qps = nq * args.steps / total
+    gpu_ids, gpu_d = d_ids.cpu().numpy(), d_dists.cpu().numpy()
+
+    # one more batch under the library's stage timers (HIP events on the search
+    # stream); the scan's bytes are the codes of the lists each query probes
+    lib = hip_lib()
+    stage_times(lib)
+    lib.qann_hip_profile_enable(1)
+    step()
+    torch.cuda.synchronize()
+    lib.qann_hip_profile_enable(0)
+    st = stage_times(lib)
+    scan_ms, scan_launches = st.get("scan", (0.0, 0))
+    cents = torch.from_numpy(ivf.centroids()).cuda()
+    csq = (cents * cents).sum(1)
+    lbytes = torch.from_numpy(sizes.astype(np.int64) * dim).cuda()
+    scan_bytes = 0
+    for s in range(0, nq, 1000):
+        d2 = csq[None, :] - 2.0 * (dq[s:s + 1000] @ cents.T)
+        scan_bytes += int(lbytes[d2.topk(nprobe, dim=1, largest=False).indices].sum().item())
+    achieved = scan_bytes / (scan_ms * 1e-3) / 1e9 if scan_ms else 0.0
+
+    # exact top-10 of the first 1,000 queries (FP32 on the GPU, rows in chunks)
+    nt = 1000
+    qt = dq[:nt]
+    best_d = torch.full((nt, K), float("inf"), device="cuda")
+    best_i = torch.zeros((nt, K), dtype=torch.int64, device="cuda")
+    for s in range(0, n, chunk):
+        xs = torch.from_numpy(x[s:s + chunk]).cuda()
+        d2 = (xs * xs).sum(1)[None, :] - 2.0 * (qt @ xs.T)
+        dd, ii = d2.topk(K, dim=1, largest=False)
+        cd, ci = torch.cat([best_d, dd], 1), torch.cat([best_i, ii + s], 1)
+        o = cd.topk(K, dim=1, largest=False).indices
+        best_d, best_i = torch.gather(cd, 1, o), torch.gather(ci, 1, o)
+        del xs, d2
+    rec = recall(gpu_ids[:nt], best_i.cpu().numpy())
+    log(f"sq8: {qps:,.0f} QPS, p50 {np.percentile(ms, 50):.2f} ms p99 {np.percentile(ms, 99):.2f} ms per "
+        f"{nq}-query batch, recall@10 {rec:.4f}; scan {scan_ms:.2f} ms in {scan_launches} launches, "
+        f"{scan_bytes / 1e9:.1f} GB, {achieved:,.0f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        del x
+        tmpdir = "/dev/shm" if os.path.isdir("/dev/shm") else None
+        with tempfile.TemporaryDirectory(dir=tmpdir) as tmp:
+            path = os.path.join(tmp, "sq8.qann")
+            index.save(path)
+            cpu = cpu_baseline(path, q, nprobe, gpu_ids, gpu_d, args.cpu_budget)
+            log(f"cpu baseline ({cpu['kind']}, {cpu['cores']} threads): {cpu['value']:,.0f} QPS; rows identical "
+                f"{cpu['parity_rows_identical']:.4f}, bit-exact {cpu['parity_rows_bit_exact']:.4f}")
+    print(json.dumps({
+        "metric": "QPS (20M×128 L2, IVF16384 + SQ8 + refine, nprobe 32) vs CPU; p50/p99 ms",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8 SQ codes, fp32 distances and re-rank",
+        "data": "synthetic (seeded Zipf(1.1) 4,096-centre Gaussian mixture, index trained here)",
+        "config": {"workload": "IVF16384 + SQ8 + RefineIndex(k_factor=10), 20M x 128 L2, Zipf(1.1) 4096-centre "
+                               "mixture, 20k-query batch, nprobe 32, k=10", "recall@10": round(rec, 4),
+                   "p50_ms_per_batch": round(float(np.percentile(ms, 50)), 3),
+                   "p99_ms_per_batch": round(float(np.percentile(ms, 99)), 3),
+                   "stage_ms_per_batch": {k2: round(v[0], 4) for k2, v in st.items() if v[1]}},
+        "roofline": {"bound": "hbm", "kernel": "k_sq8_dists", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_batch": scan_bytes, "launches_per_step": int(scan_launches),
+                     "launch_ms_total": round(scan_ms, 4)},
+        "cpu_baseline": cpu,
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,8 +415,9 @@ def main():
     ap.add_argument("--nprobe", type=int, default=0, help="skip the sweep and use this nprobe")
     ap.add_argument("--copy-check-mib", type=int, default=0,
                     help="after timing, copy this many MiB device to device (calibrates HBM byte counters)")
-    ap.add_argument("--workload", choices=("headline", "cpu100k"), default="headline",
-                    help="cpu100k: the CPU-runnable 100k x 128 configuration instead of the headline one")
+    ap.add_argument("--workload", choices=("headline", "cpu100k", "sq8"), default="headline",
+                    help="cpu100k: the CPU-runnable 100k x 128 configuration; sq8: 20M x 128 IVF16384 with SQ8 "
+                         "lists; instead of the headline one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the search path has no multi-GPU mode (the IVF lists would shard "
@@ -279,6 +427,9 @@ def main():
 
     if args.workload == "cpu100k":
         run_cpu_config(args)
+        return
+    if args.workload == "sq8":
+        run_sq8_config(args)
         return
 
     import torch
