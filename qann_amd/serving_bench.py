@@ -5,9 +5,11 @@ Index: 10,000,000 x 96 float32 L2 vectors from a 4,096-centre Gaussian mixture
 fast-scan; a RefineIndex (k_factor 10) whose raw vectors stay resident in HBM.
 Load: a Poisson stream of requests, each a batch of 1-256 queries (log-uniform),
 k from {10, 100}, nprobe from {16, 64}; 30% of requests carry an allowed-id
-set of 10% of the ids (one of eight seeded sets). The saturated throughput is
-measured first (requests offered far faster than served), then p50 / p99
-latency at 25%, 50% and 90% of it. recall@10 is measured on 1,000 queries
+set of 10% of the ids (one of eight seeded sets). The plain path is measured
+first on the same request mix: each request one batch_query of its own, one
+at a time, FP32 coarse GEMM, no graph capture. Then the serving loop's
+saturated throughput (requests offered far faster than served), then p50 /
+p99 latency at 25%, 50% and 90% of it. recall@10 is measured on 1,000 queries
 against exact search, and a sample of served results is checked against the
 same queries run alone.
 
@@ -18,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -90,6 +93,31 @@ def main():
     filters = [np.sort(frng.choice(ids_all, n // 10, replace=False)) for _ in range(8)]
     spec = dict(ks=[10, 100], nprobes=[16, 64], filtered_fraction=0.3, filters=filters, max_nq=256)
 
+    # the plain FP32 one-launch-per-request path on the same request mix: each
+    # request a batch_query of its own, one at a time, FP32 coarse GEMM, no
+    # graph capture, back to back (its saturated rate)
+    os.environ["QANN_AMD_COARSE"], os.environ["QANN_AMD_GRAPHS"] = "fp32", "0"
+    prng = np.random.default_rng(12)
+    lat, p_queries = [], 0
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < args.seconds / 2:
+        nq = int(np.clip(np.rint(np.exp(prng.random() * np.log(256))), 1, 256))
+        lo = int(prng.integers(0, pool.shape[0] - nq))
+        k, npb = int(prng.choice([10, 100])), int(prng.choice([16, 64]))
+        allowed = filters[int(prng.integers(0, 8))] if prng.random() < 0.3 else None
+        ivf.nprobe = npb
+        a = time.perf_counter()
+        index.batch_query(pool[lo:lo + nq], k, allowed=allowed)
+        lat.append(time.perf_counter() - a)
+        p_queries += nq
+    p_elapsed = time.perf_counter() - t_start
+    del os.environ["QANN_AMD_COARSE"], os.environ["QANN_AMD_GRAPHS"]
+    plain = {"qps": round(p_queries / p_elapsed, 1), "requests": len(lat),
+             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3)}
+    log(f"plain FP32, one launch per request: {plain['qps']:,.0f} queries/s over {plain['requests']} requests "
+        f"(p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms per request)")
+
     server = qann_amd.Server(index, workers=args.workers, max_batch=4096)
     # served results are each query's own: a sample checked against single calls
     checks = []
@@ -111,7 +139,8 @@ def main():
     mean_nq = float(np.mean(np.clip(np.rint(np.exp(np.random.default_rng(0).random(200_000) * np.log(256))), 1, 256)))
     sat = qann_amd.serve_load(server, pool, rate=20_000.0, seconds=args.seconds / 2, seed=11, **spec)
     sat_qps = sat["qps"]
-    log(f"saturated: {sat_qps:,.0f} queries/s over {sat['requests']} requests in {sat['launches']} launches")
+    log(f"saturated: {sat_qps:,.0f} queries/s over {sat['requests']} requests in {sat['launches']} launches "
+        f"({sat_qps / plain['qps']:.2f}x the plain path)")
     levels = {}
     for frac in (0.25, 0.5, 0.9):
         rate = frac * sat_qps / mean_nq
@@ -150,6 +179,8 @@ def main():
         "workload": f"serving: {n:,} x {dim} L2, IVF{nlist} + PQ{m}x4 fast-scan + RefineIndex(k_factor={kf}) in HBM; "
                     "Poisson requests of 1-256 queries, k in {10, 100}, nprobe in {16, 64}, 30% filtered at 10%",
         "saturated_qps": round(sat_qps, 1),
+        "plain_fp32_one_launch_per_request": plain,
+        "saturated_over_plain": round(sat_qps / plain["qps"], 2),
         "levels": levels,
         "recall@10": recall,
         "served_equal_to_single_calls": f"{same}/{len(checks)}",
