@@ -1359,19 +1359,8 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs
         for (int r = 0; r < kSq8Rows; ++r)
 #pragma unroll
             for (int l = 0; l < 8; ++l) acc[r][l] = 0.0f;
-        // chunk c + 1 of every row is in flight while chunk c is scored (512
-        // bytes between chunks; the last pass reloads the last chunk, unused)
-        const int nc = dim / 16;
-        uint4 cw[kSq8Rows];
-#pragma unroll
-        for (int r = 0; r < kSq8Rows; ++r) cw[r] = row[r][0];
-        for (int c = 0; c < nc; ++c) {
-            uint4 nx[kSq8Rows];
-            const int cn = min(c + 1, nc - 1);
-#pragma unroll
-            for (int r = 0; r < kSq8Rows; ++r) nx[r] = row[r][cn * 32];
-            // keeps those loads here: without it they are sunk to the next pass's start
-            asm volatile("" ::: "memory");
+        // 16 bytes of each row: chunk c, dimensions 16c .. 16c + 15
+        auto score = [&](const uint4 (&cw)[kSq8Rows], int c) {
 #pragma unroll
             for (int w4 = 0; w4 < 4; ++w4) {
                 const int d0 = c * 16 + w4 * 4;
@@ -1397,9 +1386,28 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs
                     }
                 }
             }
+        };
+        // two register buffers: chunk c + 1 of every row is in flight while
+        // chunk c is scored (512 bytes between chunks; the empty asm keeps each
+        // prefetch where it is, which would otherwise be sunk to its use; a
+        // last pass of an even count reloads the final chunk, unused)
+        const int nc = dim / 16;
+        uint4 ca[kSq8Rows], cb[kSq8Rows];
 #pragma unroll
-            for (int r = 0; r < kSq8Rows; ++r) cw[r] = nx[r];
+        for (int r = 0; r < kSq8Rows; ++r) ca[r] = row[r][0];
+        int c = 0;
+        for (; c + 2 <= nc; c += 2) {
+#pragma unroll
+            for (int r = 0; r < kSq8Rows; ++r) cb[r] = row[r][(c + 1) * 32];
+            asm volatile("" ::: "memory");
+            score(ca, c);
+            const int cn = min(c + 2, nc - 1);
+#pragma unroll
+            for (int r = 0; r < kSq8Rows; ++r) ca[r] = row[r][cn * 32];
+            asm volatile("" ::: "memory");
+            score(cb, c + 1);
         }
+        if (c < nc) score(ca, c); // an odd count's last chunk
 #pragma unroll
         for (int r = 0; r < kSq8Rows; ++r) {
             float h[4];
