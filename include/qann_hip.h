@@ -163,6 +163,25 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
                             uint32_t* out);
 
 /*
+ * The two integer stages of the fast-scan on their own, run by the same
+ * device code the search kernels use, so that each can be checked against the
+ * reference's functions directly. Host buffers; synchronous.
+ *
+ * qann_hip_fs_quantize_lut: detail::quantize_lut (src/fast_scan.cpp:22-46,
+ * as build_quantize_lut :223-254 calls it) of n tables of m x 16 floats
+ * (m <= 1024): m x 16 LUT bytes, the scale and the offset of each.
+ *
+ * qann_hip_fs_block_sums: detail::fast_scan_block (src/fast_scan.cpp:375-377,
+ * whose arithmetic fast_scan_block_scalar :256-262 spells out; the sums
+ * fast_scan_block_mask, :387-397, compares with its limit) of
+ * nblocks code blocks in the reference's layout (m x 16 bytes each,
+ * src/fast_scan.hpp:19-34), block b against LUT b (m x 16 bytes): 32 u16
+ * sums per block.
+ */
+int qann_hip_fs_quantize_lut(const float* tables, uint64_t n, uint32_t m, uint8_t* lut, float* scale, float* offset);
+int qann_hip_fs_block_sums(const uint8_t* luts, uint32_t m, const uint8_t* blocks, uint64_t nblocks, uint16_t* sums);
+
+/*
  * Per-stage kernel timing, the device counterpart of the reference's
  * compile-time section timers (src/profile.hpp: coarse, table, scan, drain,
  * rerank). QANN_STAGE_SCAN is the fast-scan sums kernel, QANN_STAGE_REPLAY the

@@ -68,6 +68,9 @@ struct Scratch {
         hipGraphExec_t exec = nullptr;
         char* base = nullptr;
         char* io = nullptr;
+        // data generations of the index and of its base at capture: a setter
+        // that replaces lists, ids or deleted flags makes the capture stale
+        uint64_t gen = 0, base_gen = 0;
     };
     std::map<std::tuple<uint64_t, uint32_t, uint32_t, uint32_t>, Graph> graphs;
     std::map<std::tuple<uint64_t, uint32_t, uint32_t, uint32_t>, int> graph_seen;
@@ -77,6 +80,9 @@ struct Scratch {
     hipEvent_t ev_in = nullptr;
     std::vector<hipEvent_t> ev_scan, ev_done;
     bool pipe_warm = false; // a first pipelined search runs its parts one after another
+    // held for a whole search on this Scratch: two host threads calling
+    // qann_hip_search_async on one stream share it one after the other
+    std::mutex mu;
 
     void drop_graphs() {
         for (auto& kv : graphs)
@@ -234,6 +240,8 @@ struct qann_hip_index {
     uint16_t* cent_split = nullptr;
     int split_kp = 0;
     qann_hip_index* base = nullptr;
+    // bumped by every setter that replaces device data a captured search reads
+    std::atomic<uint64_t> gen{0};
     std::vector<void*> allocs;
     uint64_t bytes = 0;
 
@@ -278,6 +286,30 @@ struct qann_hip_index {
 };
 
 namespace {
+
+// One fast-scan block from the reference's layout (m rows of 16 bytes,
+// src/fast_scan.hpp:19-28) to the device's: word (subspace s, column g) at
+// (s / 4) * 64 + g * 16 + (s % 4) * 4. dst holds m4 * 16 zeroed bytes.
+void pack_fs_block(const uint8_t* src, uint8_t* dst, size_t m) {
+    for (size_t sp = 0; sp < m; ++sp)
+        for (size_t g = 0; g < 4; ++g) std::memcpy(dst + (sp / 4) * 64 + g * 16 + (sp % 4) * 4, src + sp * 16 + 4 * g, 4);
+}
+
+// Device buffers freed when a synchronous helper call ends.
+struct TempBuffers {
+    std::vector<void*> ptrs;
+    template <class T>
+    T* get(size_t n) {
+        void* p = nullptr;
+        check(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 4)), "hipMalloc");
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~TempBuffers() {
+        (void)hipDeviceSynchronize();
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
 
 // Deleted flags (one byte per slot) as a bitmap.
 std::vector<uint64_t> to_bitmap(const uint8_t* flags, uint64_t slots, uint64_t& set) {
@@ -1144,13 +1176,15 @@ constexpr int kGraphAfter = 3;
 void run_graphed(qann_hip_index* h, const float* dq, uint64_t nr, const qann_hip_search_params& p, int64_t* dids,
                  float* dd, Scratch& io) {
     const auto key = std::make_tuple(nr, p.k, p.nprobe, p.k_factor);
+    const uint64_t gen = h->gen.load(), base_gen = h->base ? h->base->gen.load() : 0;
     auto it = io.graphs.find(key);
     if (it != io.graphs.end()) {
-        if (it->second.base == io.base && it->second.io == io.io) {
+        if (it->second.base == io.base && it->second.io == io.io && it->second.gen == gen &&
+            it->second.base_gen == base_gen) {
             check(hipGraphLaunch(it->second.exec, io.stream), "hipGraphLaunch");
             return;
         }
-        // captured over buffers since replaced: capture again
+        // captured over buffers or index data since replaced: capture again
         (void)hipGraphExecDestroy(it->second.exec);
         io.graphs.erase(it);
         io.graph_seen[key] = kGraphAfter;
@@ -1186,7 +1220,7 @@ void run_graphed(qann_hip_index* h, const float* dq, uint64_t nr, const qann_hip
         run_search(h, dq, nr, p, nullptr, dids, dd, io);
         return;
     }
-    io.graphs[key] = Scratch::Graph{ex, io.base, io.io};
+    io.graphs[key] = Scratch::Graph{ex, io.base, io.io, gen, base_gen};
     check(hipGraphLaunch(ex, io.stream), "hipGraphLaunch");
 }
 
@@ -1301,6 +1335,7 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
         qd::DevIvf& ix = h->ivf;
         const size_t nl = static_cast<size_t>(ix.nlist), m = static_cast<size_t>(ix.m);
         const size_t bb = m * 16, lb = static_cast<size_t>(ix.m4) * 16;
+        h->gen.fetch_add(1);
         std::vector<uint32_t> off(nl + 1, 0);
         std::vector<uint64_t> coff(nl, 0), src_off(nl, 0);
         uint64_t total = 0, cbytes = 0, src_bytes = 0;
@@ -1322,13 +1357,8 @@ int qann_hip_ivf_set_lists(qann_hip_index* h, const uint32_t* sizes, const uint3
             packed.assign(std::max<uint64_t>(cbytes, 1), 0);
             for (size_t l = 0; l < nl; ++l) {
                 const uint64_t nblk = (sizes[l] + 31) / 32;
-                for (uint64_t b = 0; b < nblk; ++b) {
-                    const uint8_t* sb = codes + src_off[l] + b * bb;
-                    uint8_t* db = packed.data() + coff[l] + b * lb;
-                    for (size_t sp = 0; sp < m; ++sp)
-                        for (size_t g = 0; g < 4; ++g)
-                            std::memcpy(db + (sp / 4) * 64 + g * 16 + (sp % 4) * 4, sb + sp * 16 + 4 * g, 4);
-                }
+                for (uint64_t b = 0; b < nblk; ++b)
+                    pack_fs_block(codes + src_off[l] + b * bb, packed.data() + coff[l] + b * lb, m);
             }
         } else if (ix.sq_chunked) {
             // 8-bit SQ rows of a multiple of 16 dimensions: each group of 32
@@ -1436,6 +1466,7 @@ int qann_hip_set_ids(qann_hip_index* h, uint64_t slots, const int64_t* labels, c
     return guarded([&] {
         if (!h) invalid("null index");
         if (slots > 0xFFFFFFFFull) invalid("an index holds at most 2^32 - 1 vectors");
+        h->gen.fetch_add(1);
         h->free_ptr(h->labels);
         h->free_ptr(h->deleted);
         h->labels = labels ? h->upload(labels, slots) : nullptr;
@@ -1538,6 +1569,7 @@ int qann_hip_search_async(qann_hip_index* h, const float* d_queries, uint64_t nq
                 sc = it->second;
             }
         }
+        std::lock_guard<std::mutex> g(sc->mu);
         search_any(h, d_queries, nq, *params, params->allowed_words, d_out_ids, d_out_dists, *sc);
     });
 }
@@ -1593,6 +1625,55 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
             check(hipStreamSynchronize(st), "hipStreamSynchronize");
             for (uint64_t i = 0; i < b; ++i) out[b0 + i] = hidx[i];
         }
+    });
+}
+
+int qann_hip_fs_quantize_lut(const float* tables, uint64_t n, uint32_t m, uint8_t* lut, float* scale, float* offset) {
+    return guarded([&] {
+        if (m == 0 || m > 1024) invalid("m must be in [1, 1024]");
+        if (n == 0) return;
+        if (n > 0x7FFFFFFFull) invalid("too many tables");
+        TempBuffers t;
+        const size_t tf = static_cast<size_t>(m) * 16;
+        float* dt = t.get<float>(n * tf);
+        float* nz = t.get<float>(tf);
+        uint8_t* dl = t.get<uint8_t>(n * tf);
+        float* ds = t.get<float>(n);
+        float* dof = t.get<float>(n);
+        check(hipMemcpy(dt, tables, n * tf * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        const std::vector<float> negzero(tf, -0.0f);
+        check(hipMemcpy(nz, negzero.data(), tf * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        qd::DevIvf ix{};
+        ix.m = static_cast<int>(m);
+        ix.m4 = (ix.m + 3) & ~3;
+        ix.B = dt;
+        ix.centroids = dt;
+        qd::launch_fs_lut_check(ix, static_cast<int>(n), nz, dl, ds, dof, nullptr);
+        check_launch("k_fs_lut_check");
+        check(hipMemcpy(lut, dl, n * tf, hipMemcpyDeviceToHost), "hipMemcpy");
+        check(hipMemcpy(scale, ds, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        check(hipMemcpy(offset, dof, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int qann_hip_fs_block_sums(const uint8_t* luts, uint32_t m, const uint8_t* blocks, uint64_t nblocks, uint16_t* sums) {
+    return guarded([&] {
+        if (m == 0 || m > 1024) invalid("m must be in [1, 1024]");
+        if (nblocks == 0) return;
+        if (nblocks > 0x7FFFFFFFull) invalid("too many blocks");
+        const size_t bb = static_cast<size_t>(m) * 16, m4 = (m + 3) & ~3u, lb = m4 * 16;
+        std::vector<uint8_t> packed(nblocks * lb, 0);
+        for (uint64_t b = 0; b < nblocks; ++b) pack_fs_block(blocks + b * bb, packed.data() + b * lb, m);
+        TempBuffers t;
+        uint8_t* dl = t.get<uint8_t>(nblocks * bb);
+        uint8_t* db = t.get<uint8_t>(nblocks * lb);
+        uint16_t* ds = t.get<uint16_t>(nblocks * qd::kFsBlock);
+        check(hipMemcpy(dl, luts, nblocks * bb, hipMemcpyHostToDevice), "hipMemcpy");
+        check(hipMemcpy(db, packed.data(), nblocks * lb, hipMemcpyHostToDevice), "hipMemcpy");
+        qd::launch_fs_block_check(dl, static_cast<int>(m), static_cast<int>(m4), db, static_cast<int>(nblocks), ds,
+                                  nullptr);
+        check_launch("k_fs_block_check");
+        check(hipMemcpy(sums, ds, nblocks * qd::kFsBlock * 2, hipMemcpyDeviceToHost), "hipMemcpy");
     });
 }
 

@@ -913,6 +913,44 @@ __device__ __forceinline__ void lookup8(uint32_t w, const uint4 r, uint32_t& lo8
 
 __device__ __forceinline__ uint4 load_code16(const uint8_t* p) { return *reinterpret_cast<const uint4*>(p); }
 
+// The u16 sums of one lane's eight vectors of a code block (fast_scan_block,
+// src/fast_scan.cpp:256-262): lane g of the block's four reads, at cp, column
+// g's 16 bytes of each group of four subspaces (n4 groups, 64 bytes apart),
+// and looks them up in the coded LUT rows at lut. Sums of vectors 4g..4g+3
+// come back as u16 pairs (4g, 4g+1) in l01 and (4g+2, 4g+3) in l23; those of
+// vectors 16+4g.. in h01 / h23.
+__device__ __forceinline__ void fs_block_sums(const uint8_t* cp, const uint8_t* lut, int n4, uint32_t& l01,
+                                              uint32_t& l23, uint32_t& h01, uint32_t& h23) {
+    uint32_t lo_a = 0, lo_b = 0, hi_a = 0, hi_b = 0; // u16 pairs: vectors (4g, 4g+2), (4g+1, 4g+3), +16
+    // 16 subspaces per step of the main loop, their four 16-byte loads in flight together
+    auto step4 = [&](const uint4 w, int s4) {
+        const uint8_t* rp = lut + s4 * 64;
+        uint32_t lo8 = 0, hi8 = 0; // four subspaces' bytes add to at most 4 * 63
+        lookup8(w.x, *reinterpret_cast<const uint4*>(rp), lo8, hi8);
+        lookup8(w.y, *reinterpret_cast<const uint4*>(rp + 16), lo8, hi8);
+        lookup8(w.z, *reinterpret_cast<const uint4*>(rp + 32), lo8, hi8);
+        lookup8(w.w, *reinterpret_cast<const uint4*>(rp + 48), lo8, hi8);
+        lo_a += lo8 & 0x00FF00FFu;
+        lo_b += __builtin_amdgcn_perm(0u, lo8, 0x0C030C01u);
+        hi_a += hi8 & 0x00FF00FFu;
+        hi_b += __builtin_amdgcn_perm(0u, hi8, 0x0C030C01u);
+    };
+    int s4 = 0;
+    for (; s4 + 4 <= n4; s4 += 4) {
+        const uint4 w0 = load_code16(cp + 64 * s4), w1 = load_code16(cp + 64 * (s4 + 1));
+        const uint4 w2 = load_code16(cp + 64 * (s4 + 2)), w3 = load_code16(cp + 64 * (s4 + 3));
+        step4(w0, s4);
+        step4(w1, s4 + 1);
+        step4(w2, s4 + 2);
+        step4(w3, s4 + 3);
+    }
+    for (; s4 < n4; ++s4) step4(load_code16(cp + 64 * s4), s4);
+    l01 = (lo_a & 0xFFFFu) | (lo_b << 16);
+    l23 = (lo_a >> 16) | (lo_b & 0xFFFF0000u);
+    h01 = (hi_a & 0xFFFFu) | (hi_b << 16);
+    h23 = (hi_a >> 16) | (hi_b & 0xFFFF0000u);
+}
+
 // Phase 1: one workgroup of kSumsWaves waves per (query, window of up to
 // kSumsWin consecutive probes). The waves build the window's LUTs in LDS (one
 // per probe for L2 tables; the query's one inner-product LUT otherwise), then
@@ -992,32 +1030,8 @@ __device__ __forceinline__ void sums_item(const SumsArgs& a, int qi, int p0, uns
             const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
             const uint8_t* cp = ix.codes + ix.code_off[list] + static_cast<size_t>(b) * lb + g * 16;
             const uint8_t* lut = luts + (ip ? 0 : static_cast<size_t>(p) * lb);
-            uint32_t lo_a = 0, lo_b = 0, hi_a = 0, hi_b = 0; // u16 pairs: vectors (4g, 4g+2), (4g+1, 4g+3), +16
-            // 16 subspaces per step of the main loop, their four 16-byte loads in flight together
-            auto step4 = [&](const uint4 w, int s4) {
-                const uint8_t* rp = lut + s4 * 64;
-                uint32_t lo8 = 0, hi8 = 0; // four subspaces' bytes add to at most 4 * 63
-                lookup8(w.x, *reinterpret_cast<const uint4*>(rp), lo8, hi8);
-                lookup8(w.y, *reinterpret_cast<const uint4*>(rp + 16), lo8, hi8);
-                lookup8(w.z, *reinterpret_cast<const uint4*>(rp + 32), lo8, hi8);
-                lookup8(w.w, *reinterpret_cast<const uint4*>(rp + 48), lo8, hi8);
-                lo_a += lo8 & 0x00FF00FFu;
-                lo_b += __builtin_amdgcn_perm(0u, lo8, 0x0C030C01u);
-                hi_a += hi8 & 0x00FF00FFu;
-                hi_b += __builtin_amdgcn_perm(0u, hi8, 0x0C030C01u);
-            };
-            int s4 = 0;
-            for (; s4 + 4 <= n4; s4 += 4) {
-                const uint4 w0 = load_code16(cp + 64 * s4), w1 = load_code16(cp + 64 * (s4 + 1));
-                const uint4 w2 = load_code16(cp + 64 * (s4 + 2)), w3 = load_code16(cp + 64 * (s4 + 3));
-                step4(w0, s4);
-                step4(w1, s4 + 1);
-                step4(w2, s4 + 2);
-                step4(w3, s4 + 3);
-            }
-            for (; s4 < n4; ++s4) step4(load_code16(cp + 64 * s4), s4);
-            const uint32_t l01 = (lo_a & 0xFFFFu) | (lo_b << 16), l23 = (lo_a >> 16) | (lo_b & 0xFFFF0000u);
-            const uint32_t h01 = (hi_a & 0xFFFFu) | (hi_b << 16), h23 = (hi_a >> 16) | (hi_b & 0xFFFF0000u);
+            uint32_t l01, l23, h01, h23;
+            fs_block_sums(cp, lut, n4, l01, l23, h01, h23);
             uint16_t* o = qsums + static_cast<size_t>(t) * kFsBlock;
             *reinterpret_cast<uint2*>(o + 4 * g) = make_uint2(l01, l23);
             *reinterpret_cast<uint2*>(o + 16 + 4 * g) = make_uint2(h01, h23);
@@ -1113,6 +1127,51 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_fs_sums_queue(const Sums
         ordered_item(a, x, i, qi, p0, ok);
         if (ok) sums_item(a, qi, p0, lds, w_start, w_list, w_min);
         __syncthreads();
+    }
+}
+
+// quantize_lut (src/fast_scan.cpp:22-46) of table t (m x 16 floats at
+// ix.B + t * m * 16) by the search kernels' own LUT build, one wave per
+// table: C is m x 16 entries of -0 and A is -0, so the entries reach the
+// rounding unchanged (x + -0 = x, zeros keep their sign). The LUT goes out
+// decoded (plain bytes), with its scale and offset.
+__global__ void __launch_bounds__(kWave) k_fs_lut_check(const DevIvf ix, const float* negzero, uint8_t* lut_out,
+                                                      float* scale_out, float* offset_out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    uint8_t* lut = lds;
+    float* rowmin = reinterpret_cast<float*>(lds + static_cast<size_t>(ix.m4) * 16);
+    const int t = blockIdx.x, lane = lane_id();
+    float scale, offset;
+    wave_lut_l2(ix, nullptr, t, -0.0f, negzero, rowmin, nullptr, lut, scale, offset);
+    wave_lds_sync();
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(lut);
+    uint32_t* out = reinterpret_cast<uint32_t*>(lut_out + static_cast<size_t>(t) * ix.m * 16);
+    for (int e = lane; e < ix.m * 4; e += kWave) out[e] = lut_code_word(words[e], e);
+    if (lane == 0) {
+        scale_out[t] = scale;
+        offset_out[t] = offset;
+    }
+}
+
+// fast_scan_block (src/fast_scan.cpp:256-262) of block b (the device layout,
+// m4 * 16 bytes at blocks + b * m4 * 16) against LUT b (m x 16 plain bytes at
+// luts + b * m * 16) by the search kernels' own lookups: the LUT is coded into
+// LDS as k_fs_sums keeps it, and the block's four lanes each sum their eight
+// vectors. out: 32 u16 sums per block.
+__global__ void __launch_bounds__(kWave) k_fs_block_check(const uint8_t* luts, int m, int m4, const uint8_t* blocks,
+                                                        uint16_t* out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int b = blockIdx.x, lane = lane_id();
+    uint32_t* words = reinterpret_cast<uint32_t*>(lds);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(luts + static_cast<size_t>(b) * m * 16);
+    for (int e = lane; e < m4 * 4; e += kWave) words[e] = lut_code_word(e < m * 4 ? src[e] : 0u, e);
+    wave_lds_sync();
+    if (lane < 4) {
+        uint32_t l01, l23, h01, h23;
+        fs_block_sums(blocks + static_cast<size_t>(b) * m4 * 16 + lane * 16, lds, m4 >> 2, l01, l23, h01, h23);
+        uint16_t* o = out + static_cast<size_t>(b) * kFsBlock;
+        *reinterpret_cast<uint2*>(o + 4 * lane) = make_uint2(l01, l23);
+        *reinterpret_cast<uint2*>(o + 16 + 4 * lane) = make_uint2(h01, h23);
     }
 }
 
@@ -2631,6 +2690,18 @@ void launch_query_order(const uint16_t* probe_idx, int M, int nq, int nlist, uin
     hipLaunchKernelGGL(k_order_count, dim3(cdiv(nq, 256)), dim3(256), 0, s, probe_idx, M, nq, hist);
     hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, s, hist, nlist);
     hipLaunchKernelGGL(k_order_scatter, dim3(cdiv(nq, 256)), dim3(256), 0, s, probe_idx, M, nq, hist, order);
+}
+
+void launch_fs_lut_check(const DevIvf& ix, int n, const float* negzero, uint8_t* lut, float* scale, float* offset,
+                         hipStream_t s) {
+    const size_t lds = static_cast<size_t>(ix.m4) * 16 + static_cast<size_t>(ix.m) * 4;
+    hipLaunchKernelGGL(k_fs_lut_check, dim3(n), dim3(kWave), lds, s, ix, negzero, lut, scale, offset);
+}
+
+void launch_fs_block_check(const uint8_t* luts, int m, int m4, const uint8_t* blocks, int nblocks, uint16_t* out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_fs_block_check, dim3(nblocks), dim3(kWave), static_cast<size_t>(m4) * 16, s, luts, m, m4,
+                       blocks, out);
 }
 
 size_t ivf_dists_lds(DistArgs& a) {

@@ -176,6 +176,15 @@ void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_prob
                        int* status, hipStream_t s);
 void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s);
 void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s);
+// The LUT build and the block lookups of k_fs_sums on their own, for checking
+// them against quantize_lut and fast_scan_block: n tables of m x 16 floats at
+// ix.B (negzero: m x 16 floats of -0) rounded to plain LUT bytes with their
+// scales and offsets; nblocks blocks (device layout) summed against one LUT
+// each (plain bytes), 32 u16 sums per block.
+void launch_fs_lut_check(const DevIvf& ix, int n, const float* negzero, uint8_t* lut, float* scale, float* offset,
+                         hipStream_t s);
+void launch_fs_block_check(const uint8_t* luts, int m, int m4, const uint8_t* blocks, int nblocks, uint16_t* out,
+                           hipStream_t s);
 // Dynamic LDS of k_ivf_dists (0 when a table does not fit); sets win, nwin and resid_in_lds.
 size_t ivf_dists_lds(DistArgs& a);
 void launch_ivf_dists(DistArgs a, int nq, hipStream_t s);

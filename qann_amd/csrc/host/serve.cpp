@@ -39,6 +39,9 @@ Server::Server(std::shared_ptr<DenseIndex> index, int workers, uint64_t max_batc
     uint64_t free_bytes = 0, total = 0;
     check_status(qann_hip_device_memory(&free_bytes, &total));
     budget_ = static_cast<uint64_t>(static_cast<double>(free_bytes) * memory_fraction);
+    // an index that cannot be searched (an untrained IVF, an unsupported
+    // scoring mode) raises here rather than in a worker
+    (void)estimate(1, 1, 0, false);
     for (int w = 0; w < workers; ++w) workers_.emplace_back([this] { run(); });
 }
 
@@ -87,7 +90,16 @@ void Server::run() {
             continue;
         }
         const ServeRequest& head = queue_.front().req;
-        uint64_t bytes = estimate(head.nq, head.k, head.nprobe, head.filter != nullptr);
+        uint64_t bytes = 0;
+        try {
+            bytes = estimate(head.nq, head.k, head.nprobe, head.filter != nullptr);
+        } catch (...) {
+            // the request cannot be planned: it fails alone, the loop goes on
+            Pending bad = std::move(queue_.front());
+            queue_.pop_front();
+            bad.promise.set_exception(std::current_exception());
+            continue;
+        }
         // admit by free memory: a launch waits while others hold the budget
         if (inflight_ > 0 && inflight_ + bytes > budget_) {
             cv_.wait(lk);
@@ -103,7 +115,12 @@ void Server::run() {
             for (auto it = queue_.begin(); it != queue_.end() && nq < max_batch_;) {
                 const ServeRequest& r = it->req;
                 if (!r.filter && r.k == k && r.nprobe == nprobe && nq + r.nq <= max_batch_) {
-                    const uint64_t b = estimate(nq + r.nq, k, nprobe, false);
+                    uint64_t b = 0;
+                    try {
+                        b = estimate(nq + r.nq, k, nprobe, false);
+                    } catch (...) {
+                        break; // the launch keeps what it has; the rest is planned again later
+                    }
                     if (inflight_ > 0 && inflight_ + b > budget_) break;
                     nq += r.nq;
                     bytes = b;
@@ -191,6 +208,7 @@ ServeStats run_load(Server& server, const float* pool, uint64_t pool_rows, const
     if (spec.rate <= 0 || spec.seconds <= 0) throw std::invalid_argument("rate and seconds must be positive");
     if (spec.ks.empty() || spec.nprobes.empty()) throw std::invalid_argument("ks and nprobes must not be empty");
     if (spec.filtered_fraction > 0 && spec.filters.empty()) throw std::invalid_argument("filtered requests need filters");
+    if (pool_rows == 0) throw std::invalid_argument("the query pool is empty");
     const uint64_t d = server.index().dim();
     const uint32_t max_nq = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint32_t>(spec.max_nq, 1), pool_rows));
     std::mt19937_64 rng(spec.seed);
