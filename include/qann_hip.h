@@ -153,6 +153,26 @@ int qann_hip_search_async(qann_hip_index* index, const float* d_queries, uint64_
                           const qann_hip_search_params* params, int64_t* d_out_ids, float* d_out_dists, void* stream);
 
 /*
+ * qann_hip_search_async that allocates no device memory (no hipMalloc, no
+ * hipFree, no allocating library call). No counterpart in the reference: it is
+ * for callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it. The caller still passes the
+ * queries, results and allowed words in device memory; the search's working
+ * memory is one scratch array of qann_hip_noalloc_scratch_bytes() bytes (64 MiB)
+ * that each GPU holds from library load until the process ends. Searches that
+ * need more than it holds (too many queries at once, very large k * k_factor
+ * or filters) are refused with QANN_EINVAL and a message giving the size
+ * needed; the sums of large batches are taken in chunks that fit. IVF and
+ * RefineIndex handles only (QANN_EINVAL for a FlatIndex); the index's GPU must
+ * be current. Every call on one GPU shares that GPU's array, which keeps the
+ * last call's contents: calls that overlap on one GPU race, so run them on one
+ * stream or one after another. Results equal qann_hip_search_async's.
+ */
+int qann_hip_search_noalloc(qann_hip_index* index, const float* d_queries, uint64_t nq,
+                            const qann_hip_search_params* params, int64_t* d_out_ids, float* d_out_dists, void* stream);
+uint64_t qann_hip_noalloc_scratch_bytes(void);
+
+/*
  * Nearest centroid of each of n rows (detail::assign_nearest without BLAS,
  * src/kmeans.cpp:34-100 / nearest_centroid): the smallest exact squared L2
  * distance, the lowest centroid index on ties. Used when vectors are added

@@ -83,6 +83,14 @@ struct Scratch {
     // held for a whole search on this Scratch: two host threads calling
     // qann_hip_search_async on one stream share it one after the other
     std::mutex mu;
+    // base / cap are a buffer the Scratch does not own and may not grow (the
+    // allocation-free search's): a plan that needs more is refused
+    bool fixed = false;
+    [[noreturn]] void too_small(size_t bytes) const {
+        throw Error(QANN_EINVAL, "this search needs " + std::to_string(bytes) + " bytes of scratch; the allocation-free "
+                                 "search holds " + std::to_string(cap) + " (search fewer queries at a time, or use "
+                                 "qann_hip_search_async)");
+    }
 
     void drop_graphs() {
         for (auto& kv : graphs)
@@ -127,6 +135,7 @@ struct Scratch {
 
     void reserve(size_t bytes) {
         if (bytes <= cap) return;
+        if (fixed) too_small(bytes);
         if (base) {
             check(hipStreamSynchronize(stream), "hipStreamSynchronize");
             check(hipFree(base), "hipFree");
@@ -821,9 +830,9 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];
     const uint64_t stride = std::max<uint64_t>(wblocks, 1) * qd::kFsBlock;
     const uint64_t vbytes = fs ? 2 : 4; // u16 sums or float distances per vector
-    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * vbytes)));
+    uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(nq, g_sums_budget / (stride * vbytes)));
 
-    // scratch plan
+    // scratch plan: what every query holds, then the sums of one chunk of queries
     const size_t d = ix.dim;
     size_t need = 0;
     need += ix.rotation ? pad256(nq * d * 4) : 0;
@@ -836,14 +845,23 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += filtered ? pad256(nlist * 4) : 0;
     need += 2 * pad256(nq * 4) + pad256(nq * M * 4);    // plan
     need += 3 * pad256(nq * M * 4);                     // per-probe scale, offset, min
-    need += pad256(chunk * stride * vbytes);                 // sums or distances
-    need += pad256(chunk * (stride / qd::kFsBlock) * vbytes); // block minimums
     need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + pad256(nq * 4);
     need += gtopk ? pad256(nq * 2 * kp * 4) + pad256(nq * 2 * kp * 8) : 0;
     need += grerank ? pad256(nq * rkeys * 8) : 0;
+    need += fs ? pad256(nlist * 4) : 0; // the query order's histogram
+    need += fs ? pad256(8 * 4) : 0;     // work-queue counters of the persistent sums kernel
+    if (sc.fixed) {
+        // a scratch of fixed size: chunks as large as what is left of it holds
+        // (each query of a chunk: its sums or distances, block minimums, order entry)
+        const uint64_t per_query = stride * vbytes + (stride / qd::kFsBlock) * vbytes + (fs ? 4 : 0);
+        const uint64_t left = sc.cap > need + 3 * 256 ? sc.cap - need - 3 * 256 : 0;
+        chunk = std::min<uint64_t>(chunk, left / per_query);
+        if (chunk == 0) sc.too_small(need + 3 * 256 + per_query);
+    }
+    need += pad256(chunk * stride * vbytes);                  // sums or distances
+    need += pad256(chunk * (stride / qd::kFsBlock) * vbytes); // block minimums
     const bool order_queries = fs && chunk >= kOrderMinQueries;
-    need += order_queries ? pad256(nlist * 4) + pad256(chunk * 4) : 0;
-    need += fs ? pad256(8 * 4) : 0; // work-queue counters of the persistent sums kernel
+    need += order_queries ? pad256(chunk * 4) : 0;
     sc.used = 0;
     sc.reserve(need);
     const float* qr = d_q;
@@ -1571,6 +1589,59 @@ int qann_hip_search_async(qann_hip_index* h, const float* d_queries, uint64_t nq
         }
         std::lock_guard<std::mutex> g(sc->mu);
         search_any(h, d_queries, nq, *params, params->allowed_words, d_out_ids, d_out_dists, *sc);
+    });
+}
+
+}  // extern "C"
+
+// The allocation-free search's scratch. Each GPU the library runs on holds its
+// own copy of it until the process ends; host code reaches a device's copy only
+// through the address hipGetSymbolAddress gives for that device (the array's
+// name is a host-side stand-in). Calls that overlap on one GPU share it, so
+// they race: a caller runs them one at a time per GPU (one stream, or waits).
+static constexpr size_t kNoAllocScratchBytes = size_t{64} << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_scratch[kMaxDevices] = {};
+
+// The scratch's address on the current device, looked up once per device.
+unsigned char* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(QANN_EDEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (!g_noalloc_scratch[device]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+        if (!p) throw Error(QANN_EDEVICE, "hipGetSymbolAddress gave no address for the scratch");
+        g_noalloc_scratch[device] = static_cast<unsigned char*>(p);
+    }
+    return g_noalloc_scratch[device];
+}
+
+} // namespace
+
+extern "C" {
+
+uint64_t qann_hip_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int qann_hip_search_noalloc(qann_hip_index* h, const float* d_queries, uint64_t nq,
+                            const qann_hip_search_params* params, int64_t* d_out_ids, float* d_out_dists,
+                            void* stream) {
+    return guarded([&] {
+        if (!h || !params) invalid("null argument");
+        if (h->kind == 1) invalid("the allocation-free search takes IVF and RefineIndex handles");
+        int cur = -1;
+        check(hipGetDevice(&cur), "hipGetDevice");
+        if (cur != h->device) invalid("the allocation-free search runs with the index's GPU current");
+        Scratch sc;
+        sc.stream = static_cast<hipStream_t>(stream);
+        sc.base = reinterpret_cast<char*>(noalloc_scratch(cur));
+        sc.cap = kNoAllocScratchBytes;
+        sc.fixed = true;
+        run_search(h, d_queries, nq, *params, params->allowed_words, d_out_ids, d_out_dists, sc);
     });
 }
 

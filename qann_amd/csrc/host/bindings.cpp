@@ -251,6 +251,12 @@ PYBIND11_MODULE(_qann, m) {
              py::arg("stream") = 0,
              "batch_query on prepared queries already in device memory (pointers as ints); results stay on the "
              "device, enqueued on the given HIP stream. Distances are the internal form (not halved for Cosine)")
+        .def("device_handle",
+             [](const DenseIndex& self) {
+                 return read_locked(self, [&] { return reinterpret_cast<uintptr_t>(self.device_handle()); });
+             },
+             "The qann_hip_index* of the index's device copy as an int, for calling include/qann_hip.h "
+             "directly; valid until the index is changed or freed")
         .def_property_readonly("metric", &DenseIndex::metric)
         .def_property_readonly("stores_vectors", &DenseIndex::stores_vectors);
 

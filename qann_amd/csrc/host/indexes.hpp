@@ -49,6 +49,9 @@ public:
                          int64_t* out_ids, float* out_dists) const;
     // An upper estimate of the device scratch such a search takes.
     uint64_t scratch_bytes(uint64_t nq, uint32_t k, uint32_t nprobe, bool filtered) const;
+    // The C-ABI handle of the device copy (built if needed; the caller holds
+    // the lock shared), valid until the index changes.
+    qann_hip_index* device_handle() const { return device(); }
     // the queries prepared for the metric (unit length for Cosine)
     std::vector<float> prepare(const float* q, uint64_t n) const;
 
