@@ -78,14 +78,20 @@ def prepare(q):
 
 
 def ground_truth(x, q, k):
-    """Exact top-k by FP32 inner product (the rows are unit length, so this is
-    the cosine order) with torch on the GPU, plus the time of that search with
-    everything resident in HBM: the plain GPU baseline, exact brute force."""
+    """Exact top-k against an FP64 oracle: inner products of the prepared
+    queries with the rows in float64 on the GPU (the rows are unit length, so
+    this is the cosine order). Also times exact brute force in FP32 (torch GEMM
+    + topk, everything resident in HBM), reported beside the IVF path."""
     import torch
 
     torch.backends.cuda.matmul.allow_tf32 = False
     xs = torch.from_numpy(x).cuda()
     qs = torch.from_numpy(q).cuda()
+    x64 = xs.double()
+    truth = torch.empty((q.shape[0], k), dtype=torch.int64, device="cuda")
+    for s in range(0, q.shape[0], 256):
+        truth[s:s + 256] = (qs[s:s + 256].double() @ x64.T).topk(k, dim=1).indices
+    del x64
     out = torch.empty((q.shape[0], k), dtype=torch.int64, device="cuda")
 
     def search():
@@ -101,10 +107,53 @@ def ground_truth(x, q, k):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
         best = dt if best is None else min(best, dt)
-    ids = out.cpu().numpy()
-    del xs, qs, out
+    ids = truth.cpu().numpy()
+    del xs, qs, out, truth
     torch.cuda.empty_cache()
     return ids, q.shape[0] / best
+
+
+def fp64_distance_error(x, qp, ids, dists):
+    """Every returned (query, id): the cosine distance the search returned
+    against float64 arithmetic on the same float32 prepared vectors,
+    0.5 * |q - x|^2; the largest absolute and relative differences."""
+    ok = ids >= 0
+    rows = np.repeat(np.arange(ids.shape[0]), ids.shape[1])[ok.ravel()]
+    cols = ids[ok]
+    q64 = qp.astype(np.float64)
+    err_abs, err_rel = 0.0, 0.0
+    for s in range(0, len(rows), 200_000):
+        r, c = rows[s:s + 200_000], cols[s:s + 200_000]
+        d64 = 0.5 * ((q64[r] - x[c].astype(np.float64)) ** 2).sum(1)
+        got = dists[ok][s:s + 200_000].astype(np.float64)
+        err_abs = max(err_abs, float(np.max(np.abs(got - d64))))
+        err_rel = max(err_rel, float(np.max(np.abs(got - d64) / np.maximum(np.abs(d64), 1e-12))))
+    return err_abs, err_rel
+
+
+def host_cpu():
+    """The host's CPU: model name, logical CPUs, and those this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every core this process may use, within
+    the CPU share the host gives it (OMP_NUM_THREADS when set)."""
+    usable = host_cpu()["usable"]
+    share = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(usable, int(share))) if share and share.isdigit() else max(1, usable)
 
 
 PROFILE = ROOT / "profiles" / "ivf_fastscan_1.2Mx100_nprobe128.md"
@@ -148,7 +197,7 @@ def cpu_baseline(path, q, nprobe, gpu_ids, gpu_d, budget_s):
     it, the oracle, on the host cores, over a bounded sample of the same batch."""
     from oracle import refpy
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     if refpy.REF_SO.exists():
         lib, kind = refpy.Reference(), "reference"
     else:
@@ -174,11 +223,60 @@ def cpu_baseline(path, q, nprobe, gpu_ids, gpu_d, budget_s):
         "value": round(n / best, 1),
         "unit": "queries/s",
         "cores": threads,
+        "host_cpu": host_cpu(),
         "kind": kind,
         "sample": f"first {n} of the {q.shape[0]} queries, nprobe {nprobe}, k {K}, k_factor {K_FACTOR}; "
-                  f"best of 3 passes; {kind} library built from its sources, QANN_DISABLE_AVX512=1",
+                  f"best of 3 passes on {threads} threads; {kind} library built from its sources, "
+                  f"QANN_DISABLE_AVX512=1",
         "parity_rows_identical": same_ids,
         "parity_rows_bit_exact": same_bits,
+    }
+
+
+def serving_comparison(qann_amd, index, ivf, q, nprobe, seconds, workers=4):
+    """The serving pattern on this workload: Poisson requests of 1-256 queries
+    (log-uniform), k 10, at the chosen nprobe. The plain FP32 path serves each
+    request with one synchronous search of its own, one at a time (FP32 coarse
+    GEMM, no graph capture); the serving loop (qann_amd.Server: requests
+    admitted while others run, coalesced into launches on `workers` concurrent
+    streams, the coarse form its tuner picks, graph replays) is driven to
+    saturation and then at 25/50/90% of it. A sample of served results is
+    checked against the same queries searched alone."""
+    from qann_amd import serving_bench as sb
+
+    ivf.nprobe = nprobe
+    plain = sb.plain_path(index, ivf, q, [K], [nprobe], seconds / 2)
+    log(f"plain FP32, one launch per request: {plain['qps']:,.0f} queries/s over {plain['requests']} requests "
+        f"(p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms per request)")
+    server = qann_amd.Server(index, workers=workers, max_batch=4096)
+    rng = np.random.default_rng(31)
+    checks = []
+    for _ in range(40):
+        nq = int(rng.integers(1, 257))
+        lo = int(rng.integers(0, q.shape[0] - nq))
+        checks.append((lo, nq, server.submit(q[lo:lo + nq], K, nprobe=nprobe)))
+    same = 0
+    for lo, nq, fut in checks:
+        ids, d, _ = fut.result()
+        want = index.batch_query(q[lo:lo + nq], K)
+        same += int(np.array_equal(ids, want[0]) and np.array_equal(d.view(np.uint32), want[1].view(np.uint32)))
+    spec = dict(ks=[K], nprobes=[nprobe], max_nq=256)
+    sat, levels, mean_nq = sb.served_levels(qann_amd, server, q, spec, seconds, offered=100_000.0)
+    server.close()
+    ratio = sat["qps"] / plain["qps"]
+    log(f"served: saturated {sat['qps']:,.0f} queries/s ({ratio:.2f}x the plain path), {same}/{len(checks)} served "
+        f"results equal to single calls")
+    return {
+        "requests": "Poisson, 1-256 queries each (log-uniform, mean %.1f), k %d, nprobe %d" % (mean_nq, K, nprobe),
+        "plain_fp32_one_launch_per_request": plain,
+        "served_saturated": {k2: round(v, 3) if isinstance(v, float) else v for k2, v in sat.items()},
+        "served_over_plain": round(ratio, 2),
+        "target_served_over_plain": 3.0,
+        "levels": levels,
+        "target_p99_ms_at_50pct": 5.0,
+        "p99_ms_at_50pct": levels["50%"]["p99_ms"],
+        "served_equal_to_single_calls": f"{same}/{len(checks)}",
+        "workers": workers,
     }
 
 
@@ -193,7 +291,7 @@ def run_cpu_config(args):
     import torch
     import qann_amd
 
-    qann_amd.set_num_threads(min(16, os.cpu_count() or 1))
+    qann_amd.set_num_threads(cpu_threads())
     rng = np.random.Generator(np.random.PCG64(1))
     centres = rng.random((64, 128), dtype=np.float32)
     x = centres[rng.integers(0, 64, 100_000)] + 0.05 * rng.standard_normal((100_000, 128), dtype=np.float32)
@@ -277,7 +375,7 @@ def run_sq8_config(args):
     import torch
     import qann_amd
 
-    qann_amd.set_num_threads(min(16, os.cpu_count() or 1))
+    qann_amd.set_num_threads(cpu_threads())
     n, nq, dim, nlist, centres_n, nprobe, chunk = 20_000_000, 20_000, 128, 16384, 4096, 32, 5_000_000
     t0 = time.perf_counter()
     g = torch.Generator(device="cuda")
@@ -412,6 +510,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="seconds of each served load level (0: no serving comparison)")
     ap.add_argument("--nprobe", type=int, default=0, help="skip the sweep and use this nprobe")
     ap.add_argument("--copy-check-mib", type=int, default=0,
                     help="after timing, copy this many MiB device to device (calibrates HBM byte counters)")
@@ -435,7 +535,7 @@ def main():
     import torch
     import qann_amd
 
-    qann_amd.set_num_threads(min(16, os.cpu_count() or 1))
+    qann_amd.set_num_threads(cpu_threads())
     t0 = time.perf_counter()
     x, q = make_data()
     log(f"data {x.shape} + {q.shape} in {time.perf_counter() - t0:.1f}s")
@@ -551,6 +651,13 @@ def main():
         log(f"copy check: 3 device copies of {nbytes} bytes")
         del src, dst
 
+    err_abs, err_rel = fp64_distance_error(prepare(x), qp, gpu_ids, gpu_d)  # the rows as the index stores them
+    log(f"distances against float64 on every result: largest error {err_abs:.3g} absolute, {err_rel:.3g} relative")
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_comparison(qann_amd, index, ivf, q, chosen, args.serve_seconds)
+        ivf.nprobe = chosen
+
     cpu = None
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:
@@ -587,10 +694,14 @@ def main():
             "p99_ms_per_batch": round(float(np.percentile(ms, 99)), 3),
             "sweep": {str(k): v for k, v in sweep.items()},
             "stage_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in st.items() if v[1]},
-            "plain_gpu": {"qps": round(plain_qps, 1), "recall@10": 1.0,
-                          "method": "exact brute force, torch FP32 GEMM + topk over the 1.2M rows in HBM",
-                          "speedup": round(qps / plain_qps, 2)},
+            "recall_truth": "exact top 10 by float64 inner products (FP64 oracle)",
+            "fp64_distance_error": {"max_abs": err_abs, "max_rel": err_rel,
+                                    "over": "every returned (query, id) of the timed batch"},
+            "brute_force_gpu": {"qps": round(plain_qps, 1), "recall@10": 1.0,
+                                "method": "exact brute force, torch FP32 GEMM + topk over the 1.2M rows in HBM",
+                                "speedup": round(qps / plain_qps, 2)},
         },
+        "serving": serving,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_fs_sums",

@@ -485,7 +485,7 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
             std::copy(sc.begin(), sc.end(), pq.centroids.begin() + s * pq.K * pq.sub);
         }
     }
-    if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = pick_cosine_ip(x.data(), n, assign);
+    if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = cosine_tables_by_recall(x.data(), n, assign, seed ? *seed : 0u);
     if (d_->sq_bits) {
         // detail::SQ::train (src/sq.cpp): each dimension's range over the sample
         d_->sq_vmin.assign(x.begin(), x.begin() + d);
@@ -526,113 +526,141 @@ static void round_table(const float* table, uint64_t m, uint8_t* lut, float& sca
         for (int j = 0; j < 16; ++j) lut[s * 16 + j] = static_cast<uint8_t>(std::nearbyint((table[s * 16 + j] - lo[s]) * scale));
 }
 
-// Whether Cosine fast-scan PQ ranks its own training sample better with the
-// one inner-product table than with per-list L2 tables, decided as
-// IVFIndex::pick_cosine_inner_product does (ivf_index.cpp:808-902): the last
-// 200 sample rows query the first 20,000; each query probes lists nearest
-// first until about 2,000 rows; both ways score those rows with their rounded
-// byte tables, and each way's share of the exact top 10 in its own top 20 is
-// counted. L2 tables are chosen only when they rank strictly better.
-bool IVFIndex::pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign) const {
-    constexpr uint64_t kQueries = 200, kRows = 20000, kCandidates = 2000, kTop = 10, kKept = 20;
-    if (n < kQueries + kCandidates) return true;
-    const uint64_t d = d_->dim, nl = d_->nlist, nq = kQueries, nd = std::min(kRows, n - nq);
+// The table mode of a Cosine PQ index, chosen by simulating its refined
+// search on the training data: true keeps the query's one inner-product table,
+// false per-list L2 tables of the residual q - c. 256 training rows (a seeded
+// sample) query a pool of up to 32,768 other rows. Each query takes pool rows
+// from its lists, nearest centroid first, until it holds 1/32 of the pool; each
+// mode scores them as the search does (rounded byte tables for fast-scan,
+// float tables otherwise) and shortlists its best 40, as a re-rank would; a
+// mode earns a point for every row of the query's exact top 10 over the whole
+// pool that its shortlist holds, and inner product wins ties. (The reference
+// makes the same choice by its own test, IVFIndex::pick_cosine_inner_product,
+// src/ivf_index.cpp:808-902; an index trained here is not the reference's
+// either way, and the mode is saved with it.)
+bool IVFIndex::cosine_tables_by_recall(const float* x, uint64_t n, const std::vector<uint32_t>& assign,
+                                       uint32_t seed) const {
+    constexpr uint64_t kQueries = 256, kPoolMax = 32768, kShort = 40, kTrue = 10;
+    if (n < 4 * kQueries) return true;
+    const uint64_t d = d_->dim, nlist = d_->nlist;
     const PQ& pq = d_->pq;
-    const uint64_t m = pq.m, K = pq.K, T = m * K;
-    const bool fs = d_->fast_scan;
-    const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
-    auto dot = [d](const float* a, const float* b) {
-        float s = 0.0f;
-        for (uint64_t t = 0; t < d; ++t) s += a[t] * b[t];
-        return s;
+    const uint64_t m = pq.m, K = pq.K;
+    const bool bytes = d_->fast_scan;
+    const float* centres = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
+    // queries first, then the pool: a partial shuffle of the rows
+    std::vector<uint64_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::mt19937_64 pick(0x9e3779b97f4a7c15ull ^ seed);
+    const uint64_t pool_n = std::min<uint64_t>(kPoolMax, n - kQueries), taken = kQueries + pool_n;
+    for (uint64_t i = 0; i < taken; ++i) std::swap(order[i], order[i + pick() % (n - i)]);
+    const uint64_t* queries = order.data();
+    const uint64_t* pool = order.data() + kQueries;
+
+    auto ip = [d](const float* a, const float* b) {
+        float acc = 0.0f;
+        for (uint64_t t = 0; t < d; ++t) acc = std::fma(a[t], b[t], acc);
+        return acc;
     };
+    // the pool's PQ codes (residuals to their own list's centre) and the pool rows of every list
     const std::vector<float> cent_t = pq_transpose(pq);
-    std::vector<uint8_t> codes(nd * m);
-    std::vector<std::vector<uint32_t>> by_list(nl);
-    for (uint64_t i = 0; i < nd; ++i) by_list[assign[i]].push_back(static_cast<uint32_t>(i));
-    parallel_for(nd, [&](size_t i) {
+    std::vector<uint8_t> code(pool_n * m);
+    parallel_for(pool_n, [&](size_t j) {
         std::vector<float> r(d);
-        const float* c = rc + size_t{assign[i]} * d;
-        for (uint64_t t = 0; t < d; ++t) r[t] = x[i * d + t] - c[t];
-        pq_encode(cent_t, m, pq.sub, pq.K, r.data(), codes.data() + i * m);
+        const float* row = x + pool[j] * d;
+        const float* c = centres + size_t{assign[pool[j]]} * d;
+        for (uint64_t t = 0; t < d; ++t) r[t] = row[t] - c[t];
+        pq_encode(cent_t, m, pq.sub, K, r.data(), code.data() + j * m);
     });
-    // table[s][c] = sum over the subspace of f(q_t, y_t) for centroid y = pq[s][c]
-    auto build = [&](const float* q, bool ip, float* table) {
+    std::vector<uint64_t> first(nlist + 1, 0);
+    for (uint64_t j = 0; j < pool_n; ++j) ++first[assign[pool[j]] + 1];
+    for (uint64_t l = 0; l < nlist; ++l) first[l + 1] += first[l];
+    std::vector<uint32_t> members(pool_n);
+    {
+        std::vector<uint64_t> fill(first.begin(), first.end() - 1);
+        for (uint64_t j = 0; j < pool_n; ++j) members[fill[assign[pool[j]]]++] = static_cast<uint32_t>(j);
+    }
+    // one subspace table (m x K) of a query vector: -2 q.y (inner product) or |r - y|^2
+    auto subspace_table = [&](const float* v, bool inner, std::vector<float>& out) {
         for (uint64_t s = 0; s < m; ++s)
             for (uint64_t c = 0; c < K; ++c) {
                 const float* y = pq.centroids.data() + (s * K + c) * pq.sub;
+                const float* vs = v + s * pq.sub;
                 float acc = 0.0f;
-                for (uint64_t t = 0; t < pq.sub; ++t) {
-                    const float qt = q[s * pq.sub + t];
-                    acc += ip ? -2.0f * qt * y[t] : (qt - y[t]) * (qt - y[t]);
-                }
-                table[s * K + c] = acc;
+                for (uint64_t t = 0; t < pq.sub; ++t) acc += inner ? -2.0f * vs[t] * y[t] : (vs[t] - y[t]) * (vs[t] - y[t]);
+                out[s * K + c] = acc;
             }
     };
-    std::vector<uint64_t> hits_ip(nq, 0), hits_l2(nq, 0);
-    parallel_for(nq, [&](size_t qi) {
-        const float* q = x + (n - nq + qi) * d;
-        std::vector<std::pair<float, uint32_t>> lists(nl);
-        for (uint64_t l = 0; l < nl; ++l) lists[l] = {-dot(q, d_->centroids.data() + l * d), static_cast<uint32_t>(l)};
-        std::sort(lists.begin(), lists.end());
-        std::vector<uint32_t> probed;
-        uint64_t rows = 0;
-        for (uint64_t p = 0; p < nl && rows < kCandidates; ++p) {
-            if (by_list[lists[p].second].empty()) continue;
-            probed.push_back(lists[p].second);
-            rows += by_list[lists[p].second].size();
+    // a code's approximate score from a table: the rounded bytes' sum as the
+    // fast-scan decodes it, or the float sum
+    struct Scorer {
+        std::vector<float> table;
+        std::vector<uint8_t> lut;
+        float scale = 1.0f, offset = 0.0f;
+    };
+    auto prepare_scorer = [&](Scorer& sc) {
+        if (bytes) round_table(sc.table.data(), m, sc.lut.data(), sc.scale, sc.offset);
+    };
+    auto score = [&](const Scorer& sc, const uint8_t* c) {
+        if (bytes) {
+            uint32_t sum = 0;
+            for (uint64_t s = 0; s < m; ++s) sum += sc.lut[s * 16 + c[s]];
+            return static_cast<float>(sum) / sc.scale + sc.offset;
         }
-        std::vector<std::pair<float, uint32_t>> exact, ip, l2;
-        std::vector<float> table(T), ip_table(T), resid(d);
-        std::vector<uint8_t> lut(fs ? T : 0), list_lut(fs ? T : 0);
-        float ip_scale = 1.0f, ip_offset = 0.0f;
-        build(q, true, ip_table.data());
-        if (fs) round_table(ip_table.data(), m, lut.data(), ip_scale, ip_offset);
-        for (uint32_t l : probed) {
-            const float* c = rc + size_t{l} * d;
-            const float base = 2.0f - 2.0f * dot(q, c);
-            for (uint64_t t = 0; t < d; ++t) resid[t] = q[t] - c[t];
-            build(resid.data(), false, table.data());
-            float l2_scale = 1.0f, l2_offset = 0.0f;
-            if (fs) round_table(table.data(), m, list_lut.data(), l2_scale, l2_offset);
-            for (uint32_t i : by_list[l]) {
-                const uint8_t* code = codes.data() + size_t{i} * m;
-                float s_ip = 0.0f, s_l2 = 0.0f;
-                if (fs) {
-                    uint32_t a = 0, b = 0;
-                    for (uint64_t s = 0; s < m; ++s) {
-                        a += lut[s * 16 + code[s]];
-                        b += list_lut[s * 16 + code[s]];
-                    }
-                    s_ip = static_cast<float>(a) / ip_scale + ip_offset;
-                    s_l2 = static_cast<float>(b) / l2_scale + l2_offset;
-                } else {
-                    for (uint64_t s = 0; s < m; ++s) {
-                        s_ip += ip_table[s * K + code[s]];
-                        s_l2 += table[s * K + code[s]];
-                    }
-                }
-                exact.emplace_back(-dot(q, x + size_t{i} * d), i);
-                ip.emplace_back(base + s_ip, i);
-                l2.emplace_back(s_l2, i);
+        float sum = 0.0f;
+        for (uint64_t s = 0; s < m; ++s) sum += sc.table[s * K + c[s]];
+        return sum;
+    };
+    std::vector<uint32_t> points_inner(kQueries, 0), points_l2(kQueries, 0);
+    parallel_for(kQueries, [&](size_t qi) {
+        const float* q = x + queries[qi] * d;
+        // the exact top 10 of the pool (rows are unit length: largest q.x first)
+        std::vector<std::pair<float, uint32_t>> exact(pool_n);
+        for (uint64_t j = 0; j < pool_n; ++j) exact[j] = {-ip(q, x + pool[j] * d), static_cast<uint32_t>(j)};
+        const uint64_t want = std::min<uint64_t>(kTrue, pool_n);
+        std::partial_sort(exact.begin(), exact.begin() + want, exact.end());
+        std::vector<uint8_t> in_truth(pool_n, 0);
+        for (uint64_t t = 0; t < want; ++t) in_truth[exact[t].second] = 1;
+        // its lists, nearest centroid first, until 1/32 of the pool is in hand
+        std::vector<std::pair<float, uint32_t>> near(nlist);
+        for (uint64_t l = 0; l < nlist; ++l) near[l] = {-ip(q, d_->centroids.data() + l * d), static_cast<uint32_t>(l)};
+        std::sort(near.begin(), near.end());
+        Scorer inner{std::vector<float>(m * K), std::vector<uint8_t>(bytes ? m * 16 : 0)};
+        Scorer l2{std::vector<float>(m * K), std::vector<uint8_t>(bytes ? m * 16 : 0)};
+        subspace_table(q, true, inner.table);
+        prepare_scorer(inner);
+        std::vector<std::pair<float, uint32_t>> by_inner, by_l2;
+        std::vector<float> r(d);
+        for (uint64_t p = 0; p < nlist && by_inner.size() * 32 < pool_n; ++p) {
+            const uint32_t l = near[p].second;
+            if (first[l] == first[l + 1]) continue;
+            const float* c = centres + size_t{l} * d;
+            const float base = 2.0f - 2.0f * ip(q, c);
+            for (uint64_t t = 0; t < d; ++t) r[t] = q[t] - c[t];
+            subspace_table(r.data(), false, l2.table);
+            prepare_scorer(l2);
+            for (uint64_t e = first[l]; e < first[l + 1]; ++e) {
+                const uint32_t j = members[e];
+                const uint8_t* c8 = code.data() + size_t{j} * m;
+                by_inner.emplace_back(base + score(inner, c8), j);
+                by_l2.emplace_back(score(l2, c8), j);
             }
         }
-        const size_t top = std::min<size_t>(kTop, exact.size()), kept = std::min<size_t>(kKept, exact.size());
-        std::partial_sort(exact.begin(), exact.begin() + top, exact.end());
-        std::partial_sort(ip.begin(), ip.begin() + kept, ip.end());
-        std::partial_sort(l2.begin(), l2.begin() + kept, l2.end());
-        for (size_t t = 0; t < top; ++t)
-            for (size_t j = 0; j < kept; ++j) {
-                hits_ip[qi] += ip[j].second == exact[t].second;
-                hits_l2[qi] += l2[j].second == exact[t].second;
-            }
+        auto points = [&](std::vector<std::pair<float, uint32_t>>& cands) {
+            const size_t keep = std::min<size_t>(kShort, cands.size());
+            std::partial_sort(cands.begin(), cands.begin() + keep, cands.end());
+            uint32_t got = 0;
+            for (size_t t = 0; t < keep; ++t) got += in_truth[cands[t].second];
+            return got;
+        };
+        points_inner[qi] = points(by_inner);
+        points_l2[qi] = points(by_l2);
     });
-    uint64_t total_ip = 0, total_l2 = 0;
-    for (uint64_t qi = 0; qi < nq; ++qi) {
-        total_ip += hits_ip[qi];
-        total_l2 += hits_l2[qi];
+    uint64_t inner_total = 0, l2_total = 0;
+    for (uint64_t qi = 0; qi < kQueries; ++qi) {
+        inner_total += points_inner[qi];
+        l2_total += points_l2[qi];
     }
-    return total_ip >= total_l2;
+    return inner_total >= l2_total;
 }
 
 // detail::SQ::encode (src/sq.cpp): per dimension round((x - vmin) / step)

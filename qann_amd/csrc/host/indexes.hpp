@@ -155,7 +155,7 @@ protected:
 
 private:
     friend class RefineIndex;
-    bool pick_cosine_ip(const float* x, uint64_t n, const std::vector<uint32_t>& assign) const;
+    bool cosine_tables_by_recall(const float* x, uint64_t n, const std::vector<uint32_t>& assign, uint32_t seed) const;
     std::unique_ptr<IvfData> d_;
     bool filter_probe_scaling_ = true;
     mutable qann_hip_index* dev_ = nullptr;

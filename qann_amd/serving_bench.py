@@ -42,6 +42,63 @@ def make_data(n, nq, dim, centres, sigma, seed):
     return x, q.astype(np.float32)
 
 
+def log_uniform_mean(max_nq, seed=0):
+    """The mean request size of the log-uniform [1, max_nq] mix the load draws."""
+    u = np.random.default_rng(seed).random(200_000)
+    return float(np.mean(np.clip(np.rint(np.exp(u * np.log(max_nq))), 1, max_nq)))
+
+
+def plain_path(index, ivf, pool, ks, nprobes, seconds, filters=(), filtered_fraction=0.0, max_nq=256, seed=12):
+    """The plain FP32 one-launch-per-request path on a request mix: each request
+    (1..max_nq queries, log-uniform; k and nprobe drawn from ks / nprobes; a
+    filter with probability filtered_fraction) one synchronous batch_query of
+    its own, one at a time, with the FP32 coarse GEMM and no graph capture,
+    back to back for `seconds` (its saturated rate)."""
+    saved = {k: os.environ.get(k) for k in ("QANN_AMD_COARSE", "QANN_AMD_GRAPHS")}
+    os.environ["QANN_AMD_COARSE"], os.environ["QANN_AMD_GRAPHS"] = "fp32", "0"
+    prng = np.random.default_rng(seed)
+    lat, queries = [], 0
+    try:
+        start = time.perf_counter()
+        while time.perf_counter() - start < seconds:
+            nq = int(np.clip(np.rint(np.exp(prng.random() * np.log(max_nq))), 1, max_nq))
+            lo = int(prng.integers(0, pool.shape[0] - nq))
+            k, npb = int(prng.choice(ks)), int(prng.choice(nprobes))
+            allowed = filters[int(prng.integers(0, len(filters)))] if filters and prng.random() < filtered_fraction else None
+            if ivf is not None:
+                ivf.nprobe = npb
+            a = time.perf_counter()
+            index.batch_query(pool[lo:lo + nq], k, allowed=allowed)
+            lat.append(time.perf_counter() - a)
+            queries += nq
+        elapsed = time.perf_counter() - start
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return {"qps": round(queries / elapsed, 1), "requests": len(lat),
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3)}
+
+
+def served_levels(qann_amd, server, pool, spec, seconds, fractions=(0.25, 0.5, 0.9), offered=20_000.0):
+    """The serving loop's saturated rate (`offered` requests per second, far
+    more than it serves, for seconds / 2), then open-loop Poisson loads at the
+    given fractions of it: latency percentiles per level."""
+    mean_nq = log_uniform_mean(spec.get("max_nq", 256))
+    sat = qann_amd.serve_load(server, pool, rate=offered, seconds=seconds / 2, seed=11, **spec)
+    levels = {}
+    for frac in fractions:
+        rate = frac * sat["qps"] / mean_nq
+        st = qann_amd.serve_load(server, pool, rate=rate, seconds=seconds, seed=int(frac * 100), **spec)
+        levels[f"{int(frac * 100)}%"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in st.items()}
+        log(f"{int(frac * 100)}% load ({rate:,.0f} requests/s): {st['qps']:,.0f} queries/s, p50 {st['p50_ms']:.2f} ms, "
+            f"p99 {st['p99_ms']:.2f} ms, {st['launches']} launches for {st['requests']} requests")
+    return sat, levels, mean_nq
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 10M vectors (and of nlist)")
@@ -93,28 +150,8 @@ def main():
     filters = [np.sort(frng.choice(ids_all, n // 10, replace=False)) for _ in range(8)]
     spec = dict(ks=[10, 100], nprobes=[16, 64], filtered_fraction=0.3, filters=filters, max_nq=256)
 
-    # the plain FP32 one-launch-per-request path on the same request mix: each
-    # request a batch_query of its own, one at a time, FP32 coarse GEMM, no
-    # graph capture, back to back (its saturated rate)
-    os.environ["QANN_AMD_COARSE"], os.environ["QANN_AMD_GRAPHS"] = "fp32", "0"
-    prng = np.random.default_rng(12)
-    lat, p_queries = [], 0
-    t_start = time.perf_counter()
-    while time.perf_counter() - t_start < args.seconds / 2:
-        nq = int(np.clip(np.rint(np.exp(prng.random() * np.log(256))), 1, 256))
-        lo = int(prng.integers(0, pool.shape[0] - nq))
-        k, npb = int(prng.choice([10, 100])), int(prng.choice([16, 64]))
-        allowed = filters[int(prng.integers(0, 8))] if prng.random() < 0.3 else None
-        ivf.nprobe = npb
-        a = time.perf_counter()
-        index.batch_query(pool[lo:lo + nq], k, allowed=allowed)
-        lat.append(time.perf_counter() - a)
-        p_queries += nq
-    p_elapsed = time.perf_counter() - t_start
-    del os.environ["QANN_AMD_COARSE"], os.environ["QANN_AMD_GRAPHS"]
-    plain = {"qps": round(p_queries / p_elapsed, 1), "requests": len(lat),
-             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
-             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3)}
+    plain = plain_path(index, ivf, pool, [10, 100], [16, 64], args.seconds / 2, filters=filters,
+                       filtered_fraction=0.3)
     log(f"plain FP32, one launch per request: {plain['qps']:,.0f} queries/s over {plain['requests']} requests "
         f"(p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms per request)")
 
@@ -136,18 +173,10 @@ def main():
         same += int(np.array_equal(got_ids, want_ids) and np.array_equal(got_d.view(np.uint32), want_d.view(np.uint32)))
     log(f"served results identical to single calls: {same}/{len(checks)}")
 
-    mean_nq = float(np.mean(np.clip(np.rint(np.exp(np.random.default_rng(0).random(200_000) * np.log(256))), 1, 256)))
-    sat = qann_amd.serve_load(server, pool, rate=20_000.0, seconds=args.seconds / 2, seed=11, **spec)
+    sat, levels, mean_nq = served_levels(qann_amd, server, pool, spec, args.seconds)
     sat_qps = sat["qps"]
     log(f"saturated: {sat_qps:,.0f} queries/s over {sat['requests']} requests in {sat['launches']} launches "
         f"({sat_qps / plain['qps']:.2f}x the plain path)")
-    levels = {}
-    for frac in (0.25, 0.5, 0.9):
-        rate = frac * sat_qps / mean_nq
-        st = qann_amd.serve_load(server, pool, rate=rate, seconds=args.seconds, seed=int(frac * 100), **spec)
-        levels[f"{int(frac * 100)}%"] = {k: round(v, 3) if isinstance(v, float) else v for k, v in st.items()}
-        log(f"{int(frac * 100)}% load ({rate:,.0f} requests/s): {st['qps']:,.0f} queries/s, p50 {st['p50_ms']:.2f} ms, "
-            f"p99 {st['p99_ms']:.2f} ms, {st['launches']} launches for {st['requests']} requests")
     # per-stage kernel time of the searches (the library's stage profiler: each
     # stage is synchronized and timed with HIP events, so this pass is slower)
     stages = {}
