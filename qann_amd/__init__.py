@@ -3,7 +3,8 @@ AMD MI355X (gfx950).
 
 The API mirrors the reference's Python module ``qann`` for the dense indexes
 on this path: ``FlatIndex``, ``IVFIndex`` (fast-scan PQ), ``RefineIndex``,
-``Metric`` and ``load``. Files saved by ``qann`` load here unchanged and
+``Metric``, ``load`` and the ``where=`` metadata filters (``field``, ``where``,
+``Expr``). Files saved by ``qann`` load here unchanged and
 files saved here load in ``qann``. Every query runs on the GPU through the
 HIP library ``libqann_hip.so``; there is no CPU search path, so importing
 this package fails loudly when the native libraries are not built.
@@ -30,22 +31,31 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken build
 
 from ._qann import (  # noqa: E402
     DenseIndex,
+    Expr,
+    Field,
     FlatIndex,
     IVFIndex,
     Metric,
     RefineIndex,
     Server,
     ServeFuture,
+    all_of,
+    any_of,
+    field,
+    get_max_filter_conditions,
     get_num_threads,
     load,
     serve_load,
+    set_max_filter_conditions,
     set_num_threads,
+    where,
 )
 
 Index = DenseIndex
 __version__ = "0.1.0"
 __all__ = ["DenseIndex", "Index", "FlatIndex", "IVFIndex", "RefineIndex", "Metric", "load", "set_num_threads",
-           "get_num_threads", "native_library", "Server", "ServeFuture", "serve_load"]
+           "get_num_threads", "native_library", "Server", "ServeFuture", "serve_load", "Expr", "Field", "field",
+           "where", "any_of", "all_of", "get_max_filter_conditions", "set_max_filter_conditions"]
 
 
 def native_library() -> pathlib.Path:

@@ -10,7 +10,9 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <optional>
 #include <shared_mutex>
+#include <unordered_map>
 
 namespace py = pybind11;
 using namespace qh;
@@ -60,13 +62,139 @@ void refuse_if_wrapped(const DenseIndex& idx, const char* what) {
                                     " on an index wrapped by a RefineIndex; use the RefineIndex instead");
 }
 
-void refuse_where(const py::object& where) {
-    if (!where.is_none())
-        throw py::type_error("where= metadata filters are not evaluated by qann_amd; pass allowed= ids instead");
+// A Python value as a filter value (the reference's Expr::Value): a str (or
+// NumPy str) as a string, a bool or an int (Python or NumPy) as int64, a float
+// (Python or NumPy) as double; anything else, None included, is a TypeError.
+FilterValue to_value(py::handle v) {
+    if (py::isinstance<py::str>(v)) return v.cast<std::string>();
+    py::module_ np = py::module_::import("numpy");
+    if (py::isinstance<py::bool_>(v) || py::isinstance(v, np.attr("bool_")))
+        return static_cast<int64_t>(v.cast<bool>() ? 1 : 0);
+    if (py::isinstance<py::int_>(v) || py::isinstance(v, np.attr("integer")))
+        return py::reinterpret_steal<py::object>(PyNumber_Index(v.ptr())).cast<int64_t>();
+    if (py::isinstance<py::float_>(v) || py::isinstance(v, np.attr("floating")))
+        return PyFloat_AsDouble(py::reinterpret_steal<py::object>(PyNumber_Float(v.ptr())).ptr());
+    throw py::type_error("filter values must be ints, floats or strings");
 }
 
-py::tuple batch_query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where) {
-    refuse_where(where);
+std::vector<FilterValue> to_values(py::handle seq) {
+    if (py::isinstance<py::str>(seq) || py::isinstance<py::bytes>(seq) || !py::isinstance<py::sequence>(seq))
+        throw py::type_error("filter values must be a sequence of ints, floats or strings");
+    std::vector<FilterValue> out;
+    for (py::handle v : seq) out.push_back(to_value(v));
+    return out;
+}
+
+// qann.where(sql, params): params a sequence for ? placeholders or a dict for
+// :name ones (python_bindings.cpp:330-347).
+Expr parse_sql(const std::string& sql, py::handle params) {
+    if (params.is_none()) return Expr::parse(sql, nullptr, nullptr);
+    if (py::isinstance<py::dict>(params)) {
+        std::unordered_map<std::string, FilterValue> named;
+        for (auto kv : py::reinterpret_borrow<py::dict>(params)) {
+            if (!py::isinstance<py::str>(kv.first)) throw py::type_error("parameter names must be strings");
+            named[kv.first.cast<std::string>()] = to_value(kv.second);
+        }
+        return Expr::parse(sql, nullptr, &named);
+    }
+    const std::vector<FilterValue> list = to_values(params);
+    return Expr::parse(sql, &list, nullptr);
+}
+
+// where=: a qann_amd.Expr, a SQL string, or (SQL string, params)
+std::optional<Expr> to_where(const py::object& where) {
+    if (where.is_none()) return std::nullopt;
+    if (py::isinstance<Expr>(where)) return where.cast<Expr>();
+    if (py::isinstance<py::str>(where)) return parse_sql(where.cast<std::string>(), py::none());
+    if (py::isinstance<py::tuple>(where)) {
+        py::tuple t = py::reinterpret_borrow<py::tuple>(where);
+        if (t.size() == 2 && py::isinstance<py::str>(t[0])) return parse_sql(t[0].cast<std::string>(), t[1]);
+    }
+    throw py::type_error("where must be a qann_amd.Expr, a SQL string or (sql, params)");
+}
+
+// One metadata column of n values (as_column, python_bindings.cpp:349-409):
+// values go through numpy.asarray; integer and bool arrays make int columns,
+// float arrays float columns, str arrays (or object arrays of str) string
+// columns; None marks a null and the other values decide the type.
+MetaInput to_column(const std::string& name, py::handle value, size_t n) {
+    py::module_ np = py::module_::import("numpy");
+    py::object arr = np.attr("asarray")(value);
+    if (arr.attr("ndim").cast<size_t>() != 1)
+        throw py::type_error("metadata column '" + name + "' must be one-dimensional");
+    if (arr.attr("shape").cast<py::tuple>()[0].cast<size_t>() != n)
+        throw std::invalid_argument("metadata column '" + name + "' must have one value per row of data");
+    const std::string kind = arr.attr("dtype").attr("kind").cast<std::string>();
+    // numpy.asarray(["a", 1]) makes "1" a string: a list's own items are checked
+    py::object items = py::hasattr(value, "dtype") ? arr.attr("tolist")() : py::reinterpret_borrow<py::object>(value);
+    MetaInput col;
+    col.name = name;
+    if (kind == "O") {
+        std::vector<uint8_t> valid;
+        py::list present;
+        for (py::handle it : items) {
+            valid.push_back(it.is_none() ? 0 : 1);
+            if (!it.is_none()) present.append(it);
+        }
+        if (std::find(valid.begin(), valid.end(), 0) != valid.end()) {
+            const size_t m = py::len(present);
+            MetaInput some = m ? to_column(name, present, m) : MetaInput{name, 0, {}, {}, {}, {}};
+            // the m values spread over the n rows; null rows keep a zero value
+            MetaInput full;
+            full.name = name;
+            full.type = some.type;
+            full.ints.resize(some.type == 0 ? n : 0);
+            full.floats.resize(some.type == 1 ? n : 0);
+            full.strings.resize(some.type == 2 ? n : 0);
+            for (size_t i = 0, j = 0; i < n; ++i) {
+                if (!valid[i]) continue;
+                if (some.type == 0) full.ints[i] = some.ints[j];
+                else if (some.type == 1) full.floats[i] = some.floats[j];
+                else full.strings[i] = std::move(some.strings[j]);
+                ++j;
+            }
+            full.valid = std::move(valid);
+            return full;
+        }
+    }
+    if (kind == "u" && arr.attr("size").cast<size_t>() > 0 &&
+        arr.attr("max")().cast<uint64_t>() > static_cast<uint64_t>(INT64_MAX))
+        throw std::invalid_argument("metadata column '" + name + "' has values above the int64 maximum");
+    if (kind == "i" || kind == "u" || kind == "b") {
+        auto a = np.attr("ascontiguousarray")(arr, py::arg("dtype") = "int64").cast<py::array_t<int64_t>>();
+        col.type = 0;
+        col.ints.assign(a.data(), a.data() + a.size());
+        return col;
+    }
+    if (kind == "f") {
+        auto a = np.attr("ascontiguousarray")(arr, py::arg("dtype") = "float64").cast<py::array_t<double>>();
+        col.type = 1;
+        col.floats.assign(a.data(), a.data() + a.size());
+        return col;
+    }
+    if (kind == "U" || kind == "O") {
+        col.type = 2;
+        for (py::handle it : items) {
+            if (!py::isinstance<py::str>(it))
+                throw py::type_error("metadata column '" + name + "' mixes strings with other values");
+            col.strings.push_back(it.cast<std::string>());
+        }
+        return col;
+    }
+    throw py::type_error("metadata column '" + name + "' must hold ints, floats or strings");
+}
+
+std::vector<MetaInput> to_metadata(const py::dict& md, size_t n) {
+    std::vector<MetaInput> out;
+    for (auto kv : md) {
+        if (!py::isinstance<py::str>(kv.first)) throw py::type_error("metadata column names must be strings");
+        out.push_back(to_column(kv.first.cast<std::string>(), kv.second, n));
+    }
+    return out;
+}
+
+py::tuple batch_query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where_arg) {
+    const std::optional<Expr> where = to_where(where_arg);
     if (q.ndim() != 2) throw py::type_error("query must be a 2-D array of shape (num_queries, dim)");
     if (static_cast<uint64_t>(q.shape(1)) != self.dim())
         throw std::invalid_argument("query dimension does not match index dimension");
@@ -81,15 +209,16 @@ py::tuple batch_query(DenseIndex& self, FloatArray q, uint32_t k, const py::obje
     float* dp = dists.mutable_data();
     const int64_t* ap = filtered ? al.data() : nullptr;
     const size_t an = filtered ? static_cast<size_t>(al.shape(0)) : 0;
+    const Expr* wp = where ? &*where : nullptr;
     read_locked(self, [&] {
-        self.batch_query(qp, nq, k, ap, an, filtered, ip, dp);
+        self.batch_query(qp, nq, k, ap, an, filtered, ip, dp, wp);
         return 0;
     });
     return py::make_tuple(ids, dists);
 }
 
-py::tuple query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where) {
-    refuse_where(where);
+py::tuple query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where_arg) {
+    const std::optional<Expr> where = to_where(where_arg);
     if (q.ndim() != 1) throw py::type_error("query must be a 1-D array of shape (dim,)");
     if (static_cast<uint64_t>(q.shape(0)) != self.dim())
         throw std::invalid_argument("query dimension does not match index dimension");
@@ -101,8 +230,9 @@ py::tuple query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& al
     const float* qp = q.data();
     const int64_t* ap = filtered ? al.data() : nullptr;
     const size_t an = filtered ? static_cast<size_t>(al.shape(0)) : 0;
+    const Expr* wp = where ? &*where : nullptr;
     read_locked(self, [&] {
-        self.batch_query(qp, 1, k, ap, an, filtered, ids.data(), dists.data());
+        self.batch_query(qp, 1, k, ap, an, filtered, ids.data(), dists.data(), wp);
         return 0;
     });
     size_t n = 0;
@@ -115,7 +245,6 @@ py::tuple query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& al
 }
 
 py::array_t<int64_t> add(DenseIndex& self, FloatArray data, const py::object& ids, const py::object& metadata) {
-    if (!metadata.is_none()) throw py::type_error("metadata= is not supported by qann_amd");
     if (data.ndim() != 2) throw py::type_error("data must be a 2-D array of shape (n, dim)");
     if (static_cast<uint64_t>(data.shape(1)) != self.dim())
         throw std::invalid_argument("data dimension does not match index dimension");
@@ -129,7 +258,13 @@ py::array_t<int64_t> add(DenseIndex& self, FloatArray data, const py::object& id
     const float* dp = data.data();
     const uint64_t n = static_cast<uint64_t>(data.shape(0));
     const int64_t* ip = has_ids ? idv.data() : nullptr;
-    std::vector<int64_t> out = write_locked(self, [&] { return self.add(dp, n, ip); });
+    std::vector<MetaInput> md;
+    const bool has_md = !metadata.is_none();
+    if (has_md) {
+        if (!py::isinstance<py::dict>(metadata)) throw py::type_error("metadata must be a dict of columns");
+        md = to_metadata(py::reinterpret_borrow<py::dict>(metadata), n);
+    }
+    std::vector<int64_t> out = write_locked(self, [&] { return self.add(dp, n, ip, has_md ? &md : nullptr); });
     py::array_t<int64_t> res(static_cast<py::ssize_t>(out.size()));
     std::copy(out.begin(), out.end(), res.mutable_data());
     return res;
@@ -155,15 +290,25 @@ struct ServeFuture {
     }
 };
 
-std::shared_ptr<const SlotFilter> slot_filter(const DenseIndex& idx, py::handle allowed) {
-    py::array_t<int64_t> a = as_ids(allowed);
+std::shared_ptr<const SlotFilter> slot_filter(const DenseIndex& idx, py::handle allowed,
+                                              const py::object& where_arg = py::none()) {
+    const std::optional<Expr> where = to_where(where_arg);
+    const bool filtered = !allowed.is_none();
+    py::array_t<int64_t> a;
+    if (filtered) a = as_ids(allowed);
     auto f = std::make_shared<SlotFilter>();
     read_locked(idx, [&] {
-        *f = idx.ids().filter(a.data(), static_cast<size_t>(a.size()));
+        *f = idx.filter_of(filtered ? a.data() : nullptr, filtered ? static_cast<size_t>(a.size()) : 0, filtered,
+                           where ? &*where : nullptr);
         return 0;
     });
     return f;
 }
+
+// qann.field(name): a column to compare; its operators build filters
+struct Field {
+    std::string name;
+};
 
 py::dict stats_dict(const ServeStats& s) {
     py::dict d;
@@ -193,6 +338,62 @@ PYBIND11_MODULE(_qann, m) {
     m.def("set_num_threads", &set_num_threads, py::arg("n"), "Host threads for adds and training (0 = all cores)");
     m.def("get_num_threads", &num_threads);
 
+    // Filters for where=: qann_amd.field("price") < 50, combined with & | ~
+    py::class_<Expr>(m, "Expr", "A metadata filter for where=; build one with qann_amd.field() or qann_amd.where()")
+        .def("__and__", [](const Expr& a, const Expr& b) { return Expr::join(Expr::Op::And, {a, b}); })
+        .def("__or__", [](const Expr& a, const Expr& b) { return Expr::join(Expr::Op::Or, {a, b}); })
+        .def("__invert__", [](const Expr& a) { return Expr::negate(a); })
+        // and / or / not and chained comparisons would drop part of a filter: refused
+        .def("__bool__",
+             [](const Expr&) -> bool {
+                 throw py::type_error("a filter has no truth value: combine filters with & | ~ (not and / or / "
+                                      "not), and write ranges as f.between(0, 5) or (f > 0) & (f < 5)");
+             })
+        .def("__repr__", [](const Expr& e) { return "qann.Expr(" + e.to_string() + ")"; });
+
+    auto cmp = [](Expr::Op op) {
+        return [op](const Field& f, py::handle v) { return Expr::compare(f.name, op, to_value(v)); };
+    };
+    py::class_<Field>(m, "Field", "A metadata column to compare in a where= filter; see qann_amd.field()")
+        .def_readonly("name", &Field::name)
+        .def("__eq__", cmp(Expr::Op::Eq), py::arg("value"))
+        .def("__ne__", cmp(Expr::Op::Ne), py::arg("value"))
+        .def("__lt__", cmp(Expr::Op::Lt), py::arg("value"))
+        .def("__le__", cmp(Expr::Op::Le), py::arg("value"))
+        .def("__gt__", cmp(Expr::Op::Gt), py::arg("value"))
+        .def("__ge__", cmp(Expr::Op::Ge), py::arg("value"))
+        .def("isin", [](const Field& f, py::handle values) { return Expr::isin(f.name, to_values(values)); },
+             py::arg("values"), "Matches rows whose value is one of values")
+        .def("is_null", [](const Field& f) { return Expr::is_null(f.name); }, "Matches rows with no value in this column")
+        .def("between",
+             [](const Field& f, py::handle lo, py::handle hi) { return Expr::between(f.name, to_value(lo), to_value(hi)); },
+             py::arg("lo"), py::arg("hi"), "Matches rows with lo <= value <= hi: (field >= lo) & (field <= hi)")
+        .def("__repr__", [](const Field& f) { return "qann.field('" + f.name + "')"; });
+    m.def("field", [](std::string name) { return Field{std::move(name)}; }, py::arg("name"),
+          "A metadata column, for filters such as (qann_amd.field('price') < 50) & (qann_amd.field('color') == 'red')");
+    m.def("where", [](const std::string& sql, const py::object& params) { return parse_sql(sql, params); },
+          py::arg("sql"), py::arg("params") = py::none(),
+          "A filter from a SQL WHERE clause: comparisons (= != <> < <= > >=), [NOT] IN (...), IS [NOT] NULL, "
+          "[NOT] BETWEEN x AND y, AND, OR, NOT and parentheses; ? placeholders take a list of params, :name ones "
+          "a dict. NOT follows SQL: NOT (price < 50) excludes rows with no price, unlike ~");
+    m.def("any_of",
+          [](std::vector<Expr> filters) {
+              if (filters.empty()) throw std::invalid_argument("any_of needs at least one filter");
+              return Expr::join(Expr::Op::Or, std::move(filters));
+          },
+          py::arg("filters"), "Matches rows matching any of filters, built in one step");
+    m.def("all_of",
+          [](std::vector<Expr> filters) {
+              if (filters.empty()) throw std::invalid_argument("all_of needs at least one filter");
+              return Expr::join(Expr::Op::And, std::move(filters));
+          },
+          py::arg("filters"), "Matches rows matching all of filters, built in one step");
+    m.def("get_max_filter_conditions", &Expr::max_conditions,
+          "The most conditions a SQL where= string may have (default 1024)");
+    m.def("set_max_filter_conditions", &Expr::set_max_conditions, py::arg("n"),
+          "Set the most conditions (comparisons, IN, IS NULL, BETWEEN; an IN list counts once) a SQL where= string "
+          "may have, for the whole process");
+
     py::class_<DenseIndex, std::shared_ptr<DenseIndex>>(m, "DenseIndex", "An index of dense float vectors")
         .def("add", &add, py::arg("data"), py::arg("ids") = py::none(), py::kw_only(),
              py::arg("metadata") = py::none(), "Add an (n, dim) array of vectors; returns their ids")
@@ -221,12 +422,23 @@ PYBIND11_MODULE(_qann, m) {
         .def("size", [](const DenseIndex& self) { return read_locked(self, [&] { return self.size(); }); })
         .def("dim", &DenseIndex::dim)
         .def("ids",
-             [](const DenseIndex& self) {
-                 std::vector<int64_t> v = read_locked(self, [&] { return self.live_ids(); });
+             [](const DenseIndex& self, const py::object& where_arg) {
+                 const std::optional<Expr> where = to_where(where_arg);
+                 std::vector<int64_t> v = read_locked(self, [&] { return self.live_ids(where ? &*where : nullptr); });
                  py::array_t<int64_t> out(static_cast<py::ssize_t>(v.size()));
                  std::copy(v.begin(), v.end(), out.mutable_data());
                  return out;
-             })
+             },
+             py::kw_only(), py::arg("where") = py::none(),
+             "The ids of the live vectors in ascending order; with where=, those whose metadata matches it")
+        .def_property_readonly("metadata_schema",
+                               [](const DenseIndex& self) {
+                                   auto schema = read_locked(self, [&] { return meta_schema(self.meta()); });
+                                   py::dict out;
+                                   for (const auto& kv : schema) out[py::str(kv.first)] = kv.second;
+                                   return out;
+                               },
+                               "Each metadata column's name and type ('int', 'float' or 'str'), in creation order")
         .def("get_vectors",
              [](const DenseIndex& self, const py::object& ids) {
                  py::array_t<int64_t> idv = as_ids(ids);
@@ -367,7 +579,8 @@ PYBIND11_MODULE(_qann, m) {
              py::arg("index"), py::kw_only(), py::arg("workers") = 4, py::arg("max_batch") = 4096,
              py::arg("memory_fraction") = 0.5)
         .def("submit",
-             [](Server& self, py::handle queries, uint32_t k, uint32_t nprobe, py::handle allowed) {
+             [](Server& self, py::handle queries, uint32_t k, uint32_t nprobe, py::handle allowed,
+                const py::object& where) {
                  FloatArray q = FloatArray::ensure(queries);
                  if (!q || q.ndim() != 2 || static_cast<uint64_t>(q.shape(1)) != self.index().dim())
                      throw py::value_error("queries must be a 2-D array with the index's dimension");
@@ -376,12 +589,13 @@ PYBIND11_MODULE(_qann, m) {
                  r.k = k;
                  r.nprobe = nprobe;
                  r.queries.assign(q.data(), q.data() + q.size());
-                 if (!allowed.is_none()) r.filter = slot_filter(self.index(), allowed);
+                 if (!allowed.is_none() || !where.is_none()) r.filter = slot_filter(self.index(), allowed, where);
                  const uint64_t nq = r.nq;
                  auto f = std::make_shared<std::future<ServeResult>>(self.submit(std::move(r)));
                  return ServeFuture{f, nq, k};
              },
-             py::arg("queries"), py::arg("k"), py::kw_only(), py::arg("nprobe") = 0, py::arg("allowed") = py::none())
+             py::arg("queries"), py::arg("k"), py::kw_only(), py::arg("nprobe") = 0, py::arg("allowed") = py::none(),
+             py::arg("where") = py::none())
         .def("stats", [](Server& self, bool reset) { return stats_dict(self.stats(reset)); }, py::arg("reset") = false)
         .def_property_readonly("memory_budget", &Server::memory_budget)
         .def("close", [](Server& self) {

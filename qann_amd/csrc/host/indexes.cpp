@@ -228,15 +228,33 @@ std::vector<float> DenseIndex::prepare(const float* q, uint64_t n) const {
     return out;
 }
 
+SlotFilter DenseIndex::filter_of(const int64_t* allowed, size_t n_allowed, bool filtered, const Expr* where) const {
+    if (!where) return ids().filter(allowed, n_allowed);
+    const IdMap& m = ids();
+    SlotFilter f;
+    f.words = evaluate(meta(), *where, m.slots);
+    if (filtered) {
+        const SlotFilter also = m.filter(allowed, n_allowed);
+        for (size_t w = 0; w < f.words.size(); ++w) f.words[w] &= also.words[w];
+    }
+    // IdMap::live: deleted slots never pass
+    for (uint64_t s = 0; s < m.slots; ++s)
+        if (m.deleted[s]) f.words[s >> 6] &= ~(uint64_t{1} << (s & 63));
+    for (uint64_t w : f.words) f.count += static_cast<uint64_t>(__builtin_popcountll(w));
+    return f;
+}
+
 void DenseIndex::batch_query(const float* queries, uint64_t nq, uint32_t k, const int64_t* allowed, size_t n_allowed,
-                             bool filtered, int64_t* out_ids, float* out_dists) {
+                             bool filtered, int64_t* out_ids, float* out_dists, const Expr* where) {
     const size_t total = static_cast<size_t>(nq) * k;
     std::fill_n(out_ids, total, int64_t{-1});
     std::fill_n(out_dists, total, INFINITY);
-    if (nq == 0 || k == 0) return;
+    // an unknown column or a value of the wrong kind raises even without queries
     SlotFilter f;
-    if (filtered) {
-        f = ids().filter(allowed, n_allowed);
+    if (filtered || where) f = filter_of(allowed, n_allowed, filtered, where);
+    if (nq == 0 || k == 0) return;
+    if (filtered || where) {
+        filtered = true;
         if (f.count == 0) return; // query_prepared: nothing may be returned
     }
     const std::vector<float> q = prepare(queries, nq);
@@ -293,22 +311,27 @@ void DenseIndex::search_device(const float* d_queries, uint64_t nq, uint32_t k, 
     check_status(qann_hip_search_async(device(), d_queries, nq, &p, d_ids, d_dists, stream));
 }
 
-std::vector<int64_t> DenseIndex::add(const float* vecs, uint64_t n, const int64_t* idv) {
+std::vector<int64_t> DenseIndex::add(const float* vecs, uint64_t n, const int64_t* idv,
+                                     const std::vector<MetaInput>* md) {
     const uint64_t first = slots();
+    if (md) check_meta(meta(), n, *md);
     const std::vector<float> prepared = prepare(vecs, n);
     add_prepared(prepared.data(), n, idv);
+    if (md) append_meta(meta(), first, n, *md);
     invalidate();
     std::vector<int64_t> out(n);
     for (uint64_t i = 0; i < n; ++i) out[i] = ids().label(static_cast<uint32_t>(first + i));
     return out;
 }
 
-std::vector<int64_t> DenseIndex::live_ids() const {
+std::vector<int64_t> DenseIndex::live_ids(const Expr* where) const {
     const IdMap& m = ids();
+    std::vector<uint64_t> match;
+    if (where) match = evaluate(meta(), *where, m.slots);
     std::vector<int64_t> out;
     out.reserve(m.live());
     for (uint64_t s = 0; s < m.slots; ++s)
-        if (!m.deleted[s]) out.push_back(m.label(static_cast<uint32_t>(s)));
+        if (!m.deleted[s] && (!where || (match[s >> 6] >> (s & 63) & 1))) out.push_back(m.label(static_cast<uint32_t>(s)));
     std::sort(out.begin(), out.end());
     return out;
 }

@@ -4,6 +4,7 @@
 // through the C ABI of include/qann_hip.h, and run every query there.
 #pragma once
 
+#include "metadata.hpp"
 #include "qfile.hpp"
 
 #include <cstdint>
@@ -34,9 +35,14 @@ public:
     uint64_t slots() const { return ids().slots; }
 
     // DenseIndex::query_batch (src/index.cpp:117-159, query_prepared :170-184):
-    // padded nq x k results. allowed / n_allowed: the id filter, or null.
+    // padded nq x k results. allowed / n_allowed: the id filter, or null;
+    // where: a metadata filter, or null (both: slots passing both).
     void batch_query(const float* queries, uint64_t nq, uint32_t k, const int64_t* allowed, size_t n_allowed,
-                     bool filtered, int64_t* out_ids, float* out_dists);
+                     bool filtered, int64_t* out_ids, float* out_dists, const Expr* where = nullptr);
+    // The slot filter of a query (IndexBase::where_filter, src/index.cpp:161-168,
+    // or IdMap::filter): the rows where matches, AND the allowed ids when
+    // given, without deleted slots.
+    SlotFilter filter_of(const int64_t* allowed, size_t n_allowed, bool filtered, const Expr* where) const;
 
     // batch_query on prepared queries already in HBM, results left in HBM,
     // enqueued on stream (qann_hip_search_async). No filter.
@@ -55,10 +61,16 @@ public:
     // the queries prepared for the metric (unit length for Cosine)
     std::vector<float> prepare(const float* q, uint64_t n) const;
 
-    // add_batch (src/index.cpp:33-49): returns the added ids.
-    std::vector<int64_t> add(const float* vecs, uint64_t n, const int64_t* ids);
+    // add_batch (src/index.cpp:33-49): returns the added ids; md: the rows'
+    // metadata columns, or null.
+    std::vector<int64_t> add(const float* vecs, uint64_t n, const int64_t* ids,
+                             const std::vector<MetaInput>* md = nullptr);
     virtual size_t remove(const int64_t* ids, size_t n) = 0;
-    std::vector<int64_t> live_ids() const;
+    // the live ids in ascending order; with where, those whose rows match it
+    std::vector<int64_t> live_ids(const Expr* where = nullptr) const;
+    // the metadata columns of this index (MetadataStore, one row per slot)
+    virtual const Metadata& meta() const = 0;
+    virtual Metadata& meta() = 0;
     virtual bool stores_vectors() const { return false; }
     std::vector<float> get_vectors(const int64_t* ids, size_t n) const;
     virtual void save(const std::string& path) const = 0;
@@ -97,6 +109,8 @@ public:
     bool stores_vectors() const override { return true; }
     void save(const std::string& path) const override;
     const FlatData& data() const { return *d_; }
+    const Metadata& meta() const override { return d_->meta; }
+    Metadata& meta() override { return d_->meta; }
 
 protected:
     void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) override;
@@ -146,6 +160,8 @@ public:
     // reference's format and searches identically in both)
     void train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed);
     const IvfData& data() const { return *d_; }
+    const Metadata& meta() const override { return d_->meta; }
+    Metadata& meta() override { return d_->meta; }
 
 protected:
     void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) override;
@@ -176,6 +192,8 @@ public:
     uint64_t k_factor() const { return d_->k_factor; }
     void set_k_factor(uint64_t kf);
     std::shared_ptr<DenseIndex> base() const { return base_; }
+    const Metadata& meta() const override { return d_->meta; }
+    Metadata& meta() override { return d_->meta; }
 
 protected:
     void add_prepared(const float* vecs, uint64_t n, const int64_t* ids) override;
