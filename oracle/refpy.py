@@ -86,6 +86,9 @@ class _Index:
     def set_k_factor(self, kf):
         self._lib._check(self._lib.set_k_factor(self._h, int(kf)))
 
+    def set_filter_probe_scaling(self, on):
+        self._lib._check(self._lib.set_filter_probe_scaling(self._h, int(bool(on))))
+
     def batch_query(self, queries, k, allowed=None, where=None):
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]

@@ -538,18 +538,20 @@ void flat_fp32_subset(qann_hip_index* h, const float* d_q, const std::vector<uin
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
-// *flag = 1 when any of n statuses is set.
-__global__ void k_any_set(const int* status, int n, int* flag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && status[i]) atomicOr(flag, 1);
+// The indexes i < n whose status is set, appended to list (in any order) and
+// counted in *count.
+__global__ void k_list_flagged(const int* status, uint32_t n, int* count, int* list) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && status[i]) list[atomicAdd(count, 1)] = static_cast<int>(i);
 }
 
 // The device search of an index for nq prepared queries already in HBM.
 // A filtered IVF search selects the nearest M lists of each query and probes
 // them in order until IVFIndex::scan's stopping rule holds. With d_more set,
-// M is a first guess (twice the probes the rule asks for at least) and a query
-// that would need more sets *d_more; the caller then searches again with every
-// list it may need (at most 4096). Without it, M is that limit at once.
+// M is a first guess (twice the probes the rule asks for at least), and the
+// queries that would need more are counted in d_more[0] and listed from
+// d_more[1] on (nq ints); the caller searches those again with every list.
+// Without it, M is every list at once.
 // Coarse scores by the split BF16 product (hi.hi + hi.lo + lo.hi, one BF16
 // MFMA GEMM over three times K) or by the FP32 MFMA GEMM. Both lead
 // k_coarse_select to the same probes. The split product's proven error is
@@ -597,17 +599,17 @@ void coarse_scores(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int
 // The M nearest lists of each query from those scores (the bound of the form
 // that wrote them).
 void coarse_probes(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int ip, uint64_t M, bool split,
-                   float* D, uint16_t* pidx, float* pval, hipStream_t st) {
+                   float* D, uint16_t* pidx, float* pval, uint64_t* gkeys, hipStream_t st) {
     const qd::DevIvf& ix = ivfh->ivf;
     qd::launch_coarse_select(D, qr, static_cast<int>(nq), ix.centroids, ix.cnorm_max, ix.nlist, ix.dim, ip,
-                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0);
+                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0, gkeys);
     check_launch("k_coarse_select");
 }
 
 // The coarse form of this search: forced, known for its class, or timed now
 // (both forms run on the real problem; their outputs are the same).
 bool coarse_split(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int ip, uint64_t M, uint16_t* q3,
-                  float* D, uint16_t* pidx, float* pval, hipStream_t st) {
+                  float* D, uint16_t* pidx, float* pval, uint64_t* gkeys, hipStream_t st) {
     const int forced = coarse_forced(ivfh);
     if (forced >= 0) return forced == 1;
     const qd::DevIvf& ix = ivfh->ivf;
@@ -629,7 +631,7 @@ bool coarse_split(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int 
             float ms = 0.0f;
             (void)hipEventRecord(e0, st);
             coarse_scores(ivfh, qr, nq, ip, f == 1, q3, D, st);
-            coarse_probes(ivfh, qr, nq, ip, M, f == 1, D, pidx, pval, st);
+            coarse_probes(ivfh, qr, nq, ip, M, f == 1, D, pidx, pval, gkeys, st);
             (void)hipEventRecord(e1, st);
             if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
                 t[f] = std::min(t[f], ms);
@@ -798,7 +800,6 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const uint32_t nlist = static_cast<uint32_t>(ix.nlist);
     const uint32_t nprobe = p.nprobe;
     if (nprobe == 0 || nprobe > nlist) invalid("nprobe must be in [1, nlist]");
-    if (nprobe > static_cast<uint32_t>(qd::kSelectMax)) invalid("nprobe above 4096 is not supported");
     const bool fs = ix.storage == QANN_IVF_FASTSCAN;
     const bool has_pq = fs || ix.storage == QANN_IVF_PQ;
     const bool cos_pq = has_pq && ix.metric == QANN_METRIC_COSINE && ix.cosine_ip;
@@ -821,7 +822,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             min_probes = static_cast<int>(std::clamp<size_t>(static_cast<size_t>(std::min(want, double(nlist))), nprobe, nlist));
         }
     }
-    const uint32_t M_all = std::min<uint32_t>(nlist, qd::kSelectMax);
+    const uint32_t M_all = nlist;
     const uint32_t M = !filtered ? nprobe
                        : d_more  ? std::min<uint32_t>(M_all, std::max<uint32_t>(64, 2 * static_cast<uint32_t>(min_probes)))
                                  : M_all;
@@ -840,6 +841,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const bool may_split = coarse_forced(ivfh) != 0;
     need += may_split ? pad256(nq * 3 * static_cast<size_t>(ivfh->split_kp) * 2) : 0;
     need += pad256(nq * M * 2) + 3 * pad256(nq * M * 4);
+    const bool gsel_keys = qd::coarse_select_global(static_cast<int>(M), ix.dim);
+    need += gsel_keys ? pad256(nq * qd::coarse_select_cap(static_cast<int>(M)) * 8) : 0;
     need += use_pre || (ip_pq && !fs) ? pad256(nq * tsz * 4) : 0;
     need += ip_pq && fs ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
     need += filtered ? pad256(nlist * 4) : 0;
@@ -877,12 +880,13 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     uint16_t* q3 = may_split ? sc.take<uint16_t>(nq * 3 * static_cast<size_t>(ivfh->split_kp)) : nullptr;
     uint16_t* pidx = sc.take<uint16_t>(nq * M);
     float* pval = sc.take<float>(nq * M);
-    const bool split = coarse_split(ivfh, qr, nq, ip, M, q3, D, pidx, pval, st);
+    uint64_t* gkeys = gsel_keys ? sc.take<uint64_t>(nq * qd::coarse_select_cap(static_cast<int>(M))) : nullptr;
+    const bool split = coarse_split(ivfh, qr, nq, ip, M, q3, D, pidx, pval, gkeys, st);
     StageTimer t_coarse(QANN_STAGE_COARSE, st);
     coarse_scores(ivfh, qr, nq, ip, split, q3, D, st);
     t_coarse.done();
     StageTimer t_sel(QANN_STAGE_SELECT, st);
-    coarse_probes(ivfh, qr, nq, ip, M, split, D, pidx, pval, st);
+    coarse_probes(ivfh, qr, nq, ip, M, split, D, pidx, pval, gkeys, st);
     float* pA = sc.take<float>(nq * M);
     float* pbase = sc.take<float>(nq * M);
     qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
@@ -900,18 +904,11 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
                           pidx, list_allowed, nprobe_q, item_blk, status, st);
     check_launch("k_probe_plan");
     t_sel.done();
-    if (filtered && M < nlist) {
-        if (d_more && M < M_all) {
-            hipLaunchKernelGGL(k_any_set, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0, st, status,
-                               static_cast<int>(nq), d_more);
-            check_launch("k_any_set");
-        } else {
-            std::vector<int> hs(nq);
-            check(hipMemcpyAsync(hs.data(), status, nq * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-            check(hipStreamSynchronize(st), "hipStreamSynchronize");
-            for (int v : hs)
-                if (v) throw Error(QANN_ESTATE, "a filtered query needs more than 4096 probed lists");
-        }
+    if (filtered && M < nlist && d_more) {
+        // the queries whose stopping rule needs more lists than the first guess
+        hipLaunchKernelGGL(k_list_flagged, dim3(static_cast<unsigned>((nq + 255) / 256)), dim3(256), 0, st, status,
+                           static_cast<uint32_t>(nq), d_more, d_more + 1);
+        check_launch("k_list_flagged");
     }
     float* C = nullptr;
     uint8_t* lut = nullptr;
@@ -1529,7 +1526,10 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
             // inputs and outputs in a second buffer so run_search owns the scratch
             const size_t qb = pad256(nr * h->dim * 4), ob = pad256(nr * k * 8), db = pad256(nr * k * 4);
             const size_t fb = params->allowed_words ? pad256((h->slots + 63) / 64 * 8) : 0;
-            char* buf = io->reserve_io(qb + ob + db + fb + 256);
+            // filtered: the count and list of queries to search again, their
+            // gathered rows and their results
+            const size_t mb = fb ? pad256((nr + 1) * 4) + qb + ob + db : 0;
+            char* buf = io->reserve_io(qb + ob + db + fb + mb + 256);
             float* dq = reinterpret_cast<float*>(buf);
             int64_t* dids = reinterpret_cast<int64_t*>(buf + qb);
             float* dd = reinterpret_cast<float*>(buf + qb + ob);
@@ -1556,8 +1556,25 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
             int more = 0;
             if (dmore) check(hipMemcpyAsync(&more, dmore, 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
-            // some query needs more lists than the first guess: every list it may need
-            if (more) run_search(h, dq, nq, *params, df, dids, dd, *io);
+            if (more > 0) {
+                // the queries whose stopping rule needs more lists than the first
+                // guess: those alone again, with every list, results scattered back
+                char* mbuf = buf + qb + ob + db + fb;
+                const uint32_t* list = reinterpret_cast<const uint32_t*>(dmore + 1);
+                float* gq = reinterpret_cast<float*>(mbuf + pad256((nr + 1) * 4));
+                int64_t* gids = reinterpret_cast<int64_t*>(mbuf + pad256((nr + 1) * 4) + qb);
+                float* gd = reinterpret_cast<float*>(mbuf + pad256((nr + 1) * 4) + qb + ob);
+                const uint32_t cnt = static_cast<uint32_t>(more);
+                const uint64_t ge = static_cast<uint64_t>(cnt) * h->dim;
+                hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>((ge + 255) / 256)), dim3(256), 0,
+                                   io->stream, dq, list, cnt, h->dim, gq);
+                check_launch("k_gather_rows");
+                run_search(h, gq, cnt, *params, df, gids, gd, *io);
+                const uint64_t re = static_cast<uint64_t>(cnt) * k;
+                hipLaunchKernelGGL(k_scatter_results, dim3(static_cast<unsigned>((re + 255) / 256)), dim3(256), 0,
+                                   io->stream, gids, gd, list, cnt, k, dids, dd);
+                check_launch("k_scatter_results");
+            }
             check(hipMemcpyAsync(out_ids, dids, nq * k * 8, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipMemcpyAsync(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
@@ -1797,7 +1814,10 @@ int qann_hip_search_scratch_bytes(qann_hip_index* h, uint64_t nq, const qann_hip
             const qann_hip_index* ivfh = h->kind == 3 ? h->base : h;
             const qd::DevIvf& ix = ivfh->ivf;
             const uint64_t nlist = static_cast<uint64_t>(ix.nlist);
-            const uint64_t M = p->allowed_words ? std::min<uint64_t>(nlist, qd::kSelectMax) : std::max<uint32_t>(p->nprobe, 1);
+            // filtered: the first guess of probes (a few times nprobe; the queries
+            // needing more are searched again on their own)
+            const uint64_t M = p->allowed_words ? std::min<uint64_t>(nlist, std::max<uint64_t>(256, 16ull * p->nprobe))
+                                                : std::max<uint32_t>(p->nprobe, 1);
             const uint64_t kp = h->kind == 3 ? k * std::max<uint32_t>(p->k_factor, 1) : k;
             const uint64_t vb = ix.storage == QANN_IVF_FASTSCAN ? 2 : 4;
             const uint64_t wblocks = ivfh->blocks_desc_prefix[std::min<size_t>(M, ivfh->blocks_desc_prefix.size() - 1)];

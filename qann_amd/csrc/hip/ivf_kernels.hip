@@ -29,6 +29,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <stdexcept>
 #include <tuple>
 #include <type_traits>
 
@@ -257,13 +258,18 @@ constexpr int kSelectSample = 4096;
 template <int NT, int CD, bool SAMPLE>
 __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
                                                        int dim, int ip, float cnorm_max, int M, int cap,
-                                                       uint16_t* out_idx, float* out_val, int split_kp) {
+                                                       uint16_t* out_idx, float* out_val, int split_kp,
+                                                       uint64_t* gkeys) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
     __shared__ float s_red[NT / kWave], s_red8[2 * NT / kWave];
     float* qsh = reinterpret_cast<float*>(lds);
-    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
+    // the shortlist's keys: in LDS after the query, or (gkeys, when more are
+    // wanted than LDS holds) cap keys per query in global memory, which the
+    // block's barriers order as they order LDS
+    uint64_t* keys = gkeys ? gkeys + static_cast<size_t>(blockIdx.x) * cap
+                           : reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
     const int i = blockIdx.x, tid = threadIdx.x;
     float* row = D + static_cast<size_t>(i) * nlist;
     float ss = 0.0f;
@@ -2508,11 +2514,19 @@ void launch_coarse_gemm(const float* q, int nq, const float* cents, const float*
     }
 }
 
+bool coarse_select_global(int M, int dim) {
+    return ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + coarse_select_cap(M) * 8 > kLdsMax;
+}
+
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
-                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp) {
+                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp,
+                          uint64_t* gkeys) {
     if (nq <= 0) return;
     const size_t cap = coarse_select_cap(M);
-    const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
+    const bool global = coarse_select_global(M, dim);
+    if (global && !gkeys) throw std::invalid_argument("the coarse select needs global keys for this many probes");
+    if (!global) gkeys = nullptr;
+    const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + (global ? 0 : cap * 8);
     // each query's row of scores stays in registers, 16 values a thread: with
     // 256 threads up to 4,096 lists, with 1,024 up to 16,384; larger rows are
     // read from memory, and take a threshold from a sample first (a row in
@@ -2525,24 +2539,24 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     if (nlist <= 16 * 256) {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<256, 16, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
         else
             hipLaunchKernelGGL((k_coarse_select<256, 16, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
     } else if (nlist <= 16 * 1024) {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<1024, 16, true>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
         else
             hipLaunchKernelGGL((k_coarse_select<1024, 16, false>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist,
-                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
+                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
     } else {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<256, 0, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
-                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
+                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
         else
             hipLaunchKernelGGL((k_coarse_select<256, 0, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
     }
 }
 

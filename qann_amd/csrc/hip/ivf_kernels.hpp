@@ -9,8 +9,8 @@ namespace qd {
 // Probes per k_fs_sums workgroup (a window of one query's probe order) and its waves.
 constexpr int kSumsWin = 16;
 constexpr int kSumsWaves = 4;
-// Most sorted probes one query can be given (k_select_sorted keeps them in LDS).
-constexpr int kSelectMax = 4096;
+// Most probes one query can be given: every list of the largest index.
+constexpr int kSelectMax = 65535;
 
 // An IVF fast-scan index in HBM. Lists are stored back to back: list l's
 // slots at list_ids[list_off[l] .. list_off[l + 1]), its code blocks at
@@ -144,9 +144,13 @@ size_t coarse_select_cap(int M);
 void launch_coarse_gemm(const float* q, int nq, const float* cents, const float* csq, int nlist, int dim, int ip,
                         float* D, hipStream_t s, const uint64_t* excl = nullptr, const uint64_t* incl = nullptr);
 // split_kp > 0: D holds the split BF16 product's scores over K = 3 split_kp
-// (launch_coarse_split), and the shortlist bound is that product's.
+// (launch_coarse_split), and the shortlist bound is that product's. When
+// coarse_select_global(M, dim), the shortlist does not fit LDS and gkeys must
+// hold nq * coarse_select_cap(M) keys of global memory.
+bool coarse_select_global(int M, int dim);
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
-                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp = 0);
+                          int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp = 0,
+                          uint64_t* gkeys = nullptr);
 // FlatIndex over D from launch_coarse_gemm (rows as the "centroids", masked by
 // excl / incl): the k nearest rows per query, exact, in (distance, slot) order.
 void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,
