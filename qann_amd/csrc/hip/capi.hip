@@ -853,10 +853,11 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += grerank ? pad256(nq * rkeys * 8) : 0;
     need += fs ? pad256(nlist * 4) : 0; // the query order's histogram
     need += fs ? pad256(8 * 4) : 0;     // work-queue counters of the persistent sums kernel
+    need += fs ? 3 * pad256((nlist + 1) * 4) : 0; // the matrix-core form's pair totals and offsets, unit offsets
     if (sc.fixed) {
         // a scratch of fixed size: chunks as large as what is left of it holds
         // (each query of a chunk: its sums or distances, block minimums, order entry)
-        const uint64_t per_query = stride * vbytes + (stride / qd::kFsBlock) * vbytes + (fs ? 4 : 0);
+        const uint64_t per_query = stride * vbytes + (stride / qd::kFsBlock) * vbytes + (fs ? 4 + 5ull * M + 256 : 0);
         const uint64_t left = sc.cap > need + 3 * 256 ? sc.cap - need - 3 * 256 : 0;
         chunk = std::min<uint64_t>(chunk, left / per_query);
         if (chunk == 0) sc.too_small(need + 3 * 256 + per_query);
@@ -865,6 +866,10 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += pad256(chunk * (stride / qd::kFsBlock) * vbytes); // block minimums
     const bool order_queries = fs && chunk >= kOrderMinQueries;
     need += order_queries ? pad256(chunk * 4) : 0;
+    // the matrix-core form's pairs grouped by list, their units and the grouping's counts
+    need += fs ? pad256(chunk * M * 4) + pad256((chunk * M / 32 + nlist + 1) * 4) +
+                     pad256(qd::pairs_ghist_words(static_cast<int>(nlist), static_cast<int>(chunk)) * 4)
+               : 0;
     sc.used = 0;
     sc.reserve(need);
     const float* qr = d_q;
@@ -942,6 +947,15 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         fbmin = sc.take<uint32_t>(chunk * (stride / qd::kFsBlock));
     }
     uint32_t* sums_queue = fs ? sc.take<uint32_t>(8) : nullptr;
+    qd::PairPlan pairs{};
+    if (fs) {
+        pairs.pair_off = sc.take<uint32_t>(nlist + 1);
+        pairs.unit_off = sc.take<uint32_t>(nlist + 1);
+        pairs.tot = sc.take<uint32_t>(nlist + 1);
+        pairs.items = sc.take<uint32_t>(chunk * M);
+        pairs.unit_list = sc.take<uint32_t>(chunk * M / 32 + nlist + 1);
+        pairs.ghist = sc.take<uint32_t>(qd::pairs_ghist_words(static_cast<int>(nlist), static_cast<int>(chunk)));
+    }
     uint32_t* ord_hist = order_queries ? sc.take<uint32_t>(nlist) : nullptr;
     uint32_t* order = order_queries ? sc.take<uint32_t>(chunk) : nullptr;
     float* cd = sc.take<float>(nq * kp);
@@ -1027,6 +1041,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.item_min = item_min + q0 * M;
         if (qd::fs_sums_lds(a) == 0) throw Error(QANN_EINVAL, "the fast-scan tables of this index do not fit in LDS");
         a.queue = sums_queue;
+        a.pairs = pairs;
         if (q0 == 0 && scan_wait) check(hipStreamWaitEvent(st, scan_wait, 0), "hipStreamWaitEvent");
         if (order && nc >= kOrderMinQueries) {
             qd::launch_query_order(a.probe_idx, static_cast<int>(M), static_cast<int>(nc), static_cast<int>(nlist),

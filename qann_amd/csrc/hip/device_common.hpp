@@ -49,6 +49,27 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+template <int Ctrl>
+__device__ __forceinline__ float dpp_f32(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), Ctrl, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float lane_f32(float x, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+
+// wave_max without LDS round trips, for a wave whose 64 lanes are all
+// active: DPP maxima within each row of 16 lanes (lanes i ^ 1, i ^ 2, then
+// the half-row and row mirrors), then the four rows' maxima read lane by
+// lane. fmaxf drops NaNs whatever the order, so both give the same value.
+__device__ __forceinline__ float wave_max_dpp(float v) {
+    v = fmaxf(v, dpp_f32<0xB1>(v));  // quad_perm [1, 0, 3, 2]
+    v = fmaxf(v, dpp_f32<0x4E>(v));  // quad_perm [2, 3, 0, 1]
+    v = fmaxf(v, dpp_f32<0x141>(v)); // row_half_mirror
+    v = fmaxf(v, dpp_f32<0x140>(v)); // row_mirror
+    return fmaxf(fmaxf(lane_f32(v, 0), lane_f32(v, 16)), fmaxf(lane_f32(v, 32), lane_f32(v, 48)));
+}
+
 // ---------------------------------------------------------------------------
 // Exact distances in the reference's AVX2 order (src/distance.cpp:69-153 as
 // g++ compiles them): 4 x 8 lane accumulators over chunks of 32, 8-wide steps

@@ -35,6 +35,9 @@
 
 #include "device_common.hpp"
 #include "ivf_kernels.hpp"
+
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+typedef int v16i_t __attribute__((ext_vector_type(16)));
 #include "select.hpp"
 
 namespace qd {
@@ -1178,6 +1181,415 @@ __global__ void __launch_bounds__(kWave) k_fs_block_check(const uint8_t* luts, i
         uint16_t* o = out + static_cast<size_t>(b) * kFsBlock;
         *reinterpret_cast<uint2*>(o + 4 * lane) = make_uint2(l01, l23);
         *reinterpret_cast<uint2*>(o + 16 + 4 * lane) = make_uint2(h01, h23);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phase 1 on the matrix cores: the u16 sums of a list's code blocks for 32
+// (query, probe) pairs at once, as an int8 MFMA product. For one block the
+// sums are S[v][j] = sum over subspaces s of LUT_j[s][code(v, s)]: a product
+// of the block's codes written one-hot (32 vectors x 16m, 0 or 1) with the 32
+// pairs' LUTs (16m x 32, bytes <= 63), accumulated exactly in i32. The pairs
+// are grouped by list, so one wave reads a list's blocks once for 32 queries
+// (the per-query form reads them once per query).
+// ---------------------------------------------------------------------------
+
+// The pairs (query i, probe p < nprobe_q[i]) of non-empty lists, counted per
+// list; pairs of empty lists get item_min = 0xFFFFFFFF (the replay skips them).
+// Two ways: with per-list counters in LDS (nlist <= kPairsLdsLists), each of
+// kPairGroups workgroups counting a contiguous range of the queries, the
+// counts combined without global atomics; or with global atomics.
+constexpr int kPairsLdsLists = 16384;
+constexpr int kPairGroups = 64;
+
+__device__ __forceinline__ bool pair_of(const SumsArgs& a, uint64_t t, int& list) {
+    const int qi = static_cast<int>(t / a.M), p = static_cast<int>(t % a.M);
+    if (p >= a.nprobe_q[qi]) return false;
+    list = a.probe_idx[t];
+    if (a.ix.list_off[list + 1] == a.ix.list_off[list]) {
+        a.item_min[t] = 0xFFFFFFFFu;
+        return false;
+    }
+    return true;
+}
+
+// queries [g nq / G, (g + 1) nq / G) of workgroup g: counts into ghist[g][list]
+__global__ void __launch_bounds__(1024) k_pairs_hist(const SumsArgs a, int nq, uint32_t* ghist) {
+    extern __shared__ uint32_t h[];
+    const int nlist = a.ix.nlist, g = blockIdx.x;
+    for (int l = threadIdx.x; l < nlist; l += blockDim.x) h[l] = 0u;
+    __syncthreads();
+    const uint64_t q0 = static_cast<uint64_t>(nq) * g / gridDim.x, q1 = static_cast<uint64_t>(nq) * (g + 1) / gridDim.x;
+    for (uint64_t t = q0 * a.M + threadIdx.x; t < q1 * a.M; t += blockDim.x) {
+        int list;
+        if (pair_of(a, t, list)) atomicAdd(&h[list], 1u);
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nlist; l += blockDim.x) ghist[static_cast<size_t>(g) * nlist + l] = h[l];
+}
+
+// From the per-workgroup counts ghist[g][list]: per list, each group's first
+// position within the list's range (ghist, in place) and the list's total (tot).
+__global__ void k_pairs_lists(uint32_t* ghist, int groups, int n, uint32_t* tot) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    uint32_t run = 0;
+    for (int g = 0; g < groups; ++g) {
+        const size_t x = static_cast<size_t>(g) * n + l;
+        const uint32_t c = ghist[x];
+        ghist[x] = run;
+        run += c;
+    }
+    tot[l] = run;
+}
+
+// pair_off / unit_off: exclusive prefix sums over the lists of the pair totals
+// and of their 32-pair units, with the totals at index n. One 1024-thread block.
+__global__ void __launch_bounds__(1024) k_pairs_offsets(const uint32_t* tot, int n, uint32_t* pair_off,
+                                                        uint32_t* unit_off) {
+    __shared__ uint32_t part_p[1024], part_u[1024];
+    const int t = threadIdx.x, per = (n + 1023) / 1024;
+    const int b = min(n, t * per), e = min(n, b + per);
+    uint32_t sp = 0, su = 0;
+    for (int l = b; l < e; ++l) {
+        sp += tot[l];
+        su += (tot[l] + 31) / 32;
+    }
+    part_p[t] = sp;
+    part_u[t] = su;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t vp = t >= off ? part_p[t - off] : 0u, vu = t >= off ? part_u[t - off] : 0u;
+        __syncthreads();
+        part_p[t] += vp;
+        part_u[t] += vu;
+        __syncthreads();
+    }
+    uint32_t rp = part_p[t] - sp, ru = part_u[t] - su;
+    for (int l = b; l < e; ++l) {
+        pair_off[l] = rp;
+        unit_off[l] = ru;
+        rp += tot[l];
+        ru += (tot[l] + 31) / 32;
+    }
+    if (t == 1023) {
+        pair_off[n] = part_p[1023];
+        unit_off[n] = part_u[1023];
+    }
+}
+
+// Each unit's list.
+__global__ void k_pairs_units(const uint32_t* unit_off, int n, uint32_t* unit_list) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    for (uint32_t k = unit_off[l]; k < unit_off[l + 1]; ++k) unit_list[k] = static_cast<uint32_t>(l);
+}
+
+// Each pair's item index (i * M + p) into its list's range, the workgroup's
+// cursors in LDS.
+__global__ void __launch_bounds__(1024) k_pairs_place(const SumsArgs a, int nq, const uint32_t* ghist,
+                                                      const uint32_t* pair_off, uint32_t* items) {
+    extern __shared__ uint32_t cur[];
+    const int nlist = a.ix.nlist, g = blockIdx.x;
+    for (int l = threadIdx.x; l < nlist; l += blockDim.x)
+        cur[l] = pair_off[l] + ghist[static_cast<size_t>(g) * nlist + l];
+    __syncthreads();
+    const uint64_t q0 = static_cast<uint64_t>(nq) * g / gridDim.x, q1 = static_cast<uint64_t>(nq) * (g + 1) / gridDim.x;
+    for (uint64_t t = q0 * a.M + threadIdx.x; t < q1 * a.M; t += blockDim.x) {
+        int list;
+        if (pair_of(a, t, list)) items[atomicAdd(&cur[list], 1u)] = static_cast<uint32_t>(t);
+    }
+}
+
+__global__ void k_pairs_cursors(const uint32_t* pair_off, int n, uint32_t* cur) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) cur[l] = pair_off[l];
+}
+
+// The same grouping with global counters, for more lists than LDS holds.
+__global__ void k_pairs_count(const SumsArgs a, int nq, uint32_t* cnt) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<uint64_t>(nq) * a.M) return;
+    int list;
+    if (pair_of(a, t, list)) atomicAdd(&cnt[list], 1u);
+}
+
+__global__ void k_pairs_scatter(const SumsArgs a, int nq, uint32_t* fill, uint32_t* items) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= static_cast<uint64_t>(nq) * a.M) return;
+    const int qi = static_cast<int>(t / a.M), p = static_cast<int>(t % a.M);
+    if (p >= a.nprobe_q[qi]) return;
+    const int list = a.probe_idx[t];
+    if (a.ix.list_off[list + 1] == a.ix.list_off[list]) return;
+    items[atomicAdd(&fill[list], 1u)] = static_cast<uint32_t>(t);
+}
+
+// A table row's minimum and maximum as std::min / std::max take them in entry
+// order (quantize_lut, src/fast_scan.cpp:22-46): fminf / fmaxf agree except for
+// a NaN first entry (kept) and the sign of a zero minimum (the first zero's).
+__device__ __forceinline__ void row_lo_hi(const float t[16], float& lo, float& hi) {
+    lo = t[0];
+    hi = t[0];
+#pragma unroll
+    for (int c = 1; c < 16; ++c) {
+        lo = fminf(lo, t[c]);
+        hi = fmaxf(hi, t[c]);
+    }
+    if (t[0] != t[0]) {
+        lo = t[0];
+        hi = t[0];
+    }
+    if (lo == 0.0f) {
+        for (int c = 15; c >= 0; --c)
+            if (t[c] == 0.0f) lo = t[c];
+    }
+}
+
+// The 16-byte one-hot of nibble c (byte c is 1, the others 0).
+__device__ __forceinline__ v4i_t onehot16(uint32_t c) {
+    const uint64_t bit = uint64_t{1} << ((c & 7u) << 3);
+    const bool low = c < 8u;
+    const uint32_t lo = static_cast<uint32_t>(bit), hi = static_cast<uint32_t>(bit >> 32);
+    v4i_t v;
+    v[0] = static_cast<int>(low ? lo : 0u);
+    v[1] = static_cast<int>(low ? hi : 0u);
+    v[2] = static_cast<int>(low ? 0u : lo);
+    v[3] = static_cast<int>(low ? 0u : hi);
+    return v;
+}
+
+// Units of 32 pairs, one per wave: unit u is a chunk of list unit_list[u]'s
+// pairs. Workgroups are laid out XCD by XCD (workgroup b runs on XCD b % 8),
+// so consecutive units -- mostly one list's -- share an XCD's L2.
+constexpr int kMfmaWaves = 2;
+constexpr int kMfmaSubMax = 4; // PQ sub-vector lengths whose codebook rows stay in registers
+
+// LDS of one wave of k_fs_sums_mfma: the rounded LUT rows of 16 pairs on
+// their way into MFMA operands, those pairs' row minimums, and the 32 pairs'
+// scales and offsets; later two staged code blocks
+__host__ __device__ inline size_t mfma_wave_lds(int m4) {
+    const size_t build = static_cast<size_t>(16) * m4 * 16 + 16 * 64 * 4 + 32 * 8;
+    const size_t blocks = static_cast<size_t>(2) * m4 * 16;
+    return ((build > blocks ? build : blocks) + 15) & ~size_t{15};
+}
+
+// NK = m4 / 2 MFMA steps of 2 subspaces; IP: the query's one LUT (ip_lut),
+// else the precomputed L2 table built here per pair, for PQ sub-vectors of
+// SUB floats (1, 2; 4 for 3 and 4). The 32 pairs' LUT columns stay in
+// registers (NK x 16 bytes a lane) for all the list's blocks.
+template <int NK, bool IP, int SUB>
+__global__ void __launch_bounds__(kMfmaWaves * kWave, 2) k_fs_sums_mfma(const SumsArgs a, const PairPlan pp) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const DevIvf& ix = a.ix;
+    const int m = ix.m, lb = ix.m4 * 16, lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    unsigned char* mine = lds + static_cast<size_t>(wave) * mfma_wave_lds(ix.m4);
+
+    const uint32_t groups = gridDim.x / 8;
+    const uint32_t wg = (blockIdx.x % 8) * groups + blockIdx.x / 8;
+    const uint32_t u = wg * kMfmaWaves + wave;
+    if (u >= pp.unit_off[ix.nlist]) return;
+    const int list = static_cast<int>(pp.unit_list[u]);
+    const uint32_t first = pp.pair_off[list] + (u - pp.unit_off[list]) * 32;
+    const int npairs = static_cast<int>(min(32u, pp.pair_off[list + 1] - first));
+    const int j = lane & 31, h = lane >> 5;
+    const bool valid = j < npairs;
+    // lanes past the unit's pairs work on its first pair and store nothing
+    const uint32_t item = pp.items[first + (valid ? j : 0)];
+    const int qi = static_cast<int>(item / a.M);
+
+    // the pair's LUT rows 2t + h, as MFMA B operands
+    v4i_t lut[NK];
+    float scale = 1.0f, offset = 0.0f;
+    if (IP) {
+        const uint8_t* src = a.ip_lut + static_cast<size_t>(qi) * m * 16;
+#pragma unroll
+        for (int t = 0; t < NK; ++t) {
+            const int r = 2 * t + h;
+            v4i_t v = {0, 0, 0, 0};
+            if (r < m) v = *reinterpret_cast<const v4i_t*>(src + r * 16);
+            lut[t] = v;
+        }
+        scale = a.ip_scale[qi];
+        offset = a.ip_offset[qi] + a.probe_base[item];
+    } else {
+        // one table row per lane (row r = lane): the list's B row and the PQ
+        // transpose's rows of subspace r stay in registers for all 32 pairs
+        const int sub = ix.sub, r = lane;
+        const bool row = r < m;
+        // loads are unconditional, from row 0 and entry sub - 1 where the
+        // lane or k is past the table, so that they issue together (a guarded
+        // load waits for itself before the next one)
+        const int rs = row ? r : 0;
+        float brow[16], crow[SUB][16];
+        const float* Bl = ix.B + static_cast<size_t>(list) * m * 16;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) brow[c] = Bl[rs * 16 + c];
+#pragma unroll
+        for (int k = 0; k < SUB; ++k)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) crow[k][c] = ix.pq_cent_t[(rs * sub + min(k, sub - 1)) * 16 + c];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) brow[c] = row ? brow[c] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < SUB; ++k)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) crow[k][c] = row && k < sub ? crow[k][c] : 0.0f;
+        uint8_t* rows16 = mine;                                                // 16 pairs x m4 rows x 16 bytes
+        float* rmin = reinterpret_cast<float*>(mine + static_cast<size_t>(16) * lb); // 16 pairs x 64 rows
+        float* so = rmin + 16 * 64;                                            // 32 x (scale, offset)
+        const float pairA = a.probe_A[item];
+        // each lane's entries of the query, for PF pairs at a time, loaded together
+        constexpr int PF = SUB <= 2 ? 16 : 4;
+#pragma unroll
+        for (int t = 0; t < NK; ++t) lut[t] = v4i_t{0, 0, 0, 0};
+        for (int half = 0; half < 2; ++half) {
+            for (int p0 = 0; p0 < 16; p0 += PF) {
+                float qv[PF][SUB];
+#pragma unroll
+                for (int pl = 0; pl < PF; ++pl) {
+                    const uint32_t it = static_cast<uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(item), half * 16 + p0 + pl));
+                    const float* q = a.q + static_cast<size_t>(it / a.M) * ix.dim + rs * sub;
+#pragma unroll
+                    for (int k = 0; k < SUB; ++k) qv[pl][k] = q[min(k, sub - 1)];
+                }
+#pragma unroll
+                for (int pl = 0; pl < PF; ++pl)
+#pragma unroll
+                    for (int k = 0; k < SUB; ++k) qv[pl][k] = row && k < sub ? qv[pl][k] : 0.0f;
+#pragma unroll
+                for (int pl = 0; pl < PF; ++pl) {
+                    const int pk = p0 + pl, pj = half * 16 + pk;
+                    const float A = lane_f32(pairA, pj);
+                    // row r of the table: B + C (+ A on row 0), C = compute_inner_table
+                    // (src/pq.cpp:186-204) as k_query_tables makes it
+                    float e[16];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) e[c] = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < SUB; ++k) {
+                        if (SUB <= 2 || k < sub) {
+                            const float m2q = -2.0f * qv[pl][k];
+#pragma unroll
+                            for (int c = 0; c < 16; ++c) e[c] = fmaf(m2q, crow[k][c], e[c]);
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) e[c] = brow[c] + e[c];
+                    if (r == 0) {
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) e[c] = e[c] + A;
+                    }
+                    float rlo = 0.0f, rhi = 0.0f;
+                    if (row) row_lo_hi(e, rlo, rhi);
+                    const float widest = wave_max_dpp(row ? rhi - rlo : 0.0f);
+                    const float sc = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
+                    rmin[pk * 64 + r] = rlo;
+                    if (lane == 0) so[2 * pj] = sc;
+                    if (lane < ix.m4) {
+                        uint32_t w[4] = {0u, 0u, 0u, 0u};
+                        if (row) {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                                for (int c = 0; c < 4; ++c)
+                                    w[i] = __builtin_amdgcn_cvt_pk_u8_f32((e[4 * i + c] - rlo) * sc, c, w[i]);
+                        }
+                        *reinterpret_cast<uint4*>(rows16 + static_cast<size_t>(pk) * lb + r * 16) =
+                            make_uint4(w[0], w[1], w[2], w[3]);
+                    }
+                }
+            }
+            wave_lds_sync();
+            // the half's offsets: lane 4 pk + i sums pair pk's row minimums
+            // of rows i, i + 4, ... in row order, then (p0 + p1) + (p2 + p3)
+            {
+                const int pk = lane >> 2;
+                float part = 0.0f;
+                for (int i = lane & 3; i < m; i += 4) part += rmin[pk * 64 + i];
+                const int b4 = lane & ~3;
+                const float s0 = __shfl(part, b4, kWave), s1 = __shfl(part, b4 + 1, kWave);
+                const float s2 = __shfl(part, b4 + 2, kWave), s3 = __shfl(part, b4 + 3, kWave);
+                if ((lane & 3) == 0) so[2 * (half * 16 + pk) + 1] = (s0 + s1) + (s2 + s3);
+            }
+            // this half's pairs take their rows 2t + h as operands
+            if ((j >> 4) == half) {
+                const unsigned char* src = rows16 + static_cast<size_t>(j & 15) * lb;
+#pragma unroll
+                for (int t = 0; t < NK; ++t) lut[t] = *reinterpret_cast<const v4i_t*>(src + (2 * t + h) * 16);
+            }
+            wave_lds_sync();
+        }
+        scale = so[2 * j];
+        offset = so[2 * j + 1];
+        wave_lds_sync(); // the LDS is the blocks' from here on
+    }
+
+    // the list's blocks, each staged in LDS a block ahead
+    const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+    const uint32_t nblk = (n + kFsBlock - 1) / kFsBlock;
+    const uint8_t* codes = ix.codes + ix.code_off[list];
+    const bool loader = lane * 16 < lb;
+    unsigned char* stage = mine;
+    uint4 next = make_uint4(0u, 0u, 0u, 0u);
+    if (loader) *reinterpret_cast<uint4*>(stage + lane * 16) = *reinterpret_cast<const uint4*>(codes + lane * 16);
+    if (nblk > 1 && loader) next = *reinterpret_cast<const uint4*>(codes + lb + lane * 16);
+    const uint32_t blk0 = a.item_blk[item];
+    uint16_t* qsums = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
+    uint16_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock) + blk0;
+    // vector v = lane % 32 of a block: byte v % 16 of each subspace row, the
+    // low nibble for v < 16; in the device layout, word (s, g = (v % 16) / 4)
+    // sits at (s / 4) * 64 + g * 16 + (s % 4) * 4
+    const int v = lane & 31, g = (v & 15) >> 2, shift = 8 * (v & 3) + 4 * (v >> 4);
+    uint32_t run_min = 0xFFFFFFFFu;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        const unsigned char* buf = stage + (b & 1) * lb;
+        wave_lds_sync();
+        // two accumulators (even and odd subspace groups) halve the chain of dependent MFMAs
+        v16i_t acc0 = {}, acc1 = {};
+#pragma unroll
+        for (int s4 = 0; s4 < NK / 2; ++s4) {
+            const uint4 w = *reinterpret_cast<const uint4*>(buf + s4 * 64 + g * 16);
+            const uint32_t w0 = h ? w.y : w.x, w1 = h ? w.w : w.z; // subspaces 4 s4 + h and 4 s4 + 2 + h
+            if (s4 & 1) {
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w0 >> shift) & 15u), lut[2 * s4], acc1, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w1 >> shift) & 15u), lut[2 * s4 + 1], acc1, 0, 0, 0);
+            } else {
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w0 >> shift) & 15u), lut[2 * s4], acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w1 >> shift) & 15u), lut[2 * s4 + 1], acc0, 0, 0, 0);
+            }
+        }
+        // the block after next into the buffer just read, its load in flight
+        if (b + 1 < nblk && loader) *reinterpret_cast<uint4*>(stage + ((b + 1) & 1) * lb + lane * 16) = next;
+        if (b + 2 < nblk && loader)
+            next = *reinterpret_cast<const uint4*>(codes + static_cast<size_t>(b + 2) * lb + lane * 16);
+        // lane (pair j, half h) holds vectors 8 (i / 4) + 4 h + i % 4 of the block
+        uint32_t vmin = 0xFFFFFFFFu;
+        const uint32_t left = n - b * kFsBlock;
+#pragma unroll
+        for (int v4 = 0; v4 < 4; ++v4) {
+            const int v0 = 8 * v4 + 4 * h;
+            const uint32_t s0 = static_cast<uint32_t>(acc0[4 * v4] + acc1[4 * v4]);
+            const uint32_t s1 = static_cast<uint32_t>(acc0[4 * v4 + 1] + acc1[4 * v4 + 1]);
+            const uint32_t s2 = static_cast<uint32_t>(acc0[4 * v4 + 2] + acc1[4 * v4 + 2]);
+            const uint32_t s3 = static_cast<uint32_t>(acc0[4 * v4 + 3] + acc1[4 * v4 + 3]);
+            if (valid)
+                *reinterpret_cast<uint2*>(qsums + static_cast<size_t>(b) * kFsBlock + v0) =
+                    make_uint2(s0 | (s1 << 16), s2 | (s3 << 16));
+            if (static_cast<uint32_t>(v0) < left) vmin = min(vmin, s0);
+            if (static_cast<uint32_t>(v0 + 1) < left) vmin = min(vmin, s1);
+            if (static_cast<uint32_t>(v0 + 2) < left) vmin = min(vmin, s2);
+            if (static_cast<uint32_t>(v0 + 3) < left) vmin = min(vmin, s3);
+        }
+        vmin = min(vmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmin), 32, kWave)));
+        if (valid && h == 0) qbmin[b] = static_cast<uint16_t>(min(vmin, 0xFFFFu));
+        run_min = min(run_min, vmin);
+    }
+    if (valid && h == 0) {
+        a.item_min[item] = run_min;
+        a.item_scale[item] = scale;
+        a.item_offset[item] = offset;
     }
 }
 
@@ -2604,9 +3016,9 @@ size_t fs_sums_lds(SumsArgs& a) {
 namespace {
 
 struct SumsKey {
-    int m4, win, nwin, nclass, ordered;
+    int m4, win, nwin, nclass, ordered, mfma;
     bool operator<(const SumsKey& o) const {
-        return std::tie(m4, win, nwin, nclass, ordered) < std::tie(o.m4, o.win, o.nwin, o.nclass, o.ordered);
+        return std::tie(m4, win, nwin, nclass, ordered, mfma) < std::tie(o.m4, o.win, o.nwin, o.nclass, o.ordered, o.mfma);
     }
 };
 std::mutex g_sums_mu;
@@ -2623,7 +3035,72 @@ unsigned persistent_grid(size_t lds) {
     return static_cast<unsigned>((cus * per + 7) / 8 * 8);
 }
 
+size_t mfma_lds(const SumsArgs& a) { return static_cast<size_t>(kMfmaWaves) * mfma_wave_lds(a.ix.m4); }
+
+template <bool IP>
+void launch_mfma_nk(const SumsArgs& a, unsigned grid, size_t lds, hipStream_t s) {
+    switch (a.ix.m4 / 2) {
+#define QANN_NK(N)                                                                                              \
+    case N:                                                                                                   \
+        if (IP)                                                                                               \
+            hipLaunchKernelGGL((k_fs_sums_mfma<N, true, 1>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,    \
+                               a.pairs);                                                                      \
+        else if (a.ix.sub == 1)                                                                               \
+            hipLaunchKernelGGL((k_fs_sums_mfma<N, false, 1>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,   \
+                               a.pairs);                                                                      \
+        else if (a.ix.sub == 2)                                                                               \
+            hipLaunchKernelGGL((k_fs_sums_mfma<N, false, 2>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,   \
+                               a.pairs);                                                                      \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_fs_sums_mfma<N, false, 4>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,   \
+                               a.pairs);                                                                      \
+        break;
+        QANN_NK(2) QANN_NK(4) QANN_NK(6) QANN_NK(8) QANN_NK(10) QANN_NK(12) QANN_NK(14) QANN_NK(16)
+        QANN_NK(18) QANN_NK(20) QANN_NK(22) QANN_NK(24) QANN_NK(26) QANN_NK(28) QANN_NK(30) QANN_NK(32)
+#undef QANN_NK
+    default: break;
+    }
+}
+
+// The matrix-core form: the pairs grouped by list (counted, offsets, placed),
+// then one wave per unit of 32 pairs of one list.
+void launch_mfma(const SumsArgs& a, int nq, hipStream_t s) {
+    const int nlist = a.ix.nlist;
+    const uint64_t items = static_cast<uint64_t>(nq) * a.M;
+    const PairPlan& pp = a.pairs;
+    if (nlist <= kPairsLdsLists) {
+        const int groups = static_cast<int>(std::min<uint64_t>(kPairGroups, static_cast<uint64_t>(nq)));
+        const size_t hl = static_cast<size_t>(nlist) * 4;
+        hipLaunchKernelGGL(k_pairs_hist, dim3(groups), dim3(1024), hl, s, a, nq, pp.ghist);
+        hipLaunchKernelGGL(k_pairs_lists, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.ghist, groups, nlist, pp.tot);
+        hipLaunchKernelGGL(k_pairs_offsets, dim3(1), dim3(1024), 0, s, pp.tot, nlist, pp.pair_off, pp.unit_off);
+        hipLaunchKernelGGL(k_pairs_units, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.unit_off, nlist, pp.unit_list);
+        hipLaunchKernelGGL(k_pairs_place, dim3(groups), dim3(1024), hl, s, a, nq, pp.ghist, pp.pair_off, pp.items);
+    } else {
+        hipLaunchKernelGGL(k_order_zero, dim3(cdiv(nlist + 1, 256)), dim3(256), 0, s, pp.tot, nlist + 1);
+        hipLaunchKernelGGL(k_pairs_count, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, a, nq,
+                           pp.tot);
+        hipLaunchKernelGGL(k_pairs_offsets, dim3(1), dim3(1024), 0, s, pp.tot, nlist, pp.pair_off, pp.unit_off);
+        hipLaunchKernelGGL(k_pairs_units, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.unit_off, nlist, pp.unit_list);
+        // the scatter's cursors start at each list's offset
+        hipLaunchKernelGGL(k_pairs_cursors, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.pair_off, nlist, pp.ghist);
+        hipLaunchKernelGGL(k_pairs_scatter, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, a, nq,
+                           pp.ghist, pp.items);
+    }
+    // units are at most one per 32 pairs plus one partial unit per list
+    const uint64_t units = (items + 31) / 32 + static_cast<uint64_t>(nlist);
+    const uint64_t groups = ((units + kMfmaWaves - 1) / kMfmaWaves + 7) / 8;
+    const unsigned grid = static_cast<unsigned>(groups * 8);
+    const size_t lds = mfma_lds(a);
+    if (a.ip_lut) launch_mfma_nk<true>(a, grid, lds, s);
+    else launch_mfma_nk<false>(a, grid, lds, s);
+}
+
 void sums_form(int form, const SumsArgs& a, int nq, size_t lds, hipStream_t s) {
+    if (form == 2) {
+        launch_mfma(a, nq, s);
+        return;
+    }
     if (form == 1) {
         hipLaunchKernelGGL(k_order_zero, dim3(1), dim3(64), 0, s, a.queue, 8);
         hipLaunchKernelGGL(k_fs_sums_queue, dim3(persistent_grid(lds)), dim3(kSumsWaves * kWave), lds, s, a);
@@ -2644,23 +3121,40 @@ void sums_form(int form, const SumsArgs& a, int nq, size_t lds, hipStream_t s) {
 
 } // namespace
 
+size_t pairs_ghist_words(int nlist, int nq) {
+    return nlist <= kPairsLdsLists ? static_cast<size_t>(std::min(kPairGroups, std::max(nq, 1))) * nlist
+                                   : static_cast<size_t>(nlist) + 1;
+}
+
+bool fs_mfma_ok(const SumsArgs& a) {
+    if (!a.pairs.items || a.ix.m4 > 64 || a.ix.m4 < 4) return false;
+    // per-list residual tables, and codebooks of long sub-vectors, stay on the per-query form
+    if (!a.ip_lut && (!a.ix.B || a.ix.sub > kMfmaSubMax)) return false;
+    return mfma_lds(a) <= kLdsMax;
+}
+
 void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
     const size_t lds = fs_sums_lds(a);
     a.nq = nq;
     a.per_xcd = (nq + 7) / 8;
+    const bool mfma = fs_mfma_ok(a);
+    // QANN_AMD_SUMS_FORM=grid, =queue or =mfma fixes the form (tests run each)
+    const char* force = std::getenv("QANN_AMD_SUMS_FORM");
+    if (force && std::strcmp(force, "mfma") == 0 && mfma) {
+        sums_form(2, a, nq, lds, s);
+        return;
+    }
     if (!a.queue) {
         sums_form(0, a, nq, lds, s);
         return;
     }
-    // QANN_AMD_SUMS_FORM=grid or =queue fixes the form (tests run both)
-    const char* force = std::getenv("QANN_AMD_SUMS_FORM");
     if (force && (std::strcmp(force, "grid") == 0 || std::strcmp(force, "queue") == 0)) {
         sums_form(std::strcmp(force, "queue") == 0 ? 1 : 0, a, nq, lds, s);
         return;
     }
     int nclass = 0;
     while ((1 << nclass) < nq) ++nclass;
-    const SumsKey key{a.ix.m4, a.win, a.nwin, nclass, a.order != nullptr};
+    const SumsKey key{a.ix.m4, a.win, a.nwin, nclass, a.order != nullptr, mfma ? (a.ip_lut ? 1 : 2) : 0};
     int form = -1;
     {
         std::lock_guard<std::mutex> g(g_sums_mu);
@@ -2668,12 +3162,12 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
         if (it != g_sums_form.end()) form = it->second;
     }
     if (form < 0) {
-        // both forms write the same outputs, so the trials can run on the real problem
-        float t[2] = {0.0f, 0.0f};
+        // the forms write the same outputs, so the trials can run on the real problem
+        float t[3] = {0.0f, 0.0f, 1e30f};
         hipEvent_t e0, e1;
         form = 0;
         if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-            for (int f = 0; f < 2; ++f) {
+            for (int f = 0; f < (mfma ? 3 : 2); ++f) {
                 (void)hipEventRecord(e0, s);
                 sums_form(f, a, nq, lds, s);
                 (void)hipEventRecord(e1, s);
@@ -2681,6 +3175,7 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
                     t[f] = 1e30f;
             }
             form = t[1] < t[0] ? 1 : 0;
+            if (t[2] < t[form]) form = 2;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
         }
@@ -2688,9 +3183,10 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
         if (log && *log && *log != '0')
             std::fprintf(stderr,
                          "qann_amd tune: fast-scan sums nq %d m4 %d windows %d -> %s (ms: %.3f grid, %.3f queue of %u "
-                         "workgroups)\n",
-                         nq, a.ix.m4, a.nwin, form ? "persistent queue" : "one workgroup per window", t[0], t[1],
-                         persistent_grid(lds));
+                         "workgroups, %.3f matrix cores)\n",
+                         nq, a.ix.m4, a.nwin,
+                         form == 2 ? "matrix cores" : form ? "persistent queue" : "one workgroup per window", t[0],
+                         t[1], persistent_grid(lds), t[2]);
         std::lock_guard<std::mutex> g(g_sums_mu);
         g_sums_form[key] = form;
     }

@@ -44,6 +44,22 @@ struct DevIvf {
     const uint64_t* deleted;       // bitmap over slots, or null when none is deleted
 };
 
+// The (query, probe) pairs of a scan grouped by list, for the matrix-core form
+// of phase 1: pair_off / unit_off (nlist + 1 each) are per-list prefix counts
+// of pairs and of 32-pair units, unit_list (nq * M / 32 + nlist) each unit's
+// list, ghist (pairs_ghist_words(nlist, nq)) per-group counts and cursors, tot
+// (nlist + 1) the pairs of each list,
+// items (nq * M) each pair's item index i * M + p, list by list.
+struct PairPlan {
+    uint32_t* pair_off;
+    uint32_t* unit_off;
+    uint32_t* unit_list;
+    uint32_t* ghist;
+    uint32_t* tot; // nlist + 1: pairs per list
+    uint32_t* items;
+};
+size_t pairs_ghist_words(int nlist, int nq);
+
 // Phase 1 of the fast-scan: the LUT and the u16 sums of every (query, probe).
 struct SumsArgs {
     DevIvf ix;
@@ -73,6 +89,8 @@ struct SumsArgs {
     int nq, per_xcd;
     // 8 counters (one per XCD) for the persistent form, or null for the plain launch only
     uint32_t* queue;
+    // buffers of the matrix-core form, or null pointers when it may not run
+    PairPlan pairs;
 };
 
 // Phase 1 for PQ (ADC), SQ and float lists: the float distance of every
@@ -166,10 +184,15 @@ void launch_query_tables(const float* q, const DevIvf& ix, int nq, int mode, flo
                          float* offset, hipStream_t s);
 // Dynamic LDS of k_fs_sums (0 when a table does not fit); sets win, nwin and the LDS flags.
 size_t fs_sums_lds(SumsArgs& a);
-// Phase 1, as one workgroup per (query, window) or as the persistent
-// work-queue kernel; which of the two a problem class runs is chosen by timing
-// both on its first launch (QANN_AMD_TUNE_LOG=1 prints the choice).
+// Phase 1, as one workgroup per (query, window), as the persistent
+// work-queue kernel, or on the matrix cores (pairs grouped by list, int8
+// MFMA); which one a problem class runs is chosen by timing each on its first
+// launch (QANN_AMD_TUNE_LOG=1 prints the choice; QANN_AMD_SUMS_FORM=grid,
+// queue or mfma forces one).
 void launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
+// Whether the matrix-core form can run these sums: inner-product LUTs or
+// precomputed L2 tables, m4 <= 64, the pair buffers given.
+bool fs_mfma_ok(const SumsArgs& a);
 // The queries ordered by their nearest probed list (a counting sort over the
 // lists; hist takes nlist counters): consecutive queries in it probe many of
 // the same lists.

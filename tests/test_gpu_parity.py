@@ -164,13 +164,16 @@ def test_gpu_probe_plan_forms_match_goldens(monkeypatch, name, form):
                     f"{name} {form} {stem}")
 
 
-@pytest.mark.parametrize("form", ["grid", "grid4", "grid6", "queue"])
-@pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp"])
+@pytest.mark.parametrize("form", ["grid", "grid4", "grid6", "queue", "mfma"])
+@pytest.mark.parametrize("name", ["l2_fs_refine", "ip_fs_custom", "cos_fs_refine", "l2_fs_noprecomp", "cos_l2_refine",
+                                  "l2_fs"])
 def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
     # phase 1 as one workgroup per (query, window) (registers for four or six
-    # waves per SIMD, or by batch size) and as the persistent work-queue
-    # kernel, each forced, small and large (XCD-ordered) batches
-    monkeypatch.setenv("QANN_AMD_SUMS_FORM", "queue" if form == "queue" else "grid")
+    # waves per SIMD, or by batch size), as the persistent work-queue kernel,
+    # and on the matrix cores (pairs grouped by list, int8 MFMA; inner-product
+    # and precomputed L2 tables -- per-list residual tables keep the per-query
+    # form), each forced, small and large (XCD-ordered) batches
+    monkeypatch.setenv("QANN_AMD_SUMS_FORM", form if form in ("queue", "mfma") else "grid")
     if form in ("grid4", "grid6"):
         monkeypatch.setenv("QANN_AMD_SUMS_OCC", form[-1])
     path, q, exp, params = synth_case(name)
