@@ -159,15 +159,55 @@ def cpu_threads():
 PROFILE = ROOT / "profiles" / "ivf_fastscan_1.2Mx100_nprobe128.md"
 
 
-def measured_traffic(nprobe):
-    """HBM bytes per k_fs_sums launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs of this workload (scaled by a copy of known size), or None."""
+def profile_figure(name, kernel, nprobe):
+    """A figure of the committed rocprofv3 runs of this workload: the line
+    `<name> <kernel> nprobe <nprobe>: <value>` of the profile summary, or None.
+    traffic_bytes_per_launch is HBM bytes per launch (FETCH_SIZE and WRITE_SIZE
+    runs, scaled by a copy of known size); mfma_busy and valu_busy are the
+    kernel's MfmaUtil and VALUBusy (SQ_VALU_MFMA_BUSY_CYCLES and
+    SQ_ACTIVE_INST_VALU over GRBM_GUI_ACTIVE) as fractions."""
     if not PROFILE.exists():
         return None
     for line in PROFILE.read_text().splitlines():
-        if line.startswith(f"traffic_bytes_per_launch k_fs_sums nprobe {nprobe}:"):
-            return int(line.split(":")[1])
+        if line.startswith(f"{name} {kernel} nprobe {nprobe}:"):
+            v = line.split(":")[1].strip()
+            return int(v) if v.isdigit() else float(v)
     return None
+
+
+PEAK_I8_TOPS = 5000.0  # MI355X dense int8 MFMA
+
+
+def roofline(mfma_form, nprobe, achieved_tops, achieved_gbs, scan_bytes, mfma_ops, unit_blocks, launches, scan_ms,
+             group_ms):
+    """The dominant kernel's roofline. The matrix-core form is priced on int8
+    MFMA: it reads each list's code blocks once per 32 pairs, so the per-pair
+    code bytes (the algorithmic B_scan) pass at more than HBM's peak and HBM no
+    longer bounds it; its MFMA and VALU busy fractions (rocprofv3) say what
+    does. The per-pair forms are priced on HBM over those code bytes."""
+    if mfma_form:
+        kernel = "k_fs_sums_mfma"
+        out = {"bound": "mfma", "kernel": kernel, "achieved": round(achieved_tops, 1), "peak": PEAK_I8_TOPS,
+               "unit": "TOP/s (int8)", "frac": round(achieved_tops / PEAK_I8_TOPS, 4),
+               "mfma_ops_per_launch": int(mfma_ops), "unit_blocks_per_launch": unit_blocks,
+               "algorithmic_code_bytes_per_launch": int(scan_bytes),
+               "algorithmic_code_GBps": round(achieved_gbs, 1),
+               "group_ms": round(group_ms, 4)}
+    else:
+        kernel = "k_fs_sums"
+        out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved_gbs, 1), "peak": PEAK_HBM_GBS,
+               "unit": "GB/s", "frac": round(achieved_gbs / PEAK_HBM_GBS, 4),
+               "algorithmic_bytes_per_launch": int(scan_bytes)}
+    traffic = profile_figure("traffic_bytes_per_launch", kernel, nprobe)
+    out["traffic"] = traffic
+    out["traffic_source"] = PROFILE.name if traffic else None
+    for name in ("mfma_busy", "valu_busy"):
+        v = profile_figure(name, kernel, nprobe)
+        if v is not None:
+            out[name] = v
+    out["launches_per_step"] = launches
+    out["launch_ms"] = round(scan_ms, 4)
+    return out
 
 
 def recall(ids, gt):
@@ -625,20 +665,34 @@ def main():
     st = stage_times(lib, reset=True)
     scan_ms = st["scan"][0] / max(st["scan"][1], 1)
     launches = max(st["scan"][1], 1) / prof_steps
+    # the matrix-core form groups the pairs by list first (its own stage)
+    mfma_form = st.get("group", (0.0, 0))[1] > 0
+    group_ms = st["group"][0] / st["group"][1] if mfma_form else 0.0
     # algorithmic bytes of one scan launch: the code blocks of every probed list
     # of every query, B_scan(q) = sum over probes ceil(n_l / 32) * M * 16
     cents = torch.from_numpy(ivf.centroids()).cuda()
     qs = torch.from_numpy(qp).cuda()
     blocks = torch.from_numpy(((sizes + 31) // 32).astype(np.int64)).cuda()
     total_blocks = 0
+    pairs_per_list = torch.zeros(NLIST, dtype=torch.int64, device="cuda")
     for s in range(0, N_QUERY, 1024):
         d2 = (cents * cents).sum(1)[None, :] - 2.0 * (qs[s:s + 1024] @ cents.T)
         lists = d2.topk(chosen, dim=1, largest=False).indices
         total_blocks += int(blocks[lists].sum().item())
+        pairs_per_list += torch.bincount(lists.flatten(), minlength=NLIST)
     scan_bytes = total_blocks * M * 16 / launches
-    achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
+    achieved_gbs = scan_bytes / (scan_ms * 1e-3) / 1e9
+    # the matrix-core form: one wave per unit of 32 pairs of a list streams the
+    # list's blocks once, M4 / 2 v_mfma_i32_32x32x32_i8 (65,536 ops each) per
+    # block, M4 = M rounded up to a multiple of 4
+    m4 = (M + 3) // 4 * 4
+    unit_blocks = int((((pairs_per_list + 31) // 32) * blocks).sum().item())
+    mfma_ops = unit_blocks * (m4 // 2) * 2 * 32 * 32 * 32 / launches
+    achieved_tops = mfma_ops / (scan_ms * 1e-3) / 1e12
     log("stages (ms per launch): " + ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}" for k, v in st.items() if v[1]))
-    log(f"scan: {scan_bytes / 1e9:.3f} GB of code blocks per launch, {achieved:,.0f} GB/s algorithmic")
+    log(f"scan: {scan_bytes / 1e9:.3f} GB of code blocks per (query, probe) per launch, {achieved_gbs:,.0f} GB/s of "
+        f"them; matrix-core form {'on' if mfma_form else 'off'}: {unit_blocks:,} unit blocks, "
+        f"{mfma_ops / 1e12:.3f} T int8 MFMA ops, {achieved_tops:,.0f} TOP/s")
     if args.copy_check_mib:
         # a plain device-to-device copy of known size, so that HBM counters read
         # under a profiler can be scaled to bytes
@@ -702,19 +756,8 @@ def main():
                                 "speedup": round(qps / plain_qps, 2)},
         },
         "serving": serving,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_fs_sums",
-            "achieved": round(achieved, 1),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": measured_traffic(chosen),
-            "traffic_source": PROFILE.name if measured_traffic(chosen) else None,
-            "algorithmic_bytes_per_launch": int(scan_bytes),
-            "launches_per_step": launches,
-            "launch_ms": round(scan_ms, 4),
-        },
+        "roofline": roofline(mfma_form, chosen, achieved_tops, achieved_gbs, scan_bytes, mfma_ops, unit_blocks,
+                             launches, scan_ms, group_ms),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

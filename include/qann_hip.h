@@ -208,7 +208,9 @@ int qann_hip_fs_block_sums(const uint8_t* luts, uint32_t m, const uint8_t* block
  * kernel replaying the TopK pushes over them. QANN_STAGE_FLAT is a FlatIndex
  * search by the BF16 GEMM (or, for fewer than 8 queries, by direct exact
  * scoring), QANN_STAGE_FLAT_FP32 one by the FP32 GEMM, queries the BF16 path
- * hands over included.
+ * hands over included. QANN_STAGE_GROUP is the grouping of (query, probe)
+ * pairs by list that the matrix-core form of the sums kernel runs first (its
+ * QANN_STAGE_SCAN time then excludes it); no other form records it.
  * While enabled, every search records HIP events around each stage on the
  * stream the stage runs on (no synchronization: stages of concurrent streams
  * stay concurrent, and a stage's time includes what it waited for while others
@@ -227,7 +229,8 @@ enum {
     QANN_STAGE_FLAT = 6,
     QANN_STAGE_REPLAY = 7,
     QANN_STAGE_FLAT_FP32 = 8,
-    QANN_STAGE_COUNT = 9,
+    QANN_STAGE_GROUP = 9,
+    QANN_STAGE_COUNT = 10,
 };
 int qann_hip_profile_enable(int on);
 int qann_hip_profile_read(double* ms, uint64_t* launches, int reset);

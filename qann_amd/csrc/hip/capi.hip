@@ -171,6 +171,7 @@ uint64_t g_prof_n[QANN_STAGE_COUNT] = {};
 struct PendingStage {
     int stage;
     hipEvent_t a, b;
+    bool own_b = true; // false: b starts a later pending stage, which destroys it
 };
 std::vector<PendingStage> g_prof_pending; // guarded by g_prof_mu
 
@@ -184,7 +185,7 @@ void resolve_stages_locked() {
             g_prof_n[p.stage] += 1;
         }
         (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
+        if (p.own_b) (void)hipEventDestroy(p.b);
     }
     g_prof_pending.clear();
 }
@@ -200,6 +201,23 @@ struct StageTimer {
         if (!g_prof_on.load()) return;
         check(hipEventCreate(&a), "hipEventCreate");
         check(hipEventRecord(a, st), "hipEventRecord");
+    }
+    // An event recorded inside the stage (or null) for a first part timed as
+    // stage `first`; the rest stays this stage's.
+    hipEvent_t mark() {
+        hipEvent_t m = nullptr;
+        if (a) check(hipEventCreate(&m), "hipEventCreate");
+        return m;
+    }
+    void split(int first, hipEvent_t m, bool recorded) {
+        if (!m) return;
+        if (!recorded) {
+            (void)hipEventDestroy(m);
+            return;
+        }
+        std::lock_guard<std::mutex> g(g_prof_mu);
+        g_prof_pending.push_back(PendingStage{first, a, m, false});
+        a = m;
     }
     void done() {
         if (!a) return;
@@ -1050,8 +1068,10 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             a.order = order;
         }
         StageTimer t_scan(QANN_STAGE_SCAN, st);
-        qd::launch_fs_sums(a, static_cast<int>(nc), st);
+        a.grouped = t_scan.mark();
+        const int form = qd::launch_fs_sums(a, static_cast<int>(nc), st);
         check_launch("k_fs_sums");
+        t_scan.split(QANN_STAGE_GROUP, a.grouped, form == 2);
         t_scan.done();
         if (q0 + nc >= nq && scan_done) check(hipEventRecord(scan_done, st), "hipEventRecord");
         qd::ReplayArgs r{};
@@ -1801,7 +1821,7 @@ int qann_hip_profile_read(double* ms, uint64_t* launches, int reset) {
 
 const char* qann_hip_stage_name(int stage) {
     static const char* names[QANN_STAGE_COUNT] = {"rotate", "coarse", "select", "tables", "scan", "rerank", "flat",
-                                                  "replay", "flat_fp32"};
+                                                  "replay", "flat_fp32", "group"};
     return stage >= 0 && stage < QANN_STAGE_COUNT ? names[stage] : "";
 }
 

@@ -50,24 +50,27 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 template <int Ctrl>
-__device__ __forceinline__ float dpp_f32(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), Ctrl, 0xF, 0xF, false));
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), Ctrl, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int l) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
 }
 
 __device__ __forceinline__ float lane_f32(float x, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
 }
 
-// wave_max without LDS round trips, for a wave whose 64 lanes are all
-// active: DPP maxima within each row of 16 lanes (lanes i ^ 1, i ^ 2, then
-// the half-row and row mirrors), then the four rows' maxima read lane by
-// lane. fmaxf drops NaNs whatever the order, so both give the same value.
-__device__ __forceinline__ float wave_max_dpp(float v) {
-    v = fmaxf(v, dpp_f32<0xB1>(v));  // quad_perm [1, 0, 3, 2]
-    v = fmaxf(v, dpp_f32<0x4E>(v));  // quad_perm [2, 3, 0, 1]
-    v = fmaxf(v, dpp_f32<0x141>(v)); // row_half_mirror
-    v = fmaxf(v, dpp_f32<0x140>(v)); // row_mirror
-    return fmaxf(fmaxf(lane_f32(v, 0), lane_f32(v, 16)), fmaxf(lane_f32(v, 32), lane_f32(v, 48)));
+// The wave's largest value, for a wave whose 64 lanes are all active, in a
+// scalar register: DPP maxima within each row of 16 lanes (lanes i ^ 1,
+// i ^ 2, then the half-row and row mirrors), then the four rows' maxima.
+__device__ __forceinline__ uint32_t wave_max_u32_dpp(uint32_t v) {
+    v = max(v, dpp_u32<0xB1>(v));  // quad_perm [1, 0, 3, 2]
+    v = max(v, dpp_u32<0x4E>(v));  // quad_perm [2, 3, 0, 1]
+    v = max(v, dpp_u32<0x141>(v)); // row_half_mirror
+    v = max(v, dpp_u32<0x140>(v)); // row_mirror
+    return max(max(lane_u32(v, 0), lane_u32(v, 16)), max(lane_u32(v, 32), lane_u32(v, 48)));
 }
 
 // ---------------------------------------------------------------------------

@@ -1345,50 +1345,54 @@ __device__ __forceinline__ void row_lo_hi(const float t[16], float& lo, float& h
     }
 }
 
-// The 16-byte one-hot of nibble c (byte c is 1, the others 0).
-__device__ __forceinline__ v4i_t onehot16(uint32_t c) {
-    const uint64_t bit = uint64_t{1} << ((c & 7u) << 3);
-    const bool low = c < 8u;
-    const uint32_t lo = static_cast<uint32_t>(bit), hi = static_cast<uint32_t>(bit >> 32);
-    v4i_t v;
-    v[0] = static_cast<int>(low ? lo : 0u);
-    v[1] = static_cast<int>(low ? hi : 0u);
-    v[2] = static_cast<int>(low ? 0u : lo);
-    v[3] = static_cast<int>(low ? 0u : hi);
-    return v;
+// row_lo_hi for every lane of a wave, with no branch but the zero-sign pass,
+// which the wave takes only when one of its rows has a minimum of zero;
+// lanes whose row is past the table get lo = hi = 0.
+__device__ __forceinline__ void row_lo_hi_wave(const float t[16], bool row, float& lo, float& hi) {
+    lo = t[0];
+    hi = t[0];
+#pragma unroll
+    for (int c = 1; c < 16; ++c) {
+        lo = fminf(lo, t[c]);
+        hi = fmaxf(hi, t[c]);
+    }
+    const bool nan0 = t[0] != t[0];
+    lo = nan0 ? t[0] : lo;
+    hi = nan0 ? t[0] : hi;
+    if (__ballot(row && lo == 0.0f)) {
+        if (lo == 0.0f) {
+            for (int c = 15; c >= 0; --c)
+                if (t[c] == 0.0f) lo = t[c];
+        }
+    }
+    lo = row ? lo : 0.0f;
+    hi = row ? hi : 0.0f;
 }
 
-// Units of 32 pairs, one per wave: unit u is a chunk of list unit_list[u]'s
-// pairs. Workgroups are laid out XCD by XCD (workgroup b runs on XCD b % 8),
-// so consecutive units -- mostly one list's -- share an XCD's L2.
+// Units of 32 pairs: unit u is a chunk of list unit_list[u]'s pairs.
 constexpr int kMfmaWaves = 2;
 constexpr int kMfmaSubMax = 4; // PQ sub-vector lengths whose codebook rows stay in registers
 
 // LDS of one wave of k_fs_sums_mfma: the rounded LUT rows of 16 pairs on
 // their way into MFMA operands, those pairs' row minimums, and the 32 pairs'
-// scales and offsets; later two staged code blocks
+// scales and offsets; later two staged code blocks; then the 16 one-hot rows
+// (256 bytes) the blocks' MFMA A operands are read from
 __host__ __device__ inline size_t mfma_wave_lds(int m4) {
     const size_t build = static_cast<size_t>(16) * m4 * 16 + 16 * 64 * 4 + 32 * 8;
     const size_t blocks = static_cast<size_t>(2) * m4 * 16;
-    return ((build > blocks ? build : blocks) + 15) & ~size_t{15};
+    return (((build > blocks ? build : blocks) + 15) & ~size_t{15}) + 256;
 }
 
-// NK = m4 / 2 MFMA steps of 2 subspaces; IP: the query's one LUT (ip_lut),
-// else the precomputed L2 table built here per pair, for PQ sub-vectors of
-// SUB floats (1, 2; 4 for 3 and 4). The 32 pairs' LUT columns stay in
-// registers (NK x 16 bytes a lane) for all the list's blocks.
+// One unit on one wave. NK = m4 / 2 MFMA steps of 2 subspaces; IP: the
+// query's one LUT (ip_lut), else the precomputed L2 table built here per
+// pair, for PQ sub-vectors of SUB floats (1, 2; 4 for 3 and 4). The 32 pairs'
+// LUT columns stay in registers (NK x 16 bytes a lane) for all the list's
+// blocks.
 template <int NK, bool IP, int SUB>
-__global__ void __launch_bounds__(kMfmaWaves * kWave, 2) k_fs_sums_mfma(const SumsArgs a, const PairPlan pp) {
-    extern __shared__ __align__(16) unsigned char lds[];
+__device__ __forceinline__ void mfma_unit(const SumsArgs& a, const PairPlan& pp, unsigned char* mine,
+                                          const unsigned char* onehot, uint32_t u, int lane) {
     const DevIvf& ix = a.ix;
-    const int m = ix.m, lb = ix.m4 * 16, lane = lane_id();
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-    unsigned char* mine = lds + static_cast<size_t>(wave) * mfma_wave_lds(ix.m4);
-
-    const uint32_t groups = gridDim.x / 8;
-    const uint32_t wg = (blockIdx.x % 8) * groups + blockIdx.x / 8;
-    const uint32_t u = wg * kMfmaWaves + wave;
-    if (u >= pp.unit_off[ix.nlist]) return;
+    const int m = ix.m, lb = ix.m4 * 16;
     const int list = static_cast<int>(pp.unit_list[u]);
     const uint32_t first = pp.pair_off[list] + (u - pp.unit_off[list]) * 32;
     const int npairs = static_cast<int>(min(32u, pp.pair_off[list + 1] - first));
@@ -1481,21 +1485,24 @@ __global__ void __launch_bounds__(kMfmaWaves * kWave, 2) k_fs_sums_mfma(const Su
 #pragma unroll
                         for (int c = 0; c < 16; ++c) e[c] = e[c] + A;
                     }
-                    float rlo = 0.0f, rhi = 0.0f;
-                    if (row) row_lo_hi(e, rlo, rhi);
-                    const float widest = wave_max_dpp(row ? rhi - rlo : 0.0f);
-                    const float sc = widest > 0.0f ? __fdiv_rn(kLutMax, widest) : 1.0f;
+                    float rlo, rhi;
+                    row_lo_hi_wave(e, row, rlo, rhi);
+                    // the widest row by the bits of the positive widths (ordered as
+                    // the floats are): a width of 0, below or NaN leaves the scale
+                    // what fmaxf over them gives, so it counts as 0
+                    const float wd = rhi - rlo;
+                    const uint32_t widest = wave_max_u32_dpp(row && wd > 0.0f ? __float_as_uint(wd) : 0u);
+                    const float sc = widest ? __fdiv_rn(kLutMax, __uint_as_float(widest)) : 1.0f;
                     rmin[pk * 64 + r] = rlo;
                     if (lane == 0) so[2 * pj] = sc;
+                    // rows m .. m4 - 1 hold zeros (e and rlo are 0 there), so bytes 0
                     if (lane < ix.m4) {
                         uint32_t w[4] = {0u, 0u, 0u, 0u};
-                        if (row) {
 #pragma unroll
-                            for (int i = 0; i < 4; ++i)
+                        for (int i = 0; i < 4; ++i)
 #pragma unroll
-                                for (int c = 0; c < 4; ++c)
-                                    w[i] = __builtin_amdgcn_cvt_pk_u8_f32((e[4 * i + c] - rlo) * sc, c, w[i]);
-                        }
+                            for (int c = 0; c < 4; ++c)
+                                w[i] = __builtin_amdgcn_cvt_pk_u8_f32((e[4 * i + c] - rlo) * sc, c, w[i]);
                         *reinterpret_cast<uint4*>(rows16 + static_cast<size_t>(pk) * lb + r * 16) =
                             make_uint4(w[0], w[1], w[2], w[3]);
                     }
@@ -1532,9 +1539,23 @@ __global__ void __launch_bounds__(kMfmaWaves * kWave, 2) k_fs_sums_mfma(const Su
     const uint8_t* codes = ix.codes + ix.code_off[list];
     const bool loader = lane * 16 < lb;
     unsigned char* stage = mine;
-    uint4 next = make_uint4(0u, 0u, 0u, 0u);
-    if (loader) *reinterpret_cast<uint4*>(stage + lane * 16) = *reinterpret_cast<const uint4*>(codes + lane * 16);
-    if (nblk > 1 && loader) next = *reinterpret_cast<const uint4*>(codes + lb + lane * 16);
+    // Block loads are unconditional, of block min(b, nblk - 1) and column
+    // min(lane * 16, lb - 16), so that none waits at a branch join; only the
+    // loader lanes stage what they load. Blocks 1, 2 and 3 wait in registers:
+    // each block's load is made three blocks before the loop stages it.
+    const uint32_t last = nblk > 0 ? nblk - 1 : 0;
+    const int col = min(lane * 16, lb - 16);
+    auto load_block = [&](uint32_t bb) {
+        return *reinterpret_cast<const uint4*>(codes + static_cast<size_t>(min(bb, last)) * lb + col);
+    };
+    uint4 n1 = make_uint4(0u, 0u, 0u, 0u), n2 = n1, n3 = n1;
+    if (nblk > 0) {
+        const uint4 c = load_block(0);
+        if (loader) *reinterpret_cast<uint4*>(stage + lane * 16) = c;
+        n1 = load_block(1);
+        n2 = load_block(2);
+        n3 = load_block(3);
+    }
     const uint32_t blk0 = a.item_blk[item];
     uint16_t* qsums = a.sums + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
     uint16_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock) + blk0;
@@ -1543,54 +1564,99 @@ __global__ void __launch_bounds__(kMfmaWaves * kWave, 2) k_fs_sums_mfma(const Su
     // sits at (s / 4) * 64 + g * 16 + (s % 4) * 4
     const int v = lane & 31, g = (v & 15) >> 2, shift = 8 * (v & 3) + 4 * (v >> 4);
     uint32_t run_min = 0xFFFFFFFFu;
-    for (uint32_t b = 0; b < nblk; ++b) {
+    // block b: its MFMAs, then nb (the register holding block b + 1) staged
+    // and loaded again with block b + 4, then the sums stored
+    auto block = [&](uint32_t b, uint4& nb) {
         const unsigned char* buf = stage + (b & 1) * lb;
         wave_lds_sync();
         // two accumulators (even and odd subspace groups) halve the chain of dependent MFMAs
         v16i_t acc0 = {}, acc1 = {};
 #pragma unroll
         for (int s4 = 0; s4 < NK / 2; ++s4) {
-            const uint4 w = *reinterpret_cast<const uint4*>(buf + s4 * 64 + g * 16);
-            const uint32_t w0 = h ? w.y : w.x, w1 = h ? w.w : w.z; // subspaces 4 s4 + h and 4 s4 + 2 + h
+            // subspaces 4 s4 + h and 4 s4 + 2 + h
+            const uint32_t* wp = reinterpret_cast<const uint32_t*>(buf + s4 * 64 + g * 16) + h;
+            const uint32_t w0 = wp[0], w1 = wp[2];
+            const v4i_t a0 = *reinterpret_cast<const v4i_t*>(onehot + (__builtin_amdgcn_ubfe(w0, shift, 4) << 4));
+            const v4i_t a1 = *reinterpret_cast<const v4i_t*>(onehot + (__builtin_amdgcn_ubfe(w1, shift, 4) << 4));
             if (s4 & 1) {
-                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w0 >> shift) & 15u), lut[2 * s4], acc1, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w1 >> shift) & 15u), lut[2 * s4 + 1], acc1, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, lut[2 * s4], acc1, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, lut[2 * s4 + 1], acc1, 0, 0, 0);
             } else {
-                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w0 >> shift) & 15u), lut[2 * s4], acc0, 0, 0, 0);
-                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(onehot16((w1 >> shift) & 15u), lut[2 * s4 + 1], acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, lut[2 * s4], acc0, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, lut[2 * s4 + 1], acc0, 0, 0, 0);
             }
         }
-        // the block after next into the buffer just read, its load in flight
-        if (b + 1 < nblk && loader) *reinterpret_cast<uint4*>(stage + ((b + 1) & 1) * lb + lane * 16) = next;
-        if (b + 2 < nblk && loader)
-            next = *reinterpret_cast<const uint4*>(codes + static_cast<size_t>(b + 2) * lb + lane * 16);
-        // lane (pair j, half h) holds vectors 8 (i / 4) + 4 h + i % 4 of the block
+        if (b + 1 < nblk && loader) *reinterpret_cast<uint4*>(stage + ((b + 1) & 1) * lb + lane * 16) = nb;
+        nb = load_block(b + 4);
+        // lane (pair j, half h) holds vectors 8 (i / 4) + 4 h + i % 4 of the
+        // block. Stores are unconditional (no branch, so that the waits for
+        // the block loads stay exact): lanes past the unit's pairs hold its
+        // first pair and store that pair's values again.
+        uint32_t sv[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sv[i] = static_cast<uint32_t>(acc0[i] + acc1[i]);
+#pragma unroll
+        for (int v4 = 0; v4 < 4; ++v4)
+            *reinterpret_cast<uint2*>(qsums + static_cast<size_t>(b) * kFsBlock + 8 * v4 + 4 * h) =
+                make_uint2(sv[4 * v4] | (sv[4 * v4 + 1] << 16), sv[4 * v4 + 2] | (sv[4 * v4 + 3] << 16));
+        // the block's smallest sum over its vectors (all 32 but in a list's last block)
         uint32_t vmin = 0xFFFFFFFFu;
         const uint32_t left = n - b * kFsBlock;
+        if (left >= kFsBlock) {
 #pragma unroll
-        for (int v4 = 0; v4 < 4; ++v4) {
-            const int v0 = 8 * v4 + 4 * h;
-            const uint32_t s0 = static_cast<uint32_t>(acc0[4 * v4] + acc1[4 * v4]);
-            const uint32_t s1 = static_cast<uint32_t>(acc0[4 * v4 + 1] + acc1[4 * v4 + 1]);
-            const uint32_t s2 = static_cast<uint32_t>(acc0[4 * v4 + 2] + acc1[4 * v4 + 2]);
-            const uint32_t s3 = static_cast<uint32_t>(acc0[4 * v4 + 3] + acc1[4 * v4 + 3]);
-            if (valid)
-                *reinterpret_cast<uint2*>(qsums + static_cast<size_t>(b) * kFsBlock + v0) =
-                    make_uint2(s0 | (s1 << 16), s2 | (s3 << 16));
-            if (static_cast<uint32_t>(v0) < left) vmin = min(vmin, s0);
-            if (static_cast<uint32_t>(v0 + 1) < left) vmin = min(vmin, s1);
-            if (static_cast<uint32_t>(v0 + 2) < left) vmin = min(vmin, s2);
-            if (static_cast<uint32_t>(v0 + 3) < left) vmin = min(vmin, s3);
+            for (int i = 0; i < 16; ++i) vmin = min(vmin, sv[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (static_cast<uint32_t>(8 * (i / 4) + 4 * h + i % 4) < left) vmin = min(vmin, sv[i]);
         }
         vmin = min(vmin, static_cast<uint32_t>(__shfl_xor(static_cast<int>(vmin), 32, kWave)));
-        if (valid && h == 0) qbmin[b] = static_cast<uint16_t>(min(vmin, 0xFFFFu));
+        qbmin[b] = static_cast<uint16_t>(min(vmin, 0xFFFFu)); // the same value from both halves
         run_min = min(run_min, vmin);
+    };
+    // unrolled by three so that each of n1, n2, n3 keeps one role: block b
+    // stages the register that holds block b + 1 (b + 1 = 1, 2, 0 mod 3).
+    // The first three are peeled, so that the loop is entered with the loads
+    // and stores in flight that its back edge has, and its waits for each
+    // block's load leave the two younger loads in flight.
+    uint32_t b = 0;
+    if (nblk >= 3) {
+        block(0, n1);
+        block(1, n2);
+        block(2, n3);
+        for (b = 3; b + 3 <= nblk; b += 3) {
+            block(b, n1);
+            block(b + 1, n2);
+            block(b + 2, n3);
+        }
     }
+    if (b < nblk) block(b, n1);
+    if (b + 1 < nblk) block(b + 1, n2);
     if (valid && h == 0) {
         a.item_min[item] = run_min;
         a.item_scale[item] = scale;
         a.item_offset[item] = offset;
     }
+}
+
+// One wave per unit. Workgroups are laid out XCD by XCD (workgroup b runs
+// on XCD b % 8), so consecutive units -- mostly one list's -- share an XCD's
+// L2.
+template <int NK, bool IP, int SUB>
+__global__ void __launch_bounds__(kMfmaWaves * kWave, 2) k_fs_sums_mfma(const SumsArgs a, const PairPlan pp) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    unsigned char* mine = lds + static_cast<size_t>(wave) * mfma_wave_lds(a.ix.m4);
+    const uint32_t groups = gridDim.x / 8;
+    const uint32_t wg = (blockIdx.x % 8) * groups + blockIdx.x / 8;
+    const uint32_t u = wg * kMfmaWaves + wave;
+    if (u >= pp.unit_off[a.ix.nlist]) return;
+    // one-hot row c (byte c is 1): dword lane % 4 of row lane / 4; the block
+    // loop syncs before it reads them
+    unsigned char* onehot = mine + mfma_wave_lds(a.ix.m4) - 256;
+    reinterpret_cast<uint32_t*>(onehot)[lane] = (lane & 3) == (lane >> 4) ? 1u << (8 * ((lane >> 2) & 3)) : 0u;
+    mfma_unit<NK, IP, SUB>(a, pp, mine, onehot, u, lane);
 }
 
 // Phase 1 for lists stored as PQ codes (ADC), SQ codes or raw vectors: one
@@ -3037,23 +3103,20 @@ unsigned persistent_grid(size_t lds) {
 
 size_t mfma_lds(const SumsArgs& a) { return static_cast<size_t>(kMfmaWaves) * mfma_wave_lds(a.ix.m4); }
 
+template <int NK, bool IP, int SUB>
+void launch_mfma_grid(const SumsArgs& a, unsigned grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_fs_sums_mfma<NK, IP, SUB>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a, a.pairs);
+}
+
 template <bool IP>
 void launch_mfma_nk(const SumsArgs& a, unsigned grid, size_t lds, hipStream_t s) {
     switch (a.ix.m4 / 2) {
 #define QANN_NK(N)                                                                                              \
     case N:                                                                                                   \
-        if (IP)                                                                                               \
-            hipLaunchKernelGGL((k_fs_sums_mfma<N, true, 1>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,    \
-                               a.pairs);                                                                      \
-        else if (a.ix.sub == 1)                                                                               \
-            hipLaunchKernelGGL((k_fs_sums_mfma<N, false, 1>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,   \
-                               a.pairs);                                                                      \
-        else if (a.ix.sub == 2)                                                                               \
-            hipLaunchKernelGGL((k_fs_sums_mfma<N, false, 2>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,   \
-                               a.pairs);                                                                      \
-        else                                                                                                  \
-            hipLaunchKernelGGL((k_fs_sums_mfma<N, false, 4>), dim3(grid), dim3(kMfmaWaves * kWave), lds, s, a,   \
-                               a.pairs);                                                                      \
+        if (IP) launch_mfma_grid<N, true, 1>(a, grid, lds, s);                                                \
+        else if (a.ix.sub == 1) launch_mfma_grid<N, false, 1>(a, grid, lds, s);                               \
+        else if (a.ix.sub == 2) launch_mfma_grid<N, false, 2>(a, grid, lds, s);                               \
+        else launch_mfma_grid<N, false, 4>(a, grid, lds, s);                                                  \
         break;
         QANN_NK(2) QANN_NK(4) QANN_NK(6) QANN_NK(8) QANN_NK(10) QANN_NK(12) QANN_NK(14) QANN_NK(16)
         QANN_NK(18) QANN_NK(20) QANN_NK(22) QANN_NK(24) QANN_NK(26) QANN_NK(28) QANN_NK(30) QANN_NK(32)
@@ -3087,6 +3150,7 @@ void launch_mfma(const SumsArgs& a, int nq, hipStream_t s) {
         hipLaunchKernelGGL(k_pairs_scatter, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, a, nq,
                            pp.ghist, pp.items);
     }
+    if (a.grouped) (void)hipEventRecord(a.grouped, s);
     // units are at most one per 32 pairs plus one partial unit per list
     const uint64_t units = (items + 31) / 32 + static_cast<uint64_t>(nlist);
     const uint64_t groups = ((units + kMfmaWaves - 1) / kMfmaWaves + 7) / 8;
@@ -3133,7 +3197,7 @@ bool fs_mfma_ok(const SumsArgs& a) {
     return mfma_lds(a) <= kLdsMax;
 }
 
-void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
+int launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
     const size_t lds = fs_sums_lds(a);
     a.nq = nq;
     a.per_xcd = (nq + 7) / 8;
@@ -3142,15 +3206,16 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
     const char* force = std::getenv("QANN_AMD_SUMS_FORM");
     if (force && std::strcmp(force, "mfma") == 0 && mfma) {
         sums_form(2, a, nq, lds, s);
-        return;
+        return 2;
     }
     if (!a.queue) {
         sums_form(0, a, nq, lds, s);
-        return;
+        return 0;
     }
     if (force && (std::strcmp(force, "grid") == 0 || std::strcmp(force, "queue") == 0)) {
-        sums_form(std::strcmp(force, "queue") == 0 ? 1 : 0, a, nq, lds, s);
-        return;
+        const int f = std::strcmp(force, "queue") == 0 ? 1 : 0;
+        sums_form(f, a, nq, lds, s);
+        return f;
     }
     int nclass = 0;
     while ((1 << nclass) < nq) ++nclass;
@@ -3167,9 +3232,11 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
         hipEvent_t e0, e1;
         form = 0;
         if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+            SumsArgs trial = a;
+            trial.grouped = nullptr;
             for (int f = 0; f < (mfma ? 3 : 2); ++f) {
                 (void)hipEventRecord(e0, s);
-                sums_form(f, a, nq, lds, s);
+                sums_form(f, trial, nq, lds, s);
                 (void)hipEventRecord(e1, s);
                 if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t[f], e0, e1) != hipSuccess)
                     t[f] = 1e30f;
@@ -3191,6 +3258,7 @@ void launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
         g_sums_form[key] = form;
     }
     sums_form(form, a, nq, lds, s);
+    return form;
 }
 
 void launch_query_order(const uint16_t* probe_idx, int M, int nq, int nlist, uint32_t* hist, uint32_t* order,

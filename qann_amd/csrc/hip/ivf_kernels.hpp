@@ -91,6 +91,7 @@ struct SumsArgs {
     uint32_t* queue;
     // buffers of the matrix-core form, or null pointers when it may not run
     PairPlan pairs;
+    hipEvent_t grouped; // or null: recorded after the matrix-core form's pair grouping
 };
 
 // Phase 1 for PQ (ADC), SQ and float lists: the float distance of every
@@ -189,7 +190,10 @@ size_t fs_sums_lds(SumsArgs& a);
 // MFMA); which one a problem class runs is chosen by timing each on its first
 // launch (QANN_AMD_TUNE_LOG=1 prints the choice; QANN_AMD_SUMS_FORM=grid,
 // queue or mfma forces one).
-void launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
+// Returns the form launched: 0 one workgroup per window, 1 persistent
+// queue, 2 matrix cores (which records a.grouped, when set, after its pair
+// grouping).
+int launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
 // Whether the matrix-core form can run these sums: inner-product LUTs or
 // precomputed L2 tables, m4 <= 64, the pair buffers given.
 bool fs_mfma_ok(const SumsArgs& a);
