@@ -183,6 +183,43 @@ int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const f
                             uint32_t* out);
 
 /*
+ * The assign step of PQ codebook training (PQCodebook::train, src/pq.cpp:41-73,
+ * runs detail::kmeans on each subspace; its assign_nearest without BLAS is
+ * nearest_centroid, src/kmeans.cpp:102-114) for all m subspaces at once: for
+ * each of n rows (dim floats) and each subspace s, the centroid of
+ * centroids[s][0..K)[dim / m] at the smallest exact squared L2 distance of the
+ * row's sub-vector s (l2_distance's order), the first on ties. out: n * m
+ * codes, row-major. Host buffers; synchronous.
+ */
+int qann_hip_assign_subspaces(const float* rows, uint64_t n, uint32_t dim, uint32_t m, const float* centroids,
+                              uint32_t K, uint8_t* out);
+
+/*
+ * The encoding half of IVFIndex::add_batch_impl (src/ivf_index.cpp:124-150)
+ * for n prepared (and, for a rotated index, rotated) rows: each row's nearest
+ * list (detail::assign_nearest, exact squared L2, lowest list on ties) into
+ * out_lists, and for PQ / fast-scan storage the PQ code of its residual to the
+ * list's residual centre (format 13) or centroid (PQCodebook::encode,
+ * src/pq.cpp:87-163), for SQ storage its SQ code (SQ::encode, src/sq.cpp), into
+ * out_codes (n * m, or n * SQ code bytes; untouched for float storage). desc
+ * as for qann_hip_ivf_create (use_precomputed and cosine_ip are not read).
+ * Host buffers; synchronous.
+ */
+int qann_hip_ivf_encode(const qann_hip_ivf_desc* desc, const float* rows, uint64_t n, uint32_t* out_lists,
+                        uint8_t* out_codes);
+
+/*
+ * detail::fs_set (src/fast_scan.hpp:25-28) for nblocks fast-scan blocks of 32
+ * positions: position p of block b holds the 4-bit codes (m bytes, row-major
+ * in codes, n rows) of row row_at[b * 32 + p], or zero nibbles where that is
+ * 0xFFFFFFFF. Writes nblocks * m * 16 bytes in the reference's block layout,
+ * the bytes IVFIndex::add_batch_impl (:171-175) writes into a list's blocks.
+ * Host buffers; synchronous.
+ */
+int qann_hip_fs_pack(const uint8_t* codes, uint64_t n, uint32_t m, const uint32_t* row_at, uint64_t nblocks,
+                     uint8_t* out);
+
+/*
  * The two integer stages of the fast-scan on their own, run by the same
  * device code the search kernels use, so that each can be checked against the
  * reference's functions directly. Host buffers; synchronous.

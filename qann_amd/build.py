@@ -19,8 +19,8 @@ import sysconfig
 
 PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
-HIP_SRC = [PKG / "csrc" / "hip" / n for n in ("ivf_kernels.hip", "gemm_bf16.hip", "capi.hip")]
-HOST_SRC = [PKG / "csrc" / "host" / n for n in ("qfile.cpp", "metadata.cpp", "indexes.cpp", "serve.cpp", "bindings.cpp")]
+HIP_SRC = [PKG / "csrc" / "hip" / n for n in ("ivf_kernels.hip", "gemm_bf16.hip", "build_kernels.hip", "capi.hip")]
+HOST_SRC = [PKG / "csrc" / "host" / n for n in ("qfile.cpp", "metadata.cpp", "train.cpp", "indexes.cpp", "serve.cpp", "bindings.cpp")]
 HIP_LIB = PKG / "libqann_hip.so"
 EXT = PKG / ("_qann" + sysconfig.get_config_var("EXT_SUFFIX"))
 ARCH = os.environ.get("QANN_AMD_ARCH", "gfx950")

@@ -4,6 +4,7 @@
 #include "device_common.hpp"
 #include "gemm_bf16.hpp"
 #include "ivf_kernels.hpp"
+#include "build_kernels.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -1697,57 +1698,208 @@ int qann_hip_search_noalloc(qann_hip_index* h, const float* d_queries, uint64_t 
     });
 }
 
+namespace {
+
+// A non-blocking stream of one call, synchronized and destroyed at its end.
+struct OwnStream {
+    hipStream_t st = nullptr;
+    OwnStream() { check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate"); }
+    ~OwnStream() {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    OwnStream(const OwnStream&) = delete;
+    OwnStream& operator=(const OwnStream&) = delete;
+};
+
+// The nearest of k centroids for blocks of rows on one stream: the FP32 MFMA
+// GEMM of approximate scores, then the exact shortlist of k_coarse_select
+// (exact squared L2 in l2_distance's order, the lowest index on ties). Owns
+// its device buffers; rows in blocks so the block x k score matrix stays near
+// 1 GiB.
+struct NearestRunner {
+    TempBuffers t;
+    hipStream_t st = nullptr;
+    uint32_t dim = 0, k = 0;
+    uint64_t block = 0;
+    float cmax = 0.0f;
+    float *dc = nullptr, *csq = nullptr, *dq = nullptr, *D = nullptr, *val = nullptr;
+    uint16_t* idx = nullptr;
+
+    NearestRunner(const float* centroids, uint32_t k_, uint32_t dim_, uint64_t n) : dim(dim_), k(k_) {
+        if (dim == 0 || k == 0) invalid("dim and k must be >= 1");
+        if (k > 65535) invalid("at most 65535 centroids");
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        block = std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t{1} << 28) / k));
+        dc = t.get<float>(static_cast<size_t>(k) * dim);
+        csq = t.get<float>(k);
+        dq = t.get<float>(block * dim);
+        D = t.get<float>(block * k);
+        idx = t.get<uint16_t>(block);
+        val = t.get<float>(block);
+        check(hipMemcpyAsync(dc, centroids, static_cast<size_t>(k) * dim * 4, hipMemcpyHostToDevice, st),
+              "hipMemcpyAsync");
+        qd::launch_sqnorms(dc, static_cast<int>(k), static_cast<int>(dim), csq, st);
+        check_launch("k_sqnorms");
+        cmax = norm_bound(centroids, k, dim);
+    }
+    ~NearestRunner() {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    // b <= block rows of host memory: uploads them to dq and leaves each
+    // row's nearest centroid in idx (device).
+    void run(const float* rows, uint64_t b) {
+        check(hipMemcpyAsync(dq, rows, b * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        qd::launch_coarse_gemm(dq, static_cast<int>(b), dc, csq, static_cast<int>(k), static_cast<int>(dim), 0, D, st);
+        check_launch("k_coarse_gemm");
+        qd::launch_coarse_select(D, dq, static_cast<int>(b), dc, cmax, static_cast<int>(k), static_cast<int>(dim), 0, 1,
+                                 idx, val, st);
+        check_launch("k_coarse_select");
+    }
+};
+
+} // namespace
+
 int qann_hip_assign_nearest(const float* rows, uint64_t n, uint32_t dim, const float* centroids, uint32_t k,
                             uint32_t* out) {
     return guarded([&] {
         if (dim == 0 || k == 0) invalid("dim and k must be >= 1");
         if (k > 65535) invalid("at most 65535 centroids");
         if (n == 0) return;
-        hipStream_t st = nullptr;
-        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
-        struct Cleanup {
-            std::vector<void*> ptrs;
-            hipStream_t st;
-            ~Cleanup() {
-                (void)hipStreamSynchronize(st);
-                for (void* p : ptrs) (void)hipFree(p);
-                (void)hipStreamDestroy(st);
-            }
-        } cl{{}, st};
-        // rows in blocks so the (block x k) distance matrix stays near 1 GiB
-        const uint64_t block = std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t{1} << 28) / k));
-        float *dc = nullptr, *dq = nullptr, *D = nullptr, *val = nullptr;
-        uint16_t* idx = nullptr;
-        check(hipMalloc(&dc, static_cast<size_t>(k) * dim * 4), "hipMalloc");
-        cl.ptrs.push_back(dc);
-        check(hipMalloc(&dq, block * dim * 4), "hipMalloc");
-        cl.ptrs.push_back(dq);
-        check(hipMalloc(&D, block * k * 4), "hipMalloc");
-        cl.ptrs.push_back(D);
-        check(hipMalloc(&idx, block * 2), "hipMalloc");
-        cl.ptrs.push_back(idx);
-        check(hipMalloc(&val, block * 4), "hipMalloc");
-        cl.ptrs.push_back(val);
-        check(hipMemcpyAsync(dc, centroids, static_cast<size_t>(k) * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-        float* csq = nullptr;
-        check(hipMalloc(&csq, static_cast<size_t>(k) * 4), "hipMalloc");
-        cl.ptrs.push_back(csq);
-        qd::launch_sqnorms(dc, static_cast<int>(k), static_cast<int>(dim), csq, st);
-        check_launch("k_sqnorms");
-        const float cmax = norm_bound(centroids, k, dim);
-        std::vector<uint16_t> hidx(block);
-        for (uint64_t b0 = 0; b0 < n; b0 += block) {
-            const uint64_t b = std::min(block, n - b0);
-            check(hipMemcpyAsync(dq, rows + b0 * dim, b * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-            qd::launch_coarse_gemm(dq, static_cast<int>(b), dc, csq, static_cast<int>(k), static_cast<int>(dim), 0, D, st);
-            check_launch("k_coarse_gemm");
-            qd::launch_coarse_select(D, dq, static_cast<int>(b), dc, cmax, static_cast<int>(k), static_cast<int>(dim), 0, 1,
-                                     idx, val, st);
-            check_launch("k_coarse_select");
-            check(hipMemcpyAsync(hidx.data(), idx, b * 2, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        NearestRunner nr(centroids, k, dim, n);
+        std::vector<uint16_t> hidx(nr.block);
+        for (uint64_t b0 = 0; b0 < n; b0 += nr.block) {
+            const uint64_t b = std::min(nr.block, n - b0);
+            nr.run(rows + b0 * dim, b);
+            check(hipMemcpyAsync(hidx.data(), nr.idx, b * 2, hipMemcpyDeviceToHost, nr.st), "hipMemcpyAsync");
+            check(hipStreamSynchronize(nr.st), "hipStreamSynchronize");
             for (uint64_t i = 0; i < b; ++i) out[b0 + i] = hidx[i];
         }
+    });
+}
+
+int qann_hip_assign_subspaces(const float* rows, uint64_t n, uint32_t dim, uint32_t m, const float* centroids,
+                              uint32_t K, uint8_t* out) {
+    return guarded([&] {
+        if (dim == 0 || m == 0 || dim % m != 0) invalid("dim must be a positive multiple of m");
+        if (K == 0 || K > 256) invalid("K must be in [1, 256]");
+        if (n == 0) return;
+        const uint32_t sub = dim / m;
+        TempBuffers t;
+        OwnStream os;
+        hipStream_t st = os.st;
+        const uint64_t block = std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t{1} << 27) / dim));
+        float* dc = t.get<float>(static_cast<size_t>(m) * K * sub);
+        float* dx = t.get<float>(block * dim);
+        uint8_t* dout = t.get<uint8_t>(block * m);
+        check(hipMemcpyAsync(dc, centroids, static_cast<size_t>(m) * K * sub * 4, hipMemcpyHostToDevice, st),
+              "hipMemcpyAsync");
+        for (uint64_t b0 = 0; b0 < n; b0 += block) {
+            const uint64_t b = std::min(block, n - b0);
+            check(hipMemcpyAsync(dx, rows + b0 * dim, b * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+            qd::launch_assign_subspaces(dx, b, static_cast<int>(dim), static_cast<int>(m), static_cast<int>(sub), dc,
+                                        static_cast<int>(K), dout, st);
+            check_launch("k_assign_subspaces");
+            check(hipMemcpyAsync(out + b0 * m, dout, b * m, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        }
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+int qann_hip_ivf_encode(const qann_hip_ivf_desc* desc, const float* rows, uint64_t n, uint32_t* out_lists,
+                        uint8_t* out_codes) {
+    return guarded([&] {
+        if (!desc) invalid("null descriptor");
+        const qann_hip_ivf_desc& d = *desc;
+        if (d.dim == 0) invalid("dim must be >= 1");
+        if (d.nlist == 0 || d.nlist > 65535) invalid("nlist must be in [1, 65535]");
+        if (!d.centroids) invalid("no centroids");
+        const bool pq = d.storage == QANN_IVF_FASTSCAN || d.storage == QANN_IVF_PQ;
+        const bool sq = d.storage == QANN_IVF_SQ;
+        if (d.storage > QANN_IVF_FLOAT) invalid("unknown storage");
+        const uint32_t K = d.pq_k ? d.pq_k : 16;
+        if (pq) {
+            if (d.m == 0 || d.dim % d.m != 0) invalid("dim must be a positive multiple of m");
+            if (K > 256) invalid("pq_k must be in [1, 256]");
+            if (d.storage == QANN_IVF_FASTSCAN && K != 16) invalid("fast-scan needs 16 centroids per subspace");
+            if (!d.pq_centroids) invalid("no PQ centroids");
+        }
+        if (sq && ((d.sq_bits != 4 && d.sq_bits != 8) || !d.sq_vmin || !d.sq_step)) invalid("SQ needs 4 or 8 bits, vmin and step");
+        if (n == 0) return;
+        NearestRunner nr(d.centroids, d.nlist, d.dim, n);
+        const uint32_t sub = pq ? d.dim / d.m : 0;
+        const uint64_t vbytes = pq ? d.m : sq ? (d.sq_bits == 8 ? d.dim : (d.dim + 1) / 2) : 0;
+        float *centers = nullptr, *cent = nullptr, *cent_t = nullptr, *vmin = nullptr, *step = nullptr;
+        uint8_t* dcodes = nullptr;
+        if (pq) {
+            // residuals to each list's residual centre where the index keeps them
+            // (Cosine PQ), else to its centroid (IVFIndex::residual_center)
+            centers = nr.dc;
+            if (d.residual_centers) {
+                centers = nr.t.get<float>(static_cast<size_t>(d.nlist) * d.dim);
+                check(hipMemcpyAsync(centers, d.residual_centers, static_cast<size_t>(d.nlist) * d.dim * 4,
+                                     hipMemcpyHostToDevice, nr.st),
+                      "hipMemcpyAsync");
+            }
+            const size_t ncent = static_cast<size_t>(d.m) * K * sub;
+            cent = nr.t.get<float>(ncent);
+            cent_t = nr.t.get<float>(ncent);
+            check(hipMemcpyAsync(cent, d.pq_centroids, ncent * 4, hipMemcpyHostToDevice, nr.st), "hipMemcpyAsync");
+            qd::launch_pq_transpose(cent, static_cast<int>(d.m), static_cast<int>(K), static_cast<int>(sub), cent_t, nr.st);
+            check_launch("k_pq_transpose");
+        }
+        if (sq) {
+            vmin = nr.t.get<float>(d.dim);
+            step = nr.t.get<float>(d.dim);
+            check(hipMemcpyAsync(vmin, d.sq_vmin, d.dim * 4, hipMemcpyHostToDevice, nr.st), "hipMemcpyAsync");
+            check(hipMemcpyAsync(step, d.sq_step, d.dim * 4, hipMemcpyHostToDevice, nr.st), "hipMemcpyAsync");
+        }
+        if (vbytes) dcodes = nr.t.get<uint8_t>(nr.block * vbytes);
+        std::vector<uint16_t> hidx(nr.block);
+        for (uint64_t b0 = 0; b0 < n; b0 += nr.block) {
+            const uint64_t b = std::min(nr.block, n - b0);
+            nr.run(rows + b0 * d.dim, b);
+            if (pq) {
+                qd::launch_pq_encode(nr.dq, b, static_cast<int>(d.dim), nr.idx, centers, cent_t, static_cast<int>(d.m),
+                                     static_cast<int>(sub), static_cast<int>(K), dcodes, nr.st);
+                check_launch("k_pq_encode");
+            } else if (sq) {
+                qd::launch_sq_encode(nr.dq, b, static_cast<int>(d.dim), vmin, step, static_cast<int>(d.sq_bits), dcodes,
+                                     nr.st);
+                check_launch("k_sq_encode");
+            }
+            check(hipMemcpyAsync(hidx.data(), nr.idx, b * 2, hipMemcpyDeviceToHost, nr.st), "hipMemcpyAsync");
+            if (vbytes)
+                check(hipMemcpyAsync(out_codes + b0 * vbytes, dcodes, b * vbytes, hipMemcpyDeviceToHost, nr.st),
+                      "hipMemcpyAsync");
+            check(hipStreamSynchronize(nr.st), "hipStreamSynchronize");
+            for (uint64_t i = 0; i < b; ++i) out_lists[b0 + i] = hidx[i];
+        }
+    });
+}
+
+int qann_hip_fs_pack(const uint8_t* codes, uint64_t n, uint32_t m, const uint32_t* row_at, uint64_t nblocks,
+                     uint8_t* out) {
+    return guarded([&] {
+        if (m == 0 || m > 1024) invalid("m must be in [1, 1024]");
+        if (nblocks == 0) return;
+        // every position names a row of codes or none: the kernel reads codes[row * m + s]
+        for (uint64_t p = 0; p < nblocks * 32; ++p)
+            if (row_at[p] != 0xFFFFFFFFu && row_at[p] >= n) invalid("row_at names a row beyond n");
+        TempBuffers t;
+        OwnStream os;
+        hipStream_t st = os.st;
+        const size_t bb = static_cast<size_t>(m) * 16;
+        uint8_t* dc = t.get<uint8_t>(std::max<uint64_t>(n, 1) * m);
+        uint32_t* dr = t.get<uint32_t>(nblocks * 32);
+        uint8_t* dout = t.get<uint8_t>(nblocks * bb);
+        if (n) check(hipMemcpyAsync(dc, codes, n * m, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        check(hipMemcpyAsync(dr, row_at, nblocks * 32 * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        qd::launch_fs_pack(dc, static_cast<int>(m), dr, nblocks, dout, st);
+        check_launch("k_fs_pack");
+        check(hipMemcpyAsync(out, dout, nblocks * bb, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
     });
 }
 

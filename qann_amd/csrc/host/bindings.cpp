@@ -5,6 +5,7 @@
 // (read_locked / write_locked, python_bindings.cpp:97-131).
 #include "indexes.hpp"
 #include "serve.hpp"
+#include "train.hpp"
 
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -335,6 +336,11 @@ PYBIND11_MODULE(_qann, m) {
         .value("Cosine", Metric::Cosine)
         .value("InnerProduct", Metric::InnerProduct);
 
+    // vecengine::Clustering (include/vecengine/ivf_index.hpp:27-30)
+    py::enum_<ClusteringKind>(m, "Clustering", "How IVFIndex.train learns the coarse centroids")
+        .value("EMeans", ClusteringKind::EMeans, "e-means: mini-batch k-means with EMA centroids (default)")
+        .value("Lloyd", ClusteringKind::Lloyd, "Lloyd's k-means");
+
     m.def("set_num_threads", &set_num_threads, py::arg("n"), "Host threads for adds and training (0 = all cores)");
     m.def("get_num_threads", &num_threads);
 
@@ -503,20 +509,25 @@ PYBIND11_MODULE(_qann, m) {
              },
              py::arg("bits") = 8)
         .def("train",
-             [](IVFIndex& self, FloatArray data, uint64_t max_iters, const py::object& seed) {
+             [](IVFIndex& self, FloatArray data, uint64_t max_iters, const py::object& seed, ClusteringKind clustering,
+                uint64_t passes) {
                  if (data.ndim() != 2) throw py::type_error("data must be a 2-D array of shape (n, dim)");
                  if (static_cast<uint64_t>(data.shape(1)) != self.dim())
-                     throw std::invalid_argument("data dimension does not match index dimension");
+                     throw std::invalid_argument("training vector dimension does not match index dimension");
                  std::optional<uint32_t> s;
                  if (!seed.is_none()) s = seed.cast<uint32_t>();
                  const float* p = data.data();
                  const uint64_t n = static_cast<uint64_t>(data.shape(0));
                  write_locked(self, [&] {
-                     self.train(p, n, max_iters, s);
+                     self.train(p, n, max_iters, s, clustering == ClusteringKind::EMeans, passes);
                      return 0;
                  });
              },
-             py::arg("data"), py::arg("max_iters") = 10, py::arg("seed") = py::none())
+             py::arg("data"), py::arg("max_iters") = 10, py::arg("seed") = py::none(), py::kw_only(),
+             py::arg("clustering") = ClusteringKind::EMeans, py::arg("passes") = 2,
+             "Learn the coarse centroids (e-means in `passes` passes, or Lloyd's k-means for max_iters "
+             "iterations), then the PQ codebook (25 Lloyd iterations per subspace) or SQ ranges, as qann "
+             "trains them; every assign step runs on the GPU")
         .def_property("nprobe", [](const IVFIndex& self) { return self.nprobe(); },
                       [](IVFIndex& self, uint64_t v) { write_locked(self, [&] { self.set_nprobe(v); return 0; }); })
         .def("list_sizes",

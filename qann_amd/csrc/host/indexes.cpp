@@ -1,4 +1,6 @@
 #include "indexes.hpp"
+#include "parallel.hpp"
+#include "train.hpp"
 
 #include "../../../include/qann_hip.h"
 
@@ -18,37 +20,6 @@ namespace qh {
 namespace {
 
 std::atomic<size_t> g_threads{0};
-
-// Runs fn(i) for i in [0, n) on num_threads() threads (detail::parallel_for, src/parallel.hpp:19-64).
-template <class F>
-void parallel_for(size_t n, F&& fn) {
-    const size_t t = std::min(n, num_threads());
-    if (t <= 1) {
-        for (size_t i = 0; i < n; ++i) fn(i);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::exception_ptr err;
-    std::mutex m;
-    auto work = [&] {
-        for (;;) {
-            const size_t i = next.fetch_add(1);
-            if (i >= n) return;
-            try {
-                fn(i);
-            } catch (...) {
-                std::lock_guard<std::mutex> g(m);
-                if (!err) err = std::current_exception();
-                return;
-            }
-        }
-    };
-    std::vector<std::thread> pool;
-    for (size_t i = 1; i < t; ++i) pool.emplace_back(work);
-    work();
-    for (auto& th : pool) th.join();
-    if (err) std::rethrow_exception(err);
-}
 
 // IdMap::slots_of (src/id_map.cpp:93-112)
 std::vector<uint32_t> slots_of(const IdMap& m, const int64_t* ids, size_t n) {
@@ -123,66 +94,15 @@ void assign_gpu(const float* rows, uint64_t n, uint64_t dim, const float* cents,
     check_status(qann_hip_assign_nearest(rows, n, static_cast<uint32_t>(dim), cents, static_cast<uint32_t>(k), out));
 }
 
-// Lloyd's k-means with this library's own seeding and empty-cluster rule.
-std::vector<float> kmeans(const float* x, uint64_t n, uint64_t d, uint64_t k, uint64_t iters, std::mt19937& rng,
-                          bool spherical, std::vector<uint32_t>& assign, bool gpu) {
-    std::vector<uint64_t> pick(n);
-    std::iota(pick.begin(), pick.end(), 0);
-    for (uint64_t i = 0; i < k; ++i) std::swap(pick[i], pick[i + rng() % (n - i)]);
-    std::vector<float> c(k * d);
-    for (uint64_t i = 0; i < k; ++i) std::copy_n(x + pick[i] * d, d, c.data() + i * d);
-    assign.assign(n, 0);
-    auto do_assign = [&] {
-        if (gpu) {
-            assign_gpu(x, n, d, c.data(), k, assign.data());
-            return;
-        }
-        parallel_for(n, [&](size_t i) {
-            float best = INFINITY;
-            uint32_t bi = 0;
-            for (uint64_t j = 0; j < k; ++j) {
-                float s = 0.0f;
-                for (uint64_t t = 0; t < d; ++t) {
-                    const float diff = x[i * d + t] - c[j * d + t];
-                    s = std::fma(diff, diff, s);
-                }
-                if (s < best) {
-                    best = s;
-                    bi = static_cast<uint32_t>(j);
-                }
-            }
-            assign[i] = bi;
-        });
-    };
-    for (uint64_t it = 0; it < iters; ++it) {
-        do_assign();
-        std::vector<double> sum(k * d, 0.0);
-        std::vector<uint64_t> cnt(k, 0);
-        for (uint64_t i = 0; i < n; ++i) {
-            const uint32_t a = assign[i];
-            ++cnt[a];
-            for (uint64_t t = 0; t < d; ++t) sum[a * d + t] += x[i * d + t];
-        }
-        for (uint64_t j = 0; j < k; ++j) {
-            if (cnt[j] == 0) {
-                // an empty cluster restarts at a random row
-                std::copy_n(x + (rng() % n) * d, d, c.data() + j * d);
-                continue;
-            }
-            double norm = 0.0;
-            for (uint64_t t = 0; t < d; ++t) {
-                const double v = sum[j * d + t] / double(cnt[j]);
-                c[j * d + t] = static_cast<float>(v);
-                norm += v * v;
-            }
-            if (spherical && norm > 0) {
-                const double inv = 1.0 / std::sqrt(norm);
-                for (uint64_t t = 0; t < d; ++t) c[j * d + t] = static_cast<float>(c[j * d + t] * inv);
-            }
-        }
-    }
-    do_assign();
-    return c;
+// The trainer's assign steps on the GPU (train.hpp).
+void nearest_gpu(const float* rows, uint64_t n, uint64_t d, const float* cents, uint64_t k, uint32_t* out) {
+    assign_gpu(rows, n, d, cents, k, out);
+}
+
+void nearest_subspaces_gpu(const float* rows, uint64_t n, uint64_t dim, uint64_t m, const float* cents, uint64_t K,
+                           uint8_t* out) {
+    check_status(qann_hip_assign_subspaces(rows, n, static_cast<uint32_t>(dim), static_cast<uint32_t>(m), cents,
+                                           static_cast<uint32_t>(K), out));
 }
 
 } // namespace
@@ -460,16 +380,20 @@ void IVFIndex::enable_sq(int bits) {
     d_->sq_bits = bits;
 }
 
-void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed) {
+void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed, bool emeans_coarse,
+                     uint64_t passes) {
     if (d_->ids.slots > 0) throw std::logic_error("train() cannot be called after add()");
     if (d_->trained) throw std::logic_error("train() has already been called");
     if (n < d_->nlist) throw std::invalid_argument("training set must have at least nlist vectors");
     if (d_->rotate) throw std::runtime_error("training a rotated index is not supported by qann_amd; train it with qann and load it");
     const uint64_t d = d_->dim;
+    // the clusters are learnt on the unit vectors that adds and queries use (Cosine)
     const std::vector<float> x = prepare(vecs, n);
-    std::mt19937 rng(seed ? *seed : std::random_device{}());
-    std::vector<uint32_t> assign;
-    d_->centroids = kmeans(x.data(), n, d, d_->nlist, std::max<uint64_t>(iters, 1), rng, d_->metric == Cosine, assign, true);
+    const bool spherical = d_->metric == Cosine;
+    Clusters coarse = emeans_coarse ? emeans(x.data(), n, d, d_->nlist, passes, seed, spherical, nearest_gpu)
+                                    : lloyd_kmeans(x.data(), n, d, d_->nlist, iters, seed, spherical, nearest_gpu);
+    d_->centroids = std::move(coarse.centroids);
+    const std::vector<uint32_t>& assign = coarse.assign;
     if (d_->has_pq && d_->metric == Cosine) {
         // each list's plain mean as its residual centre (ivf_index.cpp:769-785)
         d_->residual_centers.assign(d_->nlist * d, 0.0f);
@@ -487,26 +411,17 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
         }
     }
     if (d_->has_pq) {
+        // PQ codebooks on the residuals to each row's list centre (:787-802),
+        // seeded with seed + 1
         PQ& pq = d_->pq;
         const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
-        // a capped sample of residuals, 256 per centroid (pq.cpp:46-60)
-        const uint64_t cap = 256 * pq.K;
-        std::vector<uint64_t> rows(n);
-        std::iota(rows.begin(), rows.end(), 0);
-        const uint64_t ns = std::min(n, cap);
-        for (uint64_t i = 0; i < ns; ++i) std::swap(rows[i], rows[i + rng() % (n - i)]);
-        for (uint64_t s = 0; s < pq.m; ++s) {
-            std::vector<float> sx(ns * pq.sub);
-            for (uint64_t i = 0; i < ns; ++i) {
-                const uint64_t r = rows[i];
-                for (uint64_t j = 0; j < pq.sub; ++j)
-                    sx[i * pq.sub + j] = x[r * d + s * pq.sub + j] - rc[size_t{assign[r]} * d + s * pq.sub + j];
-            }
-            std::vector<uint32_t> sa;
-            std::mt19937 srng(static_cast<uint32_t>(rng() + s));
-            const std::vector<float> sc = kmeans(sx.data(), ns, pq.sub, pq.K, 25, srng, false, sa, false);
-            std::copy(sc.begin(), sc.end(), pq.centroids.begin() + s * pq.K * pq.sub);
-        }
+        std::vector<float> resid(n * d);
+        parallel_for(n, [&](size_t i) {
+            const float* c = rc + size_t{assign[i]} * d;
+            for (uint64_t t = 0; t < d; ++t) resid[i * d + t] = x[i * d + t] - c[t];
+        });
+        pq.centroids = pq_codebooks(resid.data(), n, d, pq.m, pq.K, 25,
+                                    seed ? std::optional<uint32_t>(*seed + 1) : std::nullopt, nearest_subspaces_gpu);
     }
     if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = cosine_tables_by_recall(x.data(), n, assign, seed ? *seed : 0u);
     if (d_->sq_bits) {
@@ -700,6 +615,32 @@ static void sq_encode(const IvfData& x, const float* v, uint8_t* out) {
     }
 }
 
+// The device descriptor of an index's trained parts (no lists).
+static qann_hip_ivf_desc ivf_desc(const IvfData& x) {
+    qann_hip_ivf_desc desc{};
+    desc.dim = static_cast<uint32_t>(x.dim);
+    desc.nlist = static_cast<uint32_t>(x.nlist);
+    desc.metric = x.metric;
+    desc.cosine_ip = x.cosine_ip;
+    desc.centroids = x.centroids.data();
+    desc.residual_centers = x.residual_centers.empty() ? nullptr : x.residual_centers.data();
+    desc.rotation = x.rotate && !x.rotation.empty() ? x.rotation.data() : nullptr;
+    if (x.sq_bits) {
+        desc.storage = QANN_IVF_SQ;
+        desc.sq_bits = static_cast<uint32_t>(x.sq_bits);
+        desc.sq_vmin = x.sq_vmin.data();
+        desc.sq_step = x.sq_step.data();
+    } else if (x.has_pq) {
+        desc.storage = x.fast_scan ? QANN_IVF_FASTSCAN : QANN_IVF_PQ;
+        desc.m = static_cast<uint32_t>(x.pq.m);
+        desc.pq_k = static_cast<uint32_t>(x.pq.K);
+        desc.pq_centroids = x.pq.centroids.data();
+    } else {
+        desc.storage = QANN_IVF_FLOAT;
+    }
+    return desc;
+}
+
 // IVFIndex::add_batch_impl (src/ivf_index.cpp:111-195): rotation, the nearest
 // list of each vector (on the GPU), its PQ code of the residual, SQ code or the
 // raw row, and each list filled in input order.
@@ -712,48 +653,80 @@ void IVFIndex::add_prepared(const float* vecs, uint64_t n, const int64_t* idv) {
         rot = rotate_rows(d_->rotation, vecs, n, d);
         vecs = rot.data();
     }
-    std::vector<uint32_t> assign(n);
-    assign_gpu(vecs, n, d, d_->centroids.data(), d_->nlist, assign.data());
+    // nearest lists and codes on the GPU
     const PQ& pq = d_->pq;
     const uint64_t w = d_->sq_bits ? (d_->sq_bits == 8 ? d : (d + 1) / 2) : d_->has_pq ? pq.m : 0;
+    std::vector<uint32_t> assign(n);
     std::vector<uint8_t> codes(n * w);
-    if (d_->sq_bits) {
-        parallel_for(n, [&](size_t i) { sq_encode(*d_, vecs + i * d, codes.data() + i * w); });
-    } else if (d_->has_pq) {
-        const std::vector<float> cent_t = pq_transpose(pq);
-        const float* rc = d_->residual_centers.empty() ? d_->centroids.data() : d_->residual_centers.data();
-        parallel_for(n, [&](size_t i) {
-            std::vector<float> r(d);
-            const float* c = rc + size_t{assign[i]} * d;
-            for (uint64_t t = 0; t < d; ++t) r[t] = vecs[i * d + t] - c[t];
-            pq_encode(cent_t, pq.m, pq.sub, pq.K, r.data(), codes.data() + i * w);
-        });
+    const qann_hip_ivf_desc desc = ivf_desc(*d_);
+    check_status(qann_hip_ivf_encode(&desc, vecs, n, assign.data(), codes.data()));
+
+    // each list's new rows in input order
+    const uint64_t nlist = d_->nlist;
+    std::vector<uint64_t> start(nlist + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) ++start[assign[i] + 1];
+    for (uint64_t l = 0; l < nlist; ++l) start[l + 1] += start[l];
+    std::vector<uint32_t> order(n);
+    {
+        std::vector<uint64_t> cursor(start.begin(), start.end() - 1);
+        for (uint64_t i = 0; i < n; ++i) order[cursor[assign[i]]++] = static_cast<uint32_t>(i);
     }
     const uint32_t first = static_cast<uint32_t>(d_->ids.slots);
-    std::vector<std::vector<uint32_t>> members(d_->nlist);
-    for (uint64_t i = 0; i < n; ++i) members[assign[i]].push_back(static_cast<uint32_t>(i));
+
+    // fast-scan: the blocks each list's new rows touch, packed on the GPU, from
+    // the block holding the list's next position on
+    const bool fs = d_->has_pq && d_->fast_scan;
     const uint64_t bb = pq.m * 16;
-    parallel_for(d_->nlist, [&](size_t l) {
-        IvfList& list = d_->lists[l];
-        for (uint32_t i : members[l]) {
-            const uint64_t pos = list.ids.size();
-            list.ids.push_back(first + i);
-            if (d_->has_pq && d_->fast_scan) {
-                // fs_set (src/fast_scan.hpp:25-28): vector v of a block in the low
-                // nibble of byte v (v < 16) or the high nibble of byte v - 16
-                if (pos % 32 == 0) list.codes.resize(list.codes.size() + bb, 0);
-                uint8_t* block = list.codes.data() + (pos / 32) * bb;
-                const uint64_t v = pos % 32;
-                for (uint64_t s = 0; s < pq.m; ++s) {
-                    uint8_t& b = block[s * 16 + (v % 16)];
-                    const uint8_t code = codes[i * w + s];
-                    b = v < 16 ? static_cast<uint8_t>((b & 0xF0) | code) : static_cast<uint8_t>((b & 0x0F) | (code << 4));
-                }
-            } else if (w) {
-                list.codes.insert(list.codes.end(), codes.begin() + i * w, codes.begin() + (i + 1) * w);
-            } else {
-                list.vecs.insert(list.vecs.end(), vecs + i * d, vecs + (i + 1) * d);
+    std::vector<uint64_t> block_at(nlist + 1, 0);
+    std::vector<uint8_t> packed;
+    if (fs) {
+        for (uint64_t l = 0; l < nlist; ++l) {
+            const uint64_t pos0 = d_->lists[l].ids.size(), cnt = start[l + 1] - start[l];
+            block_at[l + 1] = block_at[l] + (cnt ? (pos0 + cnt - 1) / 32 - pos0 / 32 + 1 : 0);
+        }
+        const uint64_t nblocks = block_at[nlist];
+        std::vector<uint32_t> row_at(nblocks * 32, 0xFFFFFFFFu);
+        parallel_for(nlist, [&](size_t l) {
+            const uint64_t pos0 = d_->lists[l].ids.size(), fb = pos0 / 32;
+            for (uint64_t r = start[l]; r < start[l + 1]; ++r) {
+                const uint64_t p = pos0 + (r - start[l]);
+                row_at[(block_at[l] + p / 32 - fb) * 32 + p % 32] = order[r];
             }
+        });
+        packed.resize(nblocks * bb);
+        check_status(qann_hip_fs_pack(codes.data(), n, static_cast<uint32_t>(pq.m), row_at.data(), nblocks,
+                                      packed.data()));
+    }
+
+    parallel_for(nlist, [&](size_t l) {
+        if (start[l + 1] == start[l]) return;
+        IvfList& list = d_->lists[l];
+        const uint64_t pos0 = list.ids.size(), cnt = start[l + 1] - start[l];
+        for (uint64_t r = start[l]; r < start[l + 1]; ++r) list.ids.push_back(first + order[r]);
+        if (fs) {
+            // the first touched block may hold earlier rows: only the new
+            // positions' nibbles are taken from the packed block (fs_set)
+            const uint8_t* src = packed.data() + block_at[l] * bb;
+            uint64_t nb = block_at[l + 1] - block_at[l];
+            const uint64_t a = pos0 % 32;
+            if (a != 0) {
+                uint8_t* dst = list.codes.data() + (pos0 / 32) * bb;
+                const uint64_t e = std::min<uint64_t>(32, a + cnt);
+                for (uint64_t t = 0; t < bb; ++t) {
+                    const uint64_t j = t % 16;
+                    const uint8_t mask = static_cast<uint8_t>((j >= a && j < e ? 0x0F : 0) | (j + 16 >= a && j + 16 < e ? 0xF0 : 0));
+                    dst[t] = static_cast<uint8_t>((dst[t] & ~mask) | (src[t] & mask));
+                }
+                src += bb;
+                --nb;
+            }
+            list.codes.insert(list.codes.end(), src, src + nb * bb);
+        } else if (w) {
+            for (uint64_t r = start[l]; r < start[l + 1]; ++r)
+                list.codes.insert(list.codes.end(), codes.begin() + size_t{order[r]} * w, codes.begin() + (size_t{order[r]} + 1) * w);
+        } else {
+            for (uint64_t r = start[l]; r < start[l + 1]; ++r)
+                list.vecs.insert(list.vecs.end(), vecs + size_t{order[r]} * d, vecs + (size_t{order[r]} + 1) * d);
         }
     });
     d_->ids.add(n, idv);
@@ -774,28 +747,8 @@ qann_hip_index* IVFIndex::device() const {
     if (!d_->trained) throw std::logic_error("train() has not be run yet");
     if (d_->has_pq && d_->pq_distance != 0)
         throw std::runtime_error("qann_amd does not search PQ indexes scored by SDC; this index uses SDC");
-    qann_hip_ivf_desc desc{};
-    desc.dim = static_cast<uint32_t>(d_->dim);
-    desc.nlist = static_cast<uint32_t>(d_->nlist);
-    desc.metric = d_->metric;
-    desc.cosine_ip = d_->cosine_ip;
-    desc.centroids = d_->centroids.data();
-    desc.residual_centers = d_->residual_centers.empty() ? nullptr : d_->residual_centers.data();
-    desc.rotation = d_->rotate && !d_->rotation.empty() ? d_->rotation.data() : nullptr;
-    if (d_->sq_bits) {
-        desc.storage = QANN_IVF_SQ;
-        desc.sq_bits = static_cast<uint32_t>(d_->sq_bits);
-        desc.sq_vmin = d_->sq_vmin.data();
-        desc.sq_step = d_->sq_step.data();
-    } else if (d_->has_pq) {
-        desc.storage = d_->fast_scan ? QANN_IVF_FASTSCAN : QANN_IVF_PQ;
-        desc.m = static_cast<uint32_t>(d_->pq.m);
-        desc.pq_k = static_cast<uint32_t>(d_->pq.K);
-        desc.use_precomputed = wants_precomputed(*d_);
-        desc.pq_centroids = d_->pq.centroids.data();
-    } else {
-        desc.storage = QANN_IVF_FLOAT;
-    }
+    qann_hip_ivf_desc desc = ivf_desc(*d_);
+    desc.use_precomputed = d_->has_pq && !d_->sq_bits ? wants_precomputed(*d_) : 0;
     qann_hip_index* h = nullptr;
     check_status(qann_hip_ivf_create(&desc, &h));
     std::vector<uint32_t> sizes(d_->nlist);

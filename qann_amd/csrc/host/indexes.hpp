@@ -155,10 +155,13 @@ public:
     void enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate);
     // IVFIndex::enable_sq (src/ivf_index.cpp:103-109)
     void enable_sq(int bits);
-    // IVFIndex::train (src/ivf_index.cpp:725-806), with this library's own
-    // k-means (the clustering is not restated; the index it makes is in the
-    // reference's format and searches identically in both)
-    void train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed);
+    // IVFIndex::train (src/ivf_index.cpp:725-806): e-means in `passes` passes
+    // (emeans_coarse) or Lloyd's k-means for `iters` iterations, then the PQ
+    // codebooks and SQ ranges, as the reference trains them, with every
+    // assign step on the GPU; the cosine table mode is this library's own
+    // choice (cosine_tables_by_recall)
+    void train(const float* vecs, uint64_t n, uint64_t iters, std::optional<uint32_t> seed, bool emeans_coarse = true,
+               uint64_t passes = 2);
     const IvfData& data() const { return *d_; }
     const Metadata& meta() const override { return d_->meta; }
     Metadata& meta() override { return d_->meta; }
