@@ -156,7 +156,7 @@ def cpu_threads():
     return max(1, min(usable, int(share))) if share and share.isdigit() else max(1, usable)
 
 
-PROFILE = ROOT / "profiles" / "ivf_fastscan_1.2Mx100_nprobe128.md"
+PROFILE = ROOT / "profiles" / "ivf_fastscan_1.2Mx100.md"
 
 
 def profile_figure(name, kernel, nprobe):
