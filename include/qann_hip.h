@@ -143,6 +143,32 @@ typedef struct {
 int qann_hip_search(qann_hip_index* index, const float* queries, uint64_t nq, const qann_hip_search_params* params,
                     int64_t* out_ids, float* out_dists);
 
+/* One request of qann_hip_search_requests: nq query rows, its own k, nprobe and
+ * filter (host memory, as in qann_hip_search_params). */
+typedef struct {
+    uint64_t nq;
+    uint32_t k;
+    uint32_t nprobe;
+    const uint64_t* allowed_words;
+    uint64_t allowed_count;
+} qann_hip_request;
+
+/*
+ * Several requests at once, each answered as qann_hip_search would answer it
+ * alone: the work of concurrent DenseIndex::query_batch calls on one index
+ * (each under the shared lock, bindings/python_bindings.cpp:97-113). No single
+ * reference function does this; it is for callers that batch requests, such
+ * as a serving loop. queries holds the requests' rows back to back; request
+ * r's nq * k results start after those of requests 0 .. r-1 in out_ids /
+ * out_dists. On an IVF or refine index the requests share one launch, each
+ * query with its own k, nprobe and filter (requests that take a direct scan
+ * or have an empty filter run on their own); on a FlatIndex they run one after
+ * another. k_factor and filter_probe_scaling are the index's. Synchronous.
+ */
+int qann_hip_search_requests(qann_hip_index* index, const float* queries, const qann_hip_request* requests,
+                             uint32_t nreq, uint32_t k_factor, uint32_t filter_probe_scaling, int64_t* out_ids,
+                             float* out_dists);
+
 /*
  * qann_hip_search with queries, results and allowed words in device memory,
  * enqueued on stream (a hipStream_t; NULL = the default stream) and returning

@@ -15,6 +15,7 @@
 #include <tuple>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -672,15 +673,39 @@ bool coarse_split(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int 
 // One search on sc's stream. scan_wait / scan_done (fast-scan lists only, may
 // be null): the scan waits for scan_wait and records scan_done after its last
 // launch, so that a pipelined search's parts scan one after another.
+// A launch that carries several requests of one IVF / refine index, each with
+// its own k, nprobe and filter: the per-query arrays (device memory) and the
+// largest values over the launch, which size its buffers. The caller has
+// ruled out what such a launch does not run (FlatIndex, empty filters, the
+// direct scans) and filled pq.list_allowed.
+struct MixPlan {
+    qd::PerQuery pq;
+    uint32_t k_max = 0, kp_max = 0;
+    uint32_t M = 0; // probes selected per query
+    bool any_filtered = false;
+};
+
+// The first k[i] entries of each query's candidate row (kl wide) into its result row.
+__global__ void k_copy_results_pq(const int64_t* src_ids, const float* src_d, int kl, int kmax, int nq, const int* kq,
+                                  const uint64_t* off, int64_t* ids, float* dists) {
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<uint64_t>(nq) * kmax) return;
+    const int i = static_cast<int>(t / kmax), j = static_cast<int>(t % kmax);
+    if (j >= kq[i]) return;
+    ids[off[i] + j] = src_ids[static_cast<size_t>(i) * kl + j];
+    dists[off[i] + j] = src_d[static_cast<size_t>(i) * kl + j];
+}
+
 void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip_search_params& p,
                 const uint64_t* d_filter, int64_t* d_ids, float* d_dists, Scratch& sc, int* d_more = nullptr,
-                hipEvent_t scan_wait = nullptr, hipEvent_t scan_done = nullptr) {
+                hipEvent_t scan_wait = nullptr, hipEvent_t scan_done = nullptr, const MixPlan* mix = nullptr) {
     hipStream_t st = sc.stream;
-    const uint32_t k = p.k;
+    const uint32_t k = mix ? mix->k_max : p.k;
     if (nq == 0 || k == 0) return;
-    const bool filtered = p.allowed_words != nullptr;
+    const bool filtered = mix ? mix->any_filtered : p.allowed_words != nullptr;
+    if (mix && (h->kind == 1 || d_filter)) invalid("a launch of several requests runs on IVF and refine indexes only");
     // query_prepared: an empty filter returns empty rows (src/index.cpp:172)
-    if (filtered && p.allowed_count == 0) {
+    if (!mix && filtered && p.allowed_count == 0) {
         fill_empty(d_ids, d_dists, nq * k, st);
         return;
     }
@@ -758,9 +783,9 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             return;
         }
         const uint64_t cap = filtered ? p.allowed_count : h->live;
-        kp = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(k) * p.k_factor, cap));
+        kp = mix ? mix->kp_max : static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(k) * p.k_factor, cap));
         // RefineIndex::use_direct (src/refine_index.cpp:59-69): score the allowed rows directly
-        if (filtered && p.allowed_count <= 6ull * ix.nlist) {
+        if (!mix && filtered && p.allowed_count <= 6ull * ix.nlist) {
             sc.used = 0;
             const size_t selb = qd::block_select_bytes(static_cast<int>(k), h->labels != nullptr);
             const bool gsel = qlds + selb > qd::kLdsMax;
@@ -780,6 +805,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
             return;
         }
     }
+    if (mix && !refine) kp = mix->kp_max;
     if (kp == 0) {
         fill_empty(d_ids, d_dists, nq * k, st);
         return;
@@ -787,7 +813,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const bool flat_lists = ix.storage == QANN_IVF_SQ || ix.storage == QANN_IVF_FLOAT;
     // IVFIndex::direct_scan (src/ivf_index.cpp:248-256): without PQ, a filter of at
     // most 3 nlist slots is scored slot by slot, pushed in slot order
-    if (!refine && flat_lists && filtered && p.allowed_count <= 3ull * ix.nlist) {
+    if (!mix && !refine && flat_lists && filtered && p.allowed_count <= 3ull * ix.nlist) {
         sc.used = 0;
         const size_t selb = qd::block_select_bytes(static_cast<int>(k), h->labels != nullptr);
         const bool gsel = qlds + selb > qd::kLdsMax;
@@ -817,7 +843,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         return;
     }
     const uint32_t nlist = static_cast<uint32_t>(ix.nlist);
-    const uint32_t nprobe = p.nprobe;
+    const uint32_t nprobe = mix ? mix->M : p.nprobe;
     if (nprobe == 0 || nprobe > nlist) invalid("nprobe must be in [1, nlist]");
     const bool fs = ix.storage == QANN_IVF_FASTSCAN;
     const bool has_pq = fs || ix.storage == QANN_IVF_PQ;
@@ -833,7 +859,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
 
     // IVFIndex::scan's filter parameters (src/ivf_index.cpp:462, 502, 510-514)
     int min_probes = static_cast<int>(nprobe), wanted = 0, block_filter = 0;
-    if (filtered) {
+    if (filtered && !mix) {
         wanted = static_cast<int>(std::min<uint64_t>(kp, p.allowed_count));
         block_filter = p.allowed_count * 16 < ivfh->slots;
         if (p.filter_probe_scaling && p.allowed_count > 0) {
@@ -842,7 +868,8 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         }
     }
     const uint32_t M_all = nlist;
-    const uint32_t M = !filtered ? nprobe
+    const uint32_t M = mix       ? mix->M
+                       : !filtered ? nprobe
                        : d_more  ? std::min<uint32_t>(M_all, std::max<uint32_t>(64, 2 * static_cast<uint32_t>(min_probes)))
                                  : M_all;
     // sums of one query take at most the blocks of its M largest probed lists;
@@ -864,7 +891,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     need += gsel_keys ? pad256(nq * qd::coarse_select_cap(static_cast<int>(M)) * 8) : 0;
     need += use_pre || (ip_pq && !fs) ? pad256(nq * tsz * 4) : 0;
     need += ip_pq && fs ? pad256(nq * m * 16) + 2 * pad256(nq * 4) : 0;
-    need += filtered ? pad256(nlist * 4) : 0;
+    need += filtered && !mix ? pad256(nlist * 4) : 0;
     need += 2 * pad256(nq * 4) + pad256(nq * M * 4);    // plan
     need += 3 * pad256(nq * M * 4);                     // per-probe scale, offset, min
     need += pad256(nq * kp * 4) + pad256(nq * kp * 8) + pad256(nq * 4);
@@ -916,7 +943,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
     check_launch("k_probe_base");
     uint32_t* list_allowed = nullptr;
-    if (filtered) {
+    if (filtered && !mix) {
         list_allowed = sc.take<uint32_t>(nlist);
         qd::launch_list_allowed(ix, d_filter, list_allowed, st);
         check_launch("k_list_allowed");
@@ -925,7 +952,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     int* status = sc.take<int>(nq);
     uint32_t* item_blk = sc.take<uint32_t>(nq * M);
     qd::launch_probe_plan(ix, static_cast<int>(nq), static_cast<int>(M), static_cast<int>(nprobe), min_probes, wanted,
-                          pidx, list_allowed, nprobe_q, item_blk, status, st);
+                          pidx, list_allowed, nprobe_q, item_blk, status, st, mix ? mix->pq : qd::PerQuery{});
     check_launch("k_probe_plan");
     t_sel.done();
     if (filtered && M < nlist && d_more) {
@@ -1005,6 +1032,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.item_blk = item_blk + q0 * M;
         a.C = C ? C + q0 * tsz : nullptr;
         a.filter = filtered ? d_filter : nullptr;
+        if (mix) a.pq = mix->pq.at(q0);
         a.dists = fdists;
         a.stride_slots = stride;
         a.bmin = fbmin;
@@ -1026,6 +1054,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.fbmin = fbmin;
         r.item_min = a.item_min;
         r.filter = filtered ? d_filter : nullptr;
+        if (mix) r.pq = mix->pq.at(q0);
         r.out_dist = cd + q0 * kp;
         r.out_label = cl + q0 * kp;
         r.out_count = ccount + q0;
@@ -1090,6 +1119,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.item_offset = a.item_offset;
         r.item_min = a.item_min;
         r.filter = filtered ? d_filter : nullptr;
+        if (mix) r.pq = mix->pq.at(q0);
         r.out_dist = cd + q0 * kp;
         r.out_label = cl + q0 * kp;
         r.out_count = ccount + q0;
@@ -1105,9 +1135,14 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         uint64_t* gkeys = grerank ? sc.take<uint64_t>(nq * rkeys) : nullptr;
         StageTimer t_re(QANN_STAGE_RERANK, st);
         qd::launch_rerank(d_q, static_cast<int>(nq), h->dim, ip, h->rows, cl, ccount, static_cast<int>(kp),
-                          static_cast<int>(k), h->labels, d_ids, d_dists, gkeys, st);
+                          static_cast<int>(k), h->labels, d_ids, d_dists, gkeys, st, mix ? mix->pq : qd::PerQuery{});
         check_launch("k_rerank");
         t_re.done();
+    } else if (mix) {
+        hipLaunchKernelGGL(k_copy_results_pq, dim3(static_cast<unsigned>((nq * k + 255) / 256)), dim3(256), 0, st, cl,
+                           cd, static_cast<int>(kp), static_cast<int>(k), static_cast<int>(nq), mix->pq.k,
+                           mix->pq.out_off, d_ids, d_dists);
+        check_launch("k_copy_results_pq");
     } else {
         hipLaunchKernelGGL(k_copy_results, dim3(static_cast<unsigned>((nq * k + 255) / 256)), dim3(256), 0, st, cl, cd,
                            static_cast<int>(kp), static_cast<int>(k), static_cast<int>(nq), d_ids, d_dists);
@@ -1614,6 +1649,248 @@ int qann_hip_search(qann_hip_index* h, const float* queries, uint64_t nq, const 
             check(hipMemcpyAsync(out_ids, dids, nq * k * 8, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipMemcpyAsync(out_dists, dd, nq * k * 4, hipMemcpyDeviceToHost, io->stream), "hipMemcpyAsync");
             check(hipStreamSynchronize(io->stream), "hipStreamSynchronize");
+        } catch (...) {
+            (void)hipStreamSynchronize(io->stream);
+            checkin(h, io);
+            throw;
+        }
+        checkin(h, io);
+    });
+}
+
+int qann_hip_search_requests(qann_hip_index* h, const float* queries, const qann_hip_request* reqs, uint32_t nreq,
+                             uint32_t k_factor, uint32_t filter_probe_scaling, int64_t* out_ids, float* out_dists) {
+    return guarded([&] {
+        if (!h || (nreq && (!reqs || !queries || !out_ids || !out_dists))) invalid("null argument");
+        const uint64_t dim = h->dim, nwords = (h->slots + 63) / 64;
+        std::vector<uint64_t> qoff(nreq + 1, 0), ooff(nreq + 1, 0);
+        for (uint32_t r = 0; r < nreq; ++r) {
+            const qann_hip_request& q = reqs[r];
+            if (q.allowed_words) {
+                uint64_t set = 0;
+                for (uint64_t w = 0; w < nwords; ++w) set += static_cast<uint64_t>(__builtin_popcountll(q.allowed_words[w]));
+                if (set != q.allowed_count) invalid("allowed_count does not match the allowed bitmap");
+            }
+            qoff[r + 1] = qoff[r] + q.nq;
+            ooff[r + 1] = ooff[r] + q.nq * q.k;
+        }
+        auto params_of = [&](const qann_hip_request& q) {
+            qann_hip_search_params p{};
+            p.k = q.k;
+            p.nprobe = q.nprobe;
+            p.k_factor = k_factor;
+            p.filter_probe_scaling = filter_probe_scaling;
+            p.allowed_words = q.allowed_words;
+            p.allowed_count = q.allowed_count;
+            return p;
+        };
+        auto alone = [&](uint32_t r) {
+            const qann_hip_search_params p = params_of(reqs[r]);
+            const int rc = qann_hip_search(h, queries + qoff[r] * dim, reqs[r].nq, &p, out_ids + ooff[r],
+                                           out_dists + ooff[r]);
+            if (rc != QANN_OK) throw Error(rc, g_err);
+        };
+        if (h->kind == 1) { // FlatIndex: no probes to share; each request on its own
+            for (uint32_t r = 0; r < nreq; ++r) alone(r);
+            return;
+        }
+        qann_hip_index* ivfh = h->kind == 3 ? h->base : h;
+        const qd::DevIvf& ix = ivfh->ivf;
+        const uint32_t nlist = static_cast<uint32_t>(ix.nlist);
+        const bool refine = h->kind == 3;
+        const bool flat_lists = ix.storage == QANN_IVF_SQ || ix.storage == QANN_IVF_FLOAT;
+
+        // which requests share the launch, and each query's parameters
+        std::vector<uint32_t> mixed;
+        std::vector<int> kp_q, k_q, nprobe_q, minp_q, wanted_q, bf_q;
+        std::vector<uint64_t> off_q;
+        std::vector<int> filt_q; // index of the query's filter among the distinct ones, -1 for none
+        std::vector<const uint64_t*> filters;
+        std::map<const uint64_t*, int> filter_index;
+        uint64_t nm = 0, mixed_entries = 0;
+        uint32_t k_max = 0, kp_max = 0, M = 0, minp_max = 0;
+        bool any_filtered = false;
+        for (uint32_t r = 0; r < nreq; ++r) {
+            const qann_hip_request& q = reqs[r];
+            if (q.nq == 0 || q.k == 0) continue;
+            if (q.nprobe == 0 || q.nprobe > nlist) invalid("nprobe must be in [1, nlist]");
+            const bool f = q.allowed_words != nullptr;
+            const uint64_t cap = f ? q.allowed_count : h->live;
+            const uint32_t kp = refine ? static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(q.k) * k_factor, cap))
+                                       : q.k;
+            // what the launch does not run: empty results, the direct scans
+            if ((f && q.allowed_count == 0) || kp == 0 || (refine && f && q.allowed_count <= 6ull * nlist) ||
+                (!refine && flat_lists && f && q.allowed_count <= 3ull * nlist)) {
+                alone(r);
+                continue;
+            }
+            int min_probes = static_cast<int>(q.nprobe), wanted = 0, block_filter = 0, fi = -1;
+            if (f) {
+                wanted = static_cast<int>(std::min<uint64_t>(kp, q.allowed_count));
+                block_filter = q.allowed_count * 16 < ivfh->slots;
+                if (filter_probe_scaling) {
+                    const double want = std::ceil(0.75 * double(q.nprobe) * std::cbrt(double(ivfh->live) / double(q.allowed_count)));
+                    min_probes = static_cast<int>(std::clamp<size_t>(static_cast<size_t>(std::min(want, double(nlist))), q.nprobe, nlist));
+                }
+                auto it = filter_index.find(q.allowed_words);
+                if (it == filter_index.end()) {
+                    it = filter_index.emplace(q.allowed_words, static_cast<int>(filters.size())).first;
+                    filters.push_back(q.allowed_words);
+                }
+                fi = it->second;
+                any_filtered = true;
+                minp_max = std::max<uint32_t>(minp_max, static_cast<uint32_t>(min_probes));
+            }
+            mixed.push_back(r);
+            for (uint64_t i = 0; i < q.nq; ++i) {
+                kp_q.push_back(static_cast<int>(kp));
+                k_q.push_back(static_cast<int>(q.k));
+                off_q.push_back(mixed_entries + i * q.k);
+                nprobe_q.push_back(static_cast<int>(q.nprobe));
+                minp_q.push_back(min_probes);
+                wanted_q.push_back(wanted);
+                bf_q.push_back(block_filter);
+                filt_q.push_back(fi);
+            }
+            nm += q.nq;
+            mixed_entries += q.nq * q.k;
+            k_max = std::max(k_max, q.k);
+            kp_max = std::max(kp_max, kp);
+            M = std::max(M, q.nprobe);
+        }
+        if (nm == 0) return;
+        // probes selected per query: every request's nprobe, and for filtered
+        // ones the first guess of how many their stopping rule takes (queries
+        // it falls short for are searched again with every list)
+        if (any_filtered) M = std::max(M, std::min<uint32_t>(nlist, std::max<uint32_t>(64, 2 * minp_max)));
+        M = std::min(M, nlist);
+
+        check(hipSetDevice(h->device), "hipSetDevice");
+        Scratch* io = checkout(h);
+        try {
+            hipStream_t st = io->stream;
+            const uint64_t F = filters.size();
+            // staging: queries, results, filters and their list counts, the
+            // per-query arrays (twice: the launch and the queries searched again)
+            const size_t qb = pad256(nm * dim * 4), ob = pad256(mixed_entries * 8), db = pad256(mixed_entries * 4);
+            const size_t fb = pad256(F * nwords * 8), lb = pad256(F * nlist * 4);
+            const size_t ab = 7 * pad256(nm * 8) + 2 * pad256(nm * 8); // 7 arrays (<= 8 bytes) + 2 pointer arrays
+            const size_t mb = pad256((nm + 1) * 4) + qb;
+            char* buf = io->reserve_io(qb + ob + db + fb + lb + 2 * ab + mb + 256);
+            char* at = buf;
+            auto carve = [&](size_t bytes) {
+                char* p = at;
+                at += pad256(bytes);
+                return p;
+            };
+            float* dq = reinterpret_cast<float*>(carve(nm * dim * 4));
+            int64_t* dids = reinterpret_cast<int64_t*>(carve(mixed_entries * 8));
+            float* dd = reinterpret_cast<float*>(carve(mixed_entries * 4));
+            uint64_t* dfilt = reinterpret_cast<uint64_t*>(carve(F * nwords * 8));
+            uint32_t* dla = reinterpret_cast<uint32_t*>(carve(F * nlist * 4));
+            int* dmore = reinterpret_cast<int*>(carve((nm + 1) * 4));
+            float* gq = reinterpret_cast<float*>(carve(nm * dim * 4));
+            // the queries of the shared requests, in order
+            std::vector<float> hq(nm * dim);
+            {
+                uint64_t row = 0;
+                for (uint32_t r : mixed) {
+                    std::memcpy(hq.data() + row * dim, queries + qoff[r] * dim, reqs[r].nq * dim * 4);
+                    row += reqs[r].nq;
+                }
+            }
+            check(hipMemcpyAsync(dq, hq.data(), nm * dim * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+            for (uint64_t f = 0; f < F; ++f) {
+                check(hipMemcpyAsync(dfilt + f * nwords, filters[f], nwords * 8, hipMemcpyHostToDevice, st),
+                      "hipMemcpyAsync");
+                qd::launch_list_allowed(ix, dfilt + f * nwords, dla + f * nlist, st);
+                check_launch("k_list_allowed");
+            }
+            // per-query arrays of a set of the shared queries (all, or those searched again)
+            auto upload_plan = [&](const std::vector<uint32_t>& rows, MixPlan& mp) {
+                const size_t n = rows.size();
+                std::vector<int> a_kp(n), a_k(n), a_np(n), a_mp(n), a_w(n), a_bf(n);
+                std::vector<uint64_t> a_off(n);
+                std::vector<const uint64_t*> a_f(n);
+                std::vector<const uint32_t*> a_la(n);
+                for (size_t j = 0; j < n; ++j) {
+                    const uint32_t i = rows[j];
+                    a_kp[j] = kp_q[i];
+                    a_k[j] = k_q[i];
+                    a_off[j] = off_q[i];
+                    a_np[j] = nprobe_q[i];
+                    a_mp[j] = minp_q[i];
+                    a_w[j] = wanted_q[i];
+                    a_bf[j] = bf_q[i];
+                    a_f[j] = filt_q[i] >= 0 ? dfilt + static_cast<uint64_t>(filt_q[i]) * nwords : nullptr;
+                    a_la[j] = filt_q[i] >= 0 ? dla + static_cast<uint64_t>(filt_q[i]) * nlist : nullptr;
+                }
+                auto put = [&](const void* src, size_t bytes) {
+                    void* dst = carve(bytes);
+                    check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+                    return dst;
+                };
+                mp.pq.kp = static_cast<const int*>(put(a_kp.data(), n * 4));
+                mp.pq.k = static_cast<const int*>(put(a_k.data(), n * 4));
+                mp.pq.out_off = static_cast<const uint64_t*>(put(a_off.data(), n * 8));
+                mp.pq.nprobe = static_cast<const int*>(put(a_np.data(), n * 4));
+                mp.pq.min_probes = static_cast<const int*>(put(a_mp.data(), n * 4));
+                mp.pq.wanted = static_cast<const int*>(put(a_w.data(), n * 4));
+                mp.pq.block_filter = static_cast<const int*>(put(a_bf.data(), n * 4));
+                mp.pq.filter = static_cast<const uint64_t* const*>(put(a_f.data(), n * 8));
+                mp.pq.list_allowed = static_cast<const uint32_t* const*>(put(a_la.data(), n * 8));
+                mp.k_max = k_max;
+                mp.kp_max = kp_max;
+                mp.any_filtered = any_filtered;
+                // the host arrays are read by the copies: wait for them before they go
+                check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            };
+            std::vector<uint32_t> all(nm);
+            std::iota(all.begin(), all.end(), 0u);
+            MixPlan mp;
+            upload_plan(all, mp);
+            mp.M = M;
+            qann_hip_search_params p{};
+            p.k = k_max;
+            p.nprobe = M;
+            p.k_factor = k_factor;
+            p.filter_probe_scaling = filter_probe_scaling;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(dmore), 1);
+            check_launch("k_zero_u32");
+            run_search(h, dq, nm, p, nullptr, dids, dd, *io, any_filtered ? dmore : nullptr, nullptr, nullptr, &mp);
+            int more = 0;
+            if (any_filtered) check(hipMemcpyAsync(&more, dmore, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            if (more > 0) {
+                // the queries whose stopping rule needs more lists than the first
+                // guess: those again with every list, their rows written in place
+                std::vector<uint32_t> again(static_cast<size_t>(more));
+                check(hipMemcpyAsync(again.data(), dmore + 1, again.size() * 4, hipMemcpyDeviceToHost, st),
+                      "hipMemcpyAsync");
+                check(hipStreamSynchronize(st), "hipStreamSynchronize");
+                const uint32_t cnt = static_cast<uint32_t>(again.size());
+                for (uint32_t i : again)
+                    if (i >= nm) throw Error(QANN_EDEVICE, "a flagged query index is out of range");
+                // their rows and parameters, in the order the device listed them
+                hipLaunchKernelGGL(k_gather_rows, dim3(static_cast<unsigned>((uint64_t{cnt} * dim + 255) / 256)),
+                                   dim3(256), 0, st, dq, reinterpret_cast<const uint32_t*>(dmore + 1), cnt,
+                                   static_cast<uint32_t>(dim), gq);
+                check_launch("k_gather_rows");
+                MixPlan mp2;
+                upload_plan(again, mp2);
+                mp2.M = nlist;
+                qann_hip_search_params p2 = p;
+                p2.nprobe = nlist;
+                run_search(h, gq, cnt, p2, nullptr, dids, dd, *io, nullptr, nullptr, nullptr, &mp2);
+            }
+            uint64_t moff = 0;
+            for (uint32_t r : mixed) {
+                const uint64_t e = reqs[r].nq * reqs[r].k;
+                check(hipMemcpyAsync(out_ids + ooff[r], dids + moff, e * 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+                check(hipMemcpyAsync(out_dists + ooff[r], dd + moff, e * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+                moff += e;
+            }
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
         } catch (...) {
             (void)hipStreamSynchronize(io->stream);
             checkin(h, io);

@@ -1680,6 +1680,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
     const int qi = blockIdx.x / a.nwin, p0 = (blockIdx.x % a.nwin) * win;
     const int P = a.nprobe_q[qi];
     if (p0 >= P) return;
+    const uint64_t* filter = a.pq.filter ? a.pq.filter[qi] : a.filter;
     const int np = min(win, P - p0);
     const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
     const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
@@ -1770,7 +1771,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
                 const uint32_t entry = ix.list_off[list] + j;
                 const uint32_t slot = ix.list_ids[entry];
                 const bool ok = slot != 0xFFFFFFFFu &&
-                                (a.filter ? bit_test(a.filter, slot) : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                                (filter ? bit_test(filter, slot) : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
                 if (ok) {
                     if (adc) {
                         const uint8_t* code = ix.codes + static_cast<size_t>(entry) * m;
@@ -1825,6 +1826,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs
     const int qi = blockIdx.x / a.nwin, p0 = (blockIdx.x % a.nwin) * win;
     const int P = a.nprobe_q[qi];
     if (p0 >= P) return;
+    const uint64_t* filter = a.pq.filter ? a.pq.filter[qi] : a.filter;
     const int np = min(win, P - p0);
     const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
     const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
@@ -1884,7 +1886,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs
         uint32_t slot[kSq8Rows];
 #pragma unroll
         for (int r = 0; r < kSq8Rows; ++r) slot[r] = ix.list_ids[entry[r]];
-        const uint64_t* bits = a.filter ? a.filter : ix.deleted;
+        const uint64_t* bits = filter ? filter : ix.deleted;
         if (bits) {
             uint64_t word[kSq8Rows];
 #pragma unroll
@@ -1892,7 +1894,7 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs
 #pragma unroll
             for (int r = 0; r < kSq8Rows; ++r) {
                 const bool bit = (word[r] >> (slot[r] & 63)) & 1ull;
-                ok[r] = ok[r] && (a.filter ? bit : !bit);
+                ok[r] = ok[r] && (filter ? bit : !bit);
             }
         }
 #pragma unroll
@@ -2019,9 +2021,15 @@ __global__ void k_order_scatter(const uint16_t* probe_idx, int M, int nq, uint32
 // wanted exactly when that many allowed vectors were scanned).
 __global__ void k_probe_plan(const DevIvf ix, int nq, int M, int nprobe, int min_probes, int wanted,
                              const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
-                             int* status) {
+                             int* status, const PerQuery pq) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
+    if (pq.nprobe) {
+        nprobe = pq.nprobe[i];
+        min_probes = pq.min_probes[i];
+        wanted = pq.wanted[i];
+        list_allowed = pq.list_allowed[i];
+    }
     const uint16_t* pl = probe_idx + static_cast<size_t>(i) * M;
     uint64_t seen = 0;
     uint32_t blk = 0;
@@ -2049,10 +2057,16 @@ __global__ void k_probe_plan(const DevIvf ix, int nq, int M, int nprobe, int min
 __global__ void __launch_bounds__(256) k_probe_plan_wave(const DevIvf ix, int nq, int M, int nprobe, int min_probes,
                                                           int wanted, const uint16_t* probe_idx,
                                                           const uint32_t* list_allowed, int* nprobe_q,
-                                                          uint32_t* item_blk, int* status) {
+                                                          uint32_t* item_blk, int* status, const PerQuery pq) {
     const int i = blockIdx.x * 4 + static_cast<int>(threadIdx.x) / kWave;
     const int lane = lane_id();
     if (i >= nq) return;
+    if (pq.nprobe) {
+        nprobe = pq.nprobe[i];
+        min_probes = pq.min_probes[i];
+        wanted = pq.wanted[i];
+        list_allowed = pq.list_allowed[i];
+    }
     const uint16_t* pl = probe_idx + static_cast<size_t>(i) * M;
     uint32_t blk = 0;  // blocks of the probes taken so far
     uint64_t seen = 0; // their allowed entries
@@ -2119,13 +2133,14 @@ __global__ void k_list_allowed(const DevIvf ix, const uint64_t* filter, uint32_t
 // itself by (exact distance, slot)), just the set.
 __device__ void replay_finish(WaveTopK& tk, const ReplayArgs& a, int qi) {
     const DevIvf& ix = a.ix;
-    const int kp = a.kp;
+    const int kl = a.kp;                            // row stride of the outputs
+    const int kp = a.pq.kp ? a.pq.kp[qi] : a.kp;    // this query's capacity
     const int lane = lane_id();
 
     const int n = tk.size;
     if (a.unsorted) {
-        float* od = a.out_dist + static_cast<size_t>(qi) * kp;
-        int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
+        float* od = a.out_dist + static_cast<size_t>(qi) * kl;
+        int64_t* ol = a.out_label + static_cast<size_t>(qi) * kl;
         for (int i = lane; i < kp; i += kWave) {
             od[i] = i < n ? tk.dist[i] : INFINITY;
             ol[i] = i < n ? (ix.labels ? ix.labels[static_cast<uint32_t>(tk.label[i])] : tk.label[i]) : -1;
@@ -2181,8 +2196,8 @@ __device__ void replay_finish(WaveTopK& tk, const ReplayArgs& a, int qi) {
             wave_sync();
         }
     }
-    float* od = a.out_dist + static_cast<size_t>(qi) * kp;
-    int64_t* ol = a.out_label + static_cast<size_t>(qi) * kp;
+    float* od = a.out_dist + static_cast<size_t>(qi) * kl;
+    int64_t* ol = a.out_label + static_cast<size_t>(qi) * kl;
     for (int i = lane; i < kp; i += kWave) {
         od[i] = i < n ? tk.dist[i] : INFINITY;
         ol[i] = i < n ? tk.label[i] : -1;
@@ -2208,13 +2223,16 @@ template <bool FL, bool STAGE>
 __device__ void fs_replay(const ReplayArgs& a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevIvf& ix = a.ix;
-    const int kp = a.kp;
     const int qi = blockIdx.x;
+    const int kl = a.kp;                         // buffer layout: the launch's largest capacity
+    const int kp = a.pq.kp ? a.pq.kp[qi] : kl;   // this query's TopK capacity
+    const uint64_t* filter = a.pq.filter ? a.pq.filter[qi] : a.filter;
+    const int block_filter = a.pq.block_filter ? a.pq.block_filter[qi] : a.block_filter;
     const bool tk_lds = a.g_dist == nullptr;
-    int64_t* tk_label = tk_lds ? reinterpret_cast<int64_t*>(lds) : a.g_label + static_cast<size_t>(qi) * 2 * kp;
-    float* tk_dist = tk_lds ? reinterpret_cast<float*>(tk_label + 2 * kp) : a.g_dist + static_cast<size_t>(qi) * 2 * kp;
+    int64_t* tk_label = tk_lds ? reinterpret_cast<int64_t*>(lds) : a.g_label + static_cast<size_t>(qi) * 2 * kl;
+    float* tk_dist = tk_lds ? reinterpret_cast<float*>(tk_label + 2 * kl) : a.g_dist + static_cast<size_t>(qi) * 2 * kl;
     const int lane = lane_id();
-    const bool filtered = a.filter != nullptr;
+    const bool filtered = filter != nullptr;
     __shared__ uint32_t trim_hist[256];
     WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY, trim_hist};
     const int P = a.nprobe_q[qi];
@@ -2317,8 +2335,8 @@ __device__ void fs_replay(const ReplayArgs& a) {
                                 const bool valid = mine && in;
                                 const uint32_t slot = sid;
                                 bool allowed = valid;
-                                if (!FL && a.block_filter) {
-                                    allowed = valid && bit_test(a.filter, slot);
+                                if (!FL && block_filter) {
+                                    allowed = valid && bit_test(filter, slot);
                                     if (ballot(allowed) == 0) continue;
                                 }
                                 const bool surv = allowed && passes_now();
@@ -2331,7 +2349,7 @@ __device__ void fs_replay(const ReplayArgs& a) {
                                     ok = surv;
                                 } else {
                                     dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
-                                    ok = surv && (filtered ? bit_test(a.filter, slot)
+                                    ok = surv && (filtered ? bit_test(filter, slot)
                                                            : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
                                 }
                                 while (rem) {
@@ -2402,15 +2420,15 @@ __device__ void fs_replay(const ReplayArgs& a) {
                             const bool valid = mine && in;
                             uint32_t slot = 0;
                             bool allowed = valid;
-                            if (!FL && a.block_filter) {
+                            if (!FL && block_filter) {
                                 slot = valid ? ids[e] : 0u;
-                                allowed = valid && bit_test(a.filter, slot);
+                                allowed = valid && bit_test(filter, slot);
                                 if (ballot(allowed) == 0) continue;
                             }
                             const bool surv = allowed && passes_now();
                             uint64_t rem = ballot(surv);
                             if (rem == 0) continue;
-                            if ((FL || !a.block_filter) && surv) slot = ids[e];
+                            if ((FL || !block_filter) && surv) slot = ids[e];
                             float dist;
                             bool ok;
                             if (FL) {
@@ -2418,7 +2436,7 @@ __device__ void fs_replay(const ReplayArgs& a) {
                                 ok = surv;
                             } else {
                                 dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
-                                ok = surv && (filtered ? bit_test(a.filter, slot)
+                                ok = surv && (filtered ? bit_test(filter, slot)
                                                        : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
                             }
                             while (rem) {
@@ -2525,14 +2543,17 @@ __device__ void fs_replay_win(const ReplayArgs& a) {
     using BT = typename std::conditional<FL, uint32_t, uint16_t>::type;
     extern __shared__ __align__(16) unsigned char lds[];
     const DevIvf& ix = a.ix;
-    const int kp = a.kp;
     const int qi = blockIdx.x;
+    const int kl = a.kp;                         // LDS layout: the launch's largest capacity
+    const int kp = a.pq.kp ? a.pq.kp[qi] : kl;   // this query's TopK capacity
+    const uint64_t* filter = a.pq.filter ? a.pq.filter[qi] : a.filter;
+    const int block_filter = a.pq.block_filter ? a.pq.block_filter[qi] : a.block_filter;
     int64_t* tk_label = reinterpret_cast<int64_t*>(lds);
-    float* tk_dist = reinterpret_cast<float*>(tk_label + 2 * kp);
-    BT* sb = reinterpret_cast<BT*>(tk_dist + 2 * kp);
+    float* tk_dist = reinterpret_cast<float*>(tk_label + 2 * kl);
+    BT* sb = reinterpret_cast<BT*>(tk_dist + 2 * kl);
     const int lane = lane_id();
     const int v = lane & 31;
-    const bool filtered = a.filter != nullptr;
+    const bool filtered = filter != nullptr;
     const bool need_slot = !FL && (filtered || ix.deleted != nullptr);
     __shared__ uint32_t trim_hist[256];
     __shared__ int st_blk[kWave];
@@ -2645,7 +2666,7 @@ __device__ void fs_replay_win(const ReplayArgs& a) {
 #pragma unroll
                         for (int r = 0; r < kPfRows; ++r) {
                             if (r < rows && sv[r] < lim0) {
-                                const bool okb = filtered ? bit_test(a.filter, sl[r]) : !bit_test(ix.deleted, sl[r]);
+                                const bool okb = filtered ? bit_test(filter, sl[r]) : !bit_test(ix.deleted, sl[r]);
                                 if (!okb) sv[r] |= 0x10000u;
                             }
                         }
@@ -2709,7 +2730,7 @@ __device__ void fs_replay_win(const ReplayArgs& a) {
                                 if (h == 1 && 2 * r + 1 >= nc) break;
                                 const bool valid = (lane >> 5) == h && in;
                                 bool allowed = valid;
-                                if (!FL && a.block_filter) {
+                                if (!FL && block_filter) {
                                     allowed = valid && okb;
                                     if (ballot(allowed) == 0) continue;
                                 }
@@ -2764,12 +2785,15 @@ __global__ void __launch_bounds__(64) k_fl_replay(const ReplayArgs a) { fs_repla
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip, const float* rows, const int64_t* cand, const int* cand_count,
                          int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists, uint64_t* gkeys,
-                         int gstride) {
+                         int gstride, const PerQuery pq) {
     extern __shared__ __align__(16) unsigned char lds[];
     float* sq = reinterpret_cast<float*>(lds);
     int np = 1;
     const int i = blockIdx.x;
     const int n = cand_count[i];
+    // this query's result count and row (a launch of several requests), or the launch's
+    if (pq.k) k = pq.k[i];
+    const size_t row = pq.out_off ? pq.out_off[i] : static_cast<size_t>(i) * k;
     while (np < n) np <<= 1;
     uint64_t* keys = gkeys ? gkeys + static_cast<size_t>(i) * gstride : reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
     for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
@@ -2793,8 +2817,8 @@ __global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip,
             id = labels ? labels[slot] : static_cast<int64_t>(slot);
             d = exact_score(ip != 0, sq, rows + static_cast<size_t>(slot) * dim, dim);
         }
-        out_ids[static_cast<size_t>(i) * k + t] = id;
-        out_dists[static_cast<size_t>(i) * k + t] = d;
+        out_ids[row + t] = id;
+        out_dists[row + t] = d;
     }
 }
 
@@ -3327,15 +3351,15 @@ void launch_ivf_dists(DistArgs a, int nq, hipStream_t s) {
 
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
                        const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
-                       int* status, hipStream_t s) {
+                       int* status, hipStream_t s, const PerQuery& pq) {
     // QANN_AMD_PROBE_PLAN=thread keeps one thread per query
     const char* e = std::getenv("QANN_AMD_PROBE_PLAN");
     if (e && std::strcmp(e, "thread") == 0)
         hipLaunchKernelGGL(k_probe_plan, dim3(cdiv(nq, 128)), dim3(128), 0, s, ix, nq, M, nprobe, min_probes, wanted,
-                           probe_idx, list_allowed, nprobe_q, item_blk, status);
+                           probe_idx, list_allowed, nprobe_q, item_blk, status, pq);
     else
         hipLaunchKernelGGL(k_probe_plan_wave, dim3(cdiv(nq, 4)), dim3(256), 0, s, ix, nq, M, nprobe, min_probes,
-                           wanted, probe_idx, list_allowed, nprobe_q, item_blk, status);
+                           wanted, probe_idx, list_allowed, nprobe_q, item_blk, status, pq);
 }
 
 void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s) {
@@ -3385,11 +3409,11 @@ size_t rerank_keys(int kc) {
 
 void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
                    const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
-                   uint64_t* gkeys, hipStream_t s) {
+                   uint64_t* gkeys, hipStream_t s, const PerQuery& pq) {
     const size_t np = rerank_keys(kc);
     const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + (gkeys ? 0 : np * 8);
     hipLaunchKernelGGL(k_rerank, dim3(nq), dim3(256), lds, s, q, dim, ip, rows, cand, cand_count, kc, k, labels,
-                       out_ids, out_dists, gkeys, static_cast<int>(np));
+                       out_ids, out_dists, gkeys, static_cast<int>(np), pq);
 }
 
 size_t block_select_bytes(int k, bool by_label) {

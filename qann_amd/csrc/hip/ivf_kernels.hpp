@@ -96,6 +96,36 @@ struct SumsArgs {
 
 // Phase 1 for PQ (ADC), SQ and float lists: the float distance of every
 // vector of every (query, probe), +inf for vectors a query may not return.
+// Per-query search parameters of a launch that carries several requests (each
+// request its own k, nprobe and filter). Every array has one entry per query of
+// the launch (offset with at() for a chunk); a null array means the launch's
+// single value. kp and k are at most the launch's kp and k, which size the
+// buffers; filter[i] and list_allowed[i] are null for an unfiltered query.
+struct PerQuery {
+    const int* kp;                       // TopK capacity k' (candidates)
+    const int* k;                        // results
+    const uint64_t* out_off;             // first entry of the query's result row
+    const int* nprobe;
+    const int* min_probes;
+    const int* wanted;
+    const int* block_filter;
+    const uint64_t* const* filter;       // allowed-slot bitmap
+    const uint32_t* const* list_allowed; // allowed entries per list under that bitmap
+    __host__ __device__ PerQuery at(size_t q0) const {
+        PerQuery r = *this;
+        if (kp) r.kp += q0;
+        if (k) r.k += q0;
+        if (out_off) r.out_off += q0;
+        if (nprobe) r.nprobe += q0;
+        if (min_probes) r.min_probes += q0;
+        if (wanted) r.wanted += q0;
+        if (block_filter) r.block_filter += q0;
+        if (filter) r.filter += q0;
+        if (list_allowed) r.list_allowed += q0;
+        return r;
+    }
+};
+
 struct DistArgs {
     DevIvf ix;
     int M, win, nwin;
@@ -108,6 +138,7 @@ struct DistArgs {
     const uint32_t* item_blk;
     const float* C;            // nq * m * K: the query's inner table (kDistIp: the scoring table)
     const uint64_t* filter;    // allowed slots or null
+    PerQuery pq;               // per-query filters of a launch of several requests (pq.filter)
     float* dists;              // query i's region starts at i * stride_slots
     uint64_t stride_slots;
     uint32_t* bmin;            // fkey of each block's smallest distance
@@ -133,6 +164,7 @@ struct ReplayArgs {
     const float* item_offset;
     const uint32_t* item_min;  // fast-scan: smallest sum; float: fkey of the smallest distance
     const uint64_t* filter;    // allowed slots bitmap or null
+    PerQuery pq;               // per-query kp, filter and block_filter of a launch of several requests
     float* out_dist;           // nq * kp
     int64_t* out_label;        // nq * kp
     int* out_count;            // nq
@@ -204,7 +236,7 @@ void launch_query_order(const uint16_t* probe_idx, int M, int nq, int nlist, uin
                         hipStream_t s);
 void launch_probe_plan(const DevIvf& ix, int nq, int M, int nprobe, int min_probes, int wanted,
                        const uint16_t* probe_idx, const uint32_t* list_allowed, int* nprobe_q, uint32_t* item_blk,
-                       int* status, hipStream_t s);
+                       int* status, hipStream_t s, const PerQuery& pq = PerQuery{});
 void launch_list_allowed(const DevIvf& ix, const uint64_t* filter, uint32_t* out, hipStream_t s);
 void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s);
 // The LUT build and the block lookups of k_fs_sums on their own, for checking
@@ -223,7 +255,7 @@ void launch_ivf_dists(DistArgs a, int nq, hipStream_t s);
 size_t rerank_keys(int kc);
 void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, const int64_t* cand,
                    const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
-                   uint64_t* gkeys, hipStream_t s);
+                   uint64_t* gkeys, hipStream_t s, const PerQuery& pq = PerQuery{});
 // How k_block_select scores a slot: exact distance to its float row, found
 // at rows + slot * dim or through entry_of_slot (an IVF's float lists), or
 // the SQ score of its code (an IVF's SQ lists).

@@ -218,6 +218,60 @@ py::tuple batch_query(DenseIndex& self, FloatArray q, uint32_t k, const py::obje
     return py::make_tuple(ids, dists);
 }
 
+// Several batch_query calls answered in one launch: requests is a list of
+// (nq, k, nprobe, allowed) over consecutive rows of queries (nprobe 0: the
+// index's own; allowed: ids or None). Returns one (ids, distances) per request.
+py::list search_requests(DenseIndex& self, FloatArray q, const py::list& requests) {
+    if (q.ndim() != 2) throw py::type_error("queries must be a 2-D array of shape (num_queries, dim)");
+    if (static_cast<uint64_t>(q.shape(1)) != self.dim())
+        throw std::invalid_argument("query dimension does not match index dimension");
+    const size_t nreq = requests.size();
+    std::vector<DenseIndex::Request> reqs(nreq);
+    std::vector<py::array_t<int64_t>> allowed(nreq);
+    std::vector<bool> filtered(nreq, false);
+    uint64_t rows = 0, entries = 0;
+    for (size_t r = 0; r < nreq; ++r) {
+        const py::tuple t = requests[r].cast<py::tuple>();
+        if (t.size() != 4) throw py::type_error("each request is (nq, k, nprobe, allowed)");
+        reqs[r].nq = t[0].cast<uint64_t>();
+        reqs[r].k = t[1].cast<uint32_t>();
+        reqs[r].nprobe = t[2].cast<uint32_t>();
+        if (!t[3].is_none()) {
+            allowed[r] = as_ids(t[3]);
+            filtered[r] = true;
+        }
+        rows += reqs[r].nq;
+        entries += reqs[r].nq * reqs[r].k;
+    }
+    if (rows != static_cast<uint64_t>(q.shape(0))) throw std::invalid_argument("the requests must cover every query row");
+    std::vector<int64_t> ids(entries);
+    std::vector<float> dists(entries);
+    const float* qp = q.data();
+    read_locked(self, [&] {
+        std::vector<SlotFilter> filters(nreq);
+        for (size_t r = 0; r < nreq; ++r)
+            if (filtered[r]) {
+                filters[r] = self.filter_of(allowed[r].data(), static_cast<size_t>(allowed[r].shape(0)), true, nullptr);
+                reqs[r].filter = &filters[r];
+            }
+        const std::vector<float> prepared = self.prepare(qp, rows);
+        self.search_requests(prepared.data(), reqs.data(), nreq, ids.data(), dists.data());
+        return 0;
+    });
+    py::list out;
+    uint64_t at = 0;
+    for (size_t r = 0; r < nreq; ++r) {
+        const auto n = static_cast<py::ssize_t>(reqs[r].nq), k = static_cast<py::ssize_t>(reqs[r].k);
+        py::array_t<int64_t> ri({n, k});
+        py::array_t<float> rd({n, k});
+        std::copy_n(ids.data() + at, reqs[r].nq * reqs[r].k, ri.mutable_data());
+        std::copy_n(dists.data() + at, reqs[r].nq * reqs[r].k, rd.mutable_data());
+        at += reqs[r].nq * reqs[r].k;
+        out.append(py::make_tuple(ri, rd));
+    }
+    return out;
+}
+
 py::tuple query(DenseIndex& self, FloatArray q, uint32_t k, const py::object& allowed, const py::object& where_arg) {
     const std::optional<Expr> where = to_where(where_arg);
     if (q.ndim() != 1) throw py::type_error("query must be a 1-D array of shape (dim,)");
@@ -408,6 +462,11 @@ PYBIND11_MODULE(_qann, m) {
         .def("batch_query", &batch_query, py::arg("query"), py::arg("k"), py::kw_only(),
              py::arg("allowed") = py::none(), py::arg("where") = py::none(),
              "Query k-NN for each row; returns (indices, distances) of shape (num_queries, k), padded with -1 / inf")
+        .def("search_requests", &search_requests, py::arg("queries"), py::arg("requests"),
+             "Several batch_query calls at once: requests is a list of (nq, k, nprobe, allowed) over consecutive "
+             "rows of queries (nprobe 0: the index's own; allowed: ids or None). IVF and refine indexes answer "
+             "them in one launch, each query with its own k, nprobe and filter. Returns one (indices, distances) "
+             "per request, each as batch_query would return it")
         .def("remove",
              [](DenseIndex& self, const py::object& ids) {
                  refuse_if_wrapped(self, "remove");

@@ -211,6 +211,30 @@ void DenseIndex::search_prepared(const float* prepared, uint64_t nq, uint32_t k,
         for (size_t i = 0; i < total; ++i) out_dists[i] *= 0.5f;
 }
 
+void DenseIndex::search_requests(const float* prepared, const Request* reqs, size_t nreq, int64_t* out_ids,
+                                 float* out_dists) const {
+    qann_hip_search_params base{};
+    fill_params(&base);
+    std::vector<qann_hip_request> r(nreq);
+    size_t total = 0;
+    for (size_t i = 0; i < nreq; ++i) {
+        r[i].nq = reqs[i].nq;
+        r[i].k = reqs[i].k;
+        r[i].nprobe = reqs[i].nprobe ? reqs[i].nprobe : base.nprobe;
+        if (reqs[i].filter) {
+            r[i].allowed_words = reqs[i].filter->words.data();
+            r[i].allowed_count = reqs[i].filter->count;
+        }
+        total += static_cast<size_t>(reqs[i].nq) * reqs[i].k;
+    }
+    std::fill_n(out_ids, total, int64_t{-1});
+    std::fill_n(out_dists, total, INFINITY);
+    check_status(qann_hip_search_requests(device(), prepared, r.data(), static_cast<uint32_t>(nreq), base.k_factor,
+                                          base.filter_probe_scaling, out_ids, out_dists));
+    if (metric() == Cosine)
+        for (size_t i = 0; i < total; ++i) out_dists[i] *= 0.5f;
+}
+
 uint64_t DenseIndex::scratch_bytes(uint64_t nq, uint32_t k, uint32_t nprobe, bool filtered) const {
     qann_hip_search_params p{};
     fill_params(&p);

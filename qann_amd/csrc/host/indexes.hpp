@@ -55,6 +55,16 @@ public:
                          int64_t* out_ids, float* out_dists) const;
     // An upper estimate of the device scratch such a search takes.
     uint64_t scratch_bytes(uint64_t nq, uint32_t k, uint32_t nprobe, bool filtered) const;
+    // Several search_prepared calls answered at once (qann_hip_search_requests):
+    // request r takes the next reqs[r].nq prepared rows and writes its nq * k
+    // results after those of the requests before it. The caller holds the locks.
+    struct Request {
+        uint64_t nq = 0;
+        uint32_t k = 0, nprobe = 0; // nprobe 0: the index's own
+        const SlotFilter* filter = nullptr;
+    };
+    void search_requests(const float* prepared, const Request* reqs, size_t nreq, int64_t* out_ids,
+                         float* out_dists) const;
     // The C-ABI handle of the device copy (built if needed; the caller holds
     // the lock shared), valid until the index changes.
     qann_hip_index* device_handle() const { return device(); }

@@ -109,27 +109,33 @@ void Server::run() {
         group.push_back(std::move(queue_.front()));
         queue_.pop_front();
         uint64_t nq = group[0].req.nq;
-        const uint32_t k = group[0].req.k, nprobe = group[0].req.nprobe;
-        if (!group[0].req.filter) {
-            // queued requests that can share the launch, oldest first
-            for (auto it = queue_.begin(); it != queue_.end() && nq < max_batch_;) {
-                const ServeRequest& r = it->req;
-                if (!r.filter && r.k == k && r.nprobe == nprobe && nq + r.nq <= max_batch_) {
-                    uint64_t b = 0;
-                    try {
-                        b = estimate(nq + r.nq, k, nprobe, false);
-                    } catch (...) {
-                        break; // the launch keeps what it has; the rest is planned again later
-                    }
-                    if (inflight_ > 0 && inflight_ + b > budget_) break;
-                    nq += r.nq;
-                    bytes = b;
-                    group.push_back(std::move(*it));
-                    it = queue_.erase(it);
-                } else {
-                    ++it;
-                }
+        uint32_t k_max = group[0].req.k, np_max = group[0].req.nprobe;
+        bool any_filter = group[0].req.filter != nullptr;
+        // queued requests that share the launch, oldest first: any k, nprobe
+        // and filter (each query keeps its own), while the launch's queries
+        // stay within max_batch and its scratch within the budget
+        for (auto it = queue_.begin(); it != queue_.end() && nq < max_batch_;) {
+            const ServeRequest& r = it->req;
+            if (nq + r.nq > max_batch_) {
+                ++it;
+                continue;
             }
+            const uint32_t k2 = std::max(k_max, r.k), np2 = std::max(np_max, r.nprobe);
+            const bool f2 = any_filter || r.filter != nullptr;
+            uint64_t b = 0;
+            try {
+                b = estimate(nq + r.nq, k2, np2, f2);
+            } catch (...) {
+                break; // the launch keeps what it has; the rest is planned again later
+            }
+            if (inflight_ > 0 && inflight_ + b > budget_) break;
+            nq += r.nq;
+            k_max = k2;
+            np_max = np2;
+            any_filter = f2;
+            bytes = b;
+            group.push_back(std::move(*it));
+            it = queue_.erase(it);
         }
         inflight_ += bytes;
         ++launches_;
@@ -143,34 +149,40 @@ void Server::run() {
 
 void Server::execute(std::vector<Pending>& group, uint64_t nq) {
     const uint64_t d = index_->dim();
-    const uint32_t k = group[0].req.k, nprobe = group[0].req.nprobe;
     std::vector<float> raw(nq * d);
+    std::vector<DenseIndex::Request> reqs(group.size());
+    std::vector<uint64_t> at(group.size() + 1, 0); // each request's first result entry
     uint64_t row = 0;
-    for (const Pending& p : group) {
-        std::copy(p.req.queries.begin(), p.req.queries.end(), raw.begin() + row * d);
-        row += p.req.nq;
+    for (size_t g = 0; g < group.size(); ++g) {
+        const ServeRequest& r = group[g].req;
+        std::copy(r.queries.begin(), r.queries.end(), raw.begin() + row * d);
+        row += r.nq;
+        reqs[g].nq = r.nq;
+        reqs[g].k = r.k;
+        reqs[g].nprobe = r.nprobe;
+        reqs[g].filter = r.filter.get();
+        at[g + 1] = at[g] + r.nq * r.k;
     }
-    std::vector<int64_t> ids(nq * k);
-    std::vector<float> dists(nq * k);
+    std::vector<int64_t> ids(at.back());
+    std::vector<float> dists(at.back());
     try {
         auto locks = read_chain(*index_);
         const std::vector<float> q = index_->prepare(raw.data(), nq);
-        index_->search_prepared(q.data(), nq, k, nprobe, group[0].req.filter.get(), ids.data(), dists.data());
+        index_->search_requests(q.data(), reqs.data(), reqs.size(), ids.data(), dists.data());
     } catch (...) {
         for (Pending& p : group) p.promise.set_exception(std::current_exception());
         return;
     }
     const Clock::time_point t1 = Clock::now();
     std::vector<double> lat;
-    row = 0;
-    for (Pending& p : group) {
+    for (size_t g = 0; g < group.size(); ++g) {
+        Pending& p = group[g];
         ServeResult r;
-        r.ids.assign(ids.begin() + row * k, ids.begin() + (row + p.req.nq) * k);
-        r.dists.assign(dists.begin() + row * k, dists.begin() + (row + p.req.nq) * k);
+        r.ids.assign(ids.begin() + at[g], ids.begin() + at[g + 1]);
+        r.dists.assign(dists.begin() + at[g], dists.begin() + at[g + 1]);
         r.latency_ms = std::chrono::duration<double, std::milli>(t1 - p.t0).count();
         r.launch_queries = nq;
         lat.push_back(r.latency_ms);
-        row += p.req.nq;
         p.promise.set_value(std::move(r));
     }
     std::lock_guard<std::mutex> g(mu_);

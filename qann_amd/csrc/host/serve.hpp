@@ -1,8 +1,8 @@
 // A serving loop over one index: requests of any size, k, nprobe and filter
 // queue up; worker threads admit them into searches as device memory allows,
-// coalescing queued requests that can share a launch (same k and nprobe, no
-// filter), and run several launches at once, each on its own HIP stream and
-// scratch. The reference has no such loop; its queries are synchronous calls
+// coalescing queued requests into shared launches (each query keeps its own
+// k, nprobe and filter, qann_hip_search_requests), and run several launches
+// at once, each on its own HIP stream and scratch. The reference has no such loop; its queries are synchronous calls
 // (bindings/python_bindings.cpp:599-642), which one launch here reproduces
 // bit for bit whatever else shares it.
 #pragma once
