@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 10M vectors (and of nlist)")
     ap.add_argument("--seconds", type=float, default=6.0, help="length of each load level")
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--max-batch", type=int, default=4096, help="queries one launch may take")
     ap.add_argument("--stage-seconds", type=float, default=2.0, help="length of the stage-profiled load (0: none)")
     args = ap.parse_args()
 
@@ -155,7 +156,7 @@ def main():
     log(f"plain FP32, one launch per request: {plain['qps']:,.0f} queries/s over {plain['requests']} requests "
         f"(p50 {plain['p50_ms']:.2f} ms, p99 {plain['p99_ms']:.2f} ms per request)")
 
-    server = qann_amd.Server(index, workers=args.workers, max_batch=4096)
+    server = qann_amd.Server(index, workers=args.workers, max_batch=args.max_batch)
     # served results are each query's own: a sample checked against single calls
     checks = []
     for r in range(40):
@@ -214,6 +215,7 @@ def main():
         "recall@10": recall,
         "served_equal_to_single_calls": f"{same}/{len(checks)}",
         "workers": args.workers,
+        "max_batch": args.max_batch,
         "stage_ms_per_launch_at_25pct": stages,
     }), flush=True)
 
