@@ -6,6 +6,8 @@
 //   k_gemm_bf16<..., Epi>          C = A * B^T by v_mfma_f32_32x32x16_bf16,
 //                                  LDS double-buffered, XCD-aware tile order;
 //                                  the epilogue writes scores or keeps candidates
+//   k_gemm_bf16_lds<..., Epi>      the same product, operands loaded directly
+//                                  into a ring of LDS stages
 //   k_flat_limit                   per query, an upper bound on its k-th smallest
 //                                  approximate score from a strided row sample
 //   k_flat_finish                  per query, the exact k nearest among the
@@ -269,6 +271,179 @@ __global__ void __launch_bounds__(256, 2) k_gemm_bf16(const uint16_t* __restrict
     }
 }
 
+// The same product with operands loaded straight into LDS: workgroup tiles of
+// 256 x 64 TN, four waves of 128 x 32 TN, fed through a ring of NST stages of
+// KS in LDS that direct global-to-LDS loads (global_load_lds_dwordx4) fill
+// NST - 1 stages ahead of the MFMAs, so no operand passes through registers on
+// its way in. TN = 4 takes a whole SIMD per wave (its 256 accumulators) and
+// the ring 128 KB of gfx950's 160 KB; TN = 2 runs two workgroups per CU, a
+// ring of 3 x 24 KB each (the shape the autotuner keeps for the 1M x 4,096 x
+// 768 filter GEMM). One barrier per stage. A stage holds 256 rows of A and 64
+// TN of B, 2 KS bytes each; a wave's load instruction fills 1 KB of
+// consecutive rows (lane l the 16-byte slot l % (KS / 8) of row l / (KS / 8)),
+// and the 16-byte chunk of K a slot holds is its slot number XOR a function of
+// the row, so that the 32 lanes reading one chunk of 32 consecutive rows
+// (ds_read_b128) hit distinct banks. Rows past M (N) load the last row again;
+// the epilogue drops them.
+
+typedef __attribute__((address_space(3))) unsigned char lds_u8;
+
+// One 16-byte global-to-LDS load per lane (global_load_lds_dwordx4): lane l's
+// 16 bytes land at lds + 16 l. A plain (non-template) function: the builtin
+// written straight into the templated kernel keeps the host pass from
+// emitting the kernel's launch stub.
+__device__ __forceinline__ void lds_load16(const uint16_t* g, lds_u8* lds) {
+    __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+}
+
+// Waits until at most N of this wave's vector-memory loads are outstanding.
+// The LDS buffer's address goes into the asm, which clobbers memory: the
+// compiler must assume the buffer written (it does not model the direct
+// loads as LDS writes).
+template <int N>
+__device__ __forceinline__ void wait_loads(const lds_u8* lds) {
+    asm volatile("s_waitcnt vmcnt(%0)" : : "n"(N), "v"(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(lds)))
+                 : "memory");
+}
+
+// Tile order across XCDs (each has its own L2): the small operand's tiles
+// fall into `groups` groups (1, 2, 4 or 8) small enough to stay in one L2,
+// and each XCD takes one group and a 1 / (8 / groups) share of the big
+// operand's tiles, all of its group's tiles for each of them. Every small
+// tile is then read into an XCD's L2 once, and every big tile by `groups`
+// XCDs. slot -> (big tile, small tile) within the XCD's share.
+struct XcdTiles {
+    int groups, big_per, small_per;
+};
+
+template <int KS, int TN, int NST, bool A_BIG, class Epi>
+__global__ void __launch_bounds__(256, TN == 4 ? 1 : 2) k_gemm_bf16_lds(const uint16_t* __restrict__ A, int M,
+                                                                        size_t lda, const uint16_t* __restrict__ B,
+                                                                        int N, size_t ldb, int kp, XcdTiles xt, Epi epi) {
+    constexpr int TM = 4, BM = 256, BN = 64 * TN;
+    constexpr int RB = 2 * KS;                     // bytes of one row in a stage
+    constexpr int CPR = RB / 16;                   // 16-byte chunks per row
+    constexpr int RPL = 64 / CPR;                  // rows one wave load fills
+    constexpr int LPWA = BM / RPL / 4;             // loads of A per wave per stage
+    constexpr int LPWB = BN / RPL / 4;             // loads of B per wave per stage
+    constexpr int STAGE = (BM + BN) * RB;
+    constexpr int LPS = LPWA + LPWB;               // loads per thread per stage
+    constexpr int SWZ = 16 / CPR;                  // rows sharing one swizzle value
+    static_assert(NST >= 2 && NST <= 4 && NST * STAGE <= 160 * 1024 && LPWB >= 1, "ring within gfx950's LDS");
+    __shared__ __align__(16) unsigned char smem[NST * STAGE];
+    lds_u8* const sbase = (lds_u8*)smem; // the ring as an LDS address (address space 3)
+    const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    const int slot = blockIdx.x / 8, xcd = blockIdx.x % 8;
+    if (slot >= xt.big_per * xt.small_per) return;
+    const int group = xcd % xt.groups, part = xcd / xt.groups;
+    const int big = part * xt.big_per + slot / xt.small_per, small = group * xt.small_per + slot % xt.small_per;
+    const int tm = A_BIG ? big : small, tn = A_BIG ? small : big;
+    if (tm >= tiles_m || tn >= tiles_n) return;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
+    const int wm = wave >> 1, wn = wave & 1;
+
+    // this lane's source rows: load c of the wave fills rows (4 c + wave) RPL .. + RPL - 1
+    const uint16_t* ga[LPWA];
+    const uint16_t* gb[LPWB];
+#pragma unroll
+    for (int c = 0; c < LPWA; ++c) {
+        const int r = (4 * c + wave) * RPL + lane / CPR;
+        ga[c] = A + static_cast<size_t>(min(m0 + r, M - 1)) * lda + 8 * ((lane % CPR) ^ ((r / SWZ) % CPR));
+    }
+#pragma unroll
+    for (int c = 0; c < LPWB; ++c) {
+        const int r = (4 * c + wave) * RPL + lane / CPR;
+        gb[c] = B + static_cast<size_t>(min(n0 + r, N - 1)) * ldb + 8 * ((lane % CPR) ^ ((r / SWZ) % CPR));
+    }
+    auto issue = [&](int buf, int k0) {
+        lds_u8* st = sbase + buf * STAGE;
+#pragma unroll
+        for (int c = 0; c < LPWA; ++c)
+            lds_load16(ga[c] + k0, st + (4 * c + wave) * RPL * RB);
+#pragma unroll
+        for (int c = 0; c < LPWB; ++c)
+            lds_load16(gb[c] + k0, st + BM * RB + (4 * c + wave) * RPL * RB);
+    };
+
+    f32x16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+
+    const int nk = kp / KS;
+#pragma unroll
+    for (int s = 0; s < NST - 1; ++s)
+        if (s < nk) issue(s, s * KS);
+    const int arow = wm * 32 * TM + (lane & 31), brow = wn * 32 * TN + (lane & 31), half = lane >> 5;
+    // the swizzle of every fragment row a lane reads (rows 32 apart share it)
+    const int aswz = (arow / SWZ) % CPR, bswz = (brow / SWZ) % CPR;
+    for (int kt = 0; kt < nk; ++kt) {
+        // stage kt landed; the later stages already issued may stay in flight
+        const int ahead = min(NST - 2, nk - 1 - kt);
+        if (ahead >= 2)
+            wait_loads<2 * LPS>(sbase);
+        else if (ahead == 1)
+            wait_loads<LPS>(sbase);
+        else
+            wait_loads<0>(sbase);
+        // every wave's loads of stage kt are in, and stage kt - 1 is read by all
+        // (its LDS reads fed the MFMAs before the wait): a bare barrier, since
+        // __syncthreads() would also wait for the later stages' loads; the
+        // asm fences keep LDS accesses on their side of it
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" : : : "memory");
+        if (kt + NST - 1 < nk) issue((kt + NST - 1) % NST, (kt + NST - 1) * KS);
+        const unsigned char* st = smem + (kt % NST) * STAGE;
+        // the fragments of one 16-wide K slice; the next slice's are read
+        // while this one's 16 MFMAs run
+        bf16x8 fa[2][TM], fb[2][TN];
+        auto read = [&](int ks, bf16x8 (&a)[TM], bf16x8 (&b)[TN]) {
+            const int ch = 2 * ks + half;
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+                a[i] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const u32x4*>(st + (arow + 32 * i) * RB + 16 * (ch ^ aswz)));
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                b[j] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const u32x4*>(st + BM * RB + (brow + 32 * j) * RB + 16 * (ch ^ bswz)));
+        };
+        read(0, fa[0], fb[0]);
+#pragma unroll
+        for (int ks = 0; ks < KS / 16; ++ks) {
+            if (ks + 1 < KS / 16) read(ks + 1, fa[(ks + 1) & 1], fb[(ks + 1) & 1]);
+            // keep the reads ahead of the MFMAs (the scheduler would otherwise
+            // reuse one fragment's registers and wait on each read in turn)
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[ks & 1][i], fb[ks & 1][j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads(); // the ring is free for the epilogue's reductions
+    if constexpr (Epi::kTile) {
+        epi.template tile<TM, TN>(acc, m0, n0, wm, wn, lane, M, N, reinterpret_cast<float*>(smem));
+    } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int col = n0 + wn * 32 * TN + 32 * j + (lane & 31);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int row = m0 + wm * 32 * TM + 32 * i + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
+                    if (row < M && col < N) epi(row, col, acc[i][j][v]);
+                }
+            }
+    }
+}
+
 // Scores of queries (A rows) against every stride-th row (B rows, B's row j
 // is row j * stride): ||x||^2 - 2 q.x, or -q.x, and +inf for rows the query
 // may not return.
@@ -483,14 +658,35 @@ void gemm(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t
                        A, M, lda, B, N, ldb, kp, per_xcd, epi);
 }
 
+template <int KS, int TN, int NST, bool A_BIG, class Epi>
+void gemm_lds(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp, const Epi& epi,
+              hipStream_t s) {
+    if (M <= 0 || N <= 0) return;
+    const int BN = 64 * TN;
+    const int big_tiles = static_cast<int>(A_BIG ? cdiv(M, 256) : cdiv(N, BN));
+    const int small_tiles = static_cast<int>(A_BIG ? cdiv(N, BN) : cdiv(M, 256));
+    // groups of small tiles of at most about 1.5 MB (an XCD's L2 holds 4 MB)
+    const size_t tile_bytes = static_cast<size_t>(A_BIG ? BN : 256) * kp * 2;
+    const int fit = static_cast<int>(std::max<size_t>(1, (size_t{3} << 19) / tile_bytes));
+    XcdTiles xt{1, 0, 0};
+    while (xt.groups < 8 && (small_tiles + xt.groups - 1) / xt.groups > fit) xt.groups *= 2;
+    xt.small_per = (small_tiles + xt.groups - 1) / xt.groups;
+    const int parts = 8 / xt.groups;
+    xt.big_per = (big_tiles + parts - 1) / parts;
+    const size_t per_xcd = static_cast<size_t>(xt.big_per) * xt.small_per;
+    hipLaunchKernelGGL((k_gemm_bf16_lds<KS, TN, NST, A_BIG, Epi>), dim3(static_cast<unsigned>(8 * per_xcd)), dim3(256),
+                       0, s, A, M, lda, B, N, ldb, kp, xt, epi);
+}
+
 __global__ void k_zero_cnt(uint32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
 
 // The tile shapes the launches choose from: wave tiles of (32 TM) x (32 TN)
-// with (TM, TN) = (2, 2), (2, 4), (4, 2).
-constexpr int kVariants = 3;
+// with (TM, TN) = (2, 2), (2, 4), (4, 2), and (4, 4) fed by direct loads
+// through the LDS ring (k_gemm_bf16_lds) in stages of K 32 or 64.
+constexpr int kVariants = 6;
 
 template <bool A_BIG, class Epi>
 void gemm_variant(int v, const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp,
@@ -498,6 +694,15 @@ void gemm_variant(int v, const uint16_t* A, int M, size_t lda, const uint16_t* B
     switch (v) {
     case 1: gemm<2, 4, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
     case 2: gemm<4, 2, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
+    case 3: gemm_lds<32, 4, 4, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
+    case 4:
+        // stages of 64 need K a multiple of 64
+        if (kp % 64 == 0)
+            gemm_lds<64, 4, 2, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s);
+        else
+            gemm_lds<32, 4, 4, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s);
+        break;
+    case 5: gemm_lds<32, 2, 3, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
     default: gemm<2, 2, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
     }
 }
@@ -526,6 +731,12 @@ void gemm_tuned(int kind, const uint16_t* A, int M, size_t lda, const uint16_t* 
                 const Epi& epi, Reset reset, hipStream_t s) {
     const TuneKey key{kind, kp, size_class(M), size_class(N)};
     int best = -1;
+    // QANN_AMD_GEMM_SHAPE=0..5 forces one tile shape (tests check each)
+    const char* force = std::getenv("QANN_AMD_GEMM_SHAPE");
+    if (force && *force >= '0' && *force < '0' + kVariants && force[1] == '\0') {
+        gemm_variant<A_BIG>(*force - '0', A, M, lda, B, N, ldb, kp, epi, s);
+        return;
+    }
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -553,8 +764,9 @@ void gemm_tuned(int kind, const uint16_t* A, int M, size_t lda, const uint16_t* 
             // QANN_AMD_TUNE_LOG=1 prints each choice and its trial times
             const char* log = std::getenv("QANN_AMD_TUNE_LOG");
             if (log && *log && *log != '0')
-                std::fprintf(stderr, "qann_amd tune: gemm kind %d M %d N %d K %d -> shape %d (ms: %.3f %.3f %.3f)\n", kind,
-                             M, N, kp, best, times[0], times[1], times[2]);
+                std::fprintf(stderr,
+                             "qann_amd tune: gemm kind %d M %d N %d K %d -> shape %d (ms: %.3f %.3f %.3f %.3f %.3f %.3f)\n",
+                             kind, M, N, kp, best, times[0], times[1], times[2], times[3], times[4], times[5]);
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
             if (best < 0) best = 0;

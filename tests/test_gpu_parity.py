@@ -434,6 +434,39 @@ def test_gpu_flat_bf16_matches_oracle(tmp_path, oracle):
         _profiling(False)
 
 
+@pytest.mark.parametrize("shape", ["0", "1", "2", "3", "4", "5"])
+def test_gpu_bf16_gemm_shapes_match_oracle(tmp_path, oracle, monkeypatch, shape):
+    # each tile shape of the BF16 GEMM forced (3 and 4: the 256 x 256 tiles fed
+    # by direct loads through the LDS ring, stages of K 32 and 64; K = 96 of
+    # the coarse split product takes stages of 32 either way) on the flat
+    # search's filter GEMM, with
+    # row and query counts that leave partial tiles, and on the split-BF16
+    # coarse scores of a golden index
+    monkeypatch.setenv("QANN_AMD_GEMM_SHAPE", shape)
+    rng = np.random.default_rng(771)
+    x = spectrum_rows(rng, 70_001)
+    q = spectrum_rows(rng, 300)
+    for metric in (qann_amd.Metric.InnerProduct, qann_amd.Metric.L2):
+        flat = qann_amd.FlatIndex(768, metric=metric)
+        flat.add(x)
+        path = tmp_path / "flat_shape.qann"
+        flat.save(str(path))
+        orc = oracle.load(path)
+        for k in (1, 100):
+            assert_same(flat.batch_query(q, k), orc.batch_query(q, k), f"shape {shape} {metric} k {k}")
+    monkeypatch.setenv("QANN_AMD_COARSE", "bf16")
+    path, qg, exp, params = synth_case("cos_fs_refine")
+    idx = qann_amd.load(str(path))
+    ivf = idx.base
+    qq = np.tile(qg, (2048 // qg.shape[0] + 1, 1))
+    for npb, k, fname, allowed, stem in synth_queries(exp, params):
+        if npb is not None:
+            ivf.nprobe = npb
+        r = qq.shape[0] // qg.shape[0]
+        want = (np.tile(exp[stem + "_ids"], (r, 1)), np.tile(exp[stem + "_dists"], (r, 1)))
+        assert_same(idx.batch_query(qq, k, allowed=allowed), want, f"shape {shape} coarse {stem}")
+
+
 def test_gpu_flat_bf16_hands_queries_to_fp32(tmp_path, oracle):
     # 4,500 copies of one row: the queries next to it have more rows within the
     # BF16 bound than the candidate buffer holds and go to the FP32 path, the
