@@ -88,6 +88,8 @@ typedef struct {
     uint32_t sq_bits;              /* QANN_IVF_SQ: 4 or 8 */
     const float* sq_vmin;          /* QANN_IVF_SQ: dim floats, detail::SQ::vmin */
     const float* sq_step;          /* QANN_IVF_SQ: dim floats, detail::SQ::step */
+    uint32_t pq_sdc;               /* QANN_IVF_PQ, L2 or Cosine: score by SDC (PQDistance::SDC, ivf_index.hpp:94-99;
+                                      scan at src/ivf_index.cpp:626-632); 0 = ADC. Ignored for inner product. */
 } qann_hip_ivf_desc;
 
 /* Creates the device index on the current GPU (replaces IVFIndex::load_body's

@@ -19,7 +19,8 @@
 // (tests/data: v4_ivf_fs, v11_ivf_cos_fs, v12_ivf_cos_l2, v9_ip_refine_ivf_fs,
 // v8_ivf_rot, flats, refines, ivf_pq, sq8, float), bit-equality with the
 // reference library compiled from its own sources (oracle/_ref) on synthetic
-// indexes, and the known-answer cases of tests/test_fast_scan.cpp.
+// indexes (SDC-scored PQ among them), and the known-answer cases of
+// tests/test_fast_scan.cpp.
 //
 // Exported as plain C functions (orc_*) for ctypes.
 
@@ -438,6 +439,7 @@ struct Index {
     vector<float> sq_vmin, sq_step;
     vector<List> lists;
     vector<float> precomputed; // B, nlist * m * K
+    vector<float> sdc;         // SDC table, m * K * K, when pq_distance is SDC
     vector<uint16_t> slot_list;
     vector<uint32_t> slot_pos;
 
@@ -501,6 +503,46 @@ static void adc_table(const PQ& pq, const float* r, float* out) {
                 o[k] = fmaf(d, d, o[k]);
             }
         }
+    }
+}
+
+// The SDC table (pq.cpp:275-292): sdc[i][k][l] = sum over the subspace's
+// dimensions, from the first, of fma(y_k - y_l, y_k - y_l, acc).
+static vector<float> sdc_table(const PQ& pq) {
+    vector<float> t(pq.m * pq.K * pq.K, 0.0f);
+    for (size_t i = 0; i < pq.m; ++i)
+        for (size_t j = 0; j < pq.sub; ++j) {
+            const float* y = pq.centroids_t.data() + (i * pq.sub + j) * pq.K;
+            for (size_t k = 0; k < pq.K; ++k) {
+                float* row = t.data() + (i * pq.K + k) * pq.K;
+                for (size_t l = 0; l < pq.K; ++l) {
+                    const float d = y[k] - y[l];
+                    row[l] = fmaf(d, d, row[l]);
+                }
+            }
+        }
+    return t;
+}
+
+// PQCodebook::encode (pq.cpp:88-156): per subspace, the first centroid of
+// least (r_0 - y_0)^2 + fma-accumulated (r_j - y_j)^2 over the rest.
+static void pq_encode(const PQ& pq, const float* r, uint8_t* out) {
+    for (size_t i = 0; i < pq.m; ++i) {
+        float best = 0.0f;
+        size_t pick = 0;
+        for (size_t c = 0; c < pq.K; ++c) {
+            float d = r[i * pq.sub] - pq.centroids_t[(i * pq.sub) * pq.K + c];
+            float acc = d * d;
+            for (size_t j = 1; j < pq.sub; ++j) {
+                d = r[i * pq.sub + j] - pq.centroids_t[(i * pq.sub + j) * pq.K + c];
+                acc = fmaf(d, d, acc);
+            }
+            if (c == 0 || acc < best) {
+                best = acc;
+                pick = c;
+            }
+        }
+        out[i] = static_cast<uint8_t>(pick);
     }
 }
 
@@ -627,6 +669,7 @@ static std::unique_ptr<Index> load_ivf(Reader& r, uint32_t version) {
             x->residual_sqnorms[l] = ip_exact(x->residual_center(l), x->residual_center(l), d);
     }
     build_precomputed(*x);
+    if (x->has_pq && x->pq_distance != 0) x->sdc = sdc_table(x->pq);
     if (!x->has_pq) {
         x->slot_list.assign(total, 0);
         x->slot_pos.assign(total, 0);
@@ -855,7 +898,22 @@ static vector<Neighbor> ivf_scan(const Index& x, const float* q, vector<Probe>& 
                 if (passes(j)) push(x.exact_score(q, list.vecs.data() + j * x.dim), j);
             continue;
         }
-        if (x.pq_distance != 0) throw std::runtime_error("SDC scoring is not restated by the oracle");
+        if (x.pq_distance != 0 && !ip_pq) {
+            // SDC (ivf_index.cpp:626-632): the residual's own code against each
+            // row's, summed over subspaces from the first
+            const float* c = x.residual_center(probe.list);
+            vector<float> residual(x.dim);
+            for (size_t l = 0; l < x.dim; ++l) residual[l] = q[l] - c[l];
+            vector<uint8_t> qc(m);
+            pq_encode(x.pq, residual.data(), qc.data());
+            for (size_t j = 0; j < n; ++j) {
+                const uint8_t* code = list.codes.data() + j * m;
+                float d = 0.0f;
+                for (size_t s = 0; s < m; ++s) d += x.sdc[(s * K + qc[s]) * K + code[s]];
+                push(d, j);
+            }
+            continue;
+        }
         auto adc_sum = [&](size_t j) {
             const uint8_t* c = list.codes.data() + j * m;
             float d = 0.0f;

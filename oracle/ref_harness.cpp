@@ -146,6 +146,14 @@ int ref_set_cosine_ip(void* p, int on) {
     });
 }
 
+int ref_set_pq_distance(void* p, int sdc) {
+    return guarded([&] {
+        IVFIndex* ivf = ivf_of(static_cast<Handle*>(p));
+        if (!ivf) throw std::invalid_argument("not an IVF index");
+        ivf->set_pq_distance(sdc ? PQDistance::SDC : PQDistance::ADC);
+    });
+}
+
 int ref_set_filter_probe_scaling(void* p, int on) {
     return guarded([&] {
         IVFIndex* ivf = ivf_of(static_cast<Handle*>(p));

@@ -89,6 +89,9 @@ class _Index:
     def set_filter_probe_scaling(self, on):
         self._lib._check(self._lib.set_filter_probe_scaling(self._h, int(bool(on))))
 
+    def set_pq_distance(self, sdc):
+        self._lib._check(self._lib.set_pq_distance(self._h, int(bool(sdc))))
+
     def batch_query(self, queries, k, allowed=None, where=None):
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
@@ -124,6 +127,7 @@ class Reference(_Lib):
         self.set_k_factor = self._fn("set_k_factor", C.c_int, C.c_void_p, C.c_uint64)
         self.set_precomputed = self._fn("set_precomputed", C.c_int, C.c_void_p, C.c_int)
         self.set_cosine_ip = self._fn("set_cosine_ip", C.c_int, C.c_void_p, C.c_int)
+        self.set_pq_distance = self._fn("set_pq_distance", C.c_int, C.c_void_p, C.c_int)
         self.set_filter_probe_scaling = self._fn("set_filter_probe_scaling", C.c_int, C.c_void_p, C.c_int)
         self.batch_query = self._fn("batch_query", C.c_int, C.c_void_p, _f32p, C.c_uint64, C.c_uint64, _i64p,
                                     C.c_int64, C.c_char_p, _i64p, _f32p)

@@ -37,6 +37,7 @@ from ._qann import (  # noqa: E402
     FlatIndex,
     IVFIndex,
     Metric,
+    PQDistance,
     RefineIndex,
     Server,
     ServeFuture,
@@ -54,7 +55,7 @@ from ._qann import (  # noqa: E402
 
 Index = DenseIndex
 __version__ = "0.1.0"
-__all__ = ["Clustering", "DenseIndex", "Index", "FlatIndex", "IVFIndex", "RefineIndex", "Metric", "load", "set_num_threads",
+__all__ = ["Clustering", "DenseIndex", "Index", "FlatIndex", "IVFIndex", "RefineIndex", "Metric", "PQDistance", "load", "set_num_threads",
            "get_num_threads", "native_library", "Server", "ServeFuture", "serve_load", "Expr", "Field", "field",
            "where", "any_of", "all_of", "get_max_filter_conditions", "set_max_filter_conditions"]
 

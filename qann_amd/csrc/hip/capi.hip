@@ -847,9 +847,10 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     if (nprobe == 0 || nprobe > nlist) invalid("nprobe must be in [1, nlist]");
     const bool fs = ix.storage == QANN_IVF_FASTSCAN;
     const bool has_pq = fs || ix.storage == QANN_IVF_PQ;
-    const bool cos_pq = has_pq && ix.metric == QANN_METRIC_COSINE && ix.cosine_ip;
+    // SDC scores Cosine by L2 over residuals and never uses B (src/ivf_index.cpp:430, 448-449)
+    const bool cos_pq = has_pq && ix.metric == QANN_METRIC_COSINE && ix.cosine_ip && !ix.sdc;
     const bool ip_pq = has_pq && (ix.metric == QANN_METRIC_IP || cos_pq);
-    const bool use_pre = has_pq && !ip_pq && ix.B != nullptr;
+    const bool use_pre = has_pq && !ip_pq && !ix.sdc && ix.B != nullptr;
     const uint32_t m = static_cast<uint32_t>(ix.m);
     const size_t tsz = static_cast<size_t>(m) * static_cast<size_t>(ix.K); // table floats
     // the TopK buffers go to HBM when they do not fit LDS
@@ -1013,6 +1014,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     if (!use_pre) ixs.B = nullptr;
     const size_t bb = static_cast<size_t>(m) * 16;
     const int dist_mode = ip_pq ? qd::kDistIp
+                          : ix.sdc ? qd::kDistSdc
                           : use_pre ? qd::kDistL2Pre
                           : has_pq  ? qd::kDistL2Resid
                           : ix.storage == QANN_IVF_SQ ? qd::kDistSq
@@ -1348,6 +1350,7 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
             invalid("fast_scan needs 16 centroids per subspace and an even num_subspaces <= 1024 dividing dim");
         if (dsc.storage == QANN_IVF_PQ && (dsc.m == 0 || dsc.dim % dsc.m != 0 || K > 256))
             invalid("PQ needs num_subspaces dividing dim and at most 256 centroids per subspace");
+        if (dsc.pq_sdc && dsc.storage == QANN_IVF_FASTSCAN) invalid("cannot use SDC with fast-scan");
         if (dsc.storage == QANN_IVF_SQ && ((dsc.sq_bits != 4 && dsc.sq_bits != 8) || !dsc.sq_vmin || !dsc.sq_step))
             invalid("SQ needs 4 or 8 bits and its ranges");
         if (!dsc.centroids || (pq && !dsc.pq_centroids)) invalid("centroids (and the PQ codebook with PQ) are required");
@@ -1392,7 +1395,12 @@ int qann_hip_ivf_create(const qann_hip_ivf_desc* desc, qann_hip_index** out) {
             float* pqt = h->alloc<float>(m * K * sub);
             qd::launch_pq_transpose(pqc, static_cast<int>(m), static_cast<int>(K), static_cast<int>(sub), pqt, st);
             ix.pq_cent_t = pqt;
-            if (dsc.use_precomputed) {
+            if (dsc.pq_sdc && dsc.metric != QANN_METRIC_IP) {
+                // SDC takes no precomputed tables (src/ivf_index.cpp:430)
+                float* sdc = h->alloc<float>(m * K * K);
+                qd::launch_sdc_table(pqt, static_cast<int>(m), static_cast<int>(sub), static_cast<int>(K), sdc, st);
+                ix.sdc = sdc;
+            } else if (dsc.use_precomputed) {
                 float* B = h->alloc<float>(nl * m * K);
                 const float* centers = ix.residual_centers ? ix.residual_centers : ix.centroids;
                 qd::launch_list_term(centers, pqt, static_cast<int>(nl), static_cast<int>(d), static_cast<int>(m),

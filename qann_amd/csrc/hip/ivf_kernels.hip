@@ -62,6 +62,23 @@ __global__ void k_pq_transpose(const float* cent, int m, int K, int sub, float* 
     cent_t[(static_cast<size_t>(s) * sub + j) * K + c] = cent[idx];
 }
 
+// sdc[s][k][l] = sum over the subspace's dimensions j, from the first, of
+// fma(y_skj - y_slj, y_skj - y_slj, acc) (PQCodebook::build_tables' SDC table,
+// src/pq.cpp:275-292). One thread per entry.
+__global__ void k_sdc_table(const float* cent_t, int m, int sub, int K, float* sdc) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t kk = static_cast<size_t>(K) * K;
+    if (idx >= static_cast<size_t>(m) * kk) return;
+    const int s = static_cast<int>(idx / kk), k = static_cast<int>(idx % kk) / K, l = static_cast<int>(idx % K);
+    const float* y = cent_t + static_cast<size_t>(s) * sub * K;
+    float acc = 0.0f;
+    for (int j = 0; j < sub; ++j) {
+        const float d = y[static_cast<size_t>(j) * K + k] - y[static_cast<size_t>(j) * K + l];
+        acc = fmaf(d, d, acc);
+    }
+    sdc[idx] = acc;
+}
+
 // B[l][s][c] = sum_j y_scj * (y_scj + 2 c_l,sj), fused (compute_list_term, src/pq.cpp:206-223).
 __global__ void k_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, int K,
                             float* B) {
@@ -1684,12 +1701,12 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
     const int np = min(win, P - p0);
     const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
     const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
-    const bool adc = mode <= kDistL2Resid;
+    const bool adc = mode <= kDistL2Resid || mode == kDistSdc; // a table summed over the codes
     const size_t tsz = static_cast<size_t>(m) * K; // table floats
     float* qs = reinterpret_cast<float*>(lds);                      // dim
     float* sqv = qs + dim;                                          // SQ: vmin, step (2 dim)
     float* tables = sqv + (mode == kDistSq ? 2 * dim : 0);          // ADC: 1 (inner product) or win tables
-    float* resid = tables + (adc ? (mode == kDistIp ? 1 : win) * tsz : 0); // kDistL2Resid: a residual per wave
+    float* resid = tables + (adc ? (mode == kDistIp ? 1 : win) * tsz : 0); // kDistL2Resid / Sdc: a residual per wave
     const float* q = a.q + static_cast<size_t>(qi) * dim;
     for (int t = tid; t < dim; t += blockDim.x) qs[t] = q[t];
     if (mode == kDistSq)
@@ -1710,8 +1727,9 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
         if (tid == np - 1) w_start[np] = rel + (ix.list_off[list + 1] - ix.list_off[list] + kFsBlock - 1) / kFsBlock;
     }
     __syncthreads();
-    if (mode == kDistL2Pre || mode == kDistL2Resid) {
-        // the (query, list) L2 table: B + C (+ A on row 0), or compute_adc_table of q - c
+    if (mode == kDistL2Pre || mode == kDistL2Resid || mode == kDistSdc) {
+        // the (query, list) L2 table: B + C (+ A on row 0), compute_adc_table of
+        // q - c, or for SDC the rows of the SDC table picked by the code of q - c
         for (int p = wave; p < np; p += kSumsWaves) {
             if (w_start[p + 1] == w_start[p]) continue;
             const int list = w_list[p];
@@ -1732,16 +1750,52 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
                     for (int t = lane; t < dim; t += kWave) r[t] = qs[t] - c[t];
                     wave_lds_sync();
                 }
-                for (size_t e = lane; e < tsz; e += kWave) {
-                    const int sidx = static_cast<int>(e / K), cc = static_cast<int>(e % K);
-                    float acc = 0.0f;
-                    for (int j = 0; j < ix.sub; ++j) {
-                        const int x = sidx * ix.sub + j;
-                        const float rs = r ? r[x] : qs[x] - c[x];
-                        const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
-                        acc = fmaf(d, d, acc);
+                if (mode == kDistSdc) {
+                    // PQCodebook::encode of the residual (src/pq.cpp:88-156): per
+                    // subspace the first centroid of least (r_0 - y_0)^2 +
+                    // fma-accumulated (r_j - y_j)^2, compared as int32 bits as
+                    // the reference does; lanes over centroids, then the wave's
+                    // minimum by (bits, index), the same on every lane
+                    for (int sidx = 0; sidx < m; ++sidx) {
+                        const int x0 = sidx * ix.sub;
+                        int bd = 0x7FFFFFFF;
+                        int bc = K;
+                        for (int cc = lane; cc < K; cc += kWave) {
+                            float dd = (r ? r[x0] : qs[x0] - c[x0]) - ix.pq_cent_t[static_cast<size_t>(x0) * K + cc];
+                            float acc = dd * dd;
+                            for (int j = 1; j < ix.sub; ++j) {
+                                const int x = x0 + j;
+                                dd = (r ? r[x] : qs[x] - c[x]) - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
+                                acc = fmaf(dd, dd, acc);
+                            }
+                            if (cc == lane || __float_as_int(acc) < bd) {
+                                bd = __float_as_int(acc);
+                                bc = cc;
+                            }
+                        }
+                        for (int off = kWave / 2; off > 0; off >>= 1) {
+                            const int od = __shfl_xor(bd, off);
+                            const int oc = __shfl_xor(bc, off);
+                            if (od < bd || (od == bd && oc < bc)) {
+                                bd = od;
+                                bc = oc;
+                            }
+                        }
+                        const float* row = ix.sdc + (static_cast<size_t>(sidx) * K + bc) * K;
+                        for (int cc = lane; cc < K; cc += kWave) tb[static_cast<size_t>(sidx) * K + cc] = row[cc];
                     }
-                    tb[e] = acc;
+                } else {
+                    for (size_t e = lane; e < tsz; e += kWave) {
+                        const int sidx = static_cast<int>(e / K), cc = static_cast<int>(e % K);
+                        float acc = 0.0f;
+                        for (int j = 0; j < ix.sub; ++j) {
+                            const int x = sidx * ix.sub + j;
+                            const float rs = r ? r[x] : qs[x] - c[x];
+                            const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
+                            acc = fmaf(d, d, acc);
+                        }
+                        tb[e] = acc;
+                    }
                 }
                 wave_lds_sync();
             }
@@ -2980,6 +3034,11 @@ void launch_pq_transpose(const float* cent, int m, int K, int sub, float* cent_t
                        K, sub, cent_t);
 }
 
+void launch_sdc_table(const float* cent_t, int m, int sub, int K, float* sdc, hipStream_t s) {
+    hipLaunchKernelGGL(k_sdc_table, dim3(cdiv(static_cast<size_t>(m) * K * K, 256)), dim3(256), 0, s, cent_t, m, sub, K,
+                       sdc);
+}
+
 void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, int K, float* B,
                       hipStream_t s) {
     hipLaunchKernelGGL(k_list_term, dim3(cdiv(static_cast<size_t>(m) * K, 256) * static_cast<unsigned>(nlist)),
@@ -3314,11 +3373,12 @@ size_t ivf_dists_lds(DistArgs& a) {
     a.resid_in_lds = 0;
     a.win = kSumsWin;
     if (a.mode == kDistIp) fixed += tbytes;
-    if (a.mode == kDistL2Resid && fixed + kSumsWaves * dim4 + tbytes <= kLdsMax) {
+    const bool resid = a.mode == kDistL2Resid || a.mode == kDistSdc;
+    if (resid && fixed + kSumsWaves * dim4 + tbytes <= kLdsMax) {
         a.resid_in_lds = 1;
         fixed += kSumsWaves * dim4;
     }
-    if (a.mode == kDistL2Pre || a.mode == kDistL2Resid) {
+    if (a.mode == kDistL2Pre || resid) {
         if (fixed + tbytes > kLdsMax) return 0;
         a.win = static_cast<int>(std::min<size_t>(kSumsWin, (kLdsMax - fixed) / tbytes));
         fixed += static_cast<size_t>(a.win) * tbytes;

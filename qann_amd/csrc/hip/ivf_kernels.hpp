@@ -35,6 +35,7 @@ struct DevIvf {
     const float* residual_sqnorms; // nlist or null
     const float* pq_cent_t;        // m * sub * K
     const float* B;                // nlist * m * K, or null (no precomputed tables)
+    const float* sdc;              // m * K * K code-to-code table when PQ lists are scored by SDC, else null
     const float* rotation;         // dim * dim or null
     const uint32_t* list_off;      // nlist + 1
     const uint64_t* code_off;      // nlist
@@ -145,7 +146,7 @@ struct DistArgs {
     uint32_t* item_min;        // fkey of each probe's smallest distance
     int resid_in_lds;
 };
-enum { kDistIp = 0, kDistL2Pre = 1, kDistL2Resid = 2, kDistSq = 3, kDistFloat = 4 };
+enum { kDistIp = 0, kDistL2Pre = 1, kDistL2Resid = 2, kDistSq = 3, kDistFloat = 4, kDistSdc = 5 };
 
 // Phase 2: the reference's TopK replayed over the sums.
 struct ReplayArgs {
@@ -184,6 +185,7 @@ inline size_t replay_lds_bytes(int kp) { return static_cast<size_t>(2 * kp) * 12
 
 void launch_sqnorms(const float* rows, int n, int dim, float* out, hipStream_t s);
 void launch_pq_transpose(const float* cent, int m, int K, int sub, float* cent_t, hipStream_t s);
+void launch_sdc_table(const float* cent_t, int m, int sub, int K, float* sdc, hipStream_t s);
 void launch_list_term(const float* centers, const float* cent_t, int nlist, int dim, int m, int sub, int K, float* B,
                       hipStream_t s);
 void launch_rotate(const float* q, const float* R, int nq, int dim, float* out, hipStream_t s);

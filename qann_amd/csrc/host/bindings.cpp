@@ -22,6 +22,9 @@ namespace {
 
 using FloatArray = py::array_t<float, py::array::c_style | py::array::forcecast>;
 
+// vecengine::PQDistance, as IvfData::pq_distance stores it (0 ADC, 1 SDC)
+enum class PQDistanceKind : uint8_t { ADC = 0, SDC = 1 };
+
 // lock_chain (python_bindings.cpp:97-113): the index, then the base it wraps.
 template <class Lock>
 std::vector<Lock> lock_chain(const DenseIndex& idx) {
@@ -395,6 +398,11 @@ PYBIND11_MODULE(_qann, m) {
         .value("EMeans", ClusteringKind::EMeans, "e-means: mini-batch k-means with EMA centroids (default)")
         .value("Lloyd", ClusteringKind::Lloyd, "Lloyd's k-means");
 
+    // vecengine::PQDistance (include/vecengine/pq.hpp:18)
+    py::enum_<PQDistanceKind>(m, "PQDistance", "How IVF queries score PQ codes")
+        .value("ADC", PQDistanceKind::ADC, "the query's own (residual) against each code's centroids (default)")
+        .value("SDC", PQDistanceKind::SDC, "the query's residual encoded too, then code-to-code distances");
+
     m.def("set_num_threads", &set_num_threads, py::arg("n"), "Host threads for adds and training (0 = all cores)");
     m.def("get_num_threads", &num_threads);
 
@@ -614,6 +622,12 @@ PYBIND11_MODULE(_qann, m) {
                       [](IVFIndex& self, bool v) { write_locked(self, [&] { self.set_precomputed_tables(v); return 0; }); })
         .def_property("cosine_inner_product", &IVFIndex::cosine_inner_product,
                       [](IVFIndex& self, bool v) { write_locked(self, [&] { self.set_cosine_inner_product(v); return 0; }); })
+        .def_property(
+            "pq_distance", [](const IVFIndex& self) { return static_cast<PQDistanceKind>(self.pq_distance()); },
+            [](IVFIndex& self, PQDistanceKind v) {
+                write_locked(self, [&] { self.set_pq_distance(static_cast<uint8_t>(v)); return 0; });
+            },
+            "How queries score PQ codes: PQDistance.ADC (default, more accurate) or PQDistance.SDC")
         .def_property("filter_probe_scaling", &IVFIndex::filter_probe_scaling,
                       [](IVFIndex& self, bool v) { write_locked(self, [&] { self.set_filter_probe_scaling(v); return 0; }); });
 

@@ -378,11 +378,21 @@ void IVFIndex::set_cosine_inner_product(bool on) {
     invalidate();
 }
 
+// IVFIndex::set_pq_distance (include/vecengine/ivf_index.hpp:94-100).
+void IVFIndex::set_pq_distance(uint8_t mode) {
+    if (mode > 1) throw std::invalid_argument("pq_distance must be ADC (0) or SDC (1)");
+    if (d_->fast_scan && mode == 1) throw std::invalid_argument("cannot use SDC with fast-scan");
+    if (d_->metric == InnerProduct && mode == 1) throw std::invalid_argument("cannot use SDC with Metric::InnerProduct");
+    d_->pq_distance = mode;
+    invalidate();
+}
+
 void IVFIndex::enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate) {
     if (d_->ids.slots > 0 || d_->trained) throw std::logic_error("enable_pq must be called before train()/add()");
     if (d_->sq_bits) throw std::logic_error("enable_pq cannot be combined with enable_sq");
     if (fast_scan && (K != 16 || m % 2 != 0 || m > 1024))
         throw std::invalid_argument("fast_scan needs 16 centroids per subspace and an even num_subspaces <= 1024");
+    if (fast_scan && d_->pq_distance != 0) throw std::invalid_argument("cannot use SDC with fast-scan");
     if (d_->dim == 0 || m == 0) throw std::invalid_argument("dim and num_subspaces must be >= 1");
     if (d_->dim % m != 0) throw std::invalid_argument("dim must be divisible by num_subspaces");
     if (K == 0 || K > 256) throw std::invalid_argument("centroids_per_subspace must be in [1, 256]");
@@ -447,7 +457,8 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
         pq.centroids = pq_codebooks(resid.data(), n, d, pq.m, pq.K, 25,
                                     seed ? std::optional<uint32_t>(*seed + 1) : std::nullopt, nearest_subspaces_gpu);
     }
-    if (d_->has_pq && d_->metric == Cosine) d_->cosine_ip = cosine_tables_by_recall(x.data(), n, assign, seed ? *seed : 0u);
+    if (d_->has_pq && d_->metric == Cosine && d_->pq_distance == 0)
+        d_->cosine_ip = cosine_tables_by_recall(x.data(), n, assign, seed ? *seed : 0u);
     if (d_->sq_bits) {
         // detail::SQ::train (src/sq.cpp): each dimension's range over the sample
         d_->sq_vmin.assign(x.begin(), x.begin() + d);
@@ -659,6 +670,7 @@ static qann_hip_ivf_desc ivf_desc(const IvfData& x) {
         desc.m = static_cast<uint32_t>(x.pq.m);
         desc.pq_k = static_cast<uint32_t>(x.pq.K);
         desc.pq_centroids = x.pq.centroids.data();
+        desc.pq_sdc = x.pq_distance != 0 && x.metric != InnerProduct ? 1u : 0u;
     } else {
         desc.storage = QANN_IVF_FLOAT;
     }
@@ -769,10 +781,8 @@ qann_hip_index* IVFIndex::device() const {
     std::lock_guard<std::mutex> g(dev_mu);
     if (dev_) return dev_;
     if (!d_->trained) throw std::logic_error("train() has not be run yet");
-    if (d_->has_pq && d_->pq_distance != 0)
-        throw std::runtime_error("qann_amd does not search PQ indexes scored by SDC; this index uses SDC");
     qann_hip_ivf_desc desc = ivf_desc(*d_);
-    desc.use_precomputed = d_->has_pq && !d_->sq_bits ? wants_precomputed(*d_) : 0;
+    desc.use_precomputed = d_->has_pq && !d_->sq_bits && !desc.pq_sdc ? wants_precomputed(*d_) : 0;
     qann_hip_index* h = nullptr;
     check_status(qann_hip_ivf_create(&desc, &h));
     std::vector<uint32_t> sizes(d_->nlist);

@@ -160,6 +160,7 @@ public:
     bool filter_probe_scaling() const { return filter_probe_scaling_; }
     void set_filter_probe_scaling(bool on) { filter_probe_scaling_ = on; }
     uint8_t pq_distance() const { return d_->pq_distance; }
+    void set_pq_distance(uint8_t mode); // 0 ADC, 1 SDC
 
     // IVFIndex::enable_pq (src/ivf_index.cpp:83-94)
     void enable_pq(uint64_t m, uint64_t K, bool fast_scan, bool rotate);

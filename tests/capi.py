@@ -16,7 +16,7 @@ class IvfDesc(C.Structure):
                 ("cosine_ip", C.c_uint32), ("use_precomputed", C.c_uint32),
                 ("centroids", C.c_void_p), ("residual_centers", C.c_void_p), ("pq_centroids", C.c_void_p),
                 ("rotation", C.c_void_p), ("storage", C.c_uint32), ("pq_k", C.c_uint32), ("sq_bits", C.c_uint32),
-                ("sq_vmin", C.c_void_p), ("sq_step", C.c_void_p)]
+                ("sq_vmin", C.c_void_p), ("sq_step", C.c_void_p), ("pq_sdc", C.c_uint32)]
 
 
 class SearchParams(C.Structure):

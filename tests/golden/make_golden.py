@@ -54,6 +54,14 @@ CASES = [
     ("cos_sq8_refine", dict(d=20, n=2500, nlist=100, nprobe=8, metric=COS, m=0, sq=8, kf=3, half=True)),
     ("l2_float", dict(d=19, n=3000, nlist=100, nprobe=8, metric=L2, m=0, kf=0, remove=30, half=True)),
     ("ip_float_refine", dict(d=24, n=2500, nlist=100, nprobe=6, metric=IP, m=0, kf=2, half=True)),
+    # PQ codes scored by SDC (the query's residual encoded, then code-to-code
+    # table sums); 256 and 16 centroids per subspace, L2 and Cosine
+    ("l2_pq8_sdc", dict(d=32, n=3000, nlist=100, nprobe=8, metric=L2, m=8, pq_k=256, fast=False, kf=0, remove=60,
+                        half=True, sdc=True)),
+    ("l2_pq16x16_sdc", dict(d=32, n=3000, nlist=100, nprobe=8, metric=L2, m=16, pq_k=16, fast=False, kf=0,
+                            sdc=True)),
+    ("cos_pq8_sdc_refine", dict(d=40, n=2500, nlist=128, nprobe=12, metric=COS, m=10, pq_k=256, fast=False, kf=3,
+                                sdc=True)),
 ]
 
 
@@ -91,6 +99,8 @@ def main():
                 ref._check(ref.set_cosine_ip(index._h, int(p["cosine_ip"])))
             if "precomputed" in p:
                 ref._check(ref.set_precomputed(index._h, int(p["precomputed"])))
+            if p.get("sdc"):
+                index.set_pq_distance(True)
         if p.get("remove"):
             pool = ids if ids is not None else np.arange(p["n"])
             ref.remove(index, rng.choice(pool, size=p["remove"], replace=False))

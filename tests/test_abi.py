@@ -3,6 +3,7 @@ declares (no GPU needed: nothing here launches work)."""
 import ctypes
 import re
 
+from capi import IvfDesc
 from conftest import ROOT
 
 
@@ -35,13 +36,12 @@ def test_invalid_arguments_are_rejected_before_any_device_work():
     lib.qann_hip_last_error.restype = ctypes.c_char_p
     out = ctypes.c_void_p()
     # nlist outside [100, 65535] is the reference's std::invalid_argument
-    class Desc(ctypes.Structure):
-        _fields_ = [("dim", ctypes.c_uint32), ("nlist", ctypes.c_uint32), ("metric", ctypes.c_uint32),
-                    ("m", ctypes.c_uint32), ("cosine_ip", ctypes.c_uint32), ("use_precomputed", ctypes.c_uint32),
-                    ("centroids", ctypes.c_void_p), ("residual_centers", ctypes.c_void_p),
-                    ("pq_centroids", ctypes.c_void_p), ("rotation", ctypes.c_void_p)]
-    d = Desc(dim=16, nlist=50, metric=0, m=8)
+    d = IvfDesc(dim=16, nlist=50, metric=0, m=8)
     assert lib.qann_hip_ivf_create(ctypes.byref(d), ctypes.byref(out)) == 1
     assert b"nlist" in lib.qann_hip_last_error()
-    d = Desc(dim=16, nlist=100, metric=0, m=3)
+    d = IvfDesc(dim=16, nlist=100, metric=0, m=3)
     assert lib.qann_hip_ivf_create(ctypes.byref(d), ctypes.byref(out)) == 1
+    # SDC with fast-scan lists (ivf_index.hpp:95-96)
+    d = IvfDesc(dim=16, nlist=100, metric=0, m=8, storage=0, pq_sdc=1)
+    assert lib.qann_hip_ivf_create(ctypes.byref(d), ctypes.byref(out)) == 1
+    assert b"SDC" in lib.qann_hip_last_error()

@@ -42,6 +42,8 @@ def test_loaded_properties():
     r = qann_amd.load(str(ref_fixture("v9_ip_refine_ivf_fs", "v9_ip_queries")[0]))
     assert isinstance(r, qann_amd.RefineIndex) and isinstance(r.base, qann_amd.IVFIndex)
     assert r.k_factor == 6 and r.base.pq_rotate and r.metric == qann_amd.Metric.InnerProduct
+    s = qann_amd.load(str(synth_case("cos_pq8_sdc_refine")[0]))
+    assert s.base.pq_distance == qann_amd.PQDistance.SDC and not s.base.fast_scan
 
 
 def test_corrupt_and_truncated_files_raise(tmp_path):
@@ -69,5 +71,18 @@ def test_api_validation_without_gpu():
         ivf.enable_pq(6, fast_scan=True, centroids_per_subspace=8)
     ivf.enable_pq(8, fast_scan=True)
     assert ivf.fast_scan
+    # SDC: not with fast-scan, not with inner product (ivf_index.hpp:94-100, ivf_index.cpp:89-90)
+    assert ivf.pq_distance == qann_amd.PQDistance.ADC
+    with pytest.raises(ValueError, match="fast-scan"):
+        ivf.pq_distance = qann_amd.PQDistance.SDC
+    ip = qann_amd.IVFIndex(16, 100, 8, metric=qann_amd.Metric.InnerProduct)
+    with pytest.raises(ValueError, match="InnerProduct"):
+        ip.pq_distance = qann_amd.PQDistance.SDC
+    sdc = qann_amd.IVFIndex(16, 100, 8)
+    sdc.pq_distance = qann_amd.PQDistance.SDC
+    with pytest.raises(ValueError, match="fast-scan"):
+        sdc.enable_pq(8, fast_scan=True)
+    sdc.enable_pq(8, fast_scan=False, centroids_per_subspace=256)
+    assert sdc.pq_distance == qann_amd.PQDistance.SDC and not sdc.precomputed_tables
     with pytest.raises(ValueError):
         qann_amd.RefineIndex(ivf, 0)

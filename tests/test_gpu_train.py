@@ -112,6 +112,37 @@ def test_gpu_trained_index_searches_as_the_reference(tmp_path, reference):
         np.testing.assert_array_equal(got[1].view(np.uint32), want[1].view(np.uint32))
 
 
+@pytest.mark.parametrize("metric", [qann_amd.Metric.L2, qann_amd.Metric.Cosine])
+def test_gpu_sdc_index_searches_as_the_reference(reference, metric):
+    # PQ codes scored by SDC (src/ivf_index.cpp:626-632), set before training
+    # here and after it in the reference (SDC leaves L2 training alone, and
+    # Cosine's table mode only matters under ADC): the same ids and distance
+    # bits, filtered too; then both switched back to ADC
+    rng = np.random.default_rng(17 + METRIC_ID[metric])
+    x = mixture(rng, 6000, 32)
+    q = mixture(rng, 120, 32)
+    ref = reference.build_ivf(x, 128, 10, metric=METRIC_ID[metric], m=8, pq_k=256, fast_scan=False, seed=4)
+    ref.set_pq_distance(True)
+    ivf = qann_amd.IVFIndex(32, 128, 10, metric=metric)
+    ivf.pq_distance = qann_amd.PQDistance.SDC
+    ivf.enable_pq(8, 256)
+    ivf.train(x, seed=4)
+    ivf.add(x)
+    allowed = np.sort(rng.choice(6000, 900, replace=False)).astype(np.int64)
+    for mode in ("sdc", "adc"):
+        if mode == "adc":
+            ref.set_pq_distance(False)
+            ivf.pq_distance = qann_amd.PQDistance.ADC
+            if metric == qann_amd.Metric.Cosine:
+                ivf.cosine_inner_product = False
+                reference._check(reference.set_cosine_ip(ref._h, 0))
+        for k, al in ((1, None), (10, None), (50, None), (10, allowed)):
+            got = ivf.batch_query(q, k, allowed=al)
+            want = ref.batch_query(q, k, allowed=al)
+            np.testing.assert_array_equal(got[0], want[0], err_msg=f"{mode} k={k}")
+            np.testing.assert_array_equal(got[1].view(np.uint32), want[1].view(np.uint32), err_msg=f"{mode} k={k}")
+
+
 def test_gpu_subspace_assign_and_pack_through_the_abi():
     # qann_hip_assign_subspaces against a numpy restatement of nearest_centroid
     # on sub-vectors of 2 floats (l2_distance's scalar tail: two fused steps,
