@@ -712,8 +712,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const int ip = h->metric == QANN_METRIC_IP;
     const uint64_t nwords = (h->slots + 63) / 64;
 
-    const size_t qlds = static_cast<size_t>((h->dim + 1) & ~1u) * 4;
-    if (qlds > qd::kLdsMax / 2) invalid("dim too large for the on-chip query copy");
+    const size_t qlds = qd::query_lds_bytes(static_cast<int>(h->dim)); // 0: kernels read the query from global memory
     if (h->kind == 1) { // FlatIndex::query_impl
         const uint64_t n = h->slots;
         const size_t fcap = qd::coarse_select_cap(static_cast<int>(k));
@@ -1519,11 +1518,14 @@ int qann_hip_flat_create(uint32_t dim, uint32_t metric, const float* rows, uint6
                                h->row_sq, nullptr);
             check_launch("k_sqnorms");
             h->xnorm_max = norm_bound(rows, slots, dim);
-            // QANN_AMD_FLAT_FP32=1 keeps the index on the FP32 GEMM path
+            // QANN_AMD_FLAT_FP32=1 keeps the index on the FP32 GEMM path;
+            // QANN_AMD_FLAT_BF16_ROWS=<n> moves the BF16 path's row threshold
             const char* fp32_only = std::getenv("QANN_AMD_FLAT_FP32");
             const bool bf16 = !(fp32_only && *fp32_only && std::strcmp(fp32_only, "0") != 0);
-            if (bf16 && slots >= kFlatBf16MinRows &&
-                qd::bf16_kpad(static_cast<int>(dim)) * 4 + qd::kFlatCandCap * 8 <= qd::kLdsMax) {
+            const char* min_rows = std::getenv("QANN_AMD_FLAT_BF16_ROWS");
+            const uint64_t bf16_rows = min_rows && *min_rows ? std::strtoull(min_rows, nullptr, 10) : kFlatBf16MinRows;
+            if (bf16 && slots >= bf16_rows &&
+                qd::query_lds_bytes(static_cast<int>(dim)) + qd::kFlatCandCap * 8 <= qd::kLdsMax) {
                 h->kp = qd::bf16_kpad(static_cast<int>(dim));
                 h->rows_bf16 = h->alloc<uint16_t>(slots * static_cast<uint64_t>(h->kp));
                 h->bf16_maxes = h->alloc<float>(2);

@@ -15,6 +15,14 @@ constexpr int kWave = 64;
 constexpr int kFsBlock = 32;   // vectors per fast-scan block (src/fast_scan.hpp:19)
 constexpr float kLutMax = 63.0f; // largest LUT byte (src/fast_scan.hpp:39)
 
+// Kernels that score rows against one query copy it into LDS when it has at
+// most kQueryLdsDims floats (padded to 16 bytes); a longer query is read from
+// global memory, where every thread of the workgroup reads the same cached row.
+constexpr int kQueryLdsDims = 8192;
+__host__ __device__ constexpr size_t query_lds_bytes(int dim) {
+    return dim <= kQueryLdsDims ? (static_cast<size_t>(dim) * 4 + 15) & ~static_cast<size_t>(15) : 0;
+}
+
 // Orderable key of a float (src/topk.hpp:80-83): unsigned order = float order,
 // with -0.0 below +0.0.
 __device__ __forceinline__ uint32_t fkey(float f) {

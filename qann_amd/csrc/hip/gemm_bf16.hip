@@ -603,9 +603,12 @@ __global__ void __launch_bounds__(256) k_flat_finish(const float* q, int dim, in
         if (tid == 0) ovf[atomicAdd(ovf_cnt, 1u)] = static_cast<uint32_t>(ovf_base + i);
         return;
     }
-    float* qsh = reinterpret_cast<float*>(lds);
-    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
-    for (int t = tid; t < dim; t += blockDim.x) qsh[t] = q[static_cast<size_t>(i) * dim + t];
+    float* qsl = reinterpret_cast<float*>(lds);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + query_lds_bytes(dim));
+    const bool qcopy = dim <= kQueryLdsDims;
+    if (qcopy)
+        for (int t = tid; t < dim; t += blockDim.x) qsl[t] = q[static_cast<size_t>(i) * dim + t];
+    const float* qsh = qcopy ? qsl : q + static_cast<size_t>(i) * dim;
     if (tid == 0) s_count = 0;
     __syncthreads();
     const float* crow = cd + static_cast<size_t>(i) * cap;
@@ -825,7 +828,7 @@ void launch_flat_bf16(const FlatBf16& f, const float* q, const uint16_t* qb, con
     auto reset = [&] { hipLaunchKernelGGL(k_zero_cnt, dim3(cdiv(nq, 256)), dim3(256), 0, s, cnt, nq); };
     gemm_tuned<true>(kGemmFilter, f.xb, f.n, static_cast<size_t>(f.kp), qb, nq, static_cast<size_t>(f.kp), f.kp, ef,
                      reset, s);
-    const size_t lds = ((static_cast<size_t>(f.dim) * 4 + 15) & ~size_t{15}) + static_cast<size_t>(kFlatCandCap) * 8;
+    const size_t lds = query_lds_bytes(f.dim) + static_cast<size_t>(kFlatCandCap) * 8;
     hipLaunchKernelGGL(k_flat_finish, dim3(static_cast<unsigned>(nq)), dim3(256), lds, s, q, f.dim, f.ip, f.rows,
                        qinfo, f.maxes, f.xnorm_max, k, want, kFlatCandCap, cnt, cd, ci, f.labels, out_ids, out_dists,
                        ovf, ovf_cnt, ovf_base);

@@ -289,13 +289,15 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
     // wanted than LDS holds) cap keys per query in global memory, which the
     // block's barriers order as they order LDS
     uint64_t* keys = gkeys ? gkeys + static_cast<size_t>(blockIdx.x) * cap
-                           : reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
+                           : reinterpret_cast<uint64_t*>(lds + query_lds_bytes(dim));
     const int i = blockIdx.x, tid = threadIdx.x;
     float* row = D + static_cast<size_t>(i) * nlist;
+    const bool qcopy = dim <= kQueryLdsDims;
+    const float* qv = qcopy ? qsh : q + static_cast<size_t>(i) * dim;
     float ss = 0.0f;
     for (int t = tid; t < dim; t += blockDim.x) {
         const float v = q[static_cast<size_t>(i) * dim + t];
-        qsh[t] = v;
+        if (qcopy) qsh[t] = v;
         ss += v * v;
     }
     for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, kWave);
@@ -416,7 +418,7 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
         for (int t = tid; t < np; t += blockDim.x) {
             if (t < cnt) {
                 const uint32_t l = static_cast<uint32_t>(keys[t]);
-                keys[t] = dist_key(exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim), l);
+                keys[t] = dist_key(exact_score(ip != 0, qv, cents + static_cast<size_t>(l) * dim, dim), l);
             } else {
                 keys[t] = ~0ull;
             }
@@ -430,7 +432,7 @@ __global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, 
     }
     // every list exactly, then the M smallest (ckey(A), list) keys
     for (int l = tid; l < nlist; l += blockDim.x)
-        row[l] = exact_score(ip != 0, qsh, cents + static_cast<size_t>(l) * dim, dim);
+        row[l] = exact_score(ip != 0, qv, cents + static_cast<size_t>(l) * dim, dim);
     __syncthreads();
     auto key64 = [&](int l) { return dist_key(row[l], static_cast<uint32_t>(l)); };
     const uint64_t kth =
@@ -467,16 +469,18 @@ __global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, c
     __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
     __shared__ float s_red[4], s_red8[8];
     float* qsh = reinterpret_cast<float*>(lds);
-    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}));
+    uint64_t* keys = reinterpret_cast<uint64_t*>(lds + query_lds_bytes(dim));
     const int i = blockIdx.x, tid = threadIdx.x;
     float* row = D + static_cast<size_t>(i) * n;
     auto eligible = [&](int l) {
         return !((excl && bit_test(excl, static_cast<uint32_t>(l))) || (incl && !bit_test(incl, static_cast<uint32_t>(l))));
     };
+    const bool qcopy = dim <= kQueryLdsDims;
+    const float* qv = qcopy ? qsh : q + static_cast<size_t>(i) * dim;
     float ss = 0.0f;
     for (int t = tid; t < dim; t += blockDim.x) {
         const float v = q[static_cast<size_t>(i) * dim + t];
-        qsh[t] = v;
+        if (qcopy) qsh[t] = v;
         ss += v * v;
     }
     for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, kWave);
@@ -521,7 +525,7 @@ __global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, c
         for (int t = tid; t < np; t += blockDim.x) {
             if (t < cnt) {
                 const uint32_t l = static_cast<uint32_t>(keys[t]);
-                keys[t] = dist_key(exact_score(ip != 0, qsh, rows + static_cast<size_t>(l) * dim, dim), l);
+                keys[t] = dist_key(exact_score(ip != 0, qv, rows + static_cast<size_t>(l) * dim, dim), l);
             } else {
                 keys[t] = ~0ull;
             }
@@ -530,7 +534,7 @@ __global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, c
     } else {
         // every eligible row exactly, then the M smallest (ckey, slot) keys
         for (int l = tid; l < n; l += blockDim.x)
-            if (eligible(l)) row[l] = exact_score(ip != 0, qsh, rows + static_cast<size_t>(l) * dim, dim);
+            if (eligible(l)) row[l] = exact_score(ip != 0, qv, rows + static_cast<size_t>(l) * dim, dim);
         __syncthreads();
         auto key64 = [&](int l) { return eligible(l) ? dist_key(row[l], static_cast<uint32_t>(l)) : ~0ull; };
         const uint64_t kth =
@@ -1703,17 +1707,25 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
     const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
     const bool adc = mode <= kDistL2Resid || mode == kDistSdc; // a table summed over the codes
     const size_t tsz = static_cast<size_t>(m) * K; // table floats
-    float* qs = reinterpret_cast<float*>(lds);                      // dim
-    float* sqv = qs + dim;                                          // SQ: vmin, step (2 dim)
-    float* tables = sqv + (mode == kDistSq ? 2 * dim : 0);          // ADC: 1 (inner product) or win tables
+    // the query (and SQ's vmin and step) in LDS up to kQueryLdsDims floats,
+    // else read from global memory
+    const int qfl = static_cast<int>(query_lds_bytes(dim) / 4);
+    float* qsl = reinterpret_cast<float*>(lds);                     // qfl
+    float* sqv = qsl + qfl;                                         // SQ: vmin, step (2 qfl)
+    float* tables = sqv + (mode == kDistSq ? 2 * qfl : 0);          // ADC: 1 (inner product) or win tables
     float* resid = tables + (adc ? (mode == kDistIp ? 1 : win) * tsz : 0); // kDistL2Resid / Sdc: a residual per wave
     const float* q = a.q + static_cast<size_t>(qi) * dim;
-    for (int t = tid; t < dim; t += blockDim.x) qs[t] = q[t];
-    if (mode == kDistSq)
-        for (int t = tid; t < dim; t += blockDim.x) {
-            sqv[t] = ix.sq_vmin[t];
-            sqv[dim + t] = ix.sq_step[t];
-        }
+    const float* qs = qfl ? qsl : q;
+    const float* vmin = qfl ? sqv : ix.sq_vmin;
+    const float* vstep = qfl ? sqv + dim : ix.sq_step;
+    if (qfl) {
+        for (int t = tid; t < dim; t += blockDim.x) qsl[t] = q[t];
+        if (mode == kDistSq)
+            for (int t = tid; t < dim; t += blockDim.x) {
+                sqv[t] = ix.sq_vmin[t];
+                sqv[dim + t] = ix.sq_step[t];
+            }
+    }
     if (mode == kDistIp)
         for (size_t e = tid; e < tsz; e += blockDim.x) tables[e] = a.C[static_cast<size_t>(qi) * tsz + e];
     const uint32_t blk0 = a.item_blk[item0];
@@ -1836,8 +1848,8 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
                     } else if (mode == kDistSq) {
                         int cs;
                         const uint8_t* code = sq_row(ix.codes, entry, ix.vbytes, ix.sq_chunked, cs);
-                        dist = ix.metric == 2 ? -sq_score<true>(qs, code, sqv, sqv + dim, dim, ix.sq_bits, cs)
-                                              : sq_score<false>(qs, code, sqv, sqv + dim, dim, ix.sq_bits, cs);
+                        dist = ix.metric == 2 ? -sq_score<true>(qs, code, vmin, vstep, dim, ix.sq_bits, cs)
+                                              : sq_score<false>(qs, code, vmin, vstep, dim, ix.sq_bits, cs);
                     } else {
                         const float* row = reinterpret_cast<const float*>(ix.codes) + static_cast<size_t>(entry) * dim;
                         dist = exact_score(ix.metric == 2, qs, row, dim);
@@ -1884,15 +1896,20 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_sq8_dists(const DistArgs
     const int np = min(win, P - p0);
     const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
     const size_t item0 = static_cast<size_t>(qi) * a.M + p0;
-    float* qs = reinterpret_cast<float*>(lds); // q, vmin, step: dim each
-    float* vmin = qs + dim;
-    float* step = vmin + dim;
+    // q, vmin, step: dim each in LDS up to kQueryLdsDims floats, else read
+    // from global memory (16-byte aligned: dim is a multiple of 16)
     const float* q = a.q + static_cast<size_t>(qi) * dim;
-    for (int t = tid; t < dim; t += blockDim.x) {
-        qs[t] = q[t];
-        vmin[t] = ix.sq_vmin[t];
-        step[t] = ix.sq_step[t];
-    }
+    const bool qcopy = dim <= kQueryLdsDims;
+    float* qsl = reinterpret_cast<float*>(lds);
+    const float* qs = qcopy ? qsl : q;
+    const float* vmin = qcopy ? qsl + dim : ix.sq_vmin;
+    const float* step = qcopy ? qsl + 2 * dim : ix.sq_step;
+    if (qcopy)
+        for (int t = tid; t < dim; t += blockDim.x) {
+            qsl[t] = q[t];
+            qsl[dim + t] = ix.sq_vmin[t];
+            qsl[2 * dim + t] = ix.sq_step[t];
+        }
     const uint32_t blk0 = a.item_blk[item0];
     if (tid < np) {
         const int list = a.probe_idx[item0 + tid];
@@ -2849,8 +2866,13 @@ __global__ void __launch_bounds__(256) k_rerank(const float* q, int dim, int ip,
     if (pq.k) k = pq.k[i];
     const size_t row = pq.out_off ? pq.out_off[i] : static_cast<size_t>(i) * k;
     while (np < n) np <<= 1;
-    uint64_t* keys = gkeys ? gkeys + static_cast<size_t>(i) * gstride : reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
-    for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    uint64_t* keys = gkeys ? gkeys + static_cast<size_t>(i) * gstride
+                           : reinterpret_cast<uint64_t*>(lds + query_lds_bytes(dim));
+    const bool qcopy = dim <= kQueryLdsDims;
+    if (qcopy)
+        for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    else
+        sq = const_cast<float*>(q + static_cast<size_t>(i) * dim); // read only
     __syncthreads();
     for (int c = threadIdx.x; c < np; c += blockDim.x) {
         if (c < n) {
@@ -2904,12 +2926,15 @@ __global__ void __launch_bounds__(256) k_block_select(const float* q, int dim, i
     float* sq = reinterpret_cast<float*>(lds);
     // cap (pow2) keys, then the final sort's arrays: in LDS after the query, or in HBM
     uint64_t* keys = gbuf ? reinterpret_cast<uint64_t*>(gbuf + blockIdx.x * gstride)
-                          : reinterpret_cast<uint64_t*>(sq + ((dim + 1) & ~1));
+                          : reinterpret_cast<uint64_t*>(lds + query_lds_bytes(dim));
     __shared__ int count_s;
     __shared__ uint64_t thr_s;
     __shared__ int big_s;
     const int i = blockIdx.x;
-    for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    if (dim <= kQueryLdsDims)
+        for (int t = threadIdx.x; t < dim; t += blockDim.x) sq[t] = q[static_cast<size_t>(i) * dim + t];
+    else
+        sq = const_cast<float*>(q + static_cast<size_t>(i) * dim); // read only
     if (threadIdx.x == 0) {
         count_s = 0;
         thr_s = ~0ull;
@@ -3076,7 +3101,7 @@ void launch_coarse_gemm(const float* q, int nq, const float* cents, const float*
 }
 
 bool coarse_select_global(int M, int dim) {
-    return ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + coarse_select_cap(M) * 8 > kLdsMax;
+    return query_lds_bytes(dim) + coarse_select_cap(M) * 8 > kLdsMax;
 }
 
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
@@ -3087,7 +3112,7 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     const bool global = coarse_select_global(M, dim);
     if (global && !gkeys) throw std::invalid_argument("the coarse select needs global keys for this many probes");
     if (!global) gkeys = nullptr;
-    const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + (global ? 0 : cap * 8);
+    const size_t lds = query_lds_bytes(dim) + (global ? 0 : cap * 8);
     // each query's row of scores stays in registers, 16 values a thread: with
     // 256 threads up to 4,096 lists, with 1,024 up to 16,384; larger rows are
     // read from memory, and take a threshold from a sample first (a row in
@@ -3126,7 +3151,7 @@ void launch_flat_select(float* D, const float* q, int nq, const float* rows, int
                         float* out_dists, hipStream_t s) {
     if (nq <= 0) return;
     const size_t cap = coarse_select_cap(k);
-    const size_t lds = ((static_cast<size_t>(dim) * 4 + 15) & ~size_t{15}) + cap * 8;
+    const size_t lds = query_lds_bytes(dim) + cap * 8;
     hipLaunchKernelGGL(k_flat_select, dim3(nq), dim3(256), lds, s, D, q, rows, n, dim, ip, xnorm_max, k,
                        static_cast<int>(cap), excl, incl, labels, out_ids, out_dists);
 }
@@ -3367,9 +3392,9 @@ void launch_fs_block_check(const uint8_t* luts, int m, int m4, const uint8_t* bl
 
 size_t ivf_dists_lds(DistArgs& a) {
     const DevIvf& ix = a.ix;
-    const size_t dim4 = static_cast<size_t>(ix.dim) * 4;
+    const size_t dim4 = static_cast<size_t>(ix.dim) * 4, qb = query_lds_bytes(ix.dim);
     const size_t tbytes = static_cast<size_t>(ix.m) * ix.K * 4;
-    size_t fixed = dim4 + (a.mode == kDistSq ? 2 * dim4 : 0);
+    size_t fixed = qb + (a.mode == kDistSq ? 2 * qb : 0);
     a.resid_in_lds = 0;
     a.win = kSumsWin;
     if (a.mode == kDistIp) fixed += tbytes;
@@ -3385,7 +3410,9 @@ size_t ivf_dists_lds(DistArgs& a) {
     }
     if (fixed > kLdsMax) return 0;
     a.nwin = (a.M + a.win - 1) / a.win;
-    return fixed;
+    // float and SQ lists of queries read from global memory need none; 0 is
+    // the failure value, so they ask for 16 bytes
+    return std::max<size_t>(fixed, 16);
 }
 
 // Chunk-major 8-bit SQ lists go to k_sq8_dists unless QANN_AMD_SQ8=thread
@@ -3471,7 +3498,7 @@ void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, c
                    const int* cand_count, int kc, int k, const int64_t* labels, int64_t* out_ids, float* out_dists,
                    uint64_t* gkeys, hipStream_t s, const PerQuery& pq) {
     const size_t np = rerank_keys(kc);
-    const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + (gkeys ? 0 : np * 8);
+    const size_t lds = query_lds_bytes(dim) + (gkeys ? 0 : np * 8);
     hipLaunchKernelGGL(k_rerank, dim3(nq), dim3(256), lds, s, q, dim, ip, rows, cand, cand_count, kc, k, labels,
                        out_ids, out_dists, gkeys, static_cast<int>(np), pq);
 }
@@ -3495,7 +3522,7 @@ void launch_block_select(const float* q, int nq, int dim, int ip, const SlotScor
     int cap = 1;
     while (cap < k + threads) cap <<= 1;
     const size_t buf = block_select_bytes(k, final_by_label && labels);
-    const size_t lds = static_cast<size_t>((dim + 1) & ~1) * 4 + (gbuf ? 0 : buf);
+    const size_t lds = query_lds_bytes(dim) + (gbuf ? 0 : buf);
     hipLaunchKernelGGL(k_block_select, dim3(nq), dim3(threads), lds, s, q, dim, ip, sc, slot_list, n_slots, deleted,
                        use_fkey, final_by_label, k, cap, labels, out_ids, out_dists, gbuf, buf);
 }

@@ -667,3 +667,41 @@ def test_gpu_stage_profiler_times_without_changing_results():
         assert stages[name][1] == len(runs), (name, stages[name])
         assert stages[name][0] > 0.0, (name, stages[name])
     assert all(v == 0 for v in _stage_launches().values())
+
+
+@pytest.mark.parametrize("dim", [8193, 10000])
+def test_gpu_long_queries_match_oracle(tmp_path, monkeypatch, oracle, dim):
+    # queries longer than the kernels' LDS copy (8,192 floats) are read from
+    # global memory: flat search by the FP32 GEMM, the BF16 GEMM (its row
+    # threshold lowered) and the per-slot select (fewer than 8 queries, and
+    # small filters); IVF over float lists with refine (coarse select, list
+    # distances, re-rank) -- against the oracle, ids and distance bits
+    rng = np.random.default_rng(dim)
+    x = mixture(rng, 3000, dim, centers=20, sigma=0.05)
+    q = mixture(rng, 20, dim, centers=20, sigma=0.05)
+    for metric in (qann_amd.Metric.L2, qann_amd.Metric.InnerProduct):
+        for form in ("fp32", "bf16"):
+            monkeypatch.setenv("QANN_AMD_FLAT_BF16_ROWS", "1000" if form == "bf16" else "1000000000")
+            flat = qann_amd.FlatIndex(dim, metric=metric)
+            flat.add(x)
+            flat.remove(rng.choice(3000, 50, replace=False))
+            path = tmp_path / "flat_long.qann"
+            flat.save(str(path))
+            orc = oracle.load(path)
+            for qq in (q, q[:3]):
+                for k in (1, 10, 100):
+                    assert_same(flat.batch_query(qq, k), orc.batch_query(qq, k), f"{metric} {form} nq {len(qq)} k {k}")
+            allowed = rng.choice(3000, 40, replace=False)
+            assert_same(flat.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed),
+                        f"{metric} {form} filtered")
+    ivf = qann_amd.IVFIndex(dim, 100, 6, metric=qann_amd.Metric.L2)
+    ivf.train(x, seed=1)
+    idx = qann_amd.RefineIndex(ivf, 3)
+    idx.add(x)
+    path = tmp_path / "ivf_long.qann"
+    idx.save(str(path))
+    orc = oracle.load(path)
+    for k in (1, 10):
+        assert_same(idx.batch_query(q, k), orc.batch_query(q, k), f"ivf k {k}")
+    allowed = rng.choice(3000, 900, replace=False)
+    assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "ivf filtered")
