@@ -750,7 +750,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         sc.used = 0;
         const bool f = filtered;
         const size_t selb = qd::block_select_bytes(static_cast<int>(k), false);
-        const bool gsel = qlds + selb > qd::kLdsMax;
+        const bool gsel = qlds + selb > qd::block_select_room();
         sc.reserve((f ? pad256(h->slots * 4) + 256 : 256) + (gsel ? pad256(nq * selb) : 0));
         unsigned char* gbuf = gsel ? sc.take<unsigned char>(nq * selb) : nullptr;
         uint32_t* slot_list = nullptr;
@@ -787,7 +787,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         if (!mix && filtered && p.allowed_count <= 6ull * ix.nlist) {
             sc.used = 0;
             const size_t selb = qd::block_select_bytes(static_cast<int>(k), h->labels != nullptr);
-            const bool gsel = qlds + selb > qd::kLdsMax;
+            const bool gsel = qlds + selb > qd::block_select_room();
             sc.reserve(pad256(p.allowed_count * 4) + 256 + (gsel ? pad256(nq * selb) : 0));
             unsigned char* gbuf = gsel ? sc.take<unsigned char>(nq * selb) : nullptr;
             uint32_t* slot_list = sc.take<uint32_t>(p.allowed_count);
@@ -815,7 +815,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     if (!mix && !refine && flat_lists && filtered && p.allowed_count <= 3ull * ix.nlist) {
         sc.used = 0;
         const size_t selb = qd::block_select_bytes(static_cast<int>(k), h->labels != nullptr);
-        const bool gsel = qlds + selb > qd::kLdsMax;
+        const bool gsel = qlds + selb > qd::block_select_room();
         sc.reserve(pad256(p.allowed_count * 4) + 256 + (gsel ? pad256(nq * selb) : 0));
         unsigned char* gbuf = gsel ? sc.take<unsigned char>(nq * selb) : nullptr;
         uint32_t* slot_list = sc.take<uint32_t>(p.allowed_count);
@@ -853,9 +853,13 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const uint32_t m = static_cast<uint32_t>(ix.m);
     const size_t tsz = static_cast<size_t>(m) * static_cast<size_t>(ix.K); // table floats
     // the TopK buffers go to HBM when they do not fit LDS
-    const bool gtopk = qd::replay_lds_bytes(static_cast<int>(kp)) > qd::kLdsMax;
+    // TopK buffers stay within the 64 KB budget: above it one replay wave per
+    // CU in LDS ran slower than HBM buffers at more waves (k' = 4,000 on
+    // 8,192 queries: 18.0 against 8.8 ms); the re-rank's and the selects' key
+    // sorts gain from the device's whole LDS (k' = 10,000: 29.4 against 44.3 ms)
+    const bool gtopk = qd::replay_lds_bytes(static_cast<int>(kp)) > std::min(qd::kLdsMax, qd::replay_room());
     const size_t rkeys = qd::rerank_keys(static_cast<int>(kp));
-    const bool grerank = refine && qlds + rkeys * 8 > qd::kLdsMax;
+    const bool grerank = refine && qlds + rkeys * 8 > qd::rerank_room();
 
     // IVFIndex::scan's filter parameters (src/ivf_index.cpp:462, 502, 510-514)
     int min_probes = static_cast<int>(nprobe), wanted = 0, block_filter = 0;

@@ -3100,8 +3100,63 @@ void launch_coarse_gemm(const float* q, int nq, const float* cents, const float*
     }
 }
 
+size_t lds_room(const void* fn) {
+    // the device's limit and each kernel's static LDS, looked up once
+    static std::mutex mu;
+    static std::map<int, size_t> dev_limit;
+    static std::map<const void*, size_t> static_lds;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    size_t limit = 0, fixed = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto d = dev_limit.find(dev);
+        if (d == dev_limit.end()) {
+            int v = 0;
+            if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+                v = static_cast<int>(kLdsMax);
+            d = dev_limit.emplace(dev, static_cast<size_t>(v)).first;
+        }
+        limit = d->second;
+        auto f = static_lds.find(fn);
+        if (f == static_lds.end()) {
+            hipFuncAttributes fa{};
+            const size_t v = hipFuncGetAttributes(&fa, fn) == hipSuccess ? fa.sharedSizeBytes : limit;
+            f = static_lds.emplace(fn, v).first;
+        }
+        fixed = f->second;
+    }
+    if (const char* e = std::getenv("QANN_AMD_LDS_LIMIT")) {
+        const long v = std::atol(e);
+        if (v >= 16 * 1024 && static_cast<size_t>(v) < limit) limit = static_cast<size_t>(v);
+    }
+    return fixed < limit ? limit - fixed : 0;
+}
+
+size_t replay_room() {
+    return std::min({lds_room(reinterpret_cast<const void*>(&k_fs_replay<false>)),
+                     lds_room(reinterpret_cast<const void*>(&k_fs_replay<true>)),
+                     lds_room(reinterpret_cast<const void*>(&k_fl_replay<false>)),
+                     lds_room(reinterpret_cast<const void*>(&k_fl_replay<true>))});
+}
+
+size_t rerank_room() { return lds_room(reinterpret_cast<const void*>(&k_rerank)); }
+
+size_t block_select_room() { return lds_room(reinterpret_cast<const void*>(&k_block_select)); }
+
+size_t coarse_select_room() {
+    return std::min({lds_room(reinterpret_cast<const void*>(&k_coarse_select<256, 16, true>)),
+                     lds_room(reinterpret_cast<const void*>(&k_coarse_select<256, 16, false>)),
+                     lds_room(reinterpret_cast<const void*>(&k_coarse_select<1024, 16, true>)),
+                     lds_room(reinterpret_cast<const void*>(&k_coarse_select<1024, 16, false>)),
+                     lds_room(reinterpret_cast<const void*>(&k_coarse_select<256, 0, true>)),
+                     lds_room(reinterpret_cast<const void*>(&k_coarse_select<256, 0, false>))});
+}
+
+size_t ivf_dists_room() { return lds_room(reinterpret_cast<const void*>(&k_ivf_dists)); }
+
 bool coarse_select_global(int M, int dim) {
-    return query_lds_bytes(dim) + coarse_select_cap(M) * 8 > kLdsMax;
+    return query_lds_bytes(dim) + coarse_select_cap(M) * 8 > coarse_select_room();
 }
 
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
@@ -3403,12 +3458,15 @@ size_t ivf_dists_lds(DistArgs& a) {
         a.resid_in_lds = 1;
         fixed += kSumsWaves * dim4;
     }
+    // windows of tables within the budget; a table that alone exceeds it
+    // (m K >= 16,384) takes what the device allows, a few probes at a time
+    const size_t budget = fixed + tbytes <= kLdsMax ? kLdsMax : std::max(kLdsMax, ivf_dists_room());
     if (a.mode == kDistL2Pre || resid) {
-        if (fixed + tbytes > kLdsMax) return 0;
-        a.win = static_cast<int>(std::min<size_t>(kSumsWin, (kLdsMax - fixed) / tbytes));
+        if (fixed + tbytes > budget) return 0;
+        a.win = static_cast<int>(std::min<size_t>(kSumsWin, (budget - fixed) / tbytes));
         fixed += static_cast<size_t>(a.win) * tbytes;
     }
-    if (fixed > kLdsMax) return 0;
+    if (fixed > budget) return 0;
     a.nwin = (a.M + a.win - 1) / a.win;
     // float and SQ lists of queries read from global memory need none; 0 is
     // the failure value, so they ask for 16 bytes

@@ -174,8 +174,21 @@ struct ReplayArgs {
     int window;                // block minima the window replay keeps in LDS (set by the launcher)
 };
 
-// Largest LDS a kernel here asks for (the default dynamic limit).
+// LDS budget of a workgroup where more LDS buys fewer workgroups per CU:
+// the windows of probe tables and of block minima are sized to it.
 constexpr size_t kLdsMax = 64 * 1024;
+
+// Dynamic LDS a launch of the kernel `fn` may ask for on the current device:
+// the device's per-workgroup limit (160 KB on gfx950; QANN_AMD_LDS_LIMIT=<bytes>
+// lowers it) less the kernel's static LDS. Buffers that do not fit go to HBM.
+size_t lds_room(const void* fn);
+// lds_room of the kernels that hold a query's TopK buffers (the replays), its
+// re-rank keys, its exact-select buffers, its coarse shortlist and its PQ tables
+size_t replay_room();
+size_t rerank_room();
+size_t block_select_room();
+size_t coarse_select_room();
+size_t ivf_dists_room();
 
 // Replays of batches below this many queries stage their loads through LDS.
 constexpr int kReplayStageBelow = 2048;

@@ -705,3 +705,66 @@ def test_gpu_long_queries_match_oracle(tmp_path, monkeypatch, oracle, dim):
         assert_same(idx.batch_query(q, k), orc.batch_query(q, k), f"ivf k {k}")
     allowed = rng.choice(3000, 900, replace=False)
     assert_same(idx.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "ivf filtered")
+
+
+PQ8_WIDE = [
+    # metric, dim, m (K = 256: a table of m x 256 floats, 64 KB at m = 64), precomputed, SDC
+    (qann_amd.Metric.L2, 128, 64, True, False),
+    (qann_amd.Metric.L2, 128, 64, False, False),
+    (qann_amd.Metric.InnerProduct, 128, 64, False, False),
+    (qann_amd.Metric.Cosine, 192, 96, True, False),
+    (qann_amd.Metric.L2, 192, 96, False, True),
+]
+
+
+@pytest.mark.parametrize("metric,dim,m,pre,sdc", PQ8_WIDE)
+def test_gpu_wide_pq8_tables_match_oracle(tmp_path, oracle, metric, dim, m, pre, sdc):
+    # one-byte PQ lists whose per-(query, list) table alone exceeds 64 KB take
+    # gfx950's larger LDS, a few probes per workgroup
+    rng = np.random.default_rng(dim + m)
+    x = mixture(rng, 6000, dim, centers=30)
+    q = mixture(rng, 60, dim, centers=30)
+    ivf = qann_amd.IVFIndex(dim, 100, 8, metric=metric)
+    if sdc:
+        ivf.pq_distance = qann_amd.PQDistance.SDC
+    ivf.enable_pq(m, 256)
+    ivf.train(x, seed=5)
+    ivf.precomputed_tables = pre
+    if metric == qann_amd.Metric.Cosine:
+        ivf.cosine_inner_product = False
+    ivf.add(x)
+    path = tmp_path / "wide_pq8.qann"
+    ivf.save(str(path))
+    orc = oracle.load(path)
+    for npb in (1, 8, 30):
+        ivf.nprobe = npb
+        orc.set_nprobe(npb)
+        for k in (1, 10, 50):
+            assert_same(ivf.batch_query(q, k), orc.batch_query(q, k), f"nprobe {npb} k {k}")
+    allowed = rng.choice(6000, 1500, replace=False)
+    ivf.nprobe = 8
+    orc.set_nprobe(8)
+    assert_same(ivf.batch_query(q, 10, allowed=allowed), orc.batch_query(q, 10, allowed=allowed), "filtered")
+
+
+@pytest.mark.parametrize("limit", ["device", "65536"])
+def test_gpu_large_topk_and_rerank_buffers_match_oracle(tmp_path, monkeypatch, oracle, limit):
+    # k' = 4,000 and 10,000 candidates: TopK buffers of 96 and 240 KB in HBM;
+    # re-rank keys of 32 and 128 KB in LDS under the device's limit, and with
+    # the limit lowered to 64 KB the larger in HBM
+    if limit != "device":
+        monkeypatch.setenv("QANN_AMD_LDS_LIMIT", limit)
+    rng = np.random.default_rng(44)
+    x = mixture(rng, 20000, 32, centers=40)
+    q = mixture(rng, 30, 32, centers=40)
+    ivf = qann_amd.IVFIndex(32, 100, 20)
+    ivf.enable_pq(16, fast_scan=True)
+    ivf.train(x, seed=2)
+    idx = qann_amd.RefineIndex(ivf, 10)
+    idx.add(x)
+    path = tmp_path / "big_k.qann"
+    idx.save(str(path))
+    orc = oracle.load(path)
+    orc.set_nprobe(20)
+    for k in (400, 1000):
+        assert_same(idx.batch_query(q, k), orc.batch_query(q, k), f"{limit} k {k}")
