@@ -1712,8 +1712,8 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
     const int qfl = static_cast<int>(query_lds_bytes(dim) / 4);
     float* qsl = reinterpret_cast<float*>(lds);                     // qfl
     float* sqv = qsl + qfl;                                         // SQ: vmin, step (2 qfl)
-    float* tables = sqv + (mode == kDistSq ? 2 * qfl : 0);          // ADC: 1 (inner product) or win tables
-    float* resid = tables + (adc ? (mode == kDistIp ? 1 : win) * tsz : 0); // kDistL2Resid / Sdc: a residual per wave
+    float* tables = sqv + (mode == kDistSq ? 2 * qfl : 0);          // ADC: 1 (inner product) or win tables of mc K
+    float* resid = tables + (adc ? static_cast<size_t>(mode == kDistIp ? 1 : win) * a.mc * K : 0); // L2Resid / Sdc: a residual per wave
     const float* q = a.q + static_cast<size_t>(qi) * dim;
     const float* qs = qfl ? qsl : q;
     const float* vmin = qfl ? sqv : ix.sq_vmin;
@@ -1726,8 +1726,6 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
                 sqv[dim + t] = ix.sq_step[t];
             }
     }
-    if (mode == kDistIp)
-        for (size_t e = tid; e < tsz; e += blockDim.x) tables[e] = a.C[static_cast<size_t>(qi) * tsz + e];
     const uint32_t blk0 = a.item_blk[item0];
     if (tid < np) {
         const int list = a.probe_idx[item0 + tid];
@@ -1738,134 +1736,155 @@ __global__ void __launch_bounds__(kSumsWaves * kWave) k_ivf_dists(const DistArgs
         w_start[tid] = rel;
         if (tid == np - 1) w_start[np] = rel + (ix.list_off[list + 1] - ix.list_off[list] + kFsBlock - 1) / kFsBlock;
     }
-    __syncthreads();
-    if (mode == kDistL2Pre || mode == kDistL2Resid || mode == kDistSdc) {
-        // the (query, list) L2 table: B + C (+ A on row 0), compute_adc_table of
-        // q - c, or for SDC the rows of the SDC table picked by the code of q - c
-        for (int p = wave; p < np; p += kSumsWaves) {
-            if (w_start[p + 1] == w_start[p]) continue;
-            const int list = w_list[p];
-            float* tb = tables + static_cast<size_t>(p) * tsz;
-            if (mode == kDistL2Pre) {
-                const float* B = ix.B + static_cast<size_t>(list) * tsz;
-                const float* Cq = a.C + static_cast<size_t>(qi) * tsz;
-                const float A = a.probe_A[item0 + p];
-                for (size_t e = lane; e < tsz; e += kWave) {
-                    float t = B[e] + Cq[e];
-                    if (e < static_cast<size_t>(K)) t = t + A;
-                    tb[e] = t;
-                }
-            } else {
-                const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * dim;
-                float* r = a.resid_in_lds ? resid + wave * dim : nullptr;
-                if (r) {
-                    for (int t = lane; t < dim; t += kWave) r[t] = qs[t] - c[t];
-                    wave_lds_sync();
-                }
-                if (mode == kDistSdc) {
-                    // PQCodebook::encode of the residual (src/pq.cpp:88-156): per
-                    // subspace the first centroid of least (r_0 - y_0)^2 +
-                    // fma-accumulated (r_j - y_j)^2, compared as int32 bits as
-                    // the reference does; lanes over centroids, then the wave's
-                    // minimum by (bits, index), the same on every lane
-                    for (int sidx = 0; sidx < m; ++sidx) {
-                        const int x0 = sidx * ix.sub;
-                        int bd = 0x7FFFFFFF;
-                        int bc = K;
-                        for (int cc = lane; cc < K; cc += kWave) {
-                            float dd = (r ? r[x0] : qs[x0] - c[x0]) - ix.pq_cent_t[static_cast<size_t>(x0) * K + cc];
-                            float acc = dd * dd;
-                            for (int j = 1; j < ix.sub; ++j) {
-                                const int x = x0 + j;
-                                dd = (r ? r[x] : qs[x] - c[x]) - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
-                                acc = fmaf(dd, dd, acc);
-                            }
-                            if (cc == lane || __float_as_int(acc) < bd) {
-                                bd = __float_as_int(acc);
-                                bc = cc;
-                            }
-                        }
-                        for (int off = kWave / 2; off > 0; off >>= 1) {
-                            const int od = __shfl_xor(bd, off);
-                            const int oc = __shfl_xor(bc, off);
-                            if (od < bd || (od == bd && oc < bc)) {
-                                bd = od;
-                                bc = oc;
-                            }
-                        }
-                        const float* row = ix.sdc + (static_cast<size_t>(sidx) * K + bc) * K;
-                        for (int cc = lane; cc < K; cc += kWave) tb[static_cast<size_t>(sidx) * K + cc] = row[cc];
-                    }
-                } else {
-                    for (size_t e = lane; e < tsz; e += kWave) {
-                        const int sidx = static_cast<int>(e / K), cc = static_cast<int>(e % K);
-                        float acc = 0.0f;
-                        for (int j = 0; j < ix.sub; ++j) {
-                            const int x = sidx * ix.sub + j;
-                            const float rs = r ? r[x] : qs[x] - c[x];
-                            const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
-                            acc = fmaf(d, d, acc);
-                        }
-                        tb[e] = acc;
-                    }
-                }
-                wave_lds_sync();
-            }
-        }
-        __syncthreads();
-    }
-
-    const uint32_t T = w_start[np];
     float* qd = a.dists + static_cast<size_t>(qi) * a.stride_slots + static_cast<size_t>(blk0) * kFsBlock;
     uint32_t* qbmin = a.bmin + static_cast<size_t>(qi) * (a.stride_slots / kFsBlock) + blk0;
     const int v = tid & 31;
-    for (uint32_t base = 0; base < T; base += blockDim.x / kFsBlock) {
-        const uint32_t t = base + (tid >> 5);
-        uint32_t key = 0xFFFFFFFFu;
-        int p = 0;
-        if (t < T) {
-#pragma unroll
-            for (int step = kSumsWin / 2; step > 0; step >>= 1) {
-                const int c = min(p + step, np - 1);
-                p = w_start[c] <= t ? c : p;
-            }
-            const int list = w_list[p];
-            const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
-            const uint32_t j = (t - w_start[p]) * kFsBlock + v;
-            float dist = INFINITY;
-            if (j < n) {
-                const uint32_t entry = ix.list_off[list] + j;
-                const uint32_t slot = ix.list_ids[entry];
-                const bool ok = slot != 0xFFFFFFFFu &&
-                                (filter ? bit_test(filter, slot) : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
-                if (ok) {
-                    if (adc) {
-                        const uint8_t* code = ix.codes + static_cast<size_t>(entry) * m;
-                        const float* tb = tables + (mode == kDistIp ? 0 : static_cast<size_t>(p) * tsz);
-                        float d = 0.0f;
-                        for (int sidx = 0; sidx < m; ++sidx) d += tb[sidx * K + code[sidx]];
-                        dist = mode == kDistIp ? d + w_base[p] : d;
-                    } else if (mode == kDistSq) {
-                        int cs;
-                        const uint8_t* code = sq_row(ix.codes, entry, ix.vbytes, ix.sq_chunked, cs);
-                        dist = ix.metric == 2 ? -sq_score<true>(qs, code, vmin, vstep, dim, ix.sq_bits, cs)
-                                              : sq_score<false>(qs, code, vmin, vstep, dim, ix.sq_bits, cs);
-                    } else {
-                        const float* row = reinterpret_cast<const float*>(ix.codes) + static_cast<size_t>(entry) * dim;
-                        dist = exact_score(ix.metric == 2, qs, row, dim);
+    // ADC tables hold mc subspaces at a time (all m unless a table does not
+    // fit LDS): each vector's sum runs over the subspaces in order, carried
+    // from one run of subspaces to the next through its slot in dists
+    const int mc = adc ? a.mc : 1;
+    int s0 = 0;
+    do {
+        const int s1 = adc ? min(m, s0 + mc) : 0;
+        const size_t csz = static_cast<size_t>(s1 - s0) * K; // this run's table floats
+        const bool last = !adc || s1 == m;
+        __syncthreads(); // the previous run's tables are read
+        if (mode == kDistIp)
+            for (size_t e = tid; e < csz; e += blockDim.x)
+                tables[e] = a.C[static_cast<size_t>(qi) * tsz + static_cast<size_t>(s0) * K + e];
+        if (mode == kDistL2Pre || mode == kDistL2Resid || mode == kDistSdc) {
+            // the (query, list) L2 table: B + C (+ A on row 0), compute_adc_table of
+            // q - c, or for SDC the rows of the SDC table picked by the code of q - c
+            for (int p = wave; p < np; p += kSumsWaves) {
+                if (w_start[p + 1] == w_start[p]) continue;
+                const int list = w_list[p];
+                float* tb = tables + static_cast<size_t>(p) * csz;
+                if (mode == kDistL2Pre) {
+                    const float* B = ix.B + static_cast<size_t>(list) * tsz + static_cast<size_t>(s0) * K;
+                    const float* Cq = a.C + static_cast<size_t>(qi) * tsz + static_cast<size_t>(s0) * K;
+                    const float A = a.probe_A[item0 + p];
+                    for (size_t e = lane; e < csz; e += kWave) {
+                        float t = B[e] + Cq[e];
+                        if (s0 == 0 && e < static_cast<size_t>(K)) t = t + A;
+                        tb[e] = t;
                     }
-                    key = fkey(dist);
+                } else {
+                    const float* c = (ix.residual_centers ? ix.residual_centers : ix.centroids) + static_cast<size_t>(list) * dim;
+                    float* r = a.resid_in_lds ? resid + wave * dim : nullptr;
+                    if (r) {
+                        for (int t = lane; t < dim; t += kWave) r[t] = qs[t] - c[t];
+                        wave_lds_sync();
+                    }
+                    if (mode == kDistSdc) {
+                        // PQCodebook::encode of the residual (src/pq.cpp:88-156): per
+                        // subspace the first centroid of least (r_0 - y_0)^2 +
+                        // fma-accumulated (r_j - y_j)^2, compared as int32 bits as
+                        // the reference does; lanes over centroids, then the wave's
+                        // minimum by (bits, index), the same on every lane
+                        for (int sidx = s0; sidx < s1; ++sidx) {
+                            const int x0 = sidx * ix.sub;
+                            int bd = 0x7FFFFFFF;
+                            int bc = K;
+                            for (int cc = lane; cc < K; cc += kWave) {
+                                float dd = (r ? r[x0] : qs[x0] - c[x0]) - ix.pq_cent_t[static_cast<size_t>(x0) * K + cc];
+                                float acc = dd * dd;
+                                for (int j = 1; j < ix.sub; ++j) {
+                                    const int x = x0 + j;
+                                    dd = (r ? r[x] : qs[x] - c[x]) - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
+                                    acc = fmaf(dd, dd, acc);
+                                }
+                                if (cc == lane || __float_as_int(acc) < bd) {
+                                    bd = __float_as_int(acc);
+                                    bc = cc;
+                                }
+                            }
+                            for (int off = kWave / 2; off > 0; off >>= 1) {
+                                const int od = __shfl_xor(bd, off);
+                                const int oc = __shfl_xor(bc, off);
+                                if (od < bd || (od == bd && oc < bc)) {
+                                    bd = od;
+                                    bc = oc;
+                                }
+                            }
+                            const float* row = ix.sdc + (static_cast<size_t>(sidx) * K + bc) * K;
+                            for (int cc = lane; cc < K; cc += kWave) tb[static_cast<size_t>(sidx - s0) * K + cc] = row[cc];
+                        }
+                    } else {
+                        for (size_t e = lane; e < csz; e += kWave) {
+                            const int sidx = s0 + static_cast<int>(e / K), cc = static_cast<int>(e % K);
+                            float acc = 0.0f;
+                            for (int j = 0; j < ix.sub; ++j) {
+                                const int x = sidx * ix.sub + j;
+                                const float rs = r ? r[x] : qs[x] - c[x];
+                                const float d = rs - ix.pq_cent_t[static_cast<size_t>(x) * K + cc];
+                                acc = fmaf(d, d, acc);
+                            }
+                            tb[e] = acc;
+                        }
+                    }
+                    wave_lds_sync();
                 }
             }
-            qd[static_cast<size_t>(t) * kFsBlock + v] = dist;
         }
-        // the block's 32 threads: their smallest key
-        for (int off = 16; off > 0; off >>= 1) key = min(key, static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), off, 32)));
-        if (t < T && v == 0) {
-            qbmin[t] = key;
-            atomicMin(&w_min[p], key);
+        __syncthreads();
+
+        const uint32_t nT = w_start[np];
+        for (uint32_t base = 0; base < nT; base += blockDim.x / kFsBlock) {
+            const uint32_t t = base + (tid >> 5);
+            uint32_t key = 0xFFFFFFFFu;
+            int p = 0;
+            if (t < nT) {
+#pragma unroll
+                for (int step = kSumsWin / 2; step > 0; step >>= 1) {
+                    const int c = min(p + step, np - 1);
+                    p = w_start[c] <= t ? c : p;
+                }
+                const int list = w_list[p];
+                const uint32_t n = ix.list_off[list + 1] - ix.list_off[list];
+                const uint32_t j = (t - w_start[p]) * kFsBlock + v;
+                float dist = INFINITY;
+                bool scored = false;
+                float* out = qd + static_cast<size_t>(t) * kFsBlock + v;
+                if (j < n) {
+                    const uint32_t entry = ix.list_off[list] + j;
+                    const uint32_t slot = ix.list_ids[entry];
+                    const bool ok = slot != 0xFFFFFFFFu &&
+                                    (filter ? bit_test(filter, slot) : (ix.deleted == nullptr || !bit_test(ix.deleted, slot)));
+                    if (ok) {
+                        if (adc) {
+                            const uint8_t* code = ix.codes + static_cast<size_t>(entry) * m;
+                            const float* tb = tables + (mode == kDistIp ? 0 : static_cast<size_t>(p) * csz);
+                            float d = s0 == 0 ? 0.0f : *out;
+                            for (int sidx = s0; sidx < s1; ++sidx) d += tb[(sidx - s0) * K + code[sidx]];
+                            dist = mode == kDistIp && last ? d + w_base[p] : d;
+                        } else if (mode == kDistSq) {
+                            int cs;
+                            const uint8_t* code = sq_row(ix.codes, entry, ix.vbytes, ix.sq_chunked, cs);
+                            dist = ix.metric == 2 ? -sq_score<true>(qs, code, vmin, vstep, dim, ix.sq_bits, cs)
+                                                  : sq_score<false>(qs, code, vmin, vstep, dim, ix.sq_bits, cs);
+                        } else {
+                            const float* row = reinterpret_cast<const float*>(ix.codes) + static_cast<size_t>(entry) * dim;
+                            dist = exact_score(ix.metric == 2, qs, row, dim);
+                        }
+                        key = fkey(dist);
+                        scored = true;
+                    }
+                }
+                // a run before the last leaves the partial sums; the last the distances
+                if (last || scored) *out = dist;
+            }
+            if (last) {
+                // the block's 32 threads: their smallest key
+                for (int off = 16; off > 0; off >>= 1)
+                    key = min(key, static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), off, 32)));
+                if (t < nT && v == 0) {
+                    qbmin[t] = key;
+                    atomicMin(&w_min[p], key);
+                }
+            }
         }
-    }
+        s0 = s1;
+    } while (s0 < m);
     __syncthreads();
     if (tid < np) a.item_min[item0 + tid] = w_min[tid];
 }
@@ -3448,25 +3467,33 @@ void launch_fs_block_check(const uint8_t* luts, int m, int m4, const uint8_t* bl
 size_t ivf_dists_lds(DistArgs& a) {
     const DevIvf& ix = a.ix;
     const size_t dim4 = static_cast<size_t>(ix.dim) * 4, qb = query_lds_bytes(ix.dim);
-    const size_t tbytes = static_cast<size_t>(ix.m) * ix.K * 4;
+    const size_t tbytes = static_cast<size_t>(ix.m) * ix.K * 4, sbytes = static_cast<size_t>(ix.K) * 4;
+    const bool adc = a.mode <= kDistL2Resid || a.mode == kDistSdc;
     size_t fixed = qb + (a.mode == kDistSq ? 2 * qb : 0);
     a.resid_in_lds = 0;
     a.win = kSumsWin;
-    if (a.mode == kDistIp) fixed += tbytes;
+    a.mc = ix.m;
+    // the 64 KB budget, or less under a lowered device limit
+    const size_t room = ivf_dists_room(), budget = std::min(kLdsMax, room);
     const bool resid = a.mode == kDistL2Resid || a.mode == kDistSdc;
-    if (resid && fixed + kSumsWaves * dim4 + tbytes <= kLdsMax) {
+    if (resid && fixed + kSumsWaves * dim4 + tbytes <= budget) {
         a.resid_in_lds = 1;
         fixed += kSumsWaves * dim4;
     }
-    // windows of tables within the budget; a table that alone exceeds it
-    // (m K >= 16,384) takes what the device allows, a few probes at a time
-    const size_t budget = fixed + tbytes <= kLdsMax ? kLdsMax : std::max(kLdsMax, ivf_dists_room());
-    if (a.mode == kDistL2Pre || resid) {
-        if (fixed + tbytes > budget) return 0;
-        a.win = static_cast<int>(std::min<size_t>(kSumsWin, (budget - fixed) / tbytes));
-        fixed += static_cast<size_t>(a.win) * tbytes;
+    if (adc) {
+        // windows of whole tables within the budget; a table that alone
+        // exceeds it takes what the device allows, one probe per workgroup,
+        // and beyond that the table is built mc subspaces at a time
+        if (fixed + tbytes <= budget) {
+            if (a.mode != kDistIp) a.win = static_cast<int>(std::min<size_t>(kSumsWin, (budget - fixed) / tbytes));
+        } else {
+            if (a.mode != kDistIp) a.win = 1;
+            if (fixed + sbytes > room) return 0;
+            a.mc = static_cast<int>(std::min<size_t>(static_cast<size_t>(ix.m), (room - fixed) / sbytes));
+        }
+        fixed += static_cast<size_t>(a.mode == kDistIp ? 1 : a.win) * a.mc * sbytes;
     }
-    if (fixed > budget) return 0;
+    if (fixed > room) return 0;
     a.nwin = (a.M + a.win - 1) / a.win;
     // float and SQ lists of queries read from global memory need none; 0 is
     // the failure value, so they ask for 16 bytes

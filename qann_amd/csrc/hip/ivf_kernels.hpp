@@ -145,6 +145,7 @@ struct DistArgs {
     uint32_t* bmin;            // fkey of each block's smallest distance
     uint32_t* item_min;        // fkey of each probe's smallest distance
     int resid_in_lds;
+    int mc;                    // ADC: subspaces per table in LDS (m unless a whole table does not fit)
 };
 enum { kDistIp = 0, kDistL2Pre = 1, kDistL2Resid = 2, kDistSq = 3, kDistFloat = 4, kDistSdc = 5 };
 
@@ -263,7 +264,8 @@ void launch_fs_lut_check(const DevIvf& ix, int n, const float* negzero, uint8_t*
                          hipStream_t s);
 void launch_fs_block_check(const uint8_t* luts, int m, int m4, const uint8_t* blocks, int nblocks, uint16_t* out,
                            hipStream_t s);
-// Dynamic LDS of k_ivf_dists (0 when a table does not fit); sets win, nwin and resid_in_lds.
+// Dynamic LDS of k_ivf_dists (0 when not even one subspace's table fits);
+// sets win, nwin, mc and resid_in_lds.
 size_t ivf_dists_lds(DistArgs& a);
 void launch_ivf_dists(DistArgs a, int nq, hipStream_t s);
 // Keys of one query's sort: in LDS, or gkeys (nq * rerank_keys(kc)) in HBM when too many.

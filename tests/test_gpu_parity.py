@@ -714,13 +714,21 @@ PQ8_WIDE = [
     (qann_amd.Metric.InnerProduct, 128, 64, False, False),
     (qann_amd.Metric.Cosine, 192, 96, True, False),
     (qann_amd.Metric.L2, 192, 96, False, True),
+    (qann_amd.Metric.L2, 256, 256, True, False),
+    (qann_amd.Metric.InnerProduct, 256, 256, False, False),
+    (qann_amd.Metric.Cosine, 256, 256, False, True),
 ]
 
 
+@pytest.mark.parametrize("limit", ["device", "24576"])
 @pytest.mark.parametrize("metric,dim,m,pre,sdc", PQ8_WIDE)
-def test_gpu_wide_pq8_tables_match_oracle(tmp_path, oracle, metric, dim, m, pre, sdc):
+def test_gpu_wide_pq8_tables_match_oracle(tmp_path, monkeypatch, oracle, metric, dim, m, pre, sdc, limit):
     # one-byte PQ lists whose per-(query, list) table alone exceeds 64 KB take
-    # gfx950's larger LDS, a few probes per workgroup
+    # gfx950's larger LDS, one probe per workgroup; tables beyond that (256 KB
+    # at PQ256x8), or beyond a lowered limit, are built and summed a run of
+    # subspaces at a time
+    if limit != "device":
+        monkeypatch.setenv("QANN_AMD_LDS_LIMIT", limit)
     rng = np.random.default_rng(dim + m)
     x = mixture(rng, 6000, dim, centers=30)
     q = mixture(rng, 60, dim, centers=30)
