@@ -398,6 +398,23 @@ PYBIND11_MODULE(_qann, m) {
         .value("EMeans", ClusteringKind::EMeans, "e-means: mini-batch k-means with EMA centroids (default)")
         .value("Lloyd", ClusteringKind::Lloyd, "Lloyd's k-means");
 
+    // detail::balanced_pca_rotation on a sample, as IVFIndex.train learns it
+    m.def(
+        "_balanced_pca_rotation",
+        [](FloatArray data, uint64_t num_subspaces) {
+            if (data.ndim() != 2) throw std::invalid_argument("data must be (n, dim)");
+            const uint64_t n = static_cast<uint64_t>(data.shape(0)), d = static_cast<uint64_t>(data.shape(1));
+            std::vector<float> r;
+            {
+                py::gil_scoped_release nogil;
+                r = balanced_pca_rotation(data.data(), n, d, num_subspaces);
+            }
+            py::array_t<float> out({static_cast<py::ssize_t>(d), static_cast<py::ssize_t>(d)});
+            std::copy(r.begin(), r.end(), out.mutable_data());
+            return out;
+        },
+        py::arg("data"), py::arg("num_subspaces"));
+
     // vecengine::PQDistance (include/vecengine/pq.hpp:18)
     py::enum_<PQDistanceKind>(m, "PQDistance", "How IVF queries score PQ codes")
         .value("ADC", PQDistanceKind::ADC, "the query's own (residual) against each code's centroids (default)")
@@ -613,6 +630,15 @@ PYBIND11_MODULE(_qann, m) {
                  return out;
              },
              "The coarse centroids, (nlist, dim)")
+        .def("rotation",
+             [](const IVFIndex& self) -> py::object {
+                 const auto& r = self.data().rotation;
+                 if (!self.pq_rotate() || r.empty()) return py::none();
+                 py::array_t<float> out({static_cast<py::ssize_t>(self.dim()), static_cast<py::ssize_t>(self.dim())});
+                 std::copy(r.begin(), r.end(), out.mutable_data());
+                 return std::move(out);
+             },
+             "The learnt rotation R, (dim, dim), rows rotated as x R; None without one")
         .def_property_readonly("nlist", &IVFIndex::nlist)
         .def_property_readonly("fast_scan", &IVFIndex::fast_scan)
         .def_property_readonly("pq_rotate", &IVFIndex::pq_rotate)

@@ -419,10 +419,14 @@ void IVFIndex::train(const float* vecs, uint64_t n, uint64_t iters, std::optiona
     if (d_->ids.slots > 0) throw std::logic_error("train() cannot be called after add()");
     if (d_->trained) throw std::logic_error("train() has already been called");
     if (n < d_->nlist) throw std::invalid_argument("training set must have at least nlist vectors");
-    if (d_->rotate) throw std::runtime_error("training a rotated index is not supported by qann_amd; train it with qann and load it");
     const uint64_t d = d_->dim;
-    // the clusters are learnt on the unit vectors that adds and queries use (Cosine)
-    const std::vector<float> x = prepare(vecs, n);
+    // the clusters are learnt on the unit vectors that adds and queries use (Cosine),
+    // rotated first by the learnt rotation (ivf_index.cpp:738-745)
+    std::vector<float> x = prepare(vecs, n);
+    if (d_->rotate) {
+        d_->rotation = balanced_pca_rotation(x.data(), n, d, d_->pq.m);
+        x = rotate_rows(d_->rotation, x.data(), n, d);
+    }
     const bool spherical = d_->metric == Cosine;
     Clusters coarse = emeans_coarse ? emeans(x.data(), n, d, d_->nlist, passes, seed, spherical, nearest_gpu)
                                     : lloyd_kmeans(x.data(), n, d, d_->nlist, iters, seed, spherical, nearest_gpu);

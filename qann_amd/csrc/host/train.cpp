@@ -387,4 +387,135 @@ std::vector<float> pq_codebooks(const float* residuals, uint64_t n, uint64_t dim
     return cents;
 }
 
+// ---------------------------------------------------------------------------
+// The PCA-balanced rotation. The reference's build contracts its double
+// arithmetic as follows (read from the x86 code of the reference compiled
+// here): the covariance accumulates fma(x_p, x_q, acc); the Jacobi step takes
+// t from sqrt(fma(theta, theta, 1)) and c from sqrt(fma(t, t, 1)), and updates
+// each pair as (fma(c, u, -(s v)), fma(s, u, c v)); the squared sums of the
+// convergence test round each product of a 4-wide group and add them in
+// order, fusing only the remainder. These functions are compiled with FMA
+// instructions and without any other contraction.
+// ---------------------------------------------------------------------------
+
+#pragma GCC push_options
+#pragma GCC optimize("fp-contract=off")
+
+namespace {
+
+// Sum of squares of v[0..len) onto acc, in the vectorized loop's order.
+__attribute__((target("avx2,fma"))) double sum_squares(const double* v, uint64_t len, double acc) {
+    const uint64_t body = len / 4 * 4;
+    for (uint64_t i = 0; i < body; ++i) {
+        const double sq = v[i] * v[i];
+        acc = acc + sq;
+    }
+    for (uint64_t i = body; i < len; ++i) acc = std::fma(v[i], v[i], acc);
+    return acc;
+}
+
+// symmetric_eigen (src/rotation.cpp:20-69): cyclic Jacobi sweeps over the
+// pairs p < q until the off-diagonal squares fall to 1e-24 of the total.
+__attribute__((target("avx2,fma"))) void jacobi_eigen(std::vector<double>& a, uint64_t d, std::vector<double>& values,
+                                                      std::vector<double>& vecs) {
+    vecs.assign(d * d, 0.0);
+    for (uint64_t i = 0; i < d; ++i) vecs[i * d + i] = 1.0;
+    const double stop = 1e-24 * sum_squares(a.data(), d * d, 0.0);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (uint64_t p = 0; p < d; ++p) off = sum_squares(a.data() + p * d + p + 1, d - p - 1, off);
+        if (off <= stop) break;
+        for (uint64_t p = 0; p < d; ++p)
+            for (uint64_t q = p + 1; q < d; ++q) {
+                const double apq = a[p * d + q];
+                if (apq == 0.0) continue;
+                const double theta = (a[q * d + q] - a[p * d + p]) / (2.0 * apq);
+                double t;
+                if (std::abs(theta) > 1e150) {
+                    t = 1.0 / (2.0 * theta);
+                } else {
+                    const double root = std::sqrt(std::fma(theta, theta, 1.0));
+                    t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + root);
+                }
+                const double c = 1.0 / std::sqrt(std::fma(t, t, 1.0));
+                const double s = t * c;
+                // columns p and q, then rows p and q, then the eigenvector columns
+                auto turn = [&](double& u, double& v) {
+                    const double su = s * v, cv = c * v;
+                    const double nu = std::fma(c, u, -su), nv = std::fma(s, u, cv);
+                    u = nu;
+                    v = nv;
+                };
+                for (uint64_t k = 0; k < d; ++k) turn(a[k * d + p], a[k * d + q]);
+                for (uint64_t k = 0; k < d; ++k) turn(a[p * d + k], a[q * d + k]);
+                for (uint64_t k = 0; k < d; ++k) turn(vecs[k * d + p], vecs[k * d + q]);
+            }
+    }
+    values.resize(d);
+    for (uint64_t i = 0; i < d; ++i) values[i] = a[i * d + i];
+}
+
+// The upper triangle of sum_i (x_i - mean)^T (x_i - mean), row i after row i - 1
+// for every entry, in blocks of 4,096 centred rows.
+__attribute__((target("avx2,fma"))) void covariance_upper(const float* x, uint64_t n, uint64_t d,
+                                                          const std::vector<double>& mean, std::vector<double>& cov) {
+    constexpr uint64_t kRows = 4096;
+    std::vector<double> blk(std::min(n, kRows) * d);
+    for (uint64_t start = 0; start < n; start += kRows) {
+        const uint64_t rows = std::min(kRows, n - start);
+        for (uint64_t i = 0; i < rows; ++i)
+            for (uint64_t j = 0; j < d; ++j) blk[i * d + j] = static_cast<double>(x[(start + i) * d + j]) - mean[j];
+        parallel_for(d, [&](size_t p) {
+            double* row = cov.data() + p * d;
+            for (uint64_t i = 0; i < rows; ++i) {
+                const double* b = blk.data() + i * d;
+                const double bp = b[p];
+                for (uint64_t q = p; q < d; ++q) row[q] = std::fma(bp, b[q], row[q]);
+            }
+        });
+    }
+}
+
+} // namespace
+
+std::vector<float> balanced_pca_rotation(const float* x, uint64_t n, uint64_t d, uint64_t m) {
+    if (m == 0 || d % m != 0) throw std::invalid_argument("dim must be divisible by num_subspaces");
+    if (n == 0) throw std::invalid_argument("training set must not be empty");
+    std::vector<double> mean(d, 0.0);
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint64_t j = 0; j < d; ++j) mean[j] += static_cast<double>(x[i * d + j]);
+    for (double& v : mean) v /= static_cast<double>(n);
+    std::vector<double> cov(d * d, 0.0);
+    covariance_upper(x, n, d, mean, cov);
+    const double scale = n > 1 ? 1.0 / static_cast<double>(n - 1) : 1.0;
+    for (uint64_t p = 0; p < d; ++p)
+        for (uint64_t q = p; q < d; ++q) {
+            cov[p * d + q] *= scale;
+            cov[q * d + p] = cov[p * d + q];
+        }
+    std::vector<double> values, vecs;
+    jacobi_eigen(cov, d, values, vecs);
+
+    // largest eigenvalue first (ties in index order), each to the subspace
+    // with room whose sum of log eigenvalues is smallest (the first such)
+    std::vector<uint64_t> order(d);
+    std::iota(order.begin(), order.end(), uint64_t{0});
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return values[a] > values[b]; });
+    const uint64_t width = d / m;
+    std::vector<uint64_t> filled(m, 0);
+    std::vector<double> logs(m, 0.0);
+    std::vector<float> R(d * d);
+    for (uint64_t e : order) {
+        uint64_t best = m;
+        for (uint64_t b = 0; b < m; ++b)
+            if (filled[b] < width && (best == m || logs[b] < logs[best])) best = b;
+        logs[best] += std::log(std::max(values[e], 1e-12));
+        const uint64_t col = best * width + filled[best]++;
+        for (uint64_t row = 0; row < d; ++row) R[row * d + col] = static_cast<float>(vecs[row * d + e]);
+    }
+    return R;
+}
+
+#pragma GCC pop_options
+
 } // namespace qh

@@ -54,4 +54,12 @@ std::vector<float> pq_codebooks(const float* residuals, uint64_t n, uint64_t dim
 // Squared L2 distance in l2_distance_avx2's summation order (src/distance.cpp:69-101).
 float l2_avx2(const float* a, const float* b, uint64_t n);
 
+// detail::balanced_pca_rotation (src/rotation.cpp:71-130): the sample's
+// covariance in double, its eigenvectors by cyclic Jacobi sweeps
+// (symmetric_eigen, :20-69), and each eigenvector, largest eigenvalue first,
+// placed in the subspace of room whose log-eigenvalue sum is smallest. Returns
+// the d x d row-major R (x R rotates x). With the reference's arithmetic, as
+// its g++ -O3 -mavx2 -mfma build fuses it, so R holds the same floats.
+std::vector<float> balanced_pca_rotation(const float* x, uint64_t n, uint64_t d, uint64_t m);
+
 } // namespace qh

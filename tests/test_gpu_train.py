@@ -26,25 +26,27 @@ def mixture(rng, n, d, centers=40, sigma=0.1):
 
 
 TRAINS = [
-    # name, metric, dim, nlist, storage, m, clustering, iters or passes, k_factor, custom ids
-    ("l2_fs_emeans", qann_amd.Metric.L2, 64, 200, "fs", 16, "emeans", 2, 0, False),
-    ("cos_fs_emeans_refine", qann_amd.Metric.Cosine, 100, 256, "fs", 50, "emeans", 2, 4, False),
-    ("ip_fs_lloyd_custom", qann_amd.Metric.InnerProduct, 48, 150, "fs", 24, "lloyd", 6, 0, True),
-    ("l2_fs_d16_emeans", qann_amd.Metric.L2, 16, 128, "fs", 8, "emeans", 3, 0, False),
-    ("l2_pq8_lloyd", qann_amd.Metric.L2, 32, 128, "pq8", 8, "lloyd", 5, 0, False),
-    ("cos_pq8_emeans_refine", qann_amd.Metric.Cosine, 40, 100, "pq8", 10, "emeans", 2, 3, False),
-    ("l2_sq8_emeans", qann_amd.Metric.L2, 48, 150, "sq8", 0, "emeans", 2, 0, False),
-    ("cos_sq4_lloyd", qann_amd.Metric.Cosine, 24, 100, "sq4", 0, "lloyd", 4, 0, False),
-    ("ip_float_emeans_refine", qann_amd.Metric.InnerProduct, 20, 100, "float", 0, "emeans", 3, 2, False),
+    # name, metric, dim, nlist, storage, m, clustering, iters or passes, k_factor, custom ids, rotate
+    ("l2_fs_emeans", qann_amd.Metric.L2, 64, 200, "fs", 16, "emeans", 2, 0, False, False),
+    ("cos_fs_emeans_refine", qann_amd.Metric.Cosine, 100, 256, "fs", 50, "emeans", 2, 4, False, False),
+    ("ip_fs_lloyd_custom", qann_amd.Metric.InnerProduct, 48, 150, "fs", 24, "lloyd", 6, 0, True, False),
+    ("l2_fs_d16_emeans", qann_amd.Metric.L2, 16, 128, "fs", 8, "emeans", 3, 0, False, False),
+    ("l2_pq8_lloyd", qann_amd.Metric.L2, 32, 128, "pq8", 8, "lloyd", 5, 0, False, False),
+    ("cos_pq8_emeans_refine", qann_amd.Metric.Cosine, 40, 100, "pq8", 10, "emeans", 2, 3, False, False),
+    ("l2_sq8_emeans", qann_amd.Metric.L2, 48, 150, "sq8", 0, "emeans", 2, 0, False, False),
+    ("cos_sq4_lloyd", qann_amd.Metric.Cosine, 24, 100, "sq4", 0, "lloyd", 4, 0, False, False),
+    ("ip_float_emeans_refine", qann_amd.Metric.InnerProduct, 20, 100, "float", 0, "emeans", 3, 2, False, False),
+    ("l2_fs_rot_emeans_refine", qann_amd.Metric.L2, 64, 128, "fs", 16, "emeans", 2, 3, False, True),
+    ("cos_pq8_rot_lloyd", qann_amd.Metric.Cosine, 48, 100, "pq8", 12, "lloyd", 4, 0, False, True),
 ]
 
 
-def build_ours(x, train_n, metric, dim, nlist, storage, m, clustering, iters, kf, ids):
+def build_ours(x, train_n, metric, dim, nlist, storage, m, clustering, iters, kf, ids, rot=False):
     ivf = qann_amd.IVFIndex(dim, nlist, 8, metric=metric, custom_ids=ids is not None and not kf)
     if storage == "fs":
-        ivf.enable_pq(m, 16, fast_scan=True)
+        ivf.enable_pq(m, 16, fast_scan=True, rotate=rot)
     elif storage == "pq8":
-        ivf.enable_pq(m, 256)
+        ivf.enable_pq(m, 256, rotate=rot)
     elif storage.startswith("sq"):
         ivf.enable_sq(int(storage[2:]))
     kind = qann_amd.Clustering.EMeans if clustering == "emeans" else qann_amd.Clustering.Lloyd
@@ -55,9 +57,9 @@ def build_ours(x, train_n, metric, dim, nlist, storage, m, clustering, iters, kf
     return ivf
 
 
-@pytest.mark.parametrize("name,metric,dim,nlist,storage,m,clustering,iters,kf,custom", TRAINS)
+@pytest.mark.parametrize("name,metric,dim,nlist,storage,m,clustering,iters,kf,custom,rot", TRAINS)
 def test_gpu_trained_and_added_index_is_the_reference_bytes(tmp_path, reference, name, metric, dim, nlist, storage, m,
-                                                            clustering, iters, kf, custom):
+                                                            clustering, iters, kf, custom, rot):
     rng = np.random.default_rng(len(name) * 7 + dim)
     n, train_n = 12000, 9000
     x = mixture(rng, n, dim)
@@ -68,7 +70,7 @@ def test_gpu_trained_and_added_index_is_the_reference_bytes(tmp_path, reference,
         more_ids = (np.int64(1) << 37) + np.arange(700, dtype=np.int64) * 3
     # the reference: train on the first train_n rows, add all n, then 700 more
     ref = reference.build_ivf(x, nlist, 8, metric=METRIC_ID[metric], m=m if storage in ("fs", "pq8") else 0,
-                              pq_k=16 if storage == "fs" else 256, fast_scan=storage == "fs",
+                              pq_k=16 if storage == "fs" else 256, fast_scan=storage == "fs", rotate=rot,
                               sq_bits=int(storage[2:]) if storage.startswith("sq") else 0, seed=11, k_factor=kf,
                               ids=ids, clustering=0 if clustering == "emeans" else 1, iters_or_passes=iters,
                               train_n=train_n)
@@ -77,7 +79,7 @@ def test_gpu_trained_and_added_index_is_the_reference_bytes(tmp_path, reference,
     reference.add(ref, more, more_ids)
     reference.save(ref, tmp_path / "ref2.qann")
 
-    ivf = build_ours(x, train_n, metric, dim, nlist, storage, m, clustering, iters, kf, ids)
+    ivf = build_ours(x, train_n, metric, dim, nlist, storage, m, clustering, iters, kf, ids, rot)
     if metric == qann_amd.Metric.Cosine and storage in ("fs", "pq8"):
         loaded = qann_amd.load(str(ref_first))
         ivf.cosine_inner_product = (loaded.base if kf else loaded).cosine_inner_product
