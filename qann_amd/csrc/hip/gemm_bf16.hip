@@ -316,16 +316,17 @@ struct XcdTiles {
     int groups, big_per, small_per;
 };
 
-template <int KS, int TN, int NST, bool A_BIG, class Epi>
-__global__ void __launch_bounds__(256, TN == 4 ? 1 : 2) k_gemm_bf16_lds(const uint16_t* __restrict__ A, int M,
-                                                                        size_t lda, const uint16_t* __restrict__ B,
-                                                                        int N, size_t ldb, int kp, XcdTiles xt, Epi epi) {
-    constexpr int TM = 4, BM = 256, BN = 64 * TN;
+// NW waves per workgroup (4 or 8) in a 2 x NW/2 grid of (32 TM) x (32 TN) wave tiles.
+template <int KS, int TN, int NST, bool A_BIG, class Epi, int NW>
+__global__ void __launch_bounds__(64 * NW, (TN == 4 || NW == 8) ? 1 : 2)
+    k_gemm_bf16_lds(const uint16_t* __restrict__ A, int M, size_t lda, const uint16_t* __restrict__ B, int N,
+                    size_t ldb, int kp, XcdTiles xt, Epi epi) {
+    constexpr int TM = 4, WN = NW / 2, BM = 256, BN = WN * 32 * TN;
     constexpr int RB = 2 * KS;                     // bytes of one row in a stage
     constexpr int CPR = RB / 16;                   // 16-byte chunks per row
     constexpr int RPL = 64 / CPR;                  // rows one wave load fills
-    constexpr int LPWA = BM / RPL / 4;             // loads of A per wave per stage
-    constexpr int LPWB = BN / RPL / 4;             // loads of B per wave per stage
+    constexpr int LPWA = BM / RPL / NW;            // loads of A per wave per stage
+    constexpr int LPWB = BN / RPL / NW;            // loads of B per wave per stage
     constexpr int STAGE = (BM + BN) * RB;
     constexpr int LPS = LPWA + LPWB;               // loads per thread per stage
     constexpr int SWZ = 16 / CPR;                  // rows sharing one swizzle value
@@ -341,29 +342,29 @@ __global__ void __launch_bounds__(256, TN == 4 ? 1 : 2) k_gemm_bf16_lds(const ui
     if (tm >= tiles_m || tn >= tiles_n) return;
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, wave = tid / kWave, lane = lane_id();
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WN, wn = wave % WN;
 
-    // this lane's source rows: load c of the wave fills rows (4 c + wave) RPL .. + RPL - 1
+    // this lane's source rows: load c of the wave fills rows (NW c + wave) RPL .. + RPL - 1
     const uint16_t* ga[LPWA];
     const uint16_t* gb[LPWB];
 #pragma unroll
     for (int c = 0; c < LPWA; ++c) {
-        const int r = (4 * c + wave) * RPL + lane / CPR;
+        const int r = (NW * c + wave) * RPL + lane / CPR;
         ga[c] = A + static_cast<size_t>(min(m0 + r, M - 1)) * lda + 8 * ((lane % CPR) ^ ((r / SWZ) % CPR));
     }
 #pragma unroll
     for (int c = 0; c < LPWB; ++c) {
-        const int r = (4 * c + wave) * RPL + lane / CPR;
+        const int r = (NW * c + wave) * RPL + lane / CPR;
         gb[c] = B + static_cast<size_t>(min(n0 + r, N - 1)) * ldb + 8 * ((lane % CPR) ^ ((r / SWZ) % CPR));
     }
     auto issue = [&](int buf, int k0) {
         lds_u8* st = sbase + buf * STAGE;
 #pragma unroll
         for (int c = 0; c < LPWA; ++c)
-            lds_load16(ga[c] + k0, st + (4 * c + wave) * RPL * RB);
+            lds_load16(ga[c] + k0, st + (NW * c + wave) * RPL * RB);
 #pragma unroll
         for (int c = 0; c < LPWB; ++c)
-            lds_load16(gb[c] + k0, st + BM * RB + (4 * c + wave) * RPL * RB);
+            lds_load16(gb[c] + k0, st + BM * RB + (NW * c + wave) * RPL * RB);
     };
 
     f32x16 acc[TM][TN];
@@ -661,11 +662,11 @@ void gemm(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t
                        A, M, lda, B, N, ldb, kp, per_xcd, epi);
 }
 
-template <int KS, int TN, int NST, bool A_BIG, class Epi>
+template <int KS, int TN, int NST, bool A_BIG, int NW = 4, class Epi>
 void gemm_lds(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp, const Epi& epi,
               hipStream_t s) {
     if (M <= 0 || N <= 0) return;
-    const int BN = 64 * TN;
+    const int BN = NW / 2 * 32 * TN;
     const int big_tiles = static_cast<int>(A_BIG ? cdiv(M, 256) : cdiv(N, BN));
     const int small_tiles = static_cast<int>(A_BIG ? cdiv(N, BN) : cdiv(M, 256));
     // groups of small tiles of at most about 1.5 MB (an XCD's L2 holds 4 MB)
@@ -677,8 +678,8 @@ void gemm_lds(const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, si
     const int parts = 8 / xt.groups;
     xt.big_per = (big_tiles + parts - 1) / parts;
     const size_t per_xcd = static_cast<size_t>(xt.big_per) * xt.small_per;
-    hipLaunchKernelGGL((k_gemm_bf16_lds<KS, TN, NST, A_BIG, Epi>), dim3(static_cast<unsigned>(8 * per_xcd)), dim3(256),
-                       0, s, A, M, lda, B, N, ldb, kp, xt, epi);
+    hipLaunchKernelGGL((k_gemm_bf16_lds<KS, TN, NST, A_BIG, Epi, NW>), dim3(static_cast<unsigned>(8 * per_xcd)),
+                       dim3(64 * NW), 0, s, A, M, lda, B, N, ldb, kp, xt, epi);
 }
 
 __global__ void k_zero_cnt(uint32_t* p, int n) {
@@ -688,8 +689,9 @@ __global__ void k_zero_cnt(uint32_t* p, int n) {
 
 // The tile shapes the launches choose from: wave tiles of (32 TM) x (32 TN)
 // with (TM, TN) = (2, 2), (2, 4), (4, 2), and (4, 4) fed by direct loads
-// through the LDS ring (k_gemm_bf16_lds) in stages of K 32 or 64.
-constexpr int kVariants = 6;
+// through the LDS ring (k_gemm_bf16_lds) in stages of K 32 or 64, with four
+// waves per workgroup or (256 x 256 tiles of 128 x 64 wave tiles) eight.
+constexpr int kVariants = 7;
 
 template <bool A_BIG, class Epi>
 void gemm_variant(int v, const uint16_t* A, int M, size_t lda, const uint16_t* B, int N, size_t ldb, int kp,
@@ -706,6 +708,7 @@ void gemm_variant(int v, const uint16_t* A, int M, size_t lda, const uint16_t* B
             gemm_lds<32, 4, 4, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s);
         break;
     case 5: gemm_lds<32, 2, 3, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
+    case 6: gemm_lds<32, 2, 4, A_BIG, 8>(A, M, lda, B, N, ldb, kp, epi, s); break;
     default: gemm<2, 2, A_BIG>(A, M, lda, B, N, ldb, kp, epi, s); break;
     }
 }
@@ -734,7 +737,7 @@ void gemm_tuned(int kind, const uint16_t* A, int M, size_t lda, const uint16_t* 
                 const Epi& epi, Reset reset, hipStream_t s) {
     const TuneKey key{kind, kp, size_class(M), size_class(N)};
     int best = -1;
-    // QANN_AMD_GEMM_SHAPE=0..5 forces one tile shape (tests check each)
+    // QANN_AMD_GEMM_SHAPE=0..6 forces one tile shape (tests check each)
     const char* force = std::getenv("QANN_AMD_GEMM_SHAPE");
     if (force && *force >= '0' && *force < '0' + kVariants && force[1] == '\0') {
         gemm_variant<A_BIG>(*force - '0', A, M, lda, B, N, ldb, kp, epi, s);

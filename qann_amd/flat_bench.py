@@ -52,10 +52,32 @@ def main():
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--oracle-queries", type=int, default=256)
+    ap.add_argument("--library-gemm-rows", type=int, default=0,
+                    help="only time the vendor BF16 GEMM (torch.mm, hipBLASLt) on this many rows x nq x dim")
     args = ap.parse_args()
 
     import torch
     import qann_amd
+
+    if args.library_gemm_rows:
+        # the same GEMM shape through the library: BF16 operands, BF16 and FP32 outputs
+        a = torch.randn(args.library_gemm_rows, args.dim, device="cuda", dtype=torch.bfloat16)
+        b = torch.randn(args.dim, args.nq, device="cuda", dtype=torch.bfloat16)
+        out = {}
+        for name, fn in (("bf16_out", lambda: torch.mm(a, b)), ("fp32_out", lambda: torch.mm(a, b, out_dtype=torch.float32))):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(10):
+                fn()
+            torch.cuda.synchronize()
+            s = (time.perf_counter() - t) / 10
+            out[name] = {"ms": round(1e3 * s, 3), "tflops": round(2 * args.library_gemm_rows * args.nq * args.dim / s / 1e12, 1)}
+            log(f"torch.mm {args.library_gemm_rows} x {args.nq} x {args.dim} {name}: {1e3 * s:.3f} ms, "
+                f"{out[name]['tflops']} TFLOP/s")
+        print(json.dumps({"library_gemm": out}))
+        return
 
     rng = np.random.Generator(np.random.PCG64(3))
     lam_sqrt = np.sqrt(np.arange(1, args.dim + 1, dtype=np.float64) ** -0.8).astype(np.float32)

@@ -434,7 +434,7 @@ def test_gpu_flat_bf16_matches_oracle(tmp_path, oracle):
         _profiling(False)
 
 
-@pytest.mark.parametrize("shape", ["0", "1", "2", "3", "4", "5"])
+@pytest.mark.parametrize("shape", ["0", "1", "2", "3", "4", "5", "6"])
 def test_gpu_bf16_gemm_shapes_match_oracle(tmp_path, oracle, monkeypatch, shape):
     # each tile shape of the BF16 GEMM forced (3 and 4: the 256 x 256 tiles fed
     # by direct loads through the LDS ring, stages of K 32 and 64; K = 96 of
