@@ -175,7 +175,15 @@ int qann_hip_search_requests(qann_hip_index* index, const float* queries, const 
  * qann_hip_search with queries, results and allowed words in device memory,
  * enqueued on stream (a hipStream_t; NULL = the default stream) and returning
  * before the work finishes. Calls on one stream run in order; calls on
- * different streams may overlap on the device.
+ * different streams may overlap on the device. Each stream has its own
+ * scratch, held by one call at a time (host threads sharing a stream take
+ * turns). The host waits on the device only where it must read a result to
+ * go on: a FlatIndex search on its BF16 path (it reads which queries
+ * overflowed the candidate cap and re-scores them on the FP32 path), the first
+ * search of a problem class (the autotuners time their forms with HIP
+ * events), and a call that grows its stream's scratch. A filtered IVF search
+ * here probes every list its stopping rule may need up front, so it does not
+ * wait either.
  */
 int qann_hip_search_async(qann_hip_index* index, const float* d_queries, uint64_t nq,
                           const qann_hip_search_params* params, int64_t* d_out_ids, float* d_out_dists, void* stream);
