@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(256) k_coarse_gemm32(const float* q, int nq, c
 // tau comes first from a sample of at most kSelectSample scores of the row.
 constexpr int kSelectSample = 4096;
 template <int NT, int CD, bool SAMPLE>
-__global__ void __launch_bounds__(NT) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE ? 4 : 5))) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
                                                        int dim, int ip, float cnorm_max, int M, int cap,
                                                        uint16_t* out_idx, float* out_val, int split_kp,
                                                        uint64_t* gkeys) {
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, c
 // A per probe: with residual centres the list's own exact term
 // (scan, src/ivf_index.cpp:526-529), else the coarse distance. For the
 // inner-product tables, base = A, or for Cosine (A + 1) - |c|^2 (:567-570).
-__global__ void k_probe_base(const float* q, const DevIvf ix, int nq, int M, const uint16_t* probe_idx,
+__global__ void __attribute__((amdgpu_waves_per_eu(6))) k_probe_base(const float* q, const DevIvf ix, int nq, int M, const uint16_t* probe_idx,
                              const float* probe_val, float* A_out, float* base_out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nq * M) return;
@@ -1251,17 +1251,32 @@ __global__ void __launch_bounds__(1024) k_pairs_hist(const SumsArgs a, int nq, u
 
 // From the per-workgroup counts ghist[g][list]: per list, each group's first
 // position within the list's range (ghist, in place) and the list's total (tot).
-__global__ void k_pairs_lists(uint32_t* ghist, int groups, int n, uint32_t* tot) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per list, lane j holding groups kPerLane j .. kPerLane j + kPerLane - 1.
+constexpr int kPerLane = kPairGroups / kWave;
+static_assert(kPairGroups % kWave == 0, "whole groups per lane");
+__global__ void __launch_bounds__(256) k_pairs_lists(uint32_t* ghist, int groups, int n, uint32_t* tot) {
+    const int l = blockIdx.x * (256 / kWave) + static_cast<int>(threadIdx.x) / kWave, lane = lane_id();
     if (l >= n) return;
-    uint32_t run = 0;
-    for (int g = 0; g < groups; ++g) {
-        const size_t x = static_cast<size_t>(g) * n + l;
-        const uint32_t c = ghist[x];
-        ghist[x] = run;
-        run += c;
+    uint32_t c[kPerLane], own = 0;
+#pragma unroll
+    for (int i = 0; i < kPerLane; ++i) {
+        const int g = lane * kPerLane + i;
+        c[i] = g < groups ? ghist[static_cast<size_t>(g) * n + l] : 0u;
+        own += c[i];
     }
-    tot[l] = run;
+    uint32_t run = own; // inclusive prefix of the lanes' sums
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(run), off, kWave));
+        if (lane >= off) run += o;
+    }
+    uint32_t at = run - own;
+#pragma unroll
+    for (int i = 0; i < kPerLane; ++i) {
+        const int g = lane * kPerLane + i;
+        if (g < groups) ghist[static_cast<size_t>(g) * n + l] = at;
+        at += c[i];
+    }
+    if (lane == kWave - 1) tot[l] = run;
 }
 
 // pair_off / unit_off: exclusive prefix sums over the lists of the pair totals
@@ -1299,11 +1314,13 @@ __global__ void __launch_bounds__(1024) k_pairs_offsets(const uint32_t* tot, int
     }
 }
 
-// Each unit's list.
-__global__ void k_pairs_units(const uint32_t* unit_off, int n, uint32_t* unit_list) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+// Each unit's list: one wave per list, its lanes over the list's units.
+__global__ void __launch_bounds__(256) k_pairs_units(const uint32_t* unit_off, int n, uint32_t* unit_list) {
+    const int l = blockIdx.x * (256 / kWave) + static_cast<int>(threadIdx.x) / kWave;
     if (l >= n) return;
-    for (uint32_t k = unit_off[l]; k < unit_off[l + 1]; ++k) unit_list[k] = static_cast<uint32_t>(l);
+    const uint32_t end = unit_off[l + 1];
+    for (uint32_t k = unit_off[l] + static_cast<uint32_t>(lane_id()); k < end; k += kWave)
+        unit_list[k] = static_cast<uint32_t>(l);
 }
 
 // Each pair's item index (i * M + p) into its list's range, the workgroup's
@@ -3317,16 +3334,19 @@ void launch_mfma(const SumsArgs& a, int nq, hipStream_t s) {
         const int groups = static_cast<int>(std::min<uint64_t>(kPairGroups, static_cast<uint64_t>(nq)));
         const size_t hl = static_cast<size_t>(nlist) * 4;
         hipLaunchKernelGGL(k_pairs_hist, dim3(groups), dim3(1024), hl, s, a, nq, pp.ghist);
-        hipLaunchKernelGGL(k_pairs_lists, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.ghist, groups, nlist, pp.tot);
+        hipLaunchKernelGGL(k_pairs_lists, dim3(cdiv(nlist, 256 / kWave)), dim3(256), 0, s, pp.ghist, groups, nlist,
+                           pp.tot);
         hipLaunchKernelGGL(k_pairs_offsets, dim3(1), dim3(1024), 0, s, pp.tot, nlist, pp.pair_off, pp.unit_off);
-        hipLaunchKernelGGL(k_pairs_units, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.unit_off, nlist, pp.unit_list);
+        hipLaunchKernelGGL(k_pairs_units, dim3(cdiv(nlist, 256 / kWave)), dim3(256), 0, s, pp.unit_off, nlist,
+                           pp.unit_list);
         hipLaunchKernelGGL(k_pairs_place, dim3(groups), dim3(1024), hl, s, a, nq, pp.ghist, pp.pair_off, pp.items);
     } else {
         hipLaunchKernelGGL(k_order_zero, dim3(cdiv(nlist + 1, 256)), dim3(256), 0, s, pp.tot, nlist + 1);
         hipLaunchKernelGGL(k_pairs_count, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, a, nq,
                            pp.tot);
         hipLaunchKernelGGL(k_pairs_offsets, dim3(1), dim3(1024), 0, s, pp.tot, nlist, pp.pair_off, pp.unit_off);
-        hipLaunchKernelGGL(k_pairs_units, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.unit_off, nlist, pp.unit_list);
+        hipLaunchKernelGGL(k_pairs_units, dim3(cdiv(nlist, 256 / kWave)), dim3(256), 0, s, pp.unit_off, nlist,
+                           pp.unit_list);
         // the scatter's cursors start at each list's offset
         hipLaunchKernelGGL(k_pairs_cursors, dim3(cdiv(nlist, 256)), dim3(256), 0, s, pp.pair_off, nlist, pp.ghist);
         hipLaunchKernelGGL(k_pairs_scatter, dim3(static_cast<unsigned>((items + 255) / 256)), dim3(256), 0, s, a, nq,
@@ -3584,7 +3604,10 @@ void launch_rerank(const float* q, int nq, int dim, int ip, const float* rows, c
                    uint64_t* gkeys, hipStream_t s, const PerQuery& pq) {
     const size_t np = rerank_keys(kc);
     const size_t lds = query_lds_bytes(dim) + (gkeys ? 0 : np * 8);
-    hipLaunchKernelGGL(k_rerank, dim3(nq), dim3(256), lds, s, q, dim, ip, rows, cand, cand_count, kc, k, labels,
+    // one thread per candidate up to 256 (k' = 100: two waves, not four with
+    // more than half of them idle)
+    const unsigned threads = np <= 64 ? 64u : np <= 128 ? 128u : 256u;
+    hipLaunchKernelGGL(k_rerank, dim3(nq), dim3(threads), lds, s, q, dim, ip, rows, cand, cand_count, kc, k, labels,
                        out_ids, out_dists, gkeys, static_cast<int>(np), pq);
 }
 
