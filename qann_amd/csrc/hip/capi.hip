@@ -1097,10 +1097,12 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         a.pairs = pairs;
         if (q0 == 0 && scan_wait) check(hipStreamWaitEvent(st, scan_wait, 0), "hipStreamWaitEvent");
         if (order && nc >= kOrderMinQueries) {
-            qd::launch_query_order(a.probe_idx, static_cast<int>(M), static_cast<int>(nc), static_cast<int>(nlist),
-                                   ord_hist, order, st);
-            check_launch("k_order");
             a.order = order;
+            if (qd::fs_sums_reads_order(a, static_cast<int>(nc))) {
+                qd::launch_query_order(a.probe_idx, static_cast<int>(M), static_cast<int>(nc), static_cast<int>(nlist),
+                                       ord_hist, order, st);
+                check_launch("k_order");
+            }
         }
         StageTimer t_scan(QANN_STAGE_SCAN, st);
         a.grouped = t_scan.mark();

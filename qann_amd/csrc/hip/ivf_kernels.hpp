@@ -241,6 +241,8 @@ size_t fs_sums_lds(SumsArgs& a);
 // Returns the form launched: 0 one workgroup per window, 1 persistent
 // queue, 2 matrix cores (which records a.grouped, when set, after its pair
 // grouping).
+// Whether the sums form launch_fs_sums will run reads a.order (the query order).
+bool fs_sums_reads_order(SumsArgs a, int nq);
 int launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
 // Whether the matrix-core form can run these sums: inner-product LUTs or
 // precomputed L2 tables, m4 <= 64, the pair buffers given.
