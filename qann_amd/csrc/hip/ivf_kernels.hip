@@ -15,10 +15,12 @@
 //                     every 4-bit code of those lists (fast_scan_block, :256-262) with
 //                     v_perm lookups, plus block minima (k_fs_sums_queue: the same work
 //                     pulled from per-XCD queues)
-//   k_fs_replay_win   one wave per query: the reference's scan replayed push by push over
+//   k_fs_replay_gather one wave per query: the reference's scan replayed push by push over
 //                     those sums -- limit16, survivors in bit order, TopK trims
-//                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543);
-//                     k_fs_replay the same without LDS windows (large TopK buffers)
+//                     (src/topk.hpp:26-138, scan_fast / push, src/ivf_index.cpp:466-543),
+//                     candidate blocks of several probes loaded as one batch;
+//                     k_fs_replay_win a probe's blocks at a time; k_fs_replay the same
+//                     without LDS windows (large TopK buffers)
 //   k_ivf_dists,      PQ (ADC) / SQ / float lists: float distances, then the same replay
 //   k_fl_replay_win     over them (:552-611)
 //   k_rerank          exact FP32 re-rank of the candidates (RefineIndex::rerank, src/refine_index.cpp:112-140)
@@ -2881,7 +2883,267 @@ __device__ void fs_replay_win(const ReplayArgs& a) {
 
 template <bool STAGE>
 __global__ void __launch_bounds__(64) k_fs_replay(const ReplayArgs a) { fs_replay<false, STAGE>(a); }
+// fs_replay_win with the candidate blocks of consecutive probes gathered into
+// one batch of up to 2 kPfRows blocks before their sums are loaded: a probe
+// no longer costs a round trip to memory of its own. Each gathered block keeps
+// its probe's scale, offset and entry range, and the batch is pushed block by
+// block in probe order, so the pushes and the TopK are those of the probe-by-
+// probe replay. A block gathered under a limit that a later push lowers is
+// still visited; its entries fail the new limit, as they would have at its turn.
+// Rows of two candidate blocks a gathered batch holds.
+constexpr int kGatherRows = 8;
+
+template <bool FL>
+__device__ void fs_replay_gather(const ReplayArgs& a) {
+    using BT = typename std::conditional<FL, uint32_t, uint16_t>::type;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const DevIvf& ix = a.ix;
+    const int qi = blockIdx.x;
+    const int kl = a.kp;                         // LDS layout: the launch's largest capacity
+    const int kp = a.pq.kp ? a.pq.kp[qi] : kl;   // this query's TopK capacity
+    const uint64_t* filter = a.pq.filter ? a.pq.filter[qi] : a.filter;
+    const int block_filter = a.pq.block_filter ? a.pq.block_filter[qi] : a.block_filter;
+    int64_t* tk_label = reinterpret_cast<int64_t*>(lds);
+    float* tk_dist = reinterpret_cast<float*>(tk_label + 2 * kl);
+    BT* sb = reinterpret_cast<BT*>(tk_dist + 2 * kl);
+    const int lane = lane_id();
+    const int v = lane & 31, h = lane >> 5;
+    const bool filtered = filter != nullptr;
+    const bool need_slot = !FL && (filtered || ix.deleted != nullptr);
+    constexpr int kB = 2 * kGatherRows; // blocks a batch holds
+    __shared__ uint32_t trim_hist[256];
+    __shared__ uint32_t g_vbase[kB];  // the block's first sum / distance, within the query's region
+    __shared__ uint32_t g_ebase[kB];  // its first list entry
+    __shared__ int g_cnt[kB];         // its entries (32, fewer in a list's last block)
+    __shared__ float g_scale[kB], g_off[kB];
+    __shared__ uint32_t st_val[kGatherRows * kWave];
+    WaveTopK tk{tk_dist, tk_label, 0, kp, INFINITY, trim_hist};
+    const int P = a.nprobe_q[qi];
+    const size_t qoff = static_cast<size_t>(qi) * a.stride_slots;
+    const size_t qboff = static_cast<size_t>(qi) * (a.stride_slots / kFsBlock);
+    const BT* gbmin;
+    if constexpr (FL) gbmin = a.fbmin + qboff;
+    else gbmin = a.bmin + qboff;
+    auto limit_of = [&](float scale, float offset) -> uint32_t {
+        if (FL) return fkey(tk.thr);
+        const float limit = (tk.thr - offset) * scale;
+        return limit >= 65535.0f ? 65535u : limit <= 0.0f ? 0u : static_cast<uint32_t>(ceilf(limit));
+    };
+    // the batch's blocks pushed in order: rows of two blocks (lanes 0-31, 32-63)
+    int nbatch = 0;
+    auto flush = [&]() {
+        const int rows = (nbatch + 1) / 2;
+        const int bi0 = h; // this lane's block in row r is 2 r + h
+        auto bl = [&](int r) { return 2 * r + bi0; };
+        uint32_t sv[kGatherRows];
+#pragma unroll
+        for (int r = 0; r < kGatherRows; ++r) {
+            sv[r] = FL ? __float_as_uint(INFINITY) : 0xFFFFu;
+            if (r < rows && bl(r) < nbatch && v < g_cnt[bl(r)]) {
+                const size_t e = qoff + g_vbase[bl(r)] + static_cast<uint32_t>(v);
+                sv[r] = FL ? __float_as_uint(a.fdists[e]) : static_cast<uint32_t>(a.sums[e]);
+            }
+        }
+        if (need_slot) {
+            // bit 16 set: the entry may not be returned (filtered out or deleted);
+            // read for the entries passing their limit now (later limits only fall)
+            uint32_t sl[kGatherRows];
+#pragma unroll
+            for (int r = 0; r < kGatherRows; ++r) {
+                sl[r] = 0u;
+                if (r < rows && bl(r) < nbatch && v < g_cnt[bl(r)] && sv[r] < limit_of(g_scale[bl(r)], g_off[bl(r)]))
+                    sl[r] = ix.list_ids[g_ebase[bl(r)] + static_cast<uint32_t>(v)];
+            }
+#pragma unroll
+            for (int r = 0; r < kGatherRows; ++r) {
+                if (r < rows && bl(r) < nbatch && v < g_cnt[bl(r)] && sv[r] < limit_of(g_scale[bl(r)], g_off[bl(r)])) {
+                    const bool okb = filtered ? bit_test(filter, sl[r]) : !bit_test(ix.deleted, sl[r]);
+                    if (!okb) sv[r] |= 0x10000u;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kGatherRows; ++r)
+            if (r < rows) st_val[r * kWave + lane] = sv[r];
+        wave_sync();
+        // row r's entry in this lane: in its block and list, may be returned,
+        // its sum or distance, its block's scale and offset
+        auto row_entry = [&](int r, bool& in, bool& okb, uint32_t& sum, float& fd, float& scale, float& offset,
+                             int64_t& label) {
+            const int b = bl(r);
+            in = b < nbatch && v < g_cnt[b];
+            const uint32_t x = st_val[r * kWave + lane];
+            okb = FL || !(x & 0x10000u);
+            sum = 0xFFFFu;
+            fd = INFINITY;
+            scale = 1.0f;
+            offset = 0.0f;
+            label = 0;
+            if (in) {
+                if (FL) fd = __uint_as_float(x);
+                else sum = x & 0xFFFFu;
+                scale = g_scale[b];
+                offset = g_off[b];
+                label = static_cast<int64_t>(g_ebase[b] + static_cast<uint32_t>(v));
+            }
+        };
+        // rows whose pushes cannot fill the buffer leave the threshold (so every
+        // limit) as it is: their survivors are written at once, in push order;
+        // the row that would fill it replays push by push, trims included
+        int r = 0;
+        while (r < rows) {
+            const float thr = tk.thr;
+            for (; r < rows; ++r) {
+                bool in, okb;
+                uint32_t sum;
+                float fd, scale, offset;
+                int64_t label;
+                row_entry(r, in, okb, sum, fd, scale, offset, label);
+                const float dist = FL ? fd : __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                const bool surv = in && okb && (FL ? fd < thr : sum < limit_of(scale, offset)) && dist < thr;
+                const uint64_t m = ballot(surv);
+                const int c = __popcll(m);
+                if (tk.size + c >= 2 * kp) break;
+                if (surv) {
+                    const int pos = tk.size + static_cast<int>(lanes_below(m));
+                    tk.dist[pos] = dist;
+                    tk.label[pos] = label;
+                }
+                tk.size += c;
+            }
+            wave_sync();
+            if (r >= rows) break;
+            {
+                bool in, okb;
+                uint32_t sum;
+                float fd, scale, offset;
+                int64_t label;
+                row_entry(r, in, okb, sum, fd, scale, offset, label);
+                for (int hh = 0; hh < 2; ++hh) {
+                    if (2 * r + hh >= nbatch) break;
+                    const bool valid = h == hh && in;
+                    bool allowed = valid;
+                    if (!FL && block_filter) {
+                        allowed = valid && okb;
+                        if (ballot(allowed) == 0) continue;
+                    }
+                    const bool surv = allowed && (FL ? fd < tk.thr : sum < limit_of(scale, offset));
+                    const uint64_t rem = ballot(surv);
+                    if (rem == 0) continue;
+                    float dist;
+                    bool ok;
+                    if (FL) {
+                        dist = fd; // +inf already marks what may not be returned
+                        ok = surv;
+                    } else {
+                        dist = __fadd_rn(__fdiv_rn(static_cast<float>(sum), scale), offset);
+                        ok = surv && okb;
+                    }
+                    push_survivors(tk, rem, ok, dist, label);
+                }
+            }
+            ++r;
+        }
+        wave_sync(); // the next batch overwrites the g_* arrays and st_val
+        nbatch = 0;
+    };
+
+    uint32_t win0 = 0, win1 = 0; // staged blocks [win0, win1) of the query
+    for (int c0 = 0; c0 < P; c0 += kWave) {
+        int m_list = 0;
+        uint32_t m_n = 0, m_blk = 0, m_min = 0xFFFFFFFFu, m_first = 0;
+        float m_scale = 1.0f, m_off = 0.0f;
+        if (c0 + lane < P) {
+            const size_t item = static_cast<size_t>(qi) * a.M + c0 + lane;
+            m_list = a.probe_idx[item];
+            m_first = ix.list_off[m_list];
+            m_n = ix.list_off[m_list + 1] - m_first;
+            m_blk = a.item_blk[item];
+            m_min = a.item_min[item];
+            if (!FL) {
+                m_scale = a.item_scale[item];
+                m_off = a.item_offset[item];
+            }
+        }
+        const int cn = min(kWave, P - c0);
+        const uint32_t cend = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_blk), cn - 1)) +
+                              (static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_n), cn - 1)) +
+                               kFsBlock - 1) / kFsBlock;
+        // the probes whose list minimum passes their limit now, one per lane
+        auto live_now = [&]() -> uint64_t {
+            return ballot(c0 + lane < P && m_n != 0 && m_min < limit_of(m_scale, m_off));
+        };
+        uint64_t todo = live_now();
+        while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const uint32_t n = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_n), j));
+            const uint32_t nb = (n + kFsBlock - 1) / kFsBlock;
+            const float scale = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_scale), j));
+            const float offset = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m_off), j));
+            const uint32_t first = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_first), j));
+            const uint32_t blk0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(m_blk), j));
+            // the window: restaged from this probe on when it does not hold the
+            // probe's blocks (the gathered blocks keep their sums' places, not
+            // their minima, so restaging leaves the batch as it is)
+            if ((blk0 < win0 || blk0 + nb > win1) && nb <= static_cast<uint32_t>(a.window)) {
+                win0 = blk0;
+                win1 = min(blk0 + static_cast<uint32_t>(a.window), cend);
+                wave_sync();
+                for (uint32_t i = static_cast<uint32_t>(lane); i < win1 - win0; i += kWave) sb[i] = gbmin[win0 + i];
+                wave_sync();
+            }
+            const bool staged = blk0 >= win0 && blk0 + nb <= win1;
+            for (uint32_t bc = 0; bc < nb; bc += kWave) {
+                const uint32_t b = bc + static_cast<uint32_t>(lane);
+                uint32_t bm = 0xFFFFFFFFu;
+                if (b < nb) bm = staged ? static_cast<uint32_t>(sb[blk0 - win0 + b]) : static_cast<uint32_t>(gbmin[blk0 + b]);
+                uint64_t cand = ballot(bm < limit_of(scale, offset));
+                while (cand) {
+                    const int take = min(__popcll(cand), kB - nbatch);
+                    const int rank = static_cast<int>(lanes_below(cand));
+                    if (((cand >> lane) & 1ull) && rank < take) {
+                        const int gi = nbatch + rank;
+                        g_vbase[gi] = (blk0 + b) * kFsBlock;
+                        g_ebase[gi] = first + b * kFsBlock;
+                        g_cnt[gi] = static_cast<int>(min(n - b * kFsBlock, static_cast<uint32_t>(kFsBlock)));
+                        g_scale[gi] = scale;
+                        g_off[gi] = offset;
+                    }
+                    nbatch += take;
+                    // drop the blocks just gathered
+                    if (take == __popcll(cand)) {
+                        cand = 0;
+                    } else {
+                        const uint64_t taken = ballot(((cand >> lane) & 1ull) && rank < take);
+                        cand &= ~taken;
+                    }
+                    if (nbatch == kB) {
+                        wave_sync();
+                        flush();
+                        // a lower threshold drops blocks, and probes, that no longer pass
+                        cand &= ballot(bm < limit_of(scale, offset));
+                        todo &= live_now();
+                    }
+                }
+            }
+        }
+    }
+    // the last batch (a batch may span 64-probe chunks: its blocks carry their own places)
+    if (nbatch) {
+        wave_sync();
+        flush();
+    }
+    // entries become slots
+    if (tk.size > kp) wave_trim(tk);
+    for (int i = lane; i < tk.size; i += kWave) tk.label[i] = ix.list_ids[static_cast<uint32_t>(tk.label[i])];
+    wave_sync();
+    replay_finish(tk, a, qi);
+}
+
 __global__ void __launch_bounds__(64) k_fs_replay_win(const ReplayArgs a) { fs_replay_win<false>(a); }
+__global__ void __launch_bounds__(64) k_fs_replay_gather(const ReplayArgs a) { fs_replay_gather<false>(a); }
+__global__ void __launch_bounds__(64) k_fl_replay_gather(const ReplayArgs a) { fs_replay_gather<true>(a); }
 __global__ void __launch_bounds__(64) k_fl_replay_win(const ReplayArgs a) { fs_replay_win<true>(a); }
 template <bool STAGE>
 __global__ void __launch_bounds__(64) k_fl_replay(const ReplayArgs a) { fs_replay<true, STAGE>(a); }
@@ -3583,6 +3845,7 @@ void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
         if (std::strcmp(e, "plain") == 0) forced = 0;
         else if (std::strcmp(e, "stage") == 0) forced = 1;
         else if (std::strcmp(e, "win") == 0) forced = 2;
+        else if (std::strcmp(e, "gather") == 0) forced = 3;
     }
     // the window form, whenever its TopK and window fit LDS (its fixed arrays
     // take about 3.3 KB more)
@@ -3594,9 +3857,15 @@ void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
         if (v >= 64 && v <= 8192) aw.window = v;
     }
     const size_t wl = lds + static_cast<size_t>(aw.window) * (a.fdists ? 4 : 2);
-    if ((forced < 0 || forced == 2) && !a.g_dist && wl + 4096 <= kLdsMax) {
-        if (a.fdists) hipLaunchKernelGGL(k_fl_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
-        else hipLaunchKernelGGL(k_fs_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
+    if ((forced < 0 || forced >= 2) && !a.g_dist && wl + 4096 <= kLdsMax) {
+        // the gathered form by default; QANN_AMD_REPLAY=win keeps a load per probe
+        if (forced == 2) {
+            if (a.fdists) hipLaunchKernelGGL(k_fl_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
+            else hipLaunchKernelGGL(k_fs_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
+        } else {
+            if (a.fdists) hipLaunchKernelGGL(k_fl_replay_gather, dim3(nq), dim3(kWave), wl, s, aw);
+            else hipLaunchKernelGGL(k_fs_replay_gather, dim3(nq), dim3(kWave), wl, s, aw);
+        }
         return;
     }
     // staged loads pay off while the batch leaves most of the chip's wave slots idle

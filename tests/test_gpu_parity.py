@@ -102,17 +102,18 @@ def test_gpu_pipelined_batch_matches_goldens(monkeypatch, name):
             assert_same((ids.cpu().numpy(), d.cpu().numpy()), want, f"{name} {stem} device rep {rep}")
 
 
-@pytest.mark.parametrize("form", ["plain", "stage", "win", "win64"])
+@pytest.mark.parametrize("form", ["plain", "stage", "win", "win64", "gather", "gather64"])
 @pytest.mark.parametrize("name", [n for n in SYNTH_CASES if not n.startswith("flat")])
 def test_gpu_replay_forms_match_goldens(monkeypatch, name, form):
     # the TopK replay with its loads one block pair at a time, staged through
     # LDS, and with windows of block minima in LDS, grouped candidate loads and
-    # bulk pushes (entries in the TopK, slots looked up at the end), each
-    # forced, on every golden index: filtered, deleted, custom ids, PQ / SQ /
-    # float lists; small and large batches
-    # (win64: windows of 64 block minima, restaged many times a query)
-    monkeypatch.setenv("QANN_AMD_REPLAY", "win" if form == "win64" else form)
-    if form == "win64":
+    # bulk pushes (entries in the TopK, slots looked up at the end) -- a probe's
+    # blocks at a time (win) or several probes' gathered into one batch
+    # (gather) -- each forced, on every golden index: filtered, deleted,
+    # custom ids, PQ / SQ / float lists; small and large batches
+    # (win64 / gather64: windows of 64 block minima, restaged many times a query)
+    monkeypatch.setenv("QANN_AMD_REPLAY", form[:-2] if form.endswith("64") else form)
+    if form.endswith("64"):
         monkeypatch.setenv("QANN_AMD_REPLAY_WINDOW", "64")
     path, q, exp, params = synth_case(name)
     idx = qann_amd.load(str(path))
