@@ -860,6 +860,11 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     const bool gtopk = qd::replay_lds_bytes(static_cast<int>(kp)) > std::min(qd::kLdsMax, qd::replay_room());
     const size_t rkeys = qd::rerank_keys(static_cast<int>(kp));
     const bool grerank = refine && qlds + rkeys * 8 > qd::rerank_room();
+    // the replay gathers several probes' candidate blocks into one load batch
+    // where a batch of 16 spans several probes: short lists and small TopK
+    // buffers (1.2M x 100, lists of about 10 blocks, k' = 100: 9% faster;
+    // 10M x 96, about 19 blocks a list, k' up to 1,000: 20% slower)
+    const int gather_replay = kp <= 256 && ivfh->blocks_desc_prefix.back() <= 16ull * nlist ? 1 : 0;
 
     // IVFIndex::scan's filter parameters (src/ivf_index.cpp:462, 502, 510-514)
     int min_probes = static_cast<int>(nprobe), wanted = 0, block_filter = 0;
@@ -1066,6 +1071,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.unsorted = refine ? 1 : 0;
         r.g_dist = gd ? gd + q0 * 2 * kp : nullptr;
         r.g_label = gl ? gl + q0 * 2 * kp : nullptr;
+        r.gather = gather_replay;
         StageTimer t_rep(QANN_STAGE_REPLAY, st);
         qd::launch_fs_replay(r, static_cast<int>(nc), st);
         check_launch("k_fl_replay");
@@ -1133,6 +1139,7 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
         r.unsorted = refine ? 1 : 0;
         r.g_dist = gd ? gd + q0 * 2 * kp : nullptr;
         r.g_label = gl ? gl + q0 * 2 * kp : nullptr;
+        r.gather = gather_replay;
         StageTimer t_rep(QANN_STAGE_REPLAY, st);
         qd::launch_fs_replay(r, static_cast<int>(nc), st);
         check_launch("k_fs_replay");

@@ -3858,8 +3858,9 @@ void launch_fs_replay(const ReplayArgs& a, int nq, hipStream_t s) {
     }
     const size_t wl = lds + static_cast<size_t>(aw.window) * (a.fdists ? 4 : 2);
     if ((forced < 0 || forced >= 2) && !a.g_dist && wl + 4096 <= kLdsMax) {
-        // the gathered form by default; QANN_AMD_REPLAY=win keeps a load per probe
-        if (forced == 2) {
+        // the gathered form where the search asks for it (QANN_AMD_REPLAY=gather
+        // forces it), else a probe's blocks at a time
+        if (forced == 2 || (forced < 0 && !a.gather)) {
             if (a.fdists) hipLaunchKernelGGL(k_fl_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
             else hipLaunchKernelGGL(k_fs_replay_win, dim3(nq), dim3(kWave), wl, s, aw);
         } else {

@@ -173,6 +173,7 @@ struct ReplayArgs {
     float* g_dist;             // nq * 2kp TopK buffers in HBM when they do not fit LDS, else null
     int64_t* g_label;
     int window;                // block minima the window replay keeps in LDS (set by the launcher)
+    int gather;                // 1: the window replay gathers several probes' blocks a batch
 };
 
 // LDS budget of a workgroup where more LDS buys fewer workgroups per CU:
