@@ -190,6 +190,39 @@ def test_gpu_sums_launch_forms_match_goldens(monkeypatch, name, form):
             assert_same(got, want, f"{name} {form} {stem}")
 
 
+@pytest.mark.parametrize("form", ["mfma", "grid"])
+def test_gpu_zero_row_minimums_match_oracle(tmp_path, monkeypatch, oracle, form):
+    # table rows whose minimum is exactly zero, whose sign the offset keeps:
+    # half of each cluster sits on its centre and the rest at +-o around it
+    # (integer entries, so the centres are the lists' exact means), so every
+    # PQ codebook holds an all-zero entry; queries near the centres then give
+    # rows of non-negative entries with a zero among them
+    monkeypatch.setenv("QANN_AMD_SUMS_FORM", form)
+    rng = np.random.default_rng(77)
+    d, per = 16, 80
+    centers = rng.integers(-6, 7, (100, d)).astype(np.float32) * 4.0
+    off = rng.integers(-1, 2, (per // 4, d)).astype(np.float32)
+    x = np.concatenate([np.concatenate([np.repeat(c[None], per // 2, 0), c + off, c - off]) for c in centers])
+    x = np.ascontiguousarray(x[rng.permutation(len(x))])
+    q = (centers[rng.integers(0, len(centers), 200)] + 0.01 * rng.standard_normal((200, d))).astype(np.float32)
+    ivf = qann_amd.IVFIndex(d, 100, 16)
+    ivf.enable_pq(8, fast_scan=True)
+    ivf.precomputed_tables = True
+    ivf.train(x, max_iters=6, seed=3)
+    ivf.add(x)
+    path = tmp_path / "zero_rows.qann"
+    ivf.save(str(path))
+    orc = oracle.load(path)
+    for npb in (1, 3):
+        ivf.nprobe = npb
+        orc.set_nprobe(npb)
+        reps = 2048 // q.shape[0] + 1
+        qq = np.tile(q, (reps, 1))
+        got = ivf.batch_query(qq, 10)
+        want = orc.batch_query(q, 10)
+        assert_same(got, (np.tile(want[0], (reps, 1)), np.tile(want[1], (reps, 1))), f"{form} nprobe {npb}")
+
+
 @pytest.mark.parametrize("name", ["l2_fs_refine", "cos_fs_refine", "ip_pq6_custom", "l2_sq8"])
 def test_gpu_graph_replays_match_goldens(name):
     # unfiltered searches run plainly three times, are captured as a HIP graph
