@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
                                                        uint64_t* gkeys) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count;
+    __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count, s_count2;
     __shared__ float s_red[NT / kWave], s_red8[2 * NT / kWave];
     float* qsh = reinterpret_cast<float*>(lds);
     // the shortlist's keys: in LDS after the query, or (gkeys, when more are
@@ -365,8 +365,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
     };
     bool found = false;
     if (SAMPLE) {
-        // tau from a sample of ns scores, one every nlist / ns lists: the rank r
-        // in the sample that puts about tc scores of the row at or below it. One
+        // tau from a sample of ns scores, runs of 16 consecutive lists one every
+        // 16 nlist / ns lists (a run is one 64-byte read; single scores that far
+        // apart would each cost a memory sector, as much as the whole row): the
+        // rank r in the sample that puts about tc scores of the row at or below it. One
         // pass over the row counts them and collects the candidates; when fewer
         // than M are at or below (or more than cap are candidates) the row's own
         // threshold is found below.
@@ -379,7 +381,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
 #pragma unroll
             for (int j = 0; j < kPer; ++j) {
                 const int si = tid + NT * j;
-                smp[j] = si < ns ? row[static_cast<size_t>(si) * stride] : INFINITY;
+                smp[j] = si < ns ? row[static_cast<size_t>(si >> 4) * 16 * stride + (si & 15)] : INFINITY;
             }
             const int r = std::min(ns, (tc * ns + nlist - 1) / nlist);
             auto each_smp = [&](auto&& fn) {
@@ -413,8 +415,32 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
         each_vl([&](float v, int l) { collect(v, l, limit); });
     }
     __syncthreads();
-    const int cnt = static_cast<int>(s_count);
+    int cnt = static_cast<int>(s_count);
     if (cnt >= M && cnt <= cap) { // fewer than M only when scores are NaN
+        if (cnt > M + 32 && cnt <= 16 * NT) {
+            // narrower: the M-th smallest approximate score among the candidates
+            // has M scores at or below it, so it is a tau of the same bound; only
+            // the candidates within it + 3 delta are scored exactly
+            auto akey = [&](int t) { return fkey(row[static_cast<uint32_t>(keys[t])]); };
+            const float lim2 = from_fkey(block_radix_select<uint32_t, true, decltype(akey), NT>(
+                                   cnt, static_cast<uint32_t>(M), akey, hist, s_wsum, &s_digit, &s_below)) +
+                               3.0f * delta;
+            uint32_t lj[16];
+            bool kj[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int t = tid + NT * j;
+                lj[j] = t < cnt ? static_cast<uint32_t>(keys[t]) : 0u;
+                kj[j] = t < cnt && row[lj[j]] <= lim2;
+            }
+            if (tid == 0) s_count2 = 0;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (kj[j]) keys[atomicAdd(&s_count2, 1u)] = lj[j];
+            __syncthreads();
+            cnt = static_cast<int>(s_count2);
+        }
         int np = 1;
         while (np < cnt) np <<= 1;
         for (int t = tid; t < np; t += blockDim.x) {
