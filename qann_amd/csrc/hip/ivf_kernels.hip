@@ -265,12 +265,14 @@ __global__ void __launch_bounds__(256) k_coarse_gemm32(const float* q, int nq, c
 }
 
 // Per query (one workgroup): the M smallest (A, list), A exact, sorted. The
-// candidates are the lists whose approximate score is at most tau + 3 delta,
-// tau an approximate score with at least M at or below it (so every list of
-// the M nearest is a candidate) and delta twice the bound on
-// |A - approx - const| (exact side: (n + 3) u (|q| + |c|)^2; GEMM side with
-// |c|^2 rounded: (n + 2) u (|q| + |c|)^2; inner product: 2 (n + 1) u |q| |c|;
-// plus n 2^-125 for flushed denormals). If more than `cap` lists qualify (or
+// candidates are the lists whose approximate score is at most tau + 2 delta,
+// tau an approximate score with at least M at or below it and delta twice the
+// bound on |A - approx - const| (exact side: (n + 3) u (|q| + |c|)^2; GEMM
+// side with |c|^2 rounded: (n + 2) u (|q| + |c|)^2; inner product:
+// 2 (n + 1) u |q| |c|; plus n 2^-125 for flushed denormals). tau + delta holds
+// every list of the M nearest (the M lists at or below tau have A at most
+// tau + const + delta / 2, so a list of the M nearest has an approximate score
+// at most that + delta / 2); the second delta is margin. If more than `cap` lists qualify (or
 // fewer than M, with NaN scores), every list is scored exactly and selected by
 // the full 64-bit key.
 // CD: values of the row each thread keeps in registers (the row, nlist <=
@@ -391,7 +393,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
             };
             const float ts = select_threshold_each<false, NT>(each_smp, r, r + std::max(16, r / 4), hist, s_wsum,
                                                               &s_digit, &s_below, s_red8);
-            const float limit = ts + 3.0f * delta;
+            const float limit = ts + 2.0f * delta;
             uint32_t le = 0;
             each_vl([&](float v, int l) {
                 le += v <= ts ? 1u : 0u;
@@ -411,7 +413,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
     if (!found) {
         auto each_all = [&](auto&& fn) { each_vl([&](float v, int) { fn(v); }); };
         const float tau = select_threshold_each<false, NT>(each_all, M, target, hist, s_wsum, &s_digit, &s_below, s_red8);
-        const float limit = tau + 3.0f * delta;
+        const float limit = tau + 2.0f * delta;
         each_vl([&](float v, int l) { collect(v, l, limit); });
     }
     __syncthreads();
@@ -420,11 +422,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
         if (cnt > M + 32 && cnt <= 16 * NT) {
             // narrower: the M-th smallest approximate score among the candidates
             // has M scores at or below it, so it is a tau of the same bound; only
-            // the candidates within it + 3 delta are scored exactly
+            // the candidates within it + 2 delta are scored exactly
             auto akey = [&](int t) { return fkey(row[static_cast<uint32_t>(keys[t])]); };
             const float lim2 = from_fkey(block_radix_select<uint32_t, true, decltype(akey), NT>(
                                    cnt, static_cast<uint32_t>(M), akey, hist, s_wsum, &s_digit, &s_below)) +
-                               3.0f * delta;
+                               2.0f * delta;
             uint32_t lj[16];
             bool kj[16];
 #pragma unroll
@@ -3487,10 +3489,18 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
                           int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp,
                           uint64_t* gkeys) {
     if (nq <= 0) return;
-    const size_t cap = coarse_select_cap(M);
+    size_t cap = coarse_select_cap(M);
     const bool global = coarse_select_global(M, dim);
     if (global && !gkeys) throw std::invalid_argument("the coarse select needs global keys for this many probes");
     if (!global) gkeys = nullptr;
+    // rows of more than 4,096 lists hold up to 4,096 candidates in LDS: where
+    // many lists lie about as far from a query (a large cluster split into many
+    // lists), the rounding window around the threshold holds hundreds of them,
+    // and past the cap a query scores every list exactly
+    if (!global && nlist > 4096) {
+        const size_t wide = std::max<size_t>(cap, 4096);
+        if (query_lds_bytes(dim) + wide * 8 <= coarse_select_room()) cap = wide;
+    }
     const size_t lds = query_lds_bytes(dim) + (global ? 0 : cap * 8);
     // each query's row of scores stays in registers, 16 values a thread: with
     // 256 threads up to 4,096 lists, with 1,024 up to 16,384; larger rows are
