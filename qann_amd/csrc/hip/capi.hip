@@ -973,12 +973,18 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     float* C = nullptr;
     uint8_t* lut = nullptr;
     float *lscale = nullptr, *loff = nullptr;
+    // a fast-scan search's L2 tables are made chunk by chunk below, only for
+    // the sums forms that read them (the matrix-core form builds each pair's
+    // table itself from B and the PQ codebook)
+    const bool defer_tables = fs && use_pre;
     StageTimer t_tab(QANN_STAGE_TABLES, st);
     if (use_pre || (ip_pq && !fs)) {
         C = sc.take<float>(nq * tsz);
-        qd::launch_query_tables(qr, ix, static_cast<int>(nq), use_pre ? qd::kTableInner : qd::kTableIp, C, nullptr,
-                                nullptr, nullptr, st);
-        check_launch("k_query_tables");
+        if (!defer_tables) {
+            qd::launch_query_tables(qr, ix, static_cast<int>(nq), use_pre ? qd::kTableInner : qd::kTableIp, C,
+                                    nullptr, nullptr, nullptr, st);
+            check_launch("k_query_tables");
+        }
     } else if (ip_pq) {
         lut = sc.take<uint8_t>(nq * m * 16);
         lscale = sc.take<float>(nq);
@@ -1109,6 +1115,13 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
                                        ord_hist, order, st);
                 check_launch("k_order");
             }
+        }
+        if (defer_tables && qd::fs_sums_reads_tables(a, static_cast<int>(nc))) {
+            StageTimer t_ctab(QANN_STAGE_TABLES, st);
+            qd::launch_query_tables(a.q, ix, static_cast<int>(nc), qd::kTableInner, C + q0 * bb, nullptr, nullptr,
+                                    nullptr, st);
+            check_launch("k_query_tables");
+            t_ctab.done();
         }
         StageTimer t_scan(QANN_STAGE_SCAN, st);
         a.grouped = t_scan.mark();

@@ -3714,6 +3714,25 @@ bool fs_sums_reads_order(SumsArgs a, int nq) {
     return it == g_sums_form.end() || it->second != 2;
 }
 
+bool fs_sums_reads_tables(SumsArgs a, int nq) {
+    // the matrix-core form builds each pair's L2 table from B and the PQ
+    // codebook: when it runs (forced, or kept by the tuner for this problem
+    // class) the per-query tables are unused
+    a.nq = nq;
+    if (a.ip_lut) return true;
+    const bool mfma = fs_mfma_ok(a);
+    const char* force = std::getenv("QANN_AMD_SUMS_FORM");
+    if (force && std::strcmp(force, "mfma") == 0 && mfma) return false;
+    if (force && *force) return true;
+    if (!mfma || !a.queue) return true;
+    int nclass = 0;
+    while ((1 << nclass) < nq) ++nclass;
+    const SumsKey key{a.ix.m4, a.win, a.nwin, nclass, a.order != nullptr, 2};
+    std::lock_guard<std::mutex> g(g_sums_mu);
+    const auto it = g_sums_form.find(key);
+    return it == g_sums_form.end() || it->second != 2;
+}
+
 int launch_fs_sums(SumsArgs a, int nq, hipStream_t s) {
     const size_t lds = fs_sums_lds(a);
     a.nq = nq;

@@ -244,6 +244,8 @@ size_t fs_sums_lds(SumsArgs& a);
 // grouping).
 // Whether the sums form launch_fs_sums will run reads a.order (the query order).
 bool fs_sums_reads_order(SumsArgs a, int nq);
+// Whether it reads a.C (the per-query L2 tables).
+bool fs_sums_reads_tables(SumsArgs a, int nq);
 int launch_fs_sums(SumsArgs a, int nq, hipStream_t s);
 // Whether the matrix-core form can run these sums: inner-product LUTs or
 // precomputed L2 tables, m4 <= 64, the pair buffers given.
