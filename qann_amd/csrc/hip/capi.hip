@@ -619,10 +619,11 @@ void coarse_scores(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int
 // The M nearest lists of each query from those scores (the bound of the form
 // that wrote them).
 void coarse_probes(const qann_hip_index* ivfh, const float* qr, uint64_t nq, int ip, uint64_t M, bool split,
-                   float* D, uint16_t* pidx, float* pval, uint64_t* gkeys, hipStream_t st) {
+                   float* D, uint16_t* pidx, float* pval, uint64_t* gkeys, hipStream_t st,
+                   const qd::ProbeBaseOut& pb = qd::ProbeBaseOut{}) {
     const qd::DevIvf& ix = ivfh->ivf;
     qd::launch_coarse_select(D, qr, static_cast<int>(nq), ix.centroids, ix.cnorm_max, ix.nlist, ix.dim, ip,
-                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0, gkeys);
+                             static_cast<int>(M), pidx, pval, st, split ? ivfh->split_kp : 0, gkeys, pb);
     check_launch("k_coarse_select");
 }
 
@@ -945,12 +946,11 @@ void run_search(qann_hip_index* h, const float* d_q, uint64_t nq, const qann_hip
     StageTimer t_coarse(QANN_STAGE_COARSE, st);
     coarse_scores(ivfh, qr, nq, ip, split, q3, D, st);
     t_coarse.done();
-    StageTimer t_sel(QANN_STAGE_SELECT, st);
-    coarse_probes(ivfh, qr, nq, ip, M, split, D, pidx, pval, gkeys, st);
     float* pA = sc.take<float>(nq * M);
     float* pbase = sc.take<float>(nq * M);
-    qd::launch_probe_base(qr, ix, static_cast<int>(nq), static_cast<int>(M), pidx, pval, pA, pbase, st);
-    check_launch("k_probe_base");
+    StageTimer t_sel(QANN_STAGE_SELECT, st);
+    // the probes, and each probe's A and base for the scan
+    coarse_probes(ivfh, qr, nq, ip, M, split, D, pidx, pval, gkeys, st, qd::probe_base_out(ix, pA, pbase));
     uint32_t* list_allowed = nullptr;
     if (filtered && !mix) {
         list_allowed = sc.take<uint32_t>(nlist);

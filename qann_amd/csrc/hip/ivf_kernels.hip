@@ -5,7 +5,6 @@
 //   k_coarse_gemm     approximate coarse scores for every (query, list), MFMA
 //   k_coarse_select   the M nearest lists by exact A, sorted by (A, list)
 //                     (IVFIndex::coarse / coarse_batch, src/ivf_index.cpp:305-419)
-//   k_probe_base      per-probe A (residual centres) and the IP/cosine base term
 //   k_query_tables    C = -2<q_s, y_sj> (PQCodebook::compute_inner_table, src/pq.cpp:186-204),
 //                     or the inner-product LUT quantized once per query (scan, :448-456)
 //   k_probe_plan      probes each query scans (filters: min_probes / wanted) and sum offsets
@@ -283,7 +282,7 @@ template <int NT, int CD, bool SAMPLE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE ? 4 : 5))) k_coarse_select(float* D, const float* q, const float* cents, int nlist,
                                                        int dim, int ip, float cnorm_max, int M, int cap,
                                                        uint16_t* out_idx, float* out_val, int split_kp,
-                                                       uint64_t* gkeys) {
+                                                       uint64_t* gkeys, const ProbeBaseOut pb) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t s_wsum[4], s_digit, s_below, s_count, s_count2;
@@ -298,6 +297,24 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
     float* row = D + static_cast<size_t>(i) * nlist;
     const bool qcopy = dim <= kQueryLdsDims;
     const float* qv = qcopy ? qsh : q + static_cast<size_t>(i) * dim;
+    // probe t of this query: the list and its coarse distance, and with pb.A
+    // the probe's A and base (the query already in LDS)
+    auto put_probe = [&](uint64_t key, int t) {
+        const uint32_t l = key_index(key);
+        const float v = key_dist(key);
+        const size_t o = static_cast<size_t>(i) * M + t;
+        out_idx[o] = static_cast<uint16_t>(l);
+        out_val[o] = v;
+        if (pb.A) {
+            const float A = pb.residual_centers
+                                ? exact_score(pb.ip != 0, qv, pb.residual_centers + static_cast<size_t>(l) * dim, dim)
+                                : v;
+            pb.A[o] = A;
+            pb.base[o] = pb.cosine_base
+                             ? (A + 1.0f) - (pb.residual_centers ? pb.residual_sqnorms[l] : pb.centroid_sqnorms[l])
+                             : A;
+        }
+    };
     float ss = 0.0f;
     for (int t = tid; t < dim; t += blockDim.x) {
         const float v = q[static_cast<size_t>(i) * dim + t];
@@ -454,10 +471,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
             }
         }
         block_bitonic_sort(keys, np);
-        for (int t = tid; t < M; t += blockDim.x) {
-            out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(key_index(keys[t]));
-            out_val[static_cast<size_t>(i) * M + t] = key_dist(keys[t]);
-        }
+        for (int t = tid; t < M; t += blockDim.x) put_probe(keys[t], t);
         return;
     }
     // every list exactly, then the M smallest (ckey(A), list) keys
@@ -477,10 +491,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
         if (k <= kth) keys[atomicAdd(&s_count, 1u)] = k;
     }
     block_bitonic_sort(keys, Mp);
-    for (int t = tid; t < M; t += blockDim.x) {
-        out_idx[static_cast<size_t>(i) * M + t] = static_cast<uint16_t>(key_index(keys[t]));
-        out_val[static_cast<size_t>(i) * M + t] = key_dist(keys[t]);
-    }
+    for (int t = tid; t < M; t += blockDim.x) put_probe(keys[t], t);
 }
 
 // FlatIndex::query_impl (src/flat_index.cpp:33-64) from approximate scores:
@@ -592,28 +603,6 @@ __global__ void __launch_bounds__(256) k_flat_select(float* D, const float* q, c
         out_ids[static_cast<size_t>(i) * k + t] = id;
         out_dists[static_cast<size_t>(i) * k + t] = d;
     }
-}
-
-// A per probe: with residual centres the list's own exact term
-// (scan, src/ivf_index.cpp:526-529), else the coarse distance. For the
-// inner-product tables, base = A, or for Cosine (A + 1) - |c|^2 (:567-570).
-__global__ void __attribute__((amdgpu_waves_per_eu(6))) k_probe_base(const float* q, const DevIvf ix, int nq, int M, const uint16_t* probe_idx,
-                             const float* probe_val, float* A_out, float* base_out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nq * M) return;
-    const int i = t / M;
-    const int l = probe_idx[t];
-    float A = probe_val[t];
-    if (ix.residual_centers)
-        A = exact_score(ix.metric == 2, q + static_cast<size_t>(i) * ix.dim,
-                        ix.residual_centers + static_cast<size_t>(l) * ix.dim, ix.dim);
-    A_out[t] = A;
-    float base = A;
-    if (ix.metric == 1 && ix.cosine_ip) {
-        const float sq = ix.residual_centers ? ix.residual_sqnorms[l] : ix.centroid_sqnorms[l];
-        base = (A + 1.0f) - sq;
-    }
-    base_out[t] = base;
 }
 
 // Row minimum / maximum / widest / offset and the bytes of one m x 16 table
@@ -3485,9 +3474,21 @@ bool coarse_select_global(int M, int dim) {
     return query_lds_bytes(dim) + coarse_select_cap(M) * 8 > coarse_select_room();
 }
 
+ProbeBaseOut probe_base_out(const DevIvf& ix, float* A, float* base) {
+    ProbeBaseOut pb;
+    pb.residual_centers = ix.residual_centers;
+    pb.residual_sqnorms = ix.residual_sqnorms;
+    pb.centroid_sqnorms = ix.centroid_sqnorms;
+    pb.ip = ix.metric == 2 ? 1 : 0;
+    pb.cosine_base = ix.metric == 1 && ix.cosine_ip ? 1 : 0;
+    pb.A = A;
+    pb.base = base;
+    return pb;
+}
+
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
                           int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp,
-                          uint64_t* gkeys) {
+                          uint64_t* gkeys, const ProbeBaseOut& pb) {
     if (nq <= 0) return;
     size_t cap = coarse_select_cap(M);
     const bool global = coarse_select_global(M, dim);
@@ -3514,24 +3515,24 @@ void launch_coarse_select(float* D, const float* q, int nq, const float* cents, 
     if (nlist <= 16 * 256) {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<256, 16, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys, pb);
         else
             hipLaunchKernelGGL((k_coarse_select<256, 16, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys, pb);
     } else if (nlist <= 16 * 1024) {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<1024, 16, true>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys, pb);
         else
             hipLaunchKernelGGL((k_coarse_select<1024, 16, false>), dim3(nq), dim3(1024), lds, s, D, q, cents, nlist,
-                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
+                               dim, ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys, pb);
     } else {
         if (smp)
             hipLaunchKernelGGL((k_coarse_select<256, 0, true>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim, ip,
-                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
+                               cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys, pb);
         else
             hipLaunchKernelGGL((k_coarse_select<256, 0, false>), dim3(nq), dim3(256), lds, s, D, q, cents, nlist, dim,
-                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys);
+                               ip, cnorm_max, M, static_cast<int>(cap), out_idx, out_val, split_kp, gkeys, pb);
     }
 }
 
@@ -3545,11 +3546,6 @@ void launch_flat_select(float* D, const float* q, int nq, const float* rows, int
                        static_cast<int>(cap), excl, incl, labels, out_ids, out_dists);
 }
 
-void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
-                       float* A, float* base, hipStream_t s) {
-    hipLaunchKernelGGL(k_probe_base, dim3(cdiv(static_cast<size_t>(nq) * M, 256)), dim3(256), 0, s, q, ix, nq, M, idx,
-                       val, A, base);
-}
 
 void launch_query_tables(const float* q, const DevIvf& ix, int nq, int mode, float* C, uint8_t* lut, float* scale,
                          float* offset, hipStream_t s) {

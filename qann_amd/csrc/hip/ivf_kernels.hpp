@@ -216,16 +216,29 @@ void launch_coarse_gemm(const float* q, int nq, const float* cents, const float*
 // coarse_select_global(M, dim), the shortlist does not fit LDS and gkeys must
 // hold nq * coarse_select_cap(M) keys of global memory.
 bool coarse_select_global(int M, int dim);
+// Each probe's A and base as the scan takes them (src/ivf_index.cpp:526-529,
+// 567-570), written by the select beside the probe (null A: not wanted): A is
+// the list's own exact term when the index has residual centres, else the
+// coarse distance; base = A, or for Cosine inner-product tables
+// (A + 1) - |c|^2.
+struct ProbeBaseOut {
+    const float* residual_centers = nullptr; // nlist * dim or null
+    const float* residual_sqnorms = nullptr;
+    const float* centroid_sqnorms = nullptr;
+    int ip = 0;          // residual terms by inner product (Metric::InnerProduct)
+    int cosine_base = 0; // base = (A + 1) - |c|^2
+    float* A = nullptr;  // nq * M
+    float* base = nullptr;
+};
+ProbeBaseOut probe_base_out(const DevIvf& ix, float* A, float* base);
 void launch_coarse_select(float* D, const float* q, int nq, const float* cents, float cnorm_max, int nlist, int dim,
                           int ip, int M, uint16_t* out_idx, float* out_val, hipStream_t s, int split_kp = 0,
-                          uint64_t* gkeys = nullptr);
+                          uint64_t* gkeys = nullptr, const ProbeBaseOut& pb = ProbeBaseOut{});
 // FlatIndex over D from launch_coarse_gemm (rows as the "centroids", masked by
 // excl / incl): the k nearest rows per query, exact, in (distance, slot) order.
 void launch_flat_select(float* D, const float* q, int nq, const float* rows, int n, int dim, int ip, float xnorm_max,
                         int k, const uint64_t* excl, const uint64_t* incl, const int64_t* labels, int64_t* out_ids,
                         float* out_dists, hipStream_t s);
-void launch_probe_base(const float* q, const DevIvf& ix, int nq, int M, const uint16_t* idx, const float* val,
-                       float* A, float* base, hipStream_t s);
 // Per query: kTableInner the inner table C (m * K), kTableIp the inner-product
 // scoring table (C, halved for Metric::InnerProduct), kTableIpLut that table
 // rounded to fast-scan bytes (K = 16) with its scale and offset.
