@@ -174,6 +174,93 @@ __device__ __forceinline__ float exact_score(bool ip, const float* a, const floa
     return ip ? -exact_avx2<true>(a, b, n) : exact_avx2<false>(a, b, n);
 }
 
+// A float from lane (this lane ^ 1) or (this lane ^ 2) of its quad.
+__device__ __forceinline__ float quad_xor1(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float quad_xor2(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x4E, 0xF, 0xF, true));
+}
+
+// exact_avx2 computed by the four lanes of a quad (g = lane % 4, all four
+// active): lane g holds accumulators 8 g .. 8 g + 7 and reads elements
+// 8 g .. 8 g + 7 of each chunk of 32, lane 0 the 8-wide steps after the
+// chunks; the fold (acc[l] + acc[8 + l]) + (acc[16 + l] + acc[24 + l]) pairs
+// lanes g ^ 1, then g ^ 2 (each add commutes, so every lane gets the same
+// bits); the horizontal sum and the tail are computed by every lane. Every
+// lane returns exact_avx2's result, with a quarter of its loads and
+// accumulators.
+template <bool IP>
+__device__ __forceinline__ float exact_avx2_quad(const float* __restrict__ a, const float* __restrict__ b, int n,
+                                                 int g) {
+    float acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = 0.0f;
+    int i = 0;
+    if (((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0) {
+        const float4* a4 = reinterpret_cast<const float4*>(a);
+        const float4* b4 = reinterpret_cast<const float4*>(b);
+        for (; i + 32 <= n; i += 32) {
+            const int o = (i + 8 * g) / 4;
+            exact_step4<IP>(acc, a4[o], b4[o]);
+            exact_step4<IP>(acc + 4, a4[o + 1], b4[o + 1]);
+        }
+        for (; i + 8 <= n; i += 8) {
+            if (g == 0) {
+                exact_step4<IP>(acc, a4[i / 4], b4[i / 4]);
+                exact_step4<IP>(acc + 4, a4[i / 4 + 1], b4[i / 4 + 1]);
+            }
+        }
+    } else {
+        for (; i + 32 <= n; i += 32) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) exact_step<IP>(acc[t], a[i + 8 * g + t], b[i + 8 * g + t]);
+        }
+        for (; i + 8 <= n; i += 8) {
+            if (g == 0) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) exact_step<IP>(acc[t], a[i + t], b[i + t]);
+            }
+        }
+    }
+    float v[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        const float s = acc[l] + quad_xor1(acc[l]);
+        v[l] = s + quad_xor2(s);
+    }
+    float res = ((v[0] + v[4]) + (v[1] + v[5])) + ((v[2] + v[6]) + (v[3] + v[7]));
+    const int r = n - i;
+    int j = 0;
+    if (r >= 4) {
+        for (; j < 4; ++j) {
+            const float x = a[i + j], y = b[i + j];
+            float p;
+            if (IP) {
+                p = x * y;
+            } else {
+                const float d = x - y;
+                p = d * d;
+            }
+            res = res + p;
+        }
+    }
+    for (; j < r; ++j) {
+        const float x = a[i + j], y = b[i + j];
+        if (IP) {
+            res = fmaf(x, y, res);
+        } else {
+            const float d = x - y;
+            res = fmaf(d, d, res);
+        }
+    }
+    return res;
+}
+
+__device__ __forceinline__ float exact_score_quad(bool ip, const float* a, const float* b, int n, int g) {
+    return ip ? -exact_avx2_quad<true>(a, b, n, g) : exact_avx2_quad<false>(a, b, n, g);
+}
+
 // SQ::score (src/sq.cpp:126-211) as the AVX2 build computes it: each value
 // decoded as fma(code, step, vmin), 8 lane accumulators fed by fma (the
 // difference squared, or q times the value), folded as ((l0+l4)+(l2+l6)) +

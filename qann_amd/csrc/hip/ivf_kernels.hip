@@ -297,22 +297,29 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
     float* row = D + static_cast<size_t>(i) * nlist;
     const bool qcopy = dim <= kQueryLdsDims;
     const float* qv = qcopy ? qsh : q + static_cast<size_t>(i) * dim;
-    // probe t of this query: the list and its coarse distance, and with pb.A
-    // the probe's A and base (the query already in LDS)
-    auto put_probe = [&](uint64_t key, int t) {
-        const uint32_t l = key_index(key);
-        const float v = key_dist(key);
-        const size_t o = static_cast<size_t>(i) * M + t;
-        out_idx[o] = static_cast<uint16_t>(l);
-        out_val[o] = v;
-        if (pb.A) {
-            const float A = pb.residual_centers
-                                ? exact_score(pb.ip != 0, qv, pb.residual_centers + static_cast<size_t>(l) * dim, dim)
-                                : v;
-            pb.A[o] = A;
-            pb.base[o] = pb.cosine_base
-                             ? (A + 1.0f) - (pb.residual_centers ? pb.residual_sqnorms[l] : pb.centroid_sqnorms[l])
-                             : A;
+    // the M probes of this query, by quads of lanes: each probe's list and
+    // coarse distance, and with pb.A its A and base (the residual-centre term
+    // by the quad's four lanes, the query already in LDS)
+    auto put_probes = [&]() {
+        const int g = tid & 3;
+        for (int t = tid >> 2; t < M; t += NT / 4) {
+            const uint64_t key = keys[t];
+            const uint32_t l = key_index(key);
+            const float v = key_dist(key);
+            const size_t o = static_cast<size_t>(i) * M + t;
+            float A = v;
+            if (pb.A && pb.residual_centers)
+                A = exact_score_quad(pb.ip != 0, qv, pb.residual_centers + static_cast<size_t>(l) * dim, dim, g);
+            if (g == 0) {
+                out_idx[o] = static_cast<uint16_t>(l);
+                out_val[o] = v;
+                if (pb.A) {
+                    pb.A[o] = A;
+                    pb.base[o] = pb.cosine_base ? (A + 1.0f) - (pb.residual_centers ? pb.residual_sqnorms[l]
+                                                                                     : pb.centroid_sqnorms[l])
+                                                : A;
+                }
+            }
         }
     };
     float ss = 0.0f;
@@ -471,7 +478,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
             }
         }
         block_bitonic_sort(keys, np);
-        for (int t = tid; t < M; t += blockDim.x) put_probe(keys[t], t);
+        put_probes();
         return;
     }
     // every list exactly, then the M smallest (ckey(A), list) keys
@@ -491,7 +498,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SAMPLE 
         if (k <= kth) keys[atomicAdd(&s_count, 1u)] = k;
     }
     block_bitonic_sort(keys, Mp);
-    for (int t = tid; t < M; t += blockDim.x) put_probe(keys[t], t);
+    put_probes();
 }
 
 // FlatIndex::query_impl (src/flat_index.cpp:33-64) from approximate scores:
