@@ -1306,37 +1306,72 @@ __global__ void __launch_bounds__(256) k_pairs_lists(uint32_t* ghist, int groups
 }
 
 // pair_off / unit_off: exclusive prefix sums over the lists of the pair totals
-// and of their 32-pair units, with the totals at index n. One 1024-thread block.
+// and of their 32-pair units, with the totals at index n. One 1024-thread
+// block over chunks of 4,096 lists: a chunk's
+// totals are read into LDS in one coalesced pass, thread t takes lists
+// 4t .. 4t + 3 of it, an inclusive scan within each wave and then over the
+// 16 waves' sums gives each thread its place, and the running totals carry
+// to the next chunk.
+constexpr int kOffChunk = 4096;
 __global__ void __launch_bounds__(1024) k_pairs_offsets(const uint32_t* tot, int n, uint32_t* pair_off,
                                                         uint32_t* unit_off) {
-    __shared__ uint32_t part_p[1024], part_u[1024];
-    const int t = threadIdx.x, per = (n + 1023) / 1024;
-    const int b = min(n, t * per), e = min(n, b + per);
-    uint32_t sp = 0, su = 0;
-    for (int l = b; l < e; ++l) {
-        sp += tot[l];
-        su += (tot[l] + 31) / 32;
-    }
-    part_p[t] = sp;
-    part_u[t] = su;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t vp = t >= off ? part_p[t - off] : 0u, vu = t >= off ? part_u[t - off] : 0u;
+    __shared__ uint32_t s_tot[kOffChunk];
+    __shared__ uint32_t w_p[16], w_u[16];
+    const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
+    uint32_t carry_p = 0, carry_u = 0;
+    for (int base = 0; base < n; base += kOffChunk) {
+        const int cnt = min(kOffChunk, n - base);
+        for (int j = t; j < cnt; j += 1024) s_tot[j] = tot[base + j];
         __syncthreads();
-        part_p[t] += vp;
-        part_u[t] += vu;
+        uint32_t c[4], sp = 0, su = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int l = 4 * t + k;
+            c[k] = l < cnt ? s_tot[l] : 0u;
+            sp += c[k];
+            su += (c[k] + 31) / 32;
+        }
+        uint32_t ip = sp, iu = su; // inclusive, within the wave
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t yp = static_cast<uint32_t>(__shfl_up(static_cast<int>(ip), off, kWave));
+            const uint32_t yu = static_cast<uint32_t>(__shfl_up(static_cast<int>(iu), off, kWave));
+            if (lane >= off) {
+                ip += yp;
+                iu += yu;
+            }
+        }
+        if (lane == kWave - 1) {
+            w_p[w] = ip;
+            w_u[w] = iu;
+        }
         __syncthreads();
+        uint32_t bp = carry_p, bu = carry_u, tp = 0, tu = 0;
+        for (int v = 0; v < 16; ++v) {
+            if (v < w) {
+                bp += w_p[v];
+                bu += w_u[v];
+            }
+            tp += w_p[v];
+            tu += w_u[v];
+        }
+        uint32_t rp = bp + ip - sp, ru = bu + iu - su;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int l = 4 * t + k;
+            if (l < cnt) {
+                pair_off[base + l] = rp;
+                unit_off[base + l] = ru;
+                rp += c[k];
+                ru += (c[k] + 31) / 32;
+            }
+        }
+        carry_p += tp;
+        carry_u += tu;
+        __syncthreads(); // s_tot and the wave sums are rewritten by the next chunk
     }
-    uint32_t rp = part_p[t] - sp, ru = part_u[t] - su;
-    for (int l = b; l < e; ++l) {
-        pair_off[l] = rp;
-        unit_off[l] = ru;
-        rp += tot[l];
-        ru += (tot[l] + 31) / 32;
-    }
-    if (t == 1023) {
-        pair_off[n] = part_p[1023];
-        unit_off[n] = part_u[1023];
+    if (t == 0) {
+        pair_off[n] = carry_p;
+        unit_off[n] = carry_u;
     }
 }
 
