@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--max-batch", type=int, default=4096, help="queries one launch may take")
     ap.add_argument("--stage-seconds", type=float, default=2.0, help="length of the stage-profiled load (0: none)")
+    ap.add_argument("--offered", type=float, default=60_000.0,
+                    help="requests per second offered to find the saturated rate (far more than are served)")
     args = ap.parse_args()
 
     import torch  # noqa: F401  (first: torch and the library share the HIP runtime)
@@ -174,10 +176,14 @@ def main():
         same += int(np.array_equal(got_ids, want_ids) and np.array_equal(got_d.view(np.uint32), want_d.view(np.uint32)))
     log(f"served results identical to single calls: {same}/{len(checks)}")
 
-    sat, levels, mean_nq = served_levels(qann_amd, server, pool, spec, args.seconds)
+    sat, levels, mean_nq = served_levels(qann_amd, server, pool, spec, args.seconds, offered=args.offered)
     sat_qps = sat["qps"]
+    # saturated only if the loop served clearly fewer requests than were offered
+    served_rate = sat_qps / mean_nq
+    saturated = served_rate < 0.9 * args.offered
     log(f"saturated: {sat_qps:,.0f} queries/s over {sat['requests']} requests in {sat['launches']} launches "
-        f"({sat_qps / plain['qps']:.2f}x the plain path)")
+        f"({sat_qps / plain['qps']:.2f}x the plain path; {served_rate:,.0f} of {args.offered:,.0f} requests/s offered"
+        f"{'' if saturated else ' -- NOT saturated: raise --offered'})")
     # per-stage kernel time of the searches (the library's stage profiler: each
     # stage is synchronized and timed with HIP events, so this pass is slower)
     stages = {}
@@ -209,6 +215,8 @@ def main():
         "workload": f"serving: {n:,} x {dim} L2, IVF{nlist} + PQ{m}x4 fast-scan + RefineIndex(k_factor={kf}) in HBM; "
                     "Poisson requests of 1-256 queries, k in {10, 100}, nprobe in {16, 64}, 30% filtered at 10%",
         "saturated_qps": round(sat_qps, 1),
+        "offered_requests_per_s": args.offered,
+        "saturated": saturated,
         "plain_fp32_one_launch_per_request": plain,
         "saturated_over_plain": round(sat_qps / plain["qps"], 2),
         "levels": levels,
