@@ -271,7 +271,9 @@ __global__ void __launch_bounds__(256) k_coarse_gemm32(const float* q, int nq, c
 // 2 (n + 1) u |q| |c|; plus n 2^-125 for flushed denormals). tau + delta holds
 // every list of the M nearest (the M lists at or below tau have A at most
 // tau + const + delta / 2, so a list of the M nearest has an approximate score
-// at most that + delta / 2); the second delta is margin. If more than `cap` lists qualify (or
+// at most that + delta / 2); the second delta is margin. Before the exact
+// scores the shortlist is narrowed the same way from the M-th smallest
+// approximate score among the candidates. If more than `cap` lists qualify (or
 // fewer than M, with NaN scores), every list is scored exactly and selected by
 // the full 64-bit key.
 // CD: values of the row each thread keeps in registers (the row, nlist <=
@@ -1307,11 +1309,10 @@ __global__ void __launch_bounds__(256) k_pairs_lists(uint32_t* ghist, int groups
 
 // pair_off / unit_off: exclusive prefix sums over the lists of the pair totals
 // and of their 32-pair units, with the totals at index n. One 1024-thread
-// block over chunks of 4,096 lists: a chunk's
-// totals are read into LDS in one coalesced pass, thread t takes lists
-// 4t .. 4t + 3 of it, an inclusive scan within each wave and then over the
-// 16 waves' sums gives each thread its place, and the running totals carry
-// to the next chunk.
+// block over chunks of 4,096 lists: a chunk's totals are read into LDS in one
+// coalesced pass, thread t takes lists 4t .. 4t + 3 of it, an inclusive scan
+// within each wave and then over the 16 waves' sums gives each thread its
+// place, and the running totals carry to the next chunk.
 constexpr int kOffChunk = 4096;
 __global__ void __launch_bounds__(1024) k_pairs_offsets(const uint32_t* tot, int n, uint32_t* pair_off,
                                                         uint32_t* unit_off) {
