@@ -39,7 +39,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "QPS @ recall@10≥0.95 (1.2M×100 cosine) vs plain GPU & CPU; p50/p99 ms"
 N_BASE, N_QUERY, DIM, N_CENTRES, SIGMA, ZIPF, DECAY = 1_200_000, 10_000, 100, 2048, 0.1, 1.1, 0.8
 NLIST, M, K_FACTOR, K = 4096, 50, 10, 10
-NPROBES = (8, 16, 32, 40, 48, 56, 64, 96, 128)
+NPROBES = (8, 16, 32, 40, 48, 50, 52, 54, 56, 64, 96, 128)
 TRAIN_ROWS = 64 * NLIST
 SEED = 1
 PEAK_HBM_GBS = 8000.0
